@@ -1,0 +1,215 @@
+"""The slice of Newton's Model/State/ModelBuilder the DFSPH plug-in reads, on torch tensors.
+
+SolverDFSPH reads a Newton ``Model`` through these field names (alloy/solvers/sph/solver.py:188-196,
+layout.py:47-72, boundary.py:79-95): ``particle_q/qd/mass/flags/world/radius``, ``particle_world_start``,
+``gravity[W]``, ``sph.rest_density/phase`` and the particle-colliding shapes ``shape_type/scale/transform/body/
+world/flags/material_mu/collision_radius``. This module provides exactly those fields, built on the host and
+finalised to torch tensors on one device, so scenes and the solver need neither Warp nor Newton.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from enum import IntEnum
+
+import numpy as np
+import torch
+
+
+class ParticleFlags(IntEnum):
+    """Newton's particle flags (alloy/_vendor/newton/_src/geometry/flags.py:9-24)."""
+
+    ACTIVE = 1 << 0
+    PROXY = 1 << 1
+
+
+class ShapeFlags(IntEnum):
+    VISIBLE = 1 << 0
+    COLLIDE_SHAPES = 1 << 1
+    COLLIDE_PARTICLES = 1 << 2
+
+
+class GeoType(IntEnum):
+    """Newton's GeoType values (alloy/_vendor/newton/_src/geometry/types.py:73-100) for the shapes handled here."""
+
+    PLANE = 1
+    SPHERE = 3
+    BOX = 7
+
+
+@dataclass
+class SPHAttributes:
+    """model.sph: per-particle rest density [kg/m^3], viscosity [Pa s] and phase (config.py:244-261)."""
+
+    rest_density: torch.Tensor
+    viscosity: torch.Tensor
+    phase: torch.Tensor
+
+
+@dataclass
+class State:
+    particle_q: torch.Tensor   # [n, 3] float32
+    particle_qd: torch.Tensor  # [n, 3] float32
+    particle_f: torch.Tensor | None = None
+
+
+@dataclass
+class Model:
+    device: torch.device
+    particle_count: int
+    world_count: int
+    particle_q: torch.Tensor
+    particle_qd: torch.Tensor
+    particle_mass: torch.Tensor
+    particle_radius: torch.Tensor
+    particle_flags: torch.Tensor
+    particle_world: torch.Tensor
+    particle_world_start: torch.Tensor  # [W + 1] int32 (host mirror in world_start_np)
+    gravity: torch.Tensor               # [W, 3]
+    sph: SPHAttributes
+    shape_count: int
+    shape_type: np.ndarray
+    shape_scale: np.ndarray
+    shape_transform: np.ndarray         # [S, 7] p, q (xyzw)
+    shape_body: np.ndarray
+    shape_world: np.ndarray
+    shape_flags: np.ndarray
+    shape_material_mu: np.ndarray
+    shape_collision_radius: np.ndarray
+    world_start_np: np.ndarray = field(default_factory=lambda: np.zeros(1, np.int64))
+
+    def state(self) -> State:
+        """A new State holding copies of the model's initial particle positions and velocities."""
+        return State(self.particle_q.clone(), self.particle_qd.clone(), None)
+
+
+class ModelBuilder:
+    """Accumulates worlds of particles and shapes on the host; ``finalize`` moves them to a device.
+
+    Worlds are contiguous particle ranges (begin_world / end_world), as Newton's builder lays them out
+    (layout.py:47-72 requires it)."""
+
+    def __init__(self, gravity: tuple[float, float, float] = (0.0, 0.0, -9.81)):
+        self.gravity = tuple(float(g) for g in gravity)
+        self._q: list = []
+        self._qd: list = []
+        self._mass: list = []
+        self._radius: list = []
+        self._flags: list = []
+        self._world: list = []
+        self._rho0: list = []
+        self._mu: list = []
+        self._phase: list = []
+        self._shapes: list[dict] = []
+        self._world_gravity: list = []
+        self.current_world = -1
+        self.world_count = 0
+        self._sph_registered = False
+
+    # ---------------------------------------------------------------------------------------- worlds
+    def begin_world(self, gravity: tuple[float, float, float] | None = None) -> int:
+        if self.current_world >= 0:
+            raise ValueError("ModelBuilder: begin_world inside a world")
+        self.current_world = self.world_count
+        self.world_count += 1
+        self._world_gravity.append(tuple(float(g) for g in (gravity or self.gravity)))
+        return self.current_world
+
+    def end_world(self) -> None:
+        if self.current_world < 0:
+            raise ValueError("ModelBuilder: end_world without begin_world")
+        self.current_world = -1
+
+    def register_sph_attributes(self) -> None:
+        self._sph_registered = True
+
+    @property
+    def particle_count(self) -> int:
+        return len(self._mass)
+
+    # ---------------------------------------------------------------------------------------- content
+    def add_particles(self, pos, vel, mass, radius, flags, rest_density=1000.0, viscosity=1.0e-3,
+                      phase=-1) -> range:
+        if self.current_world < 0:
+            raise ValueError("ModelBuilder: particles must be added inside a world")
+        pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+        n = len(pos)
+        vel = np.broadcast_to(np.asarray(vel, dtype=np.float64), (n, 3))
+        first = self.particle_count
+        bc = lambda v: list(np.broadcast_to(np.asarray(v), (n,)))  # noqa: E731
+        self._q.extend(pos.tolist())
+        self._qd.extend(vel.tolist())
+        self._mass.extend(bc(mass))
+        self._radius.extend(bc(radius))
+        self._flags.extend(bc(flags))
+        self._world.extend([self.current_world] * n)
+        self._rho0.extend(bc(rest_density))
+        self._mu.extend(bc(viscosity))
+        self._phase.extend(bc(phase))
+        return range(first, first + n)
+
+    def add_shape(self, geo: int, center, half, quat=(0.0, 0.0, 0.0, 1.0), body: int = -1, mu: float = 0.5,
+                  radius: float | None = None) -> int:
+        """A shape in the current world (or global with no world open). ``half``: box half extents, sphere
+        radius in x, plane footprint (<= 0: infinite) in x/y."""
+        c = np.asarray(center, dtype=np.float64)
+        hx = np.asarray(half, dtype=np.float64).reshape(3)
+        if radius is None:
+            radius = float(np.linalg.norm(hx)) if geo == GeoType.BOX else float(hx[0]) if geo == GeoType.SPHERE \
+                else 1.0e30
+        self._shapes.append(dict(geo=int(geo), center=c, half=hx, quat=np.asarray(quat, np.float64), body=int(body),
+                                 world=self.current_world, mu=float(mu), radius=float(radius),
+                                 flags=int(ShapeFlags.COLLIDE_PARTICLES | ShapeFlags.COLLIDE_SHAPES)))
+        return len(self._shapes) - 1
+
+    def add_shape_box(self, body: int, center, half, mu: float = 0.5) -> int:
+        return self.add_shape(GeoType.BOX, center, half, body=body, mu=mu)
+
+    # ---------------------------------------------------------------------------------------- finalize
+    def finalize(self, device: str | torch.device = "cpu") -> Model:
+        if self.current_world >= 0:
+            raise ValueError("ModelBuilder: finalize inside a world")
+        dev = torch.device(device)
+        n = self.particle_count
+        W = max(self.world_count, 1)
+        world = np.asarray(self._world, dtype=np.int32)
+        if n and np.any(np.diff(world) < 0):
+            raise ValueError("particles must be contiguous per world")
+        ws = np.zeros(W + 1, dtype=np.int64)
+        for w in range(W):
+            ws[w + 1] = ws[w] + int(np.count_nonzero(world == w))
+        f32 = lambda a, shp=None: torch.tensor(np.asarray(a, dtype=np.float32).reshape(shp) if shp else  # noqa: E731
+                                               np.asarray(a, dtype=np.float32), device=dev)
+        grav = np.asarray(self._world_gravity or [self.gravity], dtype=np.float32).reshape(-1, 3)
+        S = len(self._shapes)
+        sh = self._shapes
+        xform = np.zeros((S, 7), np.float32)
+        for s, d in enumerate(sh):
+            xform[s, :3] = d["center"]
+            xform[s, 3:] = d["quat"]
+        model = Model(
+            device=dev,
+            particle_count=n,
+            world_count=W,
+            particle_q=f32(self._q if n else np.zeros((0, 3)), (-1, 3)),
+            particle_qd=f32(self._qd if n else np.zeros((0, 3)), (-1, 3)),
+            particle_mass=f32(self._mass),
+            particle_radius=f32(self._radius),
+            particle_flags=torch.tensor(np.asarray(self._flags, np.int32), device=dev),
+            particle_world=torch.tensor(world, device=dev),
+            particle_world_start=torch.tensor(ws.astype(np.int32), device=dev),
+            gravity=torch.tensor(grav, device=dev),
+            sph=SPHAttributes(rest_density=f32(self._rho0), viscosity=f32(self._mu),
+                              phase=torch.tensor(np.asarray(self._phase, np.int8), device=dev)),
+            shape_count=S,
+            shape_type=np.asarray([d["geo"] for d in sh], np.int32),
+            shape_scale=np.asarray([d["half"] for d in sh], np.float32).reshape(S, 3),
+            shape_transform=xform,
+            shape_body=np.asarray([d["body"] for d in sh], np.int32),
+            shape_world=np.asarray([d["world"] for d in sh], np.int32),
+            shape_flags=np.asarray([d["flags"] for d in sh], np.int32),
+            shape_material_mu=np.asarray([d["mu"] for d in sh], np.float32),
+            shape_collision_radius=np.asarray([d["radius"] for d in sh], np.float32),
+            world_start_np=ws,
+        )
+        return model

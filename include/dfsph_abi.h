@@ -1,0 +1,215 @@
+/* C-ABI of the MI355X DFSPH liquid step (libdfsph_hip.so).
+ *
+ * The entry points replace the Warp launches of alloy/solvers/sph that SolverDFSPH.step issues
+ * (alloy/solvers/sph/solver.py:218-268). Python binds them with ctypes (alloy_amd/sph/_abi.py); a
+ * maintainer's binding recipe is in INTEGRATION.md.
+ *
+ * Conventions
+ *  - Status 0 is success. A negative status comes with a thread-local message (dfsph_last_error):
+ *      DFSPH_EINVAL    -> ValueError     (bad argument, bad layout, dt <= 0)
+ *      DFSPH_ECAPACITY -> CapacityError  (a size beyond what the handle was created for)
+ *      DFSPH_EDEVICE   -> RuntimeError   (a HIP call failed)
+ *  - Particle, world and state pointers are device pointers (torch data_ptr() values). Shape tables and
+ *    the world layout are host arrays copied at create time. The library never frees caller memory.
+ *  - Every launch goes to the stream passed in (NULL = the default stream). The only host syncs are
+ *    dfsph_report, dfsph_read_buffer/dfsph_write_buffer and, per iteration of a pressure solve, the
+ *    per-world convergence count (the device-side loop replacing wp.capture_while).
+ *  - A handle is thread-compatible, not thread-safe.
+ */
+#ifndef DFSPH_ABI_H
+#define DFSPH_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFSPH_ABI_VERSION 1
+
+#define DFSPH_OK 0
+#define DFSPH_EINVAL (-1)
+#define DFSPH_ECAPACITY (-2)
+#define DFSPH_EDEVICE (-3)
+
+/* Solver parameters. Mirrors DFSPHConfig (alloy/solvers/sph/config.py:46-131) plus the constants the
+ * reference derives from it at construction (solver.py:92-94, layout.py:195-249, solve_driver.py:91).
+ * Host doubles; each kernel parameter is rounded to float32 exactly where the reference passes a Python
+ * float to a float32 Warp kernel argument. */
+typedef struct DfsphParams {
+    double particle_radius;          /* r [m] */
+    double h;                        /* support radius = support_factor * r */
+    double spacing;                  /* 2 r */
+    int32_t kernel;                  /* 0 cubic, 1 wendland_c2 (kernels.py:27-29) */
+    int32_t pressure_solver;         /* 0 projected CG (krylov.py), 1 Jacobi (solve_driver.py:171-186) */
+    double max_density_error;
+    double max_divergence_error;
+    double min_density_error;
+    double max_error_factor;
+    double rest_jitter_speed;        /* <= 0: None (rule off, solve_driver.py:108-119) */
+    double rest_jitter_flow_ratio;
+    int32_t min_iterations;
+    int32_t max_iterations;
+    double min_neighbors_divergence;
+    int32_t divergence_solve;
+    int32_t divergence_two_sided;
+    double jacobi_omega;
+    double warm_start_density;
+    double warm_start_divergence;
+    double xsph_c;                   /* 0: no XSPH */
+    int32_t neighbor_capacity;       /* 8..64 */
+    int32_t wall_friction;
+    double n_full;                   /* full_neighbor_count (config.py:230-236) */
+    double tot_min;                  /* diagonal_floor * lattice_diagonal (solver.py:94) */
+    double lattice_grad;             /* sqrt(lattice_diagonal) (solve_driver.py:91) */
+    double compression_relief_speed; /* <= 0: None */
+    double contact_separation_speed;
+    double wall_offset;              /* boundary_fit delta (layout.py:243) */
+    double wall_amplitude;           /* boundary_fit A */
+    double taper0;                   /* r (layout.py:244) */
+    double taper1;                   /* 3 r */
+    double reach;                    /* 1.5 spacing (layout.py:195-199) */
+    int32_t curvature;               /* boundary_curvature_order (0 or 1) */
+    double project_eps;              /* PROJECT_EPS * r (solver.py:483) */
+    double sweep_len;                /* integrate.sweep_length(h, thinnest) */
+} DfsphParams;
+
+/* Static description of the particle pack and its walls (layout.py:47-72, boundary.py:79-95). */
+typedef struct DfsphLayout {
+    int32_t n;                       /* particles in the pack (liquid, spares, wall particles, others) */
+    int32_t world_count;             /* W */
+    const int32_t* world_start;      /* host [W+1], contiguous world ranges */
+    /* per-world cell grid of the neighbour search (cell side = h * (1 + 1e-5)), host arrays */
+    const float* grid_origin;        /* [W*3] */
+    const int32_t* grid_dims;        /* [W*3], each >= 1 */
+    /* shapes (boundary.py:134-157): S model shapes; per world CSR start[W+1] / ids */
+    int32_t shape_count;
+    const int32_t* shape_start;      /* host [W+1] */
+    const int32_t* shape_ids;        /* host [shape_start[W]] */
+    const int32_t* shape_geo;        /* host [S] GeoType: 1 plane, 3 sphere, 7 box */
+    const float* shape_scale;        /* host [S*3] */
+    const float* shape_xform;        /* host [S*7] p(3) q(4, xyzw) */
+    const int32_t* shape_body;       /* host [S], -1 static (only static shapes are accepted) */
+    const float* shape_radius;       /* host [S] origin radius (layout.py:151-163) */
+    const float* shape_mu;           /* host [S] Coulomb friction of the wall projection */
+    int32_t wall_particle_count;     /* particles flagged ACTIVE|PROXY acting as particle walls */
+} DfsphLayout;
+
+/* Model arrays (device pointers, length n unless stated). */
+typedef struct DfsphModel {
+    const float* mass;               /* particle_mass */
+    const float* rest_density;       /* sph:rest_density */
+    int32_t* flags;                  /* particle_flags (ACTIVE = 1, PROXY = 2) */
+    int8_t* phase;                   /* sph:phase (1 liquid, 0 spare, -1 none, -2 retired) */
+    const int32_t* world;            /* particle_world */
+    const float* gravity;            /* [W*3] */
+    const float* wall_volume;        /* V_j = 8 r^3 of particle-wall particles, 0 otherwise (particle_walls.py:227) */
+} DfsphModel;
+
+/* Host copy of the per-world report (types.py:11-28 SPHReport); arrays of length W owned by the caller. */
+typedef struct DfsphReport {
+    float* density_err;
+    float* divergence_err;
+    int32_t* iters_density;
+    int32_t* iters_divergence;
+    int32_t* neighbor_overflow;
+    float* max_speed;
+    float* density_err_max;
+    float* density_tol;
+    int32_t* projected;
+    int32_t* swept;
+    int32_t* nonfinite;
+    double* n_active;
+} DfsphReport;
+
+/* Buffer ids for dfsph_get_buffer / dfsph_read_buffer / dfsph_write_buffer. */
+enum DfsphBuffer {
+    DFSPH_BUF_NBR = 0,        /* uint16 [cap, n] world-local offsets, column-major (neighbors.py:13-18) */
+    DFSPH_BUF_NBR_HI,         /* uint8  [cap, n] high byte when a world has > 65,536 particles, else [1] */
+    DFSPH_BUF_COUNT,          /* int32  [n] */
+    DFSPH_BUF_GROUP,          /* int32  [n] world, -2 - world (particle wall), -1 (neighbors.py:36-51) */
+    DFSPH_BUF_SLOT_SHAPE,     /* int32  [n, 4] */
+    DFSPH_BUF_SLOT_PSI,       /* f32    [n, 4] */
+    DFSPH_BUF_SLOT_GPSI,      /* f32    [n, 4, 3] */
+    DFSPH_BUF_SLOT_XB,        /* f32    [n, 4, 3] */
+    DFSPH_BUF_SLOT_VB,        /* f32    [n, 4, 3] */
+    DFSPH_BUF_SLOT_PHI,       /* f32    [n, 4] */
+    DFSPH_BUF_SLOT_XBL,       /* f32    [n, 4, 3] */
+    DFSPH_BUF_SLOT_NBL,       /* f32    [n, 4, 3] */
+    DFSPH_BUF_SLOT_IMPULSE,   /* f32    [n, 4, 3] */
+    DFSPH_BUF_RATIO,          /* f32    [n] */
+    DFSPH_BUF_ALPHA,
+    DFSPH_BUF_NEFF,
+    DFSPH_BUF_ACTIVE,
+    DFSPH_BUF_REG,
+    DFSPH_BUF_SRC,
+    DFSPH_BUF_KAPPA,
+    DFSPH_BUF_ACC,            /* f32    [n, 3] */
+    DFSPH_BUF_ERR,
+    DFSPH_BUF_LOWER,
+    DFSPH_BUF_KV,             /* warm start k dt (divergence) */
+    DFSPH_BUF_KD,             /* warm start k dt^2 (density) */
+    DFSPH_BUF_VTMP,           /* f32    [n, 3] */
+    DFSPH_BUF_CONVERGED,      /* int32  [W] */
+    DFSPH_BUF_ITERS,          /* int32  [W] */
+    DFSPH_BUF_ERR_MEAN,       /* f32    [W] */
+    DFSPH_BUF_ERR_MAX,        /* f32    [W] */
+    DFSPH_BUF_N_ACTIVE,       /* f64    [W] */
+    DFSPH_BUF_TOL_SCALE,      /* f32    [W] */
+    DFSPH_BUF_OVERFLOW,       /* int32  [W] */
+    DFSPH_BUF_MAX_SPEED,      /* f32    [W] */
+    DFSPH_BUF_ITERS_DEN,      /* int32  [W] */
+    DFSPH_BUF_ITERS_DIV,      /* int32  [W] */
+    DFSPH_BUF_DEN_ERR,        /* f32    [W] */
+    DFSPH_BUF_DIV_ERR,        /* f32    [W] */
+    DFSPH_BUF_SHAPE_METER,    /* int64  [S, 3] impulse per shape, 2^50 fixed point (bodies.py:21-25) */
+    DFSPH_BUF_FEATURES,       /* f32    [n, 16] learned warm-start features (solve inputs) */
+    DFSPH_BUF_COUNT_
+};
+
+/* Library */
+int32_t dfsph_abi_version(void);
+const char* dfsph_last_error(void);
+
+/* Construction: SolverDFSPH.__init__ / _alloc (solver.py:72-168). Allocates every work buffer once. */
+int32_t dfsph_create(const DfsphParams* params, const DfsphLayout* layout, const DfsphModel* model,
+                     void** out_handle);
+int32_t dfsph_destroy(void* handle);
+
+/* One DFSPH step on the state (q, qd [n*3] f32, in place; f [n*3] or NULL): SolverDFSPH.step
+ * (solver.py:218-268). dt in seconds (> 0). */
+int32_t dfsph_step(void* handle, float* q, float* qd, const float* f, double dt, void* stream);
+
+/* Phase entry points (the same buffers as dfsph_step) for parity tests and callers that interleave work. */
+int32_t dfsph_build_neighbors(void* handle, const float* q, void* stream);              /* neighbors.py:256-283 */
+int32_t dfsph_boundary_slots(void* handle, const float* q, const float* qd, void* stream); /* boundary.py:399-414 */
+int32_t dfsph_density_alpha(void* handle, const float* q, void* stream);                 /* pressure.py:144-193 + reduce */
+/* mode 0 divergence, 1 density (solve_driver.py:121-162). fixed_iters >= 0 forces that iteration count
+ * (teacher-forced parity); -1 iterates to convergence. */
+int32_t dfsph_solve(void* handle, int32_t mode, float* q, float* qd, double dt, int32_t fixed_iters, void* stream);
+int32_t dfsph_nonpressure(void* handle, const float* q, float* qd, const float* f, double dt, void* stream); /* integrate.py:150-178 */
+int32_t dfsph_advect_project(void* handle, float* q, float* qd, double dt, void* stream);  /* integrate.py:181-309 + speed */
+/* Hydrostatic prime (prime.py:53-115): guess [n] k = p/rho0 (host-computed, device pointer). Returns
+ * iterations used through *iters. */
+int32_t dfsph_prime(void* handle, float* q, float* qd, const float* guess, double dt, double tol,
+                    int32_t max_iterations, int32_t* iters, void* stream);
+
+/* Learned warm start: an MLP (13 -> 64 -> 64 -> 64 -> 2, SiLU, bf16 weights) predicting the initial kappa
+ * of both solves in place of solve_init's warm start (pressure.py:254, 261). weights: device bf16 buffer in
+ * the layout of dfsph_mlp_layout; NULL switches the learned start off. */
+int32_t dfsph_set_warm_start_model(void* handle, const void* weights, void* stream);
+
+/* Buffers */
+int32_t dfsph_get_buffer(void* handle, int32_t which, void** ptr, int64_t* n_bytes);
+int32_t dfsph_read_buffer(void* handle, int32_t which, void* host_dst, int64_t n_bytes);   /* syncs */
+int32_t dfsph_write_buffer(void* handle, int32_t which, const void* host_src, int64_t n_bytes); /* syncs */
+int32_t dfsph_zero_stats(void* handle, void* stream);   /* WorldStats.reset counters of every world */
+
+/* Per-world report (runtime.py:136-147): the sync point of a step. */
+int32_t dfsph_report(void* handle, DfsphReport* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DFSPH_ABI_H */
