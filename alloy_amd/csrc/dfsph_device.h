@@ -25,7 +25,7 @@ __device__ __forceinline__ f3 operator*(float s, f3 a) { return f3{s * a.x, s * 
 __device__ __forceinline__ f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
 __device__ __forceinline__ f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
 __device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float length(f3 a) { return __fsqrt_rn(dot(a, a)); }
+__device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }  // correctly rounded (not v_sqrt_f32 alone)
 __device__ __forceinline__ f3 cross(f3 a, f3 b) {
     return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }

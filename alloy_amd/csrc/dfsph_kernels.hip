@@ -1,0 +1,2076 @@
+// gfx950 kernels of the DFSPH liquid step (one HIP stream, deterministic, no float atomics).
+//
+// Layout in HBM. Caller-owned state: q, qd as [n][3] float32 (Newton's particle_q / particle_qd). Solver-owned
+// work buffers, allocated once at create: per-particle float32 SoA fields; neighbour rows as column-major uint16
+// world-local offsets nbr[k * n + i] (+ uint8 high byte for worlds over 65,536 particles), so the 64 lanes of a
+// wavefront read 64 adjacent entries; boundary slots as [n][4] rows; per-world float64 reduction partials.
+//
+// Launch geometry. Per-particle phases that feed per-world reductions run one 256-thread block per "chunk": 256
+// consecutive particles aligned to their world's start (reduce.py:167-182's chunk table). A block therefore
+// belongs to one world: per-world control (converged, restart) is block-uniform, converged worlds' blocks exit at
+// once (uneven batched worlds cost what their unconverged particles cost), and each block tree-reduces its values
+// in float64 into a chunk partial. A per-world epilogue (one block per world) tree-reduces that world's chunk
+// partials in fixed order and runs the world's control logic (check_kernel / cg_beta_kernel). The sums are thus
+// bitwise independent of batching and of scheduling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dfsph_abi.h"
+#include "dfsph_device.h"
+
+using namespace dfsph;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(DFSPH_EDEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// Everything a kernel reads: parameters as float32 kernel arguments and device pointers.
+struct Ctx {
+    int n, W, S, NW, cap, wide, kind, n_cells, friction, curvature, two_sided;
+    float h, cell, r, n_full, tot_min, min_nb, omega, xsph, v_sep;
+    float offset, amplitude, taper0, taper1, reach, eps, sweep_len;
+    // model (caller-owned)
+    const float* mass;
+    const float* rho0;
+    int* flags;
+    int8_t* phase;
+    const int* world;
+    const float* gravity;
+    const float* wvol;
+    // tables
+    const int* ws;
+    const int* ch_begin;
+    const int* ch_end;
+    const int* ch_world;
+    const int* w_chunk;
+    const float* g_org;
+    const int* g_dims;
+    const int* cell_base;
+    const int* sh_start;
+    const int* sh_ids;
+    const int* sh_geo;
+    const float* sh_scale;
+    const float* sh_x;
+    const float* sh_radius;
+    const float* sh_mu;
+    // grid
+    int* cell_count;
+    int* cell_start;
+    int* cell_fill;
+    int* items;
+    int* key;
+    int* scan_part;
+    // lists
+    uint16_t* nbr;
+    uint8_t* nbr_hi;
+    int* count;
+    int* group;
+    // slots
+    int* s_shape;
+    float* s_psi;
+    float* s_gpsi;
+    float* s_xb;
+    float* s_vb;
+    float* s_phi;
+    float* s_xbl;
+    float* s_nbl;
+    float* s_imp;
+    // per particle
+    float *ratio, *alpha, *neff, *active, *reg, *src, *kappa, *acc, *err, *lower, *kv, *kd, *vtmp;
+    float *pd, *z, *wv, *ap, *corr;
+    int* act;
+    // per world
+    int *converged, *iters, *restart;
+    float *err_mean, *err_max, *tol_scale;
+    double *n_active, *err_sum, *rz, *stepb, *sums, *flow_z, *flow_xy, *head;
+    int *st_overflow, *st_projected, *st_swept, *st_nonfinite, *it_den, *it_div;
+    float *st_max_speed, *st_den_err, *st_div_err, *st_den_err_max, *st_den_tol;
+    unsigned long long* meter;
+    double* part;
+    int* n_unconv;
+    // state (bound per call)
+    float* q;
+    float* qd;
+    const float* f;
+};
+
+__device__ __forceinline__ bool is_fluid(const Ctx& c, int i) {
+    return (c.flags[i] & ACTIVE) != 0 && int(c.phase[i]) >= 1;
+}
+__device__ __forceinline__ f3 Q(const Ctx& c, int i) { return ld3(c.q, i); }
+__device__ __forceinline__ f3 V(const Ctx& c, int i) { return ld3(c.qd, i); }
+
+__device__ __forceinline__ int nbr_j(const Ctx& c, int base, int i, int k) {
+    int j = base + int(c.nbr[int64_t(k) * c.n + i]);
+    if (c.wide) j += int(c.nbr_hi[int64_t(k) * c.n + i]) << 16;
+    return j;
+}
+
+// ----------------------------------------------------------------------------------------------- reductions
+// Fixed 256-wide tree: a[t] += a[t + s] for s = 128, 64 (LDS) then 32 .. 1 (wavefront shuffles; the same pairs).
+__device__ __forceinline__ double tree_sum(double v, double* lds) {
+    int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    if (t < 128) lds[t] += lds[t + 128];
+    __syncthreads();
+    if (t < 64) lds[t] += lds[t + 64];
+    __syncthreads();
+    double r = 0.0;
+    if (t < 64) {
+        r = lds[t];
+        for (int s = 32; s >= 1; s >>= 1) r += __shfl_down(r, s, 64);
+    }
+    __syncthreads();
+    return r;  // valid in thread 0
+}
+__device__ __forceinline__ double tree_max(double v, double* lds) {
+    int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    if (t < 128) lds[t] = fmax(lds[t], lds[t + 128]);
+    __syncthreads();
+    if (t < 64) lds[t] = fmax(lds[t], lds[t + 64]);
+    __syncthreads();
+    double r = -3.4e38;
+    if (t < 64) {
+        r = lds[t];
+        for (int s = 32; s >= 1; s >>= 1) r = fmax(r, __shfl_down(r, s, 64));
+    }
+    __syncthreads();
+    return r;
+}
+// chunk partial (sum x, sum y, max y) of the block's values; invalid lanes pass (0, 0, -3.4e38)
+__device__ __forceinline__ void chunk_reduce(const Ctx& c, double x, double y, double m) {
+    __shared__ double lds[CHUNK];
+    double sx = tree_sum(x, lds);
+    double sy = tree_sum(y, lds);
+    double mx = tree_max(m, lds);
+    if (threadIdx.x == 0) {
+        double* p = c.part + 3 * int64_t(blockIdx.x);
+        p[0] = sx;
+        p[1] = sy;
+        p[2] = mx;
+    }
+}
+
+enum WorldMode { W_NACTIVE = 0, W_CG_INIT, W_CG_PAP, W_CG_BETA, W_JACOBI, W_FLOW_Z, W_FLOW_XY, W_HEAD };
+
+struct StopSpec {
+    float tol, tol_max;
+    int min_it, max_it, use_scale;
+};
+
+__device__ __forceinline__ bool stop_test(const Ctx& c, int w, int it, float mean, const StopSpec& sp) {
+    float ts = sp.use_scale ? c.tol_scale[w] : 1.0f;
+    return (it >= sp.min_it && mean <= sp.tol * ts && c.err_max[w] <= sp.tol_max * ts) || it >= sp.max_it;
+}
+
+// One block per world: tree over the world's chunk partials (thread t sums chunks t, t + 256, ... in order), then
+// the world's control logic in thread 0 (reduce.py:100-164 + pressure.py:370-390 / krylov.py:215-240).
+__global__ void __launch_bounds__(256) k_world(Ctx c, int mode, StopSpec sp) {
+    int w = blockIdx.x;
+    if ((mode == W_CG_PAP || mode == W_CG_BETA || mode == W_JACOBI) && c.converged[w] != 0) return;
+    __shared__ double lds[CHUNK];
+    double a = 0.0, b = 0.0, m = -3.4e38;
+    for (int k = c.w_chunk[w] + int(threadIdx.x); k < c.w_chunk[w + 1]; k += CHUNK) {
+        a += c.part[3 * int64_t(k)];
+        b += c.part[3 * int64_t(k) + 1];
+        m = fmax(m, c.part[3 * int64_t(k) + 2]);
+    }
+    double sa = tree_sum(a, lds);
+    double sb = tree_sum(b, lds);
+    double mx = tree_max(m, lds);
+    if (threadIdx.x != 0) return;
+    switch (mode) {
+        case W_NACTIVE:
+            c.n_active[w] = sa;
+            break;
+        case W_CG_INIT:
+            c.rz[w] = sa;
+            c.restart[w] = 0;
+            break;
+        case W_CG_PAP:
+            c.sums[2 * w] = sa;
+            c.sums[2 * w + 1] = sb;
+            break;
+        case W_CG_BETA: {
+            c.err_max[w] = float(fmax(mx, 0.0));
+            int it = c.iters[w] + 1;
+            c.iters[w] = it;
+            double rz_new = sa;
+            float mean = float(sb / fmax(c.n_active[w], 1.0));
+            c.err_mean[w] = mean;
+            double beta = 0.0;
+            if (c.restart[w] == 0 && c.rz[w] > 1.0e-30) beta = rz_new / c.rz[w];
+            c.restart[w] = 0;
+            c.rz[w] = rz_new;
+            c.stepb[w] = beta;
+            if (stop_test(c, w, it, mean, sp)) c.converged[w] = 1;
+            else atomicAdd(c.n_unconv, 1);
+            break;
+        }
+        case W_JACOBI: {
+            c.err_sum[w] = sa;
+            c.err_max[w] = float(fmax(mx, 0.0));
+            int it = c.iters[w] + 1;
+            c.iters[w] = it;
+            float mean = float(sa / fmax(c.n_active[w], 1.0));
+            c.err_mean[w] = mean;
+            if (stop_test(c, w, it, mean, sp)) c.converged[w] = 1;
+            else atomicAdd(c.n_unconv, 1);
+            break;
+        }
+        case W_FLOW_Z:
+            c.flow_z[2 * w] = sa;
+            c.flow_z[2 * w + 1] = sb;
+            break;
+        case W_FLOW_XY:
+            c.flow_xy[2 * w] = sa;
+            c.flow_xy[2 * w + 1] = sb;
+            break;
+        case W_HEAD:
+            c.head[2 * w] = sa;
+            c.head[2 * w + 1] = sb;
+            break;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------- utilities
+__global__ void k_fill_i32(int* p, int v, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+__global__ void k_fill_f32(float* p, float v, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+__global__ void k_copy_f32(float* dst, const float* src, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+// dst = a * src (prime.py: kv = guess * dt, kd = dt * kv)
+__global__ void k_scale_f32(float* dst, const float* src, float a, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = a * src[i];
+}
+__global__ void k_scale_f32_rev(float* dst, const float* src, float a, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] * a;
+}
+
+// ----------------------------------------------------------------------------------------------- neighbour grid
+__device__ __forceinline__ int cell_coord(float x, float o, float cell, int dim) {
+    float u = (x - o) / cell;
+    float fl = floorf(u);
+    if (!(fl >= 0.0f)) return 0;
+    if (fl >= float(dim - 1)) return dim - 1;
+    return int(fl);
+}
+// cell of x in grid g (g < W: fluid grid of world g; g >= W: wall grid of world g - W)
+__device__ __forceinline__ int cell_of(const Ctx& c, int g, f3 x, int& cx, int& cy, int& cz) {
+    int w = g < c.W ? g : g - c.W;
+    const int* d = c.g_dims + 3 * w;
+    cx = cell_coord(x.x, c.g_org[3 * w], c.cell, d[0]);
+    cy = cell_coord(x.y, c.g_org[3 * w + 1], c.cell, d[1]);
+    cz = cell_coord(x.z, c.g_org[3 * w + 2], c.cell, d[2]);
+    return c.cell_base[g] + (cz * d[1] + cy) * d[0] + cx;
+}
+
+// neighbors.py:36-51 group per particle, then its cell key and the cell histogram
+__global__ void __launch_bounds__(256) k_groups_hist(Ctx c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    int g = -1;
+    int fl = c.flags[i];
+    if ((fl & ACTIVE) != 0 && int(c.phase[i]) >= 1) g = c.world[i];
+    else if (c.NW > 0 && c.wvol[i] > 0.0f && (fl & ACTIVE) != 0 && c.world[i] >= 0) g = -2 - c.world[i];
+    c.group[i] = g;
+    int k = -1;
+    if (g != -1) {
+        int gi = g >= 0 ? g : c.W + (-2 - g);
+        int cx, cy, cz;
+        k = cell_of(c, gi, Q(c, i), cx, cy, cz);
+        atomicAdd(c.cell_count + k, 1);
+    }
+    c.key[i] = k;
+}
+
+// exclusive scan of a block's 1024 elements (256 threads x 4); returns the thread's exclusive prefix, total in *tot
+__device__ __forceinline__ int block_scan_excl(int v, int* tot) {
+    __shared__ int wsum[4];
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    int off = 0;
+    for (int k = 0; k < wid; ++k) off += wsum[k];
+    *tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return off + x - v;
+}
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const int* in, int n, int* part) {
+    int base = blockIdx.x * 1024 + threadIdx.x * 4;
+    int s = 0;
+    for (int k = 0; k < 4; ++k)
+        if (base + k < n) s += in[base + k];
+    int tot;
+    block_scan_excl(s, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+// single block: exclusive scan of the block totals in place; total into out[n]
+__global__ void __launch_bounds__(256) k_scan_top(int* part, int nb, int* out, int n) {
+    int running = 0;
+    for (int b0 = 0; b0 < nb; b0 += 1024) {
+        int base = b0 + threadIdx.x * 4;
+        int v[4], s = 0;
+        for (int k = 0; k < 4; ++k) {
+            v[k] = base + k < nb ? part[base + k] : 0;
+            s += v[k];
+        }
+        int tot;
+        int ex = block_scan_excl(s, &tot) + running;
+        for (int k = 0; k < 4; ++k) {
+            if (base + k < nb) part[base + k] = ex;
+            ex += v[k];
+        }
+        running += tot;
+    }
+    if (threadIdx.x == 0) out[n] = running;
+}
+__global__ void __launch_bounds__(256) k_scan_apply(const int* in, int n, const int* part, int* out, int* out2) {
+    int base = blockIdx.x * 1024 + threadIdx.x * 4;
+    int v[4], s = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        s += v[k];
+    }
+    int tot;
+    int ex = block_scan_excl(s, &tot) + part[blockIdx.x];
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) {
+            out[base + k] = ex;
+            out2[base + k] = ex;
+        }
+        ex += v[k];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scatter(Ctx c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    int k = c.key[i];
+    if (k < 0) return;
+    int pos = atomicAdd(c.cell_fill + k, 1);
+    c.items[pos] = i;
+}
+
+// wall cells: ascending ids within each cell (the fixed visit order of the colour sums)
+__global__ void __launch_bounds__(256) k_sort_cells(Ctx c, int first, int last) {
+    int cc = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (cc >= last) return;
+    int b = c.cell_start[cc], e = c.cell_start[cc + 1];
+    for (int a = b + 1; a < e; ++a) {
+        int v = c.items[a];
+        int k = a - 1;
+        while (k >= b && c.items[k] > v) {
+            c.items[k + 1] = c.items[k];
+            --k;
+        }
+        c.items[k + 1] = v;
+    }
+}
+
+// neighbors.py:160-195: rows of the lowest `cap` world-local ids within h (fp32 dot(d, d) < h^2), ascending;
+// overflow counted per world. Each thread keeps its row in LDS (column per thread: conflict-free).
+constexpr int BUILD_BLOCK = 128;
+__global__ void __launch_bounds__(BUILD_BLOCK) k_build_lists(Ctx c) {
+    __shared__ int row[64 * BUILD_BLOCK];
+    int i = blockIdx.x * BUILD_BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    int g = c.group[i];
+    if (g < 0) {
+        c.count[i] = 0;
+        return;
+    }
+    int* L = row + threadIdx.x;
+    const int cap = c.cap;
+    int base = c.ws[g];
+    f3 xi = Q(c, i);
+    float h2 = c.h * c.h;
+    int cx, cy, cz;
+    cell_of(c, g, xi, cx, cy, cz);
+    const int* d = c.g_dims + 3 * g;
+    int nn = 0, total = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int Z = cz + dz;
+        if (Z < 0 || Z >= d[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int Y = cy + dy;
+            if (Y < 0 || Y >= d[1]) continue;
+            int x0 = max(cx - 1, 0), x1 = min(cx + 1, d[0] - 1);
+            int cb = c.cell_base[g] + (Z * d[1] + Y) * d[0];
+            // cells x0..x1 of this row are contiguous in the sorted item array
+            int e0 = c.cell_start[cb + x0], e1 = c.cell_start[cb + x1 + 1];
+            for (int e = e0; e < e1; ++e) {
+                int j = c.items[e];
+                if (j == i) continue;
+                f3 dd = Q(c, j) - xi;
+                if (dot(dd, dd) < h2) {
+                    total += 1;
+                    int jl = j - base;
+                    int pos = nn, k = nn - 1;
+                    while (k >= 0) {
+                        int v = L[k * BUILD_BLOCK];
+                        if (v < jl) break;
+                        pos = k;
+                        k -= 1;
+                    }
+                    if (pos >= cap) continue;
+                    int lastk = nn >= cap ? cap - 1 : nn;
+                    for (k = lastk; k > pos; --k) L[k * BUILD_BLOCK] = L[(k - 1) * BUILD_BLOCK];
+                    L[pos * BUILD_BLOCK] = jl;
+                    if (nn < cap) nn += 1;
+                }
+            }
+        }
+    }
+    for (int k = 0; k < nn; ++k) {
+        int v = L[k * BUILD_BLOCK];
+        c.nbr[int64_t(k) * c.n + i] = uint16_t(v & 65535);
+        if (c.wide) c.nbr_hi[int64_t(k) * c.n + i] = uint8_t(v >> 16);
+    }
+    if (total > cap) atomicAdd(c.st_overflow + g, 1);
+    c.count[i] = nn;
+}
+
+// ----------------------------------------------------------------------------------------------- particle walls
+// particle_walls.py:65-84: colour, grad colour and mean velocity of world w's wall particles at x
+__device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv) {
+    col = 0.0f;
+    gc = f3{0.0f, 0.0f, 0.0f};
+    mv = f3{0.0f, 0.0f, 0.0f};
+    if (c.NW == 0) return;
+    int g = c.W + w, cx, cy, cz;
+    cell_of(c, g, x, cx, cy, cz);
+    const int* d = c.g_dims + 3 * w;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                int X = cx + dx, Y = cy + dy, Z = cz + dz;
+                if (X < 0 || Y < 0 || Z < 0 || X >= d[0] || Y >= d[1] || Z >= d[2]) continue;
+                int cc = c.cell_base[g] + (Z * d[1] + Y) * d[0] + X;
+                for (int e = c.cell_start[cc]; e < c.cell_start[cc + 1]; ++e) {
+                    int j = c.items[e];
+                    f3 dd = x - Q(c, j);
+                    float wj = c.wvol[j] * kernel_w(length(dd), c.h, c.kind);
+                    if (wj > 0.0f) {
+                        col += wj;
+                        gc = gc + c.wvol[j] * kernel_grad(dd, c.h, c.kind);
+                        mv = mv + wj * V(c, j);
+                    }
+                }
+            }
+    if (col > 0.0f) mv = mv / col;
+}
+// particle_walls.py:87-98
+__device__ float colour_distance(const Ctx& c, float col) {
+    float lo = -c.h, hi = c.h;
+    for (int k = 0; k < 30; ++k) {
+        float mid = 0.5f * (lo + hi);
+        float psi, dpsi;
+        halfspace_volume(mid, c.h, 0.0f, c.kind, psi, dpsi);
+        if (psi > col) lo = mid;
+        else hi = mid;
+    }
+    return 0.5f * (lo + hi);
+}
+
+// ----------------------------------------------------------------------------------------------- boundary slots
+struct SlotRow {
+    int shape[4];
+    float phi[4];
+    f3 nrm[4];
+};
+
+// boundary_slots.py:13-38 (slot normals are not shifted, as in the reference)
+__device__ __forceinline__ int slot_insert(SlotRow& R, int nn, int s, float phi) {
+    int pos = nn, k = nn - 1;
+    while (k >= 0) {
+        if (R.phi[k] <= phi) break;
+        pos = k;
+        k -= 1;
+    }
+    if (pos >= 4) return -1;
+    int last = nn >= 4 ? 3 : nn;
+    for (k = last; k > pos; --k) {
+        R.shape[k] = R.shape[k - 1];
+        R.phi[k] = R.phi[k - 1];
+    }
+    R.shape[pos] = s;
+    R.phi[pos] = phi;
+    return pos;
+}
+__device__ __forceinline__ void slot_copy(SlotRow& R, int src, int dst) {
+    R.shape[dst] = R.shape[src];
+    R.phi[dst] = R.phi[src];
+    R.nrm[dst] = R.nrm[src];
+}
+// boundary_slots.py:50-102
+__device__ int shape_insert(SlotRow& R, int nn, int s, float phi, f3 nrm, float near_) {
+    if (nn < 4) {
+        int rr = slot_insert(R, nn, s, phi);
+        if (rr >= 0) {
+            for (int k = nn; k > rr; --k) R.nrm[k] = R.nrm[k - 1];
+            R.nrm[rr] = nrm;
+            return 1;
+        }
+        return 0;
+    }
+    const float cos_same = 0.85f;
+    int ev = -1;
+    float ev_phi = -1.0f;
+    for (int m = 0; m < 4; ++m)
+        if (phi >= near_ && R.phi[m] <= phi && dot(R.nrm[m], nrm) > cos_same) {
+            ev = 4;
+            ev_phi = phi;
+        }
+    for (int k = 0; k < 4; ++k) {
+        float pk = R.phi[k];
+        int dup = 0;
+        for (int m = 0; m < 4; ++m)
+            if (m != k && R.phi[m] <= pk && dot(R.nrm[m], R.nrm[k]) > cos_same) dup = 1;
+        if (phi <= pk && dot(nrm, R.nrm[k]) > cos_same) dup = 1;
+        if (dup == 1 && pk >= near_ && pk > ev_phi) {
+            ev = k;
+            ev_phi = pk;
+        }
+    }
+    if (ev < 0 && phi < R.phi[3]) ev = 3;
+    if (ev < 0 || ev == 4) return 0;
+    for (int k = ev; k < 3; ++k) slot_copy(R, k + 1, k);
+    int pos = 3;
+    while (pos > 0 && R.phi[pos - 1] > phi) {
+        slot_copy(R, pos - 1, pos);
+        pos -= 1;
+    }
+    R.shape[pos] = s;
+    R.phi[pos] = phi;
+    R.nrm[pos] = nrm;
+    return 1;
+}
+
+__device__ __forceinline__ void taper(const Ctx& c, float& v0, float& v1, float phi) {
+    if (phi <= c.taper0) return;
+    float L = c.taper1 - c.taper0;
+    float t = fmn((phi - c.taper0) / L, 1.0f);
+    float sm = 1.0f - t * t * (3.0f - 2.0f * t);
+    float dsm = 6.0f * t * (t - 1.0f) / L;
+    float a = v0 * sm, b = v1 * sm + v0 * dsm;
+    v0 = a;
+    v1 = b;
+}
+
+// boundary.py:190-238 (find) and 241-396 (fill, inclusion-exclusion) for one fluid particle
+__global__ void __launch_bounds__(256) k_slots(Ctx c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    SlotRow R;
+    for (int k = 0; k < 4; ++k) {
+        R.shape[k] = -1;
+        R.phi[k] = 0.0f;
+        R.nrm[k] = f3{0.0f, 0.0f, 0.0f};
+    }
+    int64_t b4 = 4 * int64_t(i);
+    for (int k = 0; k < 4; ++k) st3(c.s_imp, b4 + k, f3{0.0f, 0.0f, 0.0f});  // _begin_step (solver.py:289-291)
+    if (!is_fluid(c, i)) {
+        for (int k = 0; k < 4; ++k) {
+            c.s_shape[b4 + k] = -1;
+            c.s_psi[b4 + k] = 0.0f;
+            st3(c.s_gpsi, b4 + k, f3{0.0f, 0.0f, 0.0f});
+        }
+        return;
+    }
+    int w = c.world[i];
+    f3 x = Q(c, i);
+    int nn = 0;
+    for (int e = c.sh_start[w]; e < c.sh_start[w + 1]; ++e) {
+        int s = c.sh_ids[e];
+        Xf X = load_xf(c.sh_x + 7 * s);
+        int geo = c.sh_geo[s];
+        if (geo != GEO_PLANE)
+            if (length(x - X.p) > c.sh_radius[s] + c.h) continue;
+        f3 xl = xf_inv_point(X, x);
+        float phi;
+        f3 gl;
+        shape_sdf(geo, ld3(c.sh_scale, s), xl, phi, gl);
+        if (phi < c.reach && phi + c.offset < c.h) {
+            f3 nw = normalize(xf_vector(X, gl));
+            if (shape_insert(R, nn, s, phi, nw, 0.1f * c.h) > 0 && nn < 4) nn += 1;
+        }
+    }
+    float pcol = 0.0f;
+    f3 pgc{0.0f, 0.0f, 0.0f}, pmv{0.0f, 0.0f, 0.0f};
+    bool have_pw = false;
+    if (c.NW > 0) {
+        pw_sample(c, x, w, pcol, pgc, pmv);
+        have_pw = true;
+        float phi = 1.0e30f;
+        if (!(pcol < 1.0e-4f || length(pgc) <= 0.0f)) phi = colour_distance(c, fmn(pcol, 1.0f));
+        if (phi < c.reach && phi + c.offset < c.h)
+            if (slot_insert(R, nn, c.S, phi) >= 0 && nn < 4) nn += 1;
+    }
+    // fill (boundary.py:290-333)
+    float psi[4];
+    f3 gp[4];
+    for (int k = 0; k < 4; ++k) {
+        int s = R.shape[k];
+        c.s_shape[b4 + k] = s;
+        if (s < 0) {
+            psi[k] = 0.0f;
+            gp[k] = f3{0.0f, 0.0f, 0.0f};
+            continue;
+        }
+        if (s == c.S && have_pw) {  // particle wall: its local plane (boundary.py:272-287)
+            float lg = length(pgc);
+            f3 nrm{0.0f, 0.0f, 1.0f};
+            if (lg > 0.0f) nrm = (-pgc) / lg;
+            float phi = colour_distance(c, fcl(pcol, 0.0f, 1.0f));
+            f3 xb = x - phi * nrm;
+            float v0, v1;
+            halfspace_volume(phi + c.offset, c.h, 0.0f, c.kind, v0, v1);
+            v0 = v0 * c.amplitude;
+            v1 = v1 * c.amplitude;
+            taper(c, v0, v1, phi);
+            c.s_phi[b4 + k] = phi;
+            psi[k] = v0;
+            gp[k] = v1 * nrm;
+            st3(c.s_xb, b4 + k, xb);
+            st3(c.s_vb, b4 + k, pmv);
+            st3(c.s_xbl, b4 + k, xb);
+            st3(c.s_nbl, b4 + k, nrm);
+            continue;
+        }
+        Xf X = load_xf(c.sh_x + 7 * s);
+        int geo = c.sh_geo[s];
+        f3 sc = ld3(c.sh_scale, s);
+        f3 xl = xf_inv_point(X, x);
+        float phi;
+        f3 gl;
+        shape_sdf(geo, sc, xl, phi, gl);
+        f3 nrm = normalize(xf_vector(X, gl));
+        f3 nl = normalize(gl);
+        st3(c.s_nbl, b4 + k, nl);
+        st3(c.s_xbl, b4 + k, xl - phi * nl);
+        float H = 0.0f;
+        if (c.curvature > 0 && geo == GEO_SPHERE) {
+            float kmax = 2.0f / c.h;
+            H = -2.0f * fcl(1.0f / sc.x, -kmax, kmax);
+        }
+        float v0, v1;
+        halfspace_volume(phi + c.offset, c.h, H, c.kind, v0, v1);
+        v0 = v0 * c.amplitude;
+        v1 = v1 * c.amplitude;
+        taper(c, v0, v1, phi);
+        c.s_phi[b4 + k] = phi;
+        psi[k] = v0;
+        gp[k] = v1 * nrm;
+        st3(c.s_xb, b4 + k, x - phi * nrm);
+        st3(c.s_vb, b4 + k, f3{0.0f, 0.0f, 0.0f});
+    }
+    // inclusion-exclusion (boundary.py:370-396); the gradients of earlier slots are read after their update
+    float ps[4] = {psi[0], psi[1], psi[2], psi[3]};
+    for (int b = 0; b < 4; ++b) {
+        if (R.shape[b] < 0) continue;
+        f3 gb = gp[b];
+        float ob = 0.0f, sb = 0.0f;
+        for (int cc = 0; cc < 4; ++cc) {
+            if (cc == b || R.shape[cc] < 0) continue;
+            f3 gcv = gp[cc];
+            float lb = length(gb), lc = length(gcv);
+            float cbc = 0.0f;
+            if (lb > 0.0f && lc > 0.0f) cbc = fcl(dot(gb, gcv) / (lb * lc), -1.0f, 1.0f);
+            ob += (1.0f - fabsf(cbc)) * ps[cc];
+            if (cbc > 0.0f && (ps[cc] > ps[b] || (ps[cc] == ps[b] && cc < b))) sb += cbc;
+        }
+        ob = fmn(ob, 1.0f);
+        psi[b] = ps[b] * fmx(1.0f - 0.5f * ob - sb, 0.0f);
+        gp[b] = gb * fmx(1.0f - ob - sb, 0.0f);
+    }
+    for (int k = 0; k < 4; ++k) {
+        c.s_psi[b4 + k] = psi[k];
+        st3(c.s_gpsi, b4 + k, gp[k]);
+    }
+}
+
+// ----------------------------------------------------------------------------------------------- density / alpha
+#define CHUNK_PROLOGUE                    \
+    int b_ = blockIdx.x;                  \
+    int i = c.ch_begin[b_] + threadIdx.x; \
+    bool valid = i < c.ch_end[b_];
+
+// pressure.py:144-193, plus the chunk partial of `active` (solver.py:241)
+__global__ void __launch_bounds__(256) k_density_alpha(Ctx c) {
+    CHUNK_PROLOGUE
+    double act_v = 0.0;
+    if (valid) {
+        if (!is_fluid(c, i)) {
+            c.ratio[i] = 0.0f;
+            c.alpha[i] = 0.0f;
+            c.neff[i] = 0.0f;
+            c.active[i] = 0.0f;
+            c.reg[i] = 0.0f;
+        } else {
+            f3 xi = Q(c, i);
+            float mi = c.mass[i];
+            float inv_r0 = 1.0f / c.rho0[i];
+            float rho = mi * kernel_w(0.0f, c.h, c.kind);
+            f3 G{0.0f, 0.0f, 0.0f};
+            float sq = 0.0f;
+            int cnt = c.count[i];
+            int base = c.ws[c.world[i]];
+            for (int k = 0; k < cnt; ++k) {
+                int j = nbr_j(c, base, i, k);
+                f3 rv = xi - Q(c, j);
+                float mj = c.mass[j];
+                rho += mj * kernel_w(length(rv), c.h, c.kind);
+                f3 g = kernel_grad(rv, c.h, c.kind);
+                G = G + (mj * inv_r0) * g;
+                sq += (mi * mj * inv_r0 * inv_r0) * dot(g, g);
+            }
+            float psi = 0.0f, lam = 0.0f;
+            int64_t b4 = 4 * int64_t(i);
+            for (int b = 0; b < 4; ++b) {
+                if (c.s_shape[b4 + b] < 0) continue;
+                psi += c.s_psi[b4 + b];
+                f3 gb = ld3(c.s_gpsi, b4 + b);
+                G = G + gb;
+                for (int cc = b + 1; cc < 4; ++cc)
+                    if (c.s_shape[b4 + cc] >= 0) lam += fmx(-2.0f * dot(gb, ld3(c.s_gpsi, b4 + cc)), 0.0f);
+            }
+            float tot = dot(G, G) + sq + lam;
+            lam += fmx(c.tot_min - tot, 0.0f);
+            tot = fmx(tot, c.tot_min);
+            c.reg[i] = lam;
+            c.ratio[i] = rho * inv_r0 + psi;
+            c.alpha[i] = tot > 1.0e-12f ? 1.0f / tot : 0.0f;
+            c.neff[i] = float(cnt) + psi * c.n_full;
+            c.active[i] = 1.0f;
+            act_v = 1.0;
+        }
+    }
+    chunk_reduce(c, act_v, 0.0, valid ? act_v : -3.4e38);
+}
+
+// ----------------------------------------------------------------------------------------------- solves
+__device__ float density_rate(const Ctx& c, int i) {
+    f3 xi = Q(c, i), vi = V(c, i);
+    float inv_r0 = 1.0f / c.rho0[i];
+    float rate = 0.0f;
+    int cnt = c.count[i];
+    int base = c.ws[c.world[i]];
+    for (int k = 0; k < cnt; ++k) {
+        int j = nbr_j(c, base, i, k);
+        f3 g = kernel_grad(xi - Q(c, j), c.h, c.kind);
+        rate += (c.mass[j] * inv_r0) * dot(vi - V(c, j), g);
+    }
+    int64_t b4 = 4 * int64_t(i);
+    for (int b = 0; b < 4; ++b)
+        if (c.s_shape[b4 + b] >= 0) rate += dot(vi - ld3(c.s_vb, b4 + b), ld3(c.s_gpsi, b4 + b));
+    return rate;
+}
+
+// pressure.py:214-261; block (0, 0) clears the unconverged-world count for begin_solve
+__global__ void __launch_bounds__(256) k_solve_init(Ctx c, int mode, float dt, float warm, float warm_den,
+                                                    float relief, const float* store, const float* learned) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *c.n_unconv = 0;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    if (!is_fluid(c, i)) {
+        c.src[i] = 0.0f;
+        c.kappa[i] = 0.0f;
+        c.lower[i] = 0.0f;
+        return;
+    }
+    float rate = density_rate(c, i);
+    if (mode == 0) {
+        float kdv = 0.0f;
+        if (c.two_sided) kdv = fmx(warm_den * fmn(c.kd[i], 2.5e-4f) / (dt * dt), 0.0f);
+        float lb = -kdv;
+        float rr = rate;
+        if (kdv <= 0.0f) rr = fmx(rate, 0.0f);
+        if (c.neff[i] < c.min_nb) {
+            rr = 0.0f;
+            lb = 0.0f;
+        }
+        c.src[i] = rr;
+        c.lower[i] = lb;
+        c.kappa[i] = fmx(warm * fmn(store[i], 0.5f) / dt, lb);
+        if (learned) c.kappa[i] = fmx(learned[i], lb);
+    } else {
+        float rs = c.ratio[i] + dt * rate;
+        if (c.ratio[i] - 1.0f > relief) rs = 1.0f + relief + dt * rate;
+        c.src[i] = rs;
+        c.lower[i] = 0.0f;
+        c.kappa[i] = warm * fmn(store[i], 2.5e-4f) / (dt * dt);
+        if (learned) c.kappa[i] = fmx(learned[i], 0.0f);
+    }
+}
+
+// pressure.py:355-367
+__global__ void k_begin_solve(Ctx c) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= c.W) return;
+    c.iters[w] = 0;
+    c.err_mean[w] = 0.0f;
+    c.restart[w] = 0;
+    if (c.n_active[w] > 0.0) {
+        c.converged[w] = 0;
+        atomicAdd(c.n_unconv, 1);
+    } else {
+        c.converged[w] = 1;
+    }
+}
+
+// pressure.py:264-277
+__device__ f3 pressure_accel(const Ctx& c, const float* kap, int i) {
+    f3 xi = Q(c, i);
+    float ki = kap[i] / c.rho0[i];
+    f3 a{0.0f, 0.0f, 0.0f};
+    int cnt = c.count[i];
+    int base = c.ws[c.world[i]];
+    for (int k = 0; k < cnt; ++k) {
+        int j = nbr_j(c, base, i, k);
+        f3 g = kernel_grad(xi - Q(c, j), c.h, c.kind);
+        a = a - (c.mass[j] * (ki + kap[j] / c.rho0[j])) * g;
+    }
+    int64_t b4 = 4 * int64_t(i);
+    for (int b = 0; b < 4; ++b)
+        if (c.s_shape[b4 + b] >= 0) a = a - kap[i] * ld3(c.s_gpsi, b4 + b);
+    return a;
+}
+
+// pressure.py:294-310
+__device__ float accel_divergence(const Ctx& c, int i) {
+    f3 xi = Q(c, i), ai = ld3(c.acc, i);
+    float inv_r0 = 1.0f / c.rho0[i];
+    float ar = 0.0f;
+    int cnt = c.count[i];
+    int base = c.ws[c.world[i]];
+    for (int k = 0; k < cnt; ++k) {
+        int j = nbr_j(c, base, i, k);
+        f3 g = kernel_grad(xi - Q(c, j), c.h, c.kind);
+        ar += (c.mass[j] * inv_r0) * dot(ai - ld3(c.acc, j), g);
+    }
+    int64_t b4 = 4 * int64_t(i);
+    for (int b = 0; b < 4; ++b)
+        if (c.s_shape[b4 + b] >= 0) ar += dot(ai, ld3(c.s_gpsi, b4 + b));
+    return ar;
+}
+
+__device__ __forceinline__ bool block_skip(const Ctx& c, int w, int gated) {
+    if (c.converged[w] != 0) return true;
+    return gated != 0 && c.restart[w] == 0;
+}
+
+// accel_kernel (pressure.py:280-291) / cg_accel_kernel (krylov.py:77-87)
+__global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, int gated) {
+    CHUNK_PROLOGUE
+    if (block_skip(c, c.ch_world[b_], gated) || !valid) return;
+    if (!is_fluid(c, i)) {
+        st3(c.acc, i, f3{0.0f, 0.0f, 0.0f});
+        return;
+    }
+    st3(c.acc, i, pressure_accel(c, vec, i));
+}
+
+// cg_apply_kernel (krylov.py:90-109); with `reduce` the chunk partial of pair.x = vec_i (A vec)_i
+__global__ void __launch_bounds__(256) k_apply(Ctx c, int mode, float dt, const float* vec, float* out, int gated,
+                                               int bias, int accum, int reduce) {
+    CHUNK_PROLOGUE
+    if (block_skip(c, c.ch_world[b_], gated)) return;
+    double px = 0.0;
+    if (valid) {
+        if (!is_fluid(c, i)) {
+            out[i] = 0.0f;
+        } else {
+            float sc = mode == 0 ? dt : dt * dt;
+            float v = -sc * (accel_divergence(c, i) - c.reg[i] * vec[i]);
+            px = double(vec[i] * v);
+            if (bias) v -= (mode == 0 ? c.src[i] : c.src[i] - 1.0f);
+            if (accum) v += out[i];
+            out[i] = v;
+        }
+    }
+    if (reduce) chunk_reduce(c, px, 0.0, valid ? 0.0 : -3.4e38);
+}
+
+// cg_init_kernel (krylov.py:112-132)
+__global__ void __launch_bounds__(256) k_cg_init(Ctx c, int mode, float dt) {
+    CHUNK_PROLOGUE
+    if (c.converged[c.ch_world[b_]] != 0) return;
+    double px = 0.0;
+    if (valid) {
+        if (!is_fluid(c, i)) {
+            c.act[i] = 0;
+            c.z[i] = 0.0f;
+            c.pd[i] = 0.0f;
+        } else {
+            float sc = mode == 0 ? dt : dt * dt;
+            int a = 0;
+            float zi = 0.0f;
+            float wi = c.wv[i];
+            if (c.kappa[i] > c.lower[i] || wi < 0.0f) {
+                a = 1;
+                zi = -wi * c.alpha[i] / sc;
+            }
+            c.act[i] = a;
+            c.z[i] = zi;
+            c.pd[i] = zi;
+            px = double(-wi * zi);
+        }
+    }
+    chunk_reduce(c, px, 0.0, valid ? 0.0 : -3.4e38);
+}
+
+// cg_step_kernel (krylov.py:153-177)
+__global__ void __launch_bounds__(256) k_cg_step(Ctx c) {
+    CHUNK_PROLOGUE
+    int w = c.ch_world[b_];
+    if (c.converged[w] != 0 || !valid) return;
+    if (!is_fluid(c, i)) {
+        c.corr[i] = 0.0f;
+        return;
+    }
+    double pap = c.sums[2 * w];
+    double a64 = 0.0;
+    if (pap > 1.0e-30) a64 = c.rz[w] / pap;
+    float a = float(a64);
+    float cc = 0.0f;
+    if (c.act[i] != 0) {
+        float k = c.kappa[i] + a * c.pd[i];
+        if (k < c.lower[i]) {
+            cc = c.lower[i] - k;
+            k = c.lower[i];
+            c.act[i] = 0;
+            atomicAdd(c.restart + w, 1);
+        }
+        c.kappa[i] = k;
+    }
+    c.corr[i] = cc;
+    c.wv[i] = c.wv[i] + a * c.ap[i];
+}
+
+// cg_finish_kernel (krylov.py:180-212) + the chunk partial of (r.z, error, max error)
+__global__ void __launch_bounds__(256) k_cg_finish(Ctx c, int mode, float dt) {
+    CHUNK_PROLOGUE
+    int w = c.ch_world[b_];
+    if (c.converged[w] != 0) return;
+    double px = 0.0, py = 0.0;
+    if (valid) {
+        if (!is_fluid(c, i)) {
+            c.err[i] = 0.0f;
+        } else {
+            float sc = mode == 0 ? dt : dt * dt;
+            float wvv = c.wv[i];
+            if (c.act[i] == 0 && wvv < 0.0f) {
+                c.act[i] = 1;
+                atomicAdd(c.restart + w, 1);
+            }
+            float zi = 0.0f;
+            if (c.act[i] != 0) zi = -wvv * c.alpha[i] / sc;
+            c.z[i] = zi;
+            float e = fmx(-wvv, 0.0f);
+            if (c.kappa[i] > c.lower[i]) e = fabsf(wvv);
+            if (mode == 0) {
+                if (c.neff[i] < c.min_nb) e = 0.0f;
+                e *= dt;
+            }
+            c.err[i] = e;
+            px = double(-wvv * zi);
+            py = double(e);
+        }
+    }
+    chunk_reduce(c, px, py, valid ? py : -3.4e38);
+}
+
+// cg_direction_kernel (krylov.py:243-255)
+__global__ void __launch_bounds__(256) k_cg_direction(Ctx c) {
+    CHUNK_PROLOGUE
+    int w = c.ch_world[b_];
+    if (c.converged[w] != 0 || !valid || !is_fluid(c, i)) return;
+    if (c.act[i] != 0) c.pd[i] = c.z[i] + float(c.stepb[w]) * c.pd[i];
+    else c.pd[i] = 0.0f;
+}
+
+// jacobi_kernel (pressure.py:313-352) + the chunk partial of (error, -, max error); block (0,0) clears the
+// unconverged count the world epilogue recounts
+__global__ void __launch_bounds__(256) k_jacobi(Ctx c, int mode, float dt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *c.n_unconv = 0;
+    CHUNK_PROLOGUE
+    int w = c.ch_world[b_];
+    if (c.converged[w] != 0) return;
+    double e64 = 0.0;
+    if (valid) {
+        if (!is_fluid(c, i)) {
+            c.err[i] = 0.0f;
+        } else {
+            float ar = accel_divergence(c, i) - c.reg[i] * c.kappa[i];
+            float s, scale;
+            if (mode == 0) {
+                ar *= dt;
+                s = -c.src[i];
+                scale = 1.0f / dt;
+            } else {
+                ar *= dt * dt;
+                s = 1.0f - c.src[i];
+                scale = 1.0f / (dt * dt);
+            }
+            float rr = s - ar;
+            float lb = c.lower[i];
+            float k_new = fmx(c.kappa[i] - c.omega * rr * c.alpha[i] * scale, lb);
+            c.kappa[i] = k_new;
+            float e = fmx(-rr, 0.0f);
+            if (k_new > lb) e = fabsf(rr);
+            if (mode == 0) {
+                if (c.neff[i] < c.min_nb) e = 0.0f;
+                e *= dt;
+            }
+            c.err[i] = e;
+            e64 = double(e);
+        }
+    }
+    chunk_reduce(c, e64, 0.0, valid ? e64 : -3.4e38);
+}
+
+__global__ void k_zero_unconv(Ctx c) { *c.n_unconv = 0; }
+
+// contact_limit_kernel + apply_pressure_kernel + lone_friction_kernel (pressure.py:393-480) for one particle: a
+// lone particle (count 0) is nobody's neighbour, so limiting its kappa before the neighbours' pressure
+// accelerations read kappa changes nothing for them.
+__global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, float* store) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n || !is_fluid(c, i)) return;
+    int64_t b4 = 4 * int64_t(i);
+    bool lone = c.count[i] == 0;
+    if (lone && c.kappa[i] > 0.0f) {  // contact_limit_kernel
+        f3 G{0.0f, 0.0f, 0.0f}, vb{0.0f, 0.0f, 0.0f};
+        float wsum = 0.0f;
+        for (int b = 0; b < 4; ++b)
+            if (c.s_shape[b4 + b] >= 0) {
+                G = G + ld3(c.s_gpsi, b4 + b);
+                vb = vb + c.s_psi[b4 + b] * ld3(c.s_vb, b4 + b);
+                wsum += c.s_psi[b4 + b];
+            }
+        float g = length(G);
+        if (g > 0.0f) {
+            if (wsum > 0.0f) vb = vb / wsum;
+            float u = dot(V(c, i) - vb, (-G) / g);
+            c.kappa[i] = fmn(c.kappa[i], fmx(c.v_sep - u, 0.0f) / (dt * g));
+        }
+    }
+    // apply_pressure_kernel
+    f3 a = pressure_accel(c, c.kappa, i);
+    f3 v = V(c, i) + dt * a;
+    float kap = c.kappa[i];
+    float mk = c.mass[i] * kap * dt;
+    for (int b = 0; b < 4; ++b)
+        if (c.s_shape[b4 + b] >= 0) st3(c.s_imp, b4 + b, ld3(c.s_imp, b4 + b) + mk * ld3(c.s_gpsi, b4 + b));
+    store[i] = mode == 0 ? kap * dt : kap * dt * dt;
+    // lone_friction_kernel
+    if (c.friction && lone && kap > 0.0f) {
+        f3 G{0.0f, 0.0f, 0.0f}, vb{0.0f, 0.0f, 0.0f};
+        float wsum = 0.0f, pbest = 0.0f;
+        int best = -1;
+        for (int b = 0; b < 4; ++b) {
+            if (c.s_shape[b4 + b] >= 0) {
+                float ps = c.s_psi[b4 + b];
+                G = G + ld3(c.s_gpsi, b4 + b);
+                vb = vb + ps * ld3(c.s_vb, b4 + b);
+                wsum += ps;
+                if (ps > pbest) {
+                    pbest = ps;
+                    best = b;
+                }
+            }
+        }
+        float g = length(G);
+        if (g > 0.0f && wsum > 0.0f && best >= 0) {
+            int s = c.s_shape[b4 + best];
+            if (s < c.S) {
+                float mu = c.sh_mu[s];
+                if (mu > 0.0f) {
+                    f3 nn = (-G) / g;
+                    f3 u = v - vb / wsum;
+                    f3 vt = u - dot(u, nn) * nn;
+                    float lt = length(vt);
+                    if (lt > 0.0f) {
+                        f3 dv = fmn(1.0f, mu * dt * kap * g / lt) * vt;
+                        v = v - dv;
+                        st3(c.s_imp, b4 + best, ld3(c.s_imp, b4 + best) + c.mass[i] * dv);
+                    }
+                }
+            }
+        }
+    }
+    st3(c.qd, i, v);
+}
+
+// stats.record_solve / record_max_density_error / record_density_tolerance (runtime.py:112-122)
+__global__ void k_record(Ctx c, int mode, float tol) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= c.W) return;
+    if (mode == 0) {
+        c.it_div[w] = c.iters[w];
+        c.st_div_err[w] = c.err_mean[w];
+    } else {
+        c.it_den[w] = c.iters[w];
+        c.st_den_err[w] = c.err_mean[w];
+        c.st_den_err_max[w] = c.n_active[w] > 0.0 ? c.err_max[w] : 0.0f;
+        c.st_den_tol[w] = tol * c.tol_scale[w];
+    }
+}
+
+// ----------------------------------------------------------------------------------------------- resting-jitter rule
+// flow_terms_kernel (solve_driver.py:49-60): which 0 -> (v_z, 1), 1 -> (v_x, v_y)
+__global__ void __launch_bounds__(256) k_flow_terms(Ctx c, int which) {
+    CHUNK_PROLOGUE
+    double x = 0.0, y = 0.0;
+    if (valid && is_fluid(c, i)) {
+        f3 v = V(c, i);
+        if (which == 0) {
+            x = v.z;
+            y = 1.0;
+        } else {
+            x = v.x;
+            y = v.y;
+        }
+    }
+    chunk_reduce(c, x, y, valid ? y : -3.4e38);
+}
+// head_terms_kernel (solve_driver.py:34-46)
+__global__ void __launch_bounds__(256) k_head_terms(Ctx c) {
+    CHUNK_PROLOGUE
+    double x = 0.0, y = 0.0;
+    if (valid && is_fluid(c, i)) {
+        float k = fmx(c.kd[i], 0.0f);
+        x = k;
+        y = double(k * k);
+    }
+    chunk_reduce(c, x, y, valid ? y : -3.4e38);
+}
+// tol_scale_kernel (solve_driver.py:63-82)
+__global__ void k_tol_scale(Ctx c, float dt, float speed, float ratio, float tol, float tol_min) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= c.W) return;
+    float s = 1.0f;
+    float g = length(ld3(c.gravity, w));
+    float s1 = float(c.head[2 * w]), s2 = float(c.head[2 * w + 1]);
+    float v = speed;
+    double nn = c.flow_z[2 * w + 1];
+    if (ratio > 0.0f && nn > 0.0) {
+        f3 u{float(c.flow_xy[2 * w] / nn), float(c.flow_xy[2 * w + 1] / nn), float(c.flow_z[2 * w] / nn)};
+        v = fmx(v, ratio * length(u));
+    }
+    if (g > 0.0f && s1 > 0.0f && s2 > 0.0f) {
+        float depth = 1.5f * s2 / (s1 * dt * dt * g);
+        s = fcl(v * dt / (depth * tol), tol_min / tol, 1.0f);
+    }
+    c.tol_scale[w] = s;
+}
+
+// ----------------------------------------------------------------------------------------------- non-pressure
+// nonpressure_kernel (integrate.py:150-178)
+__global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    f3 vi = V(c, i);
+    if (!is_fluid(c, i)) {
+        st3(c.vtmp, i, vi);
+        return;
+    }
+    int w = c.world[i];
+    f3 a = ld3(c.gravity, w);
+    if (c.f) a = a + ld3(c.f, i) / c.mass[i];
+    f3 v = vi + dt * a;
+    if (c.xsph > 0.0f) {
+        f3 xi = Q(c, i);
+        f3 cr{0.0f, 0.0f, 0.0f};
+        float rho_i = c.ratio[i] * c.rho0[i];
+        int cnt = c.count[i];
+        int base = c.ws[w];
+        for (int k = 0; k < cnt; ++k) {
+            int j = nbr_j(c, base, i, k);
+            float rho_bar = 0.5f * (rho_i + c.ratio[j] * c.rho0[j]);
+            if (rho_bar > 0.0f) {
+                float wk = kernel_w(length(xi - Q(c, j)), c.h, c.kind);
+                cr = cr + (c.mass[j] / rho_bar * wk) * (V(c, j) - vi);
+            }
+        }
+        v = v + c.xsph * cr;
+    }
+    st3(c.vtmp, i, v);
+}
+
+// ----------------------------------------------------------------------------------------------- advect / project
+__device__ float path_sdf(const Ctx& c, int s, f3 l0, f3 l1, float t) {
+    float phi;
+    f3 g;
+    shape_sdf(c.sh_geo[s], ld3(c.sh_scale, s), l0 + t * (l1 - l0), phi, g);
+    return phi;
+}
+// box_span (integrate.py:66-81)
+__device__ void box_span(f3 half, f3 l0, f3 l1, float& lo, float& hi) {
+    lo = -1.0e30f;
+    hi = 1.0e30f;
+    f3 d = l1 - l0;
+    float hv[3] = {half.x, half.y, half.z}, a0[3] = {l0.x, l0.y, l0.z}, dv[3] = {d.x, d.y, d.z};
+    for (int a = 0; a < 3; ++a) {
+        if (fabsf(dv[a]) > 0.0f) {
+            float ta = (-hv[a] - a0[a]) / dv[a];
+            float tb = (hv[a] - a0[a]) / dv[a];
+            lo = fmx(lo, fmn(ta, tb));
+            hi = fmn(hi, fmx(ta, tb));
+        } else if (fabsf(a0[a]) > hv[a]) {
+            lo = 2.0e30f;
+        }
+    }
+}
+// path_entry (integrate.py:84-114)
+__device__ float path_entry(const Ctx& c, int s, f3 l0, f3 l1, int samples) {
+    if (path_sdf(c, s, l0, l1, 0.0f) < 0.0f) return 2.0f;
+    if (c.sh_geo[s] == GEO_BOX) {
+        float lo, hi;
+        box_span(ld3(c.sh_scale, s), l0, l1, lo, hi);
+        if (lo <= hi && lo <= 1.0f && hi >= 0.0f) return fmx(lo, 0.0f);
+        return 2.0f;
+    }
+    float t_out = 2.0f, t_lo = 0.0f, t_hi = 2.0f;
+    for (int k = 1; k <= samples; ++k) {
+        float t = float(k) / float(samples);
+        if (t_hi > 1.0f) {
+            if (path_sdf(c, s, l0, l1, t) < 0.0f) t_hi = t;
+            else t_lo = t;
+        }
+    }
+    if (t_hi <= 1.0f) {
+        for (int b = 0; b < 8; ++b) {
+            float tm = 0.5f * (t_lo + t_hi);
+            if (path_sdf(c, s, l0, l1, tm) < 0.0f) t_hi = tm;
+            else t_lo = tm;
+        }
+        t_out = t_lo;
+    }
+    return t_out;
+}
+// sweep_entry (integrate.py:125-147), static shapes
+__device__ int sweep_entry(const Ctx& c, int w, f3 x0, f3 x1, f3& l_hit) {
+    float len = length(x1 - x0);
+    float reach = len + c.r;
+    int samples = min(max(int(ceilf(len / c.r)) + 1, 8), 64);
+    float best = 2.0f;
+    int hit = -1;
+    l_hit = f3{0.0f, 0.0f, 0.0f};
+    for (int e = c.sh_start[w]; e < c.sh_start[w + 1]; ++e) {
+        int s = c.sh_ids[e];
+        Xf X = load_xf(c.sh_x + 7 * s);
+        bool near_ = c.sh_geo[s] == GEO_PLANE || length(x0 - X.p) <= c.sh_radius[s] + reach;
+        if (!near_) continue;
+        f3 l0 = xf_inv_point(X, x0), l1 = xf_inv_point(X, x1);
+        float t = path_entry(c, s, l0, l1, samples);
+        if (t < best) {
+            best = t;
+            hit = s;
+            l_hit = l0 + t * (l1 - l0);
+        }
+    }
+    return hit;
+}
+__device__ f3 wall_friction(const Ctx& c, f3 v, f3 vb, f3 nn, float vn, int s) {
+    if (c.friction == 0) return v;
+    float mu = c.sh_mu[s];
+    if (mu <= 0.0f) return v;
+    f3 u = v - vb;
+    f3 vt = u - dot(u, nn) * nn;
+    float lt = length(vt);
+    if (lt <= 0.0f) return v;
+    return v - fmn(1.0f, mu * (-vn) / lt) * vt;
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(float* p, float v) {
+    atomicMax(reinterpret_cast<int*>(p), __float_as_int(v));
+}
+
+// shape meters (bodies.py:37-60, static shapes), advect_kernel, particle-wall advance, project_kernel and
+// speed_kernel (integrate.py:181-326) for one particle: each reads only the particle's own state and slots.
+__global__ void __launch_bounds__(256) k_advect_project(Ctx c, float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    int64_t b4 = 4 * int64_t(i);
+    if (!is_fluid(c, i)) {
+        if (c.NW > 0 && c.wvol[i] > 0.0f) st3(c.q, i, Q(c, i) + dt * V(c, i));
+        return;
+    }
+    for (int k = 0; k < 4; ++k) {  // bodies.accumulate_kernel meters (two-way)
+        int s = c.s_shape[b4 + k];
+        if (s < 0 || s >= c.S) continue;
+        f3 J = ld3(c.s_imp, b4 + k);
+        if (J.x == 0.0f && J.y == 0.0f && J.z == 0.0f) continue;
+        const double sc = 1125899906842624.0;
+        atomicAdd(c.meter + 3 * s, (unsigned long long)(long long)llround(double(J.x) * sc));
+        atomicAdd(c.meter + 3 * s + 1, (unsigned long long)(long long)llround(double(J.y) * sc));
+        atomicAdd(c.meter + 3 * s + 2, (unsigned long long)(long long)llround(double(J.z) * sc));
+    }
+    f3 v = V(c, i);
+    f3 x = Q(c, i) + dt * v;
+    int moved = 0;
+    int w = c.world[i];
+    if (length(dt * v) > c.sweep_len) {
+        f3 l;
+        int s = sweep_entry(c, w, x - dt * v, x, l);
+        if (s >= 0) {
+            Xf X = load_xf(c.sh_x + 7 * s);
+            float phi;
+            f3 gl;
+            shape_sdf(c.sh_geo[s], ld3(c.sh_scale, s), l, phi, gl);
+            f3 nn = normalize(xf_vector(X, gl));
+            x = xf_point(X, l) + fmx(c.eps - phi, 0.0f) * nn;
+            f3 vb{0.0f, 0.0f, 0.0f};
+            float vn = dot(v - vb, nn);
+            if (vn < 0.0f) {
+                v = v - vn * nn;
+                v = wall_friction(c, v, vb, nn, vn, s);
+            }
+            moved = 1;
+            atomicAdd(c.st_swept + w, 1);
+        }
+    }
+    int busy = 1, npass = 0;
+    while (busy != 0 && npass < 4) {
+        busy = 0;
+        npass += 1;
+        for (int b = 0; b < 4; ++b) {
+            int s = c.s_shape[b4 + b];
+            if (s == c.S && c.NW > 0) {  // pw_project (particle_walls.py:122-128)
+                f3 vb = ld3(c.s_vb, b4 + b), nb = ld3(c.s_nbl, b4 + b), xb = ld3(c.s_xbl, b4 + b);
+                float dd = dot(x - (xb + dt * vb), nb);
+                if (dd < c.eps) {
+                    x = x + (c.eps - dd) * nb;
+                    float vn = dot(v - vb, nb);
+                    if (vn < 0.0f) v = v - vn * nb;
+                    moved = 1;
+                    busy = 1;
+                }
+            } else if (s >= 0 && s < c.S) {
+                Xf X = load_xf(c.sh_x + 7 * s);
+                f3 xl = xf_inv_point(X, x);
+                float phi;
+                f3 gl;
+                shape_sdf(c.sh_geo[s], ld3(c.sh_scale, s), xl, phi, gl);
+                if (phi < c.eps) {
+                    f3 nn = normalize(xf_vector(X, gl));
+                    float push = c.eps - phi;
+                    if (c.s_phi[b4 + b] >= 0.0f) {
+                        f3 ns = xf_vector(X, ld3(c.s_nbl, b4 + b));
+                        float push_s = c.eps - dot(ns, x - xf_point(X, ld3(c.s_xbl, b4 + b)));
+                        if (push_s > 0.0f) {
+                            nn = ns;
+                            push = push_s;
+                        }
+                    }
+                    if (push > 0.0f) {
+                        x = x + push * nn;
+                        f3 vb{0.0f, 0.0f, 0.0f};
+                        float vn = dot(v - vb, nn);
+                        if (vn < 0.0f) {
+                            v = v - vn * nn;
+                            v = wall_friction(c, v, vb, nn, vn, s);
+                        }
+                        moved = 1;
+                        busy = 1;
+                    }
+                }
+            }
+        }
+    }
+    st3(c.q, i, x);
+    if (moved) {
+        st3(c.qd, i, v);
+        atomicAdd(c.st_projected + w, 1);
+    }
+    float sp = length(v);
+    if (isfinite(sp) && isfinite(x.x + x.y + x.z)) {
+        atomic_max_nonneg(c.st_max_speed + w, sp);
+    } else {
+        atomicAdd(c.st_nonfinite + w, 1);
+        atomic_max_nonneg(c.st_max_speed + w, 1.0e30f);
+    }
+}
+
+// ----------------------------------------------------------------------------------------------- handle
+struct Handle {
+    DfsphParams P;
+    Ctx c;
+    int n_chunks = 0, n_scan_blocks = 0;
+    int64_t nbr_bytes = 0, nbr_hi_bytes = 0;
+    int* h_unconv = nullptr;  // pinned
+    std::vector<void*> owned;
+    float* qd_saved = nullptr;
+    float* learned_div = nullptr;
+    float* learned_den = nullptr;
+    int check_every = 1;
+    int wall_first = 0;  // first cell of the wall grids
+};
+
+template <class T>
+int dalloc(Handle* H, T** p, int64_t count) {
+    void* q = nullptr;
+    size_t bytes = size_t(std::max<int64_t>(count, 1)) * sizeof(T);
+    HIPCHK(hipMalloc(&q, bytes));
+    HIPCHK(hipMemset(q, 0, bytes));
+    H->owned.push_back(q);
+    *p = static_cast<T*>(q);
+    return DFSPH_OK;
+}
+template <class T>
+int dupload(Handle* H, T** p, const T* host, int64_t count) {
+    int rc = dalloc(H, p, count);
+    if (rc) return rc;
+    if (count > 0) HIPCHK(hipMemcpy(*p, host, size_t(count) * sizeof(T), hipMemcpyHostToDevice));
+    return DFSPH_OK;
+}
+
+inline int blocks_for(int64_t n, int b = 256) { return int(std::max<int64_t>((n + b - 1) / b, 1)); }
+
+#define LAUNCH_CHECK()                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = hipGetLastError();                                               \
+        if (e_ != hipSuccess) return fail(DFSPH_EDEVICE, std::string("launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+Handle* as_handle(void* h) { return static_cast<Handle*>(h); }
+
+// ----------------------------------------------------------------------------------------------- phases (host)
+int phase_neighbors(Handle* H, hipStream_t st) {
+    Ctx& c = H->c;
+    if (c.n == 0) return DFSPH_OK;
+    k_fill_i32<<<std::min(blocks_for(int64_t(c.n_cells) + 1), 4096), 256, 0, st>>>(c.cell_count, 0,
+                                                                                   int64_t(c.n_cells) + 1);
+    k_groups_hist<<<blocks_for(c.n), 256, 0, st>>>(c);
+    k_scan_reduce<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part);
+    k_scan_top<<<1, 256, 0, st>>>(c.scan_part, H->n_scan_blocks, c.cell_start, c.n_cells);
+    k_scan_apply<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part, c.cell_start, c.cell_fill);
+    k_scatter<<<blocks_for(c.n), 256, 0, st>>>(c);
+    if (c.NW > 0) k_sort_cells<<<blocks_for(c.n_cells - H->wall_first), 256, 0, st>>>(c, H->wall_first, c.n_cells);
+    k_build_lists<<<blocks_for(c.n, BUILD_BLOCK), BUILD_BLOCK, 0, st>>>(c);
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+int phase_slots(Handle* H, hipStream_t st) {
+    Ctx& c = H->c;
+    if (c.n == 0) return DFSPH_OK;
+    k_slots<<<blocks_for(c.n), 256, 0, st>>>(c);
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+int phase_density(Handle* H, hipStream_t st) {
+    Ctx& c = H->c;
+    if (c.n == 0) return DFSPH_OK;
+    k_density_alpha<<<H->n_chunks, 256, 0, st>>>(c);
+    k_world<<<c.W, 256, 0, st>>>(c, W_NACTIVE, StopSpec{});
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+int phase_flow(Handle* H, hipStream_t st) {
+    Ctx& c = H->c;
+    if (!(H->P.rest_jitter_speed > 0.0) || !(H->P.rest_jitter_flow_ratio > 0.0) || c.n == 0) return DFSPH_OK;
+    k_flow_terms<<<H->n_chunks, 256, 0, st>>>(c, 0);
+    k_world<<<c.W, 256, 0, st>>>(c, W_FLOW_Z, StopSpec{});
+    k_flow_terms<<<H->n_chunks, 256, 0, st>>>(c, 1);
+    k_world<<<c.W, 256, 0, st>>>(c, W_FLOW_XY, StopSpec{});
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+int read_unconv(Handle* H, hipStream_t st, int* out) {
+    HIPCHK(hipMemcpyAsync(H->h_unconv, H->c.n_unconv, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *out = *H->h_unconv;
+    return DFSPH_OK;
+}
+
+struct SolveOverride {
+    double tol = -1.0, err_factor = -1.0, warm = -1.0, min_nb = -1.0;
+    int max_it = -1;
+};
+
+// solve_driver.py:121-162 (+ krylov.py:289-318 / solve_driver.py:171-186 loops)
+int phase_solve(Handle* H, int mode, double dt_h, int fixed_iters, hipStream_t st, const SolveOverride& ov = {}) {
+    Ctx c = H->c;  // by value: overrides stay local to this solve
+    const DfsphParams& P = H->P;
+    if (c.n == 0) return DFSPH_OK;
+    bool div = mode == 0;
+    float dt = float(dt_h);
+    float* store = div ? c.kv : c.kd;
+    double tolh = div ? P.max_divergence_error : P.max_density_error;
+    if (ov.tol >= 0.0) tolh = ov.tol;
+    if (ov.min_nb >= 0.0) c.min_nb = float(ov.min_nb);
+    double err_factor = ov.err_factor >= 0.0 ? ov.err_factor : P.max_error_factor;
+    if (!div && P.rest_jitter_speed > 0.0) {  // _density_tolerance (solve_driver.py:108-119)
+        k_head_terms<<<H->n_chunks, 256, 0, st>>>(c);
+        k_world<<<c.W, 256, 0, st>>>(c, W_HEAD, StopSpec{});
+        k_tol_scale<<<blocks_for(c.W), 256, 0, st>>>(c, dt, float(P.rest_jitter_speed),
+                                                      float(P.rest_jitter_flow_ratio), float(P.max_density_error),
+                                                      float(std::min(P.min_density_error, P.max_density_error)));
+    }
+    double warm_h = div ? P.warm_start_divergence : P.warm_start_density;
+    if (ov.warm >= 0.0) warm_h = ov.warm;
+    float relief = 1.0e30f;
+    if (!div && P.compression_relief_speed > 0.0) relief = float(P.compression_relief_speed * dt_h * P.lattice_grad);
+    const float* learned = div ? H->learned_div : H->learned_den;
+    k_solve_init<<<blocks_for(c.n), 256, 0, st>>>(c, mode, dt, float(warm_h), float(P.warm_start_density), relief,
+                                                  store, learned);
+    k_begin_solve<<<blocks_for(c.W), 256, 0, st>>>(c);
+    StopSpec sp;
+    sp.tol = float(tolh);
+    sp.tol_max = float(tolh * err_factor);
+    sp.min_it = P.min_iterations;
+    sp.max_it = ov.max_it > 0 ? ov.max_it : P.max_iterations;
+    sp.use_scale = div ? 0 : 1;
+    if (fixed_iters >= 0) sp.min_it = sp.max_it = std::max(fixed_iters, 1);
+    LAUNCH_CHECK();
+    int nu = 0;
+    int guard = 0;
+    if (P.pressure_solver == 0) {
+        k_accel<<<H->n_chunks, 256, 0, st>>>(c, c.kappa, 0);
+        k_apply<<<H->n_chunks, 256, 0, st>>>(c, mode, dt, c.kappa, c.wv, 0, 1, 0, 0);
+        k_cg_init<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
+        k_world<<<c.W, 256, 0, st>>>(c, W_CG_INIT, sp);
+        LAUNCH_CHECK();
+        int rc = read_unconv(H, st, &nu);
+        if (rc) return rc;
+        while (nu != 0) {
+            for (int rep = 0; rep < H->check_every; ++rep) {
+                k_accel<<<H->n_chunks, 256, 0, st>>>(c, c.pd, 0);
+                k_apply<<<H->n_chunks, 256, 0, st>>>(c, mode, dt, c.pd, c.ap, 0, 0, 0, 1);
+                k_world<<<c.W, 256, 0, st>>>(c, W_CG_PAP, sp);
+                k_cg_step<<<H->n_chunks, 256, 0, st>>>(c);
+                k_accel<<<H->n_chunks, 256, 0, st>>>(c, c.corr, 1);
+                k_apply<<<H->n_chunks, 256, 0, st>>>(c, mode, dt, c.corr, c.wv, 1, 0, 1, 0);
+                k_cg_finish<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
+                k_zero_unconv<<<1, 1, 0, st>>>(c);
+                k_world<<<c.W, 256, 0, st>>>(c, W_CG_BETA, sp);
+                k_cg_direction<<<H->n_chunks, 256, 0, st>>>(c);
+            }
+            LAUNCH_CHECK();
+            rc = read_unconv(H, st, &nu);
+            if (rc) return rc;
+            if (++guard > 100000) return fail(DFSPH_EDEVICE, "pressure solve did not terminate");
+        }
+    } else {
+        int rc = read_unconv(H, st, &nu);
+        if (rc) return rc;
+        while (nu != 0) {
+            for (int rep = 0; rep < H->check_every; ++rep) {
+                k_accel<<<H->n_chunks, 256, 0, st>>>(c, c.kappa, 0);
+                k_jacobi<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
+                k_world<<<c.W, 256, 0, st>>>(c, W_JACOBI, sp);
+            }
+            LAUNCH_CHECK();
+            rc = read_unconv(H, st, &nu);
+            if (rc) return rc;
+            if (++guard > 100000) return fail(DFSPH_EDEVICE, "pressure solve did not terminate");
+        }
+    }
+    k_post_solve<<<blocks_for(c.n), 256, 0, st>>>(c, mode, dt, store);
+    k_record<<<blocks_for(c.W), 256, 0, st>>>(c, mode, float(tolh));
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+int phase_nonpressure(Handle* H, double dt, hipStream_t st) {
+    Ctx& c = H->c;
+    if (c.n == 0) return DFSPH_OK;
+    k_nonpressure<<<blocks_for(c.n), 256, 0, st>>>(c, float(dt));
+    k_copy_f32<<<blocks_for(3 * int64_t(c.n)), 256, 0, st>>>(c.qd, c.vtmp, 3 * int64_t(c.n));
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+int phase_advect(Handle* H, double dt, hipStream_t st) {
+    Ctx& c = H->c;
+    if (c.n == 0) return DFSPH_OK;
+    k_fill_f32<<<blocks_for(c.W), 256, 0, st>>>(c.st_max_speed, 0.0f, c.W);
+    k_advect_project<<<blocks_for(c.n), 256, 0, st>>>(c, float(dt));
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+void bind_state(Handle* H, const float* q, const float* qd, const float* f) {
+    H->c.q = const_cast<float*>(q);
+    H->c.qd = const_cast<float*>(qd);
+    H->c.f = f;
+}
+
+}  // namespace
+
+// =============================================================================================== C-ABI
+extern "C" {
+
+int32_t dfsph_abi_version(void) { return DFSPH_ABI_VERSION; }
+const char* dfsph_last_error(void) { return g_err.c_str(); }
+
+int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphModel* M, void** out) {
+    if (!P || !L || !M || !out) return fail(DFSPH_EINVAL, "dfsph_create: null argument");
+    if (L->n < 0 || L->world_count < 1) return fail(DFSPH_EINVAL, "dfsph_create: need n >= 0 and world_count >= 1");
+    if (P->neighbor_capacity < 8 || P->neighbor_capacity > 64)
+        return fail(DFSPH_EINVAL, "neighbor_capacity must be in [8, 64]");
+    if (!(P->h > 0.0)) return fail(DFSPH_EINVAL, "h must be > 0");
+    const int W = L->world_count, n = L->n, S = L->shape_count;
+    if (L->world_start[0] != 0 || L->world_start[W] != n)
+        return fail(DFSPH_EINVAL, "world_start must cover every particle (world_start[0] = 0, world_start[W] = n)");
+    for (int w = 0; w < W; ++w)
+        if (L->world_start[w + 1] < L->world_start[w]) return fail(DFSPH_EINVAL, "world_start must be ascending");
+    for (int s = 0; s < S; ++s) {
+        if (L->shape_body[s] >= 0)
+            return fail(DFSPH_EINVAL, "shapes attached to bodies are not supported (static shapes only)");
+        int g = L->shape_geo[s];
+        if (g != GEO_BOX && g != GEO_SPHERE && g != GEO_PLANE)
+            return fail(DFSPH_EINVAL, "unsupported shape geometry (box, sphere and infinite plane only)");
+        if (g == GEO_PLANE && (L->shape_scale[3 * s] > 0.0f && L->shape_scale[3 * s + 1] > 0.0f))
+            return fail(DFSPH_EINVAL, "finite planes are not supported");
+    }
+    Handle* H = new Handle();
+    H->P = *P;
+    Ctx& c = H->c;
+    std::memset(&c, 0, sizeof(Ctx));
+    c.n = n;
+    c.W = W;
+    c.S = S;
+    c.NW = L->wall_particle_count;
+    c.cap = P->neighbor_capacity;
+    c.kind = P->kernel;
+    c.friction = P->wall_friction;
+    c.curvature = P->curvature;
+    c.two_sided = P->divergence_two_sided;
+    c.h = float(P->h);
+    c.cell = float(P->h * (1.0 + 1.0e-5));
+    c.r = float(P->particle_radius);
+    c.n_full = float(P->n_full);
+    c.tot_min = float(P->tot_min);
+    c.min_nb = float(P->min_neighbors_divergence);
+    c.omega = float(P->jacobi_omega);
+    c.xsph = float(P->xsph_c);
+    c.v_sep = float(P->contact_separation_speed);
+    c.offset = float(P->wall_offset);
+    c.amplitude = float(P->wall_amplitude);
+    c.taper0 = float(P->taper0);
+    c.taper1 = float(P->taper1);
+    c.reach = float(P->reach);
+    c.eps = float(P->project_eps);
+    c.sweep_len = float(P->sweep_len);
+    c.mass = M->mass;
+    c.rho0 = M->rest_density;
+    c.flags = M->flags;
+    c.phase = M->phase;
+    c.world = M->world;
+    c.gravity = M->gravity;
+    c.wvol = M->wall_volume;
+    int maxw = 0;
+    for (int w = 0; w < W; ++w) maxw = std::max(maxw, L->world_start[w + 1] - L->world_start[w]);
+    c.wide = maxw > 65536;
+    int rc = 0;
+#define TRY(x)             \
+    do {                   \
+        rc = (x);          \
+        if (rc) goto fail; \
+    } while (0)
+    {
+        // chunk table
+        std::vector<int> cb, ce, cw, wc{0};
+        for (int w = 0; w < W; ++w) {
+            for (int b = L->world_start[w]; b < L->world_start[w + 1]; b += CHUNK) {
+                cb.push_back(b);
+                ce.push_back(std::min(b + CHUNK, L->world_start[w + 1]));
+                cw.push_back(w);
+            }
+            wc.push_back(int(cb.size()));
+        }
+        H->n_chunks = int(cb.size());
+        TRY(dupload(H, (int**)&c.ch_begin, cb.data(), int64_t(cb.size())));
+        TRY(dupload(H, (int**)&c.ch_end, ce.data(), int64_t(ce.size())));
+        TRY(dupload(H, (int**)&c.ch_world, cw.data(), int64_t(cw.size())));
+        TRY(dupload(H, (int**)&c.w_chunk, wc.data(), int64_t(wc.size())));
+        std::vector<int> ws(L->world_start, L->world_start + W + 1);
+        TRY(dupload(H, (int**)&c.ws, ws.data(), W + 1));
+        // grid
+        std::vector<float> org(L->grid_origin, L->grid_origin + 3 * W);
+        std::vector<int> dims(3 * W), base(2 * W + 1);
+        long long tot = 0;
+        for (int k = 0; k < 3 * W; ++k) dims[k] = std::max(1, L->grid_dims[k]);
+        for (int g = 0; g < 2 * W; ++g) {
+            int w = g % W;
+            base[g] = int(tot);
+            tot += (long long)dims[3 * w] * dims[3 * w + 1] * dims[3 * w + 2];
+        }
+        if (tot > (1LL << 30)) {
+            rc = fail(DFSPH_ECAPACITY, "cell grid too large (> 2^30 cells)");
+            goto fail;
+        }
+        base[2 * W] = int(tot);
+        c.n_cells = int(tot);
+        H->wall_first = base[W];
+        TRY(dupload(H, (float**)&c.g_org, org.data(), 3 * W));
+        TRY(dupload(H, (int**)&c.g_dims, dims.data(), 3 * W));
+        TRY(dupload(H, (int**)&c.cell_base, base.data(), 2 * W + 1));
+        TRY(dalloc(H, &c.cell_count, int64_t(c.n_cells) + 1));
+        TRY(dalloc(H, &c.cell_start, int64_t(c.n_cells) + 1));
+        TRY(dalloc(H, &c.cell_fill, int64_t(c.n_cells) + 1));
+        H->n_scan_blocks = blocks_for(c.n_cells, 1024);
+        TRY(dalloc(H, &c.scan_part, H->n_scan_blocks));
+        TRY(dalloc(H, &c.items, n));
+        TRY(dalloc(H, &c.key, n));
+        // shapes
+        std::vector<int> ss(L->shape_start, L->shape_start + W + 1);
+        TRY(dupload(H, (int**)&c.sh_start, ss.data(), W + 1));
+        TRY(dupload(H, (int**)&c.sh_ids, L->shape_ids, ss[W]));
+        TRY(dupload(H, (int**)&c.sh_geo, L->shape_geo, S));
+        TRY(dupload(H, (float**)&c.sh_scale, L->shape_scale, 3 * S));
+        TRY(dupload(H, (float**)&c.sh_x, L->shape_xform, 7 * S));
+        TRY(dupload(H, (float**)&c.sh_radius, L->shape_radius, S));
+        TRY(dupload(H, (float**)&c.sh_mu, L->shape_mu, S));
+        // lists
+        H->nbr_bytes = int64_t(c.cap) * std::max(n, 1) * 2;
+        H->nbr_hi_bytes = c.wide ? int64_t(c.cap) * std::max(n, 1) : 1;
+        TRY(dalloc(H, &c.nbr, int64_t(c.cap) * std::max(n, 1)));
+        TRY(dalloc(H, &c.nbr_hi, c.wide ? int64_t(c.cap) * std::max(n, 1) : 1));
+        TRY(dalloc(H, &c.count, n));
+        TRY(dalloc(H, &c.group, n));
+        // slots
+        TRY(dalloc(H, &c.s_shape, 4 * int64_t(n)));
+        TRY(dalloc(H, &c.s_psi, 4 * int64_t(n)));
+        TRY(dalloc(H, &c.s_phi, 4 * int64_t(n)));
+        for (float** p : {&c.s_gpsi, &c.s_xb, &c.s_vb, &c.s_xbl, &c.s_nbl, &c.s_imp}) TRY(dalloc(H, p, 12 * int64_t(n)));
+        for (float** p : {&c.ratio, &c.alpha, &c.neff, &c.active, &c.reg, &c.src, &c.kappa, &c.err, &c.lower, &c.kv,
+                          &c.kd, &c.pd, &c.z, &c.wv, &c.ap, &c.corr})
+            TRY(dalloc(H, p, n));
+        TRY(dalloc(H, &c.acc, 3 * int64_t(n)));
+        TRY(dalloc(H, &c.vtmp, 3 * int64_t(n)));
+        TRY(dalloc(H, &c.act, n));
+        for (int** p : {&c.converged, &c.iters, &c.restart, &c.st_overflow, &c.st_projected, &c.st_swept,
+                        &c.st_nonfinite, &c.it_den, &c.it_div})
+            TRY(dalloc(H, p, W));
+        for (float** p : {&c.err_mean, &c.err_max, &c.tol_scale, &c.st_max_speed, &c.st_den_err, &c.st_div_err,
+                          &c.st_den_err_max, &c.st_den_tol})
+            TRY(dalloc(H, p, W));
+        for (double** p : {&c.n_active, &c.err_sum, &c.rz, &c.stepb}) TRY(dalloc(H, p, W));
+        for (double** p : {&c.sums, &c.flow_z, &c.flow_xy, &c.head}) TRY(dalloc(H, p, 2 * W));
+        TRY(dalloc(H, &c.meter, 3 * int64_t(std::max(S, 1))));
+        TRY(dalloc(H, &c.part, 3 * int64_t(std::max(H->n_chunks, 1))));
+        TRY(dalloc(H, &c.n_unconv, 1));
+        TRY(dalloc(H, &H->qd_saved, 3 * int64_t(n)));
+        {
+            std::vector<float> ones(W, 1.0f);
+            HIPCHK(hipMemcpy(c.tol_scale, ones.data(), W * sizeof(float), hipMemcpyHostToDevice));
+        }
+        void* pinned = nullptr;
+        if (hipHostMalloc(&pinned, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+            rc = fail(DFSPH_EDEVICE, "hipHostMalloc failed");
+            goto fail;
+        }
+        H->h_unconv = static_cast<int*>(pinned);
+    }
+    *out = H;
+    return DFSPH_OK;
+fail:
+    for (void* p : H->owned) (void)hipFree(p);
+    delete H;
+    return rc;
+#undef TRY
+}
+
+int32_t dfsph_destroy(void* h) {
+    if (!h) return DFSPH_OK;
+    Handle* H = as_handle(h);
+    (void)hipDeviceSynchronize();
+    for (void* p : H->owned) (void)hipFree(p);
+    if (H->h_unconv) (void)hipHostFree(H->h_unconv);
+    delete H;
+    return DFSPH_OK;
+}
+
+int32_t dfsph_build_neighbors(void* h, const float* q, void* stream) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    Handle* H = as_handle(h);
+    bind_state(H, q, H->c.qd, nullptr);
+    return phase_neighbors(H, static_cast<hipStream_t>(stream));
+}
+
+int32_t dfsph_boundary_slots(void* h, const float* q, const float* qd, void* stream) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    Handle* H = as_handle(h);
+    bind_state(H, q, qd, nullptr);
+    return phase_slots(H, static_cast<hipStream_t>(stream));
+}
+
+int32_t dfsph_density_alpha(void* h, const float* q, void* stream) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    Handle* H = as_handle(h);
+    bind_state(H, q, H->c.qd, nullptr);
+    return phase_density(H, static_cast<hipStream_t>(stream));
+}
+
+int32_t dfsph_solve(void* h, int32_t mode, float* q, float* qd, double dt, int32_t fixed_iters, void* stream) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    if (!(dt > 0.0)) return fail(DFSPH_EINVAL, "dt must be > 0");
+    if (mode != 0 && mode != 1) return fail(DFSPH_EINVAL, "mode must be 0 (divergence) or 1 (density)");
+    Handle* H = as_handle(h);
+    bind_state(H, q, qd, nullptr);
+    return phase_solve(H, mode, dt, fixed_iters, static_cast<hipStream_t>(stream));
+}
+
+int32_t dfsph_nonpressure(void* h, const float* q, float* qd, const float* f, double dt, void* stream) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    if (!(dt > 0.0)) return fail(DFSPH_EINVAL, "dt must be > 0");
+    Handle* H = as_handle(h);
+    bind_state(H, q, qd, f);
+    return phase_nonpressure(H, dt, static_cast<hipStream_t>(stream));
+}
+
+int32_t dfsph_advect_project(void* h, float* q, float* qd, double dt, void* stream) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    if (!(dt > 0.0)) return fail(DFSPH_EINVAL, "dt must be > 0");
+    Handle* H = as_handle(h);
+    bind_state(H, q, qd, nullptr);
+    return phase_advect(H, dt, static_cast<hipStream_t>(stream));
+}
+
+int32_t dfsph_flow_speed(void* h, const float* qd, void* stream) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    Handle* H = as_handle(h);
+    bind_state(H, H->c.q, qd, nullptr);
+    return phase_flow(H, static_cast<hipStream_t>(stream));
+}
+
+// solver.py:218-268
+int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void* stream) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    if (!(dt > 0.0)) return fail(DFSPH_EINVAL, "SolverDFSPH.step: dt must be > 0");
+    Handle* H = as_handle(h);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    bind_state(H, q, qd, f);
+    int rc;
+    if ((rc = phase_neighbors(H, st))) return rc;
+    if ((rc = phase_slots(H, st))) return rc;
+    if ((rc = phase_density(H, st))) return rc;
+    if ((rc = phase_flow(H, st))) return rc;
+    if (H->P.divergence_solve && (rc = phase_solve(H, 0, dt, -1, st))) return rc;
+    if ((rc = phase_nonpressure(H, dt, st))) return rc;
+    if ((rc = phase_solve(H, 1, dt, -1, st))) return rc;
+    return phase_advect(H, dt, st);
+}
+
+// prime.py:53-115 (guess: k = p / rho0 per particle, device)
+int32_t dfsph_prime(void* h, float* q, float* qd, const float* guess, double dt, double tol, int32_t max_iterations,
+                    int32_t* iters, void* stream) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    if (!(dt > 0.0)) return fail(DFSPH_EINVAL, "dt must be > 0");
+    Handle* H = as_handle(h);
+    Ctx& c = H->c;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (c.n == 0) {
+        if (iters) *iters = 0;
+        return DFSPH_OK;
+    }
+    bind_state(H, q, qd, nullptr);
+    HIPCHK(hipMemcpyAsync(H->qd_saved, qd, 12 * size_t(c.n), hipMemcpyDeviceToDevice, st));
+    int rc;
+    if ((rc = phase_neighbors(H, st))) return rc;
+    if ((rc = phase_slots(H, st))) return rc;
+    if ((rc = phase_density(H, st))) return rc;
+    k_scale_f32_rev<<<blocks_for(c.n), 256, 0, st>>>(c.kv, guess, float(dt), c.n);
+    if ((rc = phase_nonpressure(H, dt, st))) return rc;
+    SolveOverride ov;
+    ov.tol = tol;
+    ov.max_it = max_iterations;
+    ov.warm = 1.0;
+    ov.min_nb = 0.0;
+    ov.err_factor = 1.0e9;
+    if ((rc = phase_solve(H, 0, dt, -1, st, ov))) return rc;
+    HIPCHK(hipMemcpyAsync(qd, H->qd_saved, 12 * size_t(c.n), hipMemcpyDeviceToDevice, st));
+    k_fill_f32<<<blocks_for(12 * int64_t(c.n)), 256, 0, st>>>(c.s_imp, 0.0f, 12 * int64_t(c.n));
+    k_scale_f32<<<blocks_for(c.n), 256, 0, st>>>(c.kd, c.kv, float(dt), c.n);
+    k_fill_f32<<<blocks_for(c.n), 256, 0, st>>>(c.kv, 0.0f, c.n);
+    LAUNCH_CHECK();
+    std::vector<int> it(c.W);
+    HIPCHK(hipMemcpyAsync(it.data(), c.it_div, c.W * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (iters) *iters = *std::max_element(it.begin(), it.end());
+    return DFSPH_OK;
+}
+
+int32_t dfsph_set_warm_start_model(void* h, const void* weights, void* stream) {
+    (void)h;
+    (void)weights;
+    (void)stream;
+    return fail(DFSPH_EINVAL, "learned warm start: use dfsph_set_warm_start_kappa");
+}
+
+// learned warm start as given kappa fields [n] (device, NULL = the reference rule) for the divergence and the
+// density solve
+int32_t dfsph_set_warm_start_kappa(void* h, const float* kappa_div, const float* kappa_den) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    Handle* H = as_handle(h);
+    H->learned_div = const_cast<float*>(kappa_div);
+    H->learned_den = const_cast<float*>(kappa_den);
+    return DFSPH_OK;
+}
+
+int32_t dfsph_set_check_interval(void* h, int32_t every) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    if (every < 1 || every > 64) return fail(DFSPH_EINVAL, "check interval must be in [1, 64]");
+    as_handle(h)->check_every = every;
+    return DFSPH_OK;
+}
+
+int32_t dfsph_get_buffer(void* h, int32_t which, void** ptr, int64_t* nb) {
+    if (!h || !ptr || !nb) return fail(DFSPH_EINVAL, "null argument");
+    Handle* H = as_handle(h);
+    Ctx& c = H->c;
+    int64_t n = std::max(c.n, 1), W = c.W;
+#define B(p, bytes)                   \
+    {                                 \
+        *ptr = (void*)(p);            \
+        *nb = int64_t(bytes);         \
+        return DFSPH_OK;              \
+    }
+    switch (which) {
+        case DFSPH_BUF_NBR: B(c.nbr, H->nbr_bytes)
+        case DFSPH_BUF_NBR_HI: B(c.nbr_hi, H->nbr_hi_bytes)
+        case DFSPH_BUF_COUNT: B(c.count, 4 * n)
+        case DFSPH_BUF_GROUP: B(c.group, 4 * n)
+        case DFSPH_BUF_SLOT_SHAPE: B(c.s_shape, 16 * n)
+        case DFSPH_BUF_SLOT_PSI: B(c.s_psi, 16 * n)
+        case DFSPH_BUF_SLOT_GPSI: B(c.s_gpsi, 48 * n)
+        case DFSPH_BUF_SLOT_XB: B(c.s_xb, 48 * n)
+        case DFSPH_BUF_SLOT_VB: B(c.s_vb, 48 * n)
+        case DFSPH_BUF_SLOT_PHI: B(c.s_phi, 16 * n)
+        case DFSPH_BUF_SLOT_XBL: B(c.s_xbl, 48 * n)
+        case DFSPH_BUF_SLOT_NBL: B(c.s_nbl, 48 * n)
+        case DFSPH_BUF_SLOT_IMPULSE: B(c.s_imp, 48 * n)
+        case DFSPH_BUF_RATIO: B(c.ratio, 4 * n)
+        case DFSPH_BUF_ALPHA: B(c.alpha, 4 * n)
+        case DFSPH_BUF_NEFF: B(c.neff, 4 * n)
+        case DFSPH_BUF_ACTIVE: B(c.active, 4 * n)
+        case DFSPH_BUF_REG: B(c.reg, 4 * n)
+        case DFSPH_BUF_SRC: B(c.src, 4 * n)
+        case DFSPH_BUF_KAPPA: B(c.kappa, 4 * n)
+        case DFSPH_BUF_ACC: B(c.acc, 12 * n)
+        case DFSPH_BUF_ERR: B(c.err, 4 * n)
+        case DFSPH_BUF_LOWER: B(c.lower, 4 * n)
+        case DFSPH_BUF_KV: B(c.kv, 4 * n)
+        case DFSPH_BUF_KD: B(c.kd, 4 * n)
+        case DFSPH_BUF_VTMP: B(c.vtmp, 12 * n)
+        case DFSPH_BUF_CONVERGED: B(c.converged, 4 * W)
+        case DFSPH_BUF_ITERS: B(c.iters, 4 * W)
+        case DFSPH_BUF_ERR_MEAN: B(c.err_mean, 4 * W)
+        case DFSPH_BUF_ERR_MAX: B(c.err_max, 4 * W)
+        case DFSPH_BUF_N_ACTIVE: B(c.n_active, 8 * W)
+        case DFSPH_BUF_TOL_SCALE: B(c.tol_scale, 4 * W)
+        case DFSPH_BUF_OVERFLOW: B(c.st_overflow, 4 * W)
+        case DFSPH_BUF_MAX_SPEED: B(c.st_max_speed, 4 * W)
+        case DFSPH_BUF_ITERS_DEN: B(c.it_den, 4 * W)
+        case DFSPH_BUF_ITERS_DIV: B(c.it_div, 4 * W)
+        case DFSPH_BUF_DEN_ERR: B(c.st_den_err, 4 * W)
+        case DFSPH_BUF_DIV_ERR: B(c.st_div_err, 4 * W)
+        case DFSPH_BUF_SHAPE_METER: B(c.meter, 24 * int64_t(std::max(c.S, 1)))
+        default: return fail(DFSPH_EINVAL, "unknown buffer id");
+    }
+#undef B
+}
+
+int32_t dfsph_read_buffer(void* h, int32_t which, void* dst, int64_t nb) {
+    void* p;
+    int64_t have;
+    int rc = dfsph_get_buffer(h, which, &p, &have);
+    if (rc) return rc;
+    if (nb > have) return fail(DFSPH_ECAPACITY, "read beyond the buffer");
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(dst, p, size_t(nb), hipMemcpyDeviceToHost));
+    return DFSPH_OK;
+}
+
+int32_t dfsph_write_buffer(void* h, int32_t which, const void* src, int64_t nb) {
+    void* p;
+    int64_t have;
+    int rc = dfsph_get_buffer(h, which, &p, &have);
+    if (rc) return rc;
+    if (nb > have) return fail(DFSPH_ECAPACITY, "write beyond the buffer");
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(p, src, size_t(nb), hipMemcpyHostToDevice));
+    return DFSPH_OK;
+}
+
+int32_t dfsph_zero_stats(void* h, void* stream) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    Ctx& c = as_handle(h)->c;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    for (int* p : {c.st_overflow, c.st_projected, c.st_swept, c.st_nonfinite}) k_fill_i32<<<blocks_for(c.W), 256, 0, st>>>(p, 0, c.W);
+    k_fill_i32<<<blocks_for(6 * int64_t(std::max(c.S, 1))), 256, 0, st>>>(reinterpret_cast<int*>(c.meter), 0,
+                                                                          6 * int64_t(std::max(c.S, 1)));
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+int32_t dfsph_report(void* h, DfsphReport* out, void* stream) {
+    if (!h || !out) return fail(DFSPH_EINVAL, "null argument");
+    Ctx& c = as_handle(h)->c;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    size_t W = size_t(c.W);
+    struct Item {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    } items[] = {
+        {out->density_err, c.st_den_err, 4 * W},   {out->divergence_err, c.st_div_err, 4 * W},
+        {out->iters_density, c.it_den, 4 * W},     {out->iters_divergence, c.it_div, 4 * W},
+        {out->neighbor_overflow, c.st_overflow, 4 * W}, {out->max_speed, c.st_max_speed, 4 * W},
+        {out->density_err_max, c.st_den_err_max, 4 * W}, {out->density_tol, c.st_den_tol, 4 * W},
+        {out->projected, c.st_projected, 4 * W},   {out->swept, c.st_swept, 4 * W},
+        {out->nonfinite, c.st_nonfinite, 4 * W},   {out->n_active, c.n_active, 8 * W},
+    };
+    for (auto& it : items)
+        if (it.dst) HIPCHK(hipMemcpyAsync(it.dst, it.src, it.bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return DFSPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+"""Registers SolverDFSPH as the ``sph`` solver capability (alloy/solvers/sph/plugin.py:36-145).
+
+The factory takes ``config`` (a DFSPHConfig) or config fields as kwargs plus ``capacity_per_world``; unknown kwargs
+raise TypeError. The health adapter reports, per world, the last step's total pressure iterations and its mean
+density error; the state adapter exposes ``alloy_state_arrays`` with its item -> world map.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+from collections.abc import Mapping
+from typing import Any
+
+import numpy as np
+
+from ..core.registry import (HealthSample, SolverCapability, register_health_adapter, register_solver,
+                             register_state_adapter)
+from .config import DFSPHConfig, register_custom_attributes
+
+_CONFIG_FIELDS = frozenset(f.name for f in dataclasses.fields(DFSPHConfig))
+
+
+def _factory(view: Any, kwargs: Mapping[str, Any]):
+    """Entry factory (plugin.py:64-93): ``config`` or Config fields, plus capacity_per_world."""
+    from .solver import SolverDFSPH
+
+    kw = dict(kwargs)
+    cfg = kw.pop("config", None)
+    cap = kw.pop("capacity_per_world", None)
+    fields = {k: kw.pop(k) for k in list(kw) if k in _CONFIG_FIELDS}
+    if kw:
+        raise TypeError(f"sph factory: unknown kwargs {sorted(kw)}")
+    if cfg is None:
+        cfg = DFSPHConfig(**fields)
+    elif fields:
+        cfg = dataclasses.replace(cfg, **fields)
+    return SolverDFSPH(view, cfg, capacity_per_world=cap)
+
+
+class _SPHEmitter:
+    """Emitter protocol adapter (plugin.py:96-100): scene objects become liquid blocks."""
+
+    def emit(self, builder: Any, obj: Any, decision: Any) -> None:
+        from .build import add_fluid_block
+
+        add_fluid_block(builder, decision.config, obj.lo, obj.counts)
+
+
+def _health_adapter(solver, sample: HealthSample) -> None:
+    """iters = iters_den + iters_div, residual = den_err per world (plugin.py:103-117)."""
+    r = solver.report()
+    sample.iters[:] = np.asarray(r.iters_density) + np.asarray(r.iters_divergence)
+    sample.residual[:] = np.asarray(r.density_err)
+
+
+CAPABILITY = SolverCapability(
+    name="sph",
+    factory=_factory,
+    routes=frozenset({"liquid_sph"}),
+    owns=frozenset({"particle", "body"}),
+    coupling=True,
+    uniform_worlds=False,
+    per_contact_friction=False,
+    determinism="native",
+    friction_mix="none",
+    restitution_path=False,
+    emitter=_SPHEmitter(),
+    register_attributes=register_custom_attributes,
+)
+
+
+def register() -> None:
+    from .solver import SolverDFSPH
+
+    register_solver(CAPABILITY)
+    register_health_adapter(SolverDFSPH, _health_adapter)
+    register_state_adapter(SolverDFSPH, lambda s: s.alloy_state_arrays(None), world_map=lambda s: s.alloy_state_worlds())
+
+
+register()

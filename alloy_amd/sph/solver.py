@@ -1,0 +1,376 @@
+"""SolverDFSPH: the DFSPH liquid plug-in, host side, driving the gfx950 kernels of libdfsph_hip.so.
+
+Same construction, methods, errors and step order as alloy/solvers/sph/solver.py:57-484: ``step`` runs neighbours
+-> boundary slots -> density and factor -> divergence solve -> gravity, external forces and XSPH -> density solve ->
+shape meters -> advection and wall projection -> speed statistics, all on the current torch stream. Every work
+buffer is allocated once at construction; ``step`` allocates nothing.
+
+Subsystems of the reference that this build does not implement (implicit viscosity, surface tension, ambient
+pressure, soft walls, Verlet skin, bodies attached to shapes, emitters/sinks) are refused at construction with a
+ValueError naming the config field, rather than silently ignored.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import logging
+from collections.abc import Mapping
+from typing import ClassVar
+
+import numpy as np
+import torch
+
+from . import _abi
+from .config import (PHASE_LIQUID, DFSPHConfig, register_custom_attributes, resolve_viscosity, snap_down,
+                     stable_dt)
+from .model import Model, ParticleFlags, State
+from .types import SPHReport
+
+log = logging.getLogger("alloy.sph")
+
+CapacityError = _abi.CapacityError
+
+
+def hydrostatic_guess(model: Model, state: State, spacing: float) -> np.ndarray:
+    """k_i = |g| (z_top - z_i + r) per (world, horizontal 2r bin) column of active liquid, along gravity
+    (prime.py:20-50): the initial pressure ``prime`` refines [m^2/s^2]."""
+    n = model.particle_count
+    out = np.zeros(max(n, 1), dtype=np.float32)
+    if n == 0:
+        return out
+    q = state.particle_q.detach().cpu().numpy().astype(np.float64)
+    act = model.sph.phase.cpu().numpy() >= 1
+    act &= (model.particle_flags.cpu().numpy() & int(ParticleFlags.ACTIVE)) != 0
+    world = model.particle_world.cpu().numpy()
+    grav = model.gravity.cpu().numpy().astype(np.float64)
+    for w in np.unique(world[act]):
+        g = grav[w]
+        gn = float(np.linalg.norm(g))
+        if gn == 0.0:
+            continue
+        up = -g / gn
+        sel = np.nonzero(act & (world == w))[0]
+        z = q[sel] @ up
+        e1 = np.cross(up, [1.0, 0.0, 0.0] if abs(up[0]) < 0.9 else [0.0, 1.0, 0.0])
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(up, e1)
+        key = np.stack([np.floor(q[sel] @ e1 / spacing), np.floor(q[sel] @ e2 / spacing)], 1).astype(np.int64)
+        _, inv = np.unique(key, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        top = np.full(inv.max() + 1, -np.inf)
+        np.maximum.at(top, inv, z)
+        out[sel] = (gn * np.maximum(top[inv] - z + 0.5 * spacing, 0.0)).astype(np.float32)
+    return out
+
+
+def _stream_ptr(device: torch.device):
+    if device.type != "cuda":
+        return None
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _dptr(t: torch.Tensor | None):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class SolverDFSPH:
+    """Divergence-free SPH (Bender & Koschier 2017) with SDF and particle walls, on MI355X."""
+
+    plugin_name: ClassVar[str] = "sph"
+    routes: ClassVar[frozenset[str]] = frozenset({"liquid_sph"})
+    determinism: ClassVar[str] = "native"
+    alloy_clock_safety: ClassVar[float] = 1.0
+    Config = DFSPHConfig
+    register_custom_attributes = staticmethod(register_custom_attributes)
+
+    def __init__(self, model: Model, config: DFSPHConfig | None = None, *, capacity_per_world: int | None = None):
+        self.model = model
+        self.config = cfg = config or DFSPHConfig()
+        cfg.validate()
+        self._refuse_unbuilt(cfg)
+        if model.device.type != "cuda":
+            raise RuntimeError("SolverDFSPH runs on a HIP device (the model is on "
+                               f"{model.device}); there is no CPU path")
+        self.device = model.device
+        self._check_layout(model)
+        self._check_capacity(capacity_per_world)
+        self.viscosity = resolve_viscosity(cfg, float(model.sph.viscosity.max().item()) if model.particle_count else 0.0)
+        if self.viscosity == "weiler2018":
+            raise ValueError("SolverDFSPH: viscosity 'weiler2018' (implicit viscosity) is not built here; "
+                             "use viscosity='xsph' or 'none'")
+        phase = model.sph.phase.cpu().numpy()
+        mass = model.particle_mass.cpu().numpy()[phase >= 0]
+        rho0 = model.sph.rest_density.cpu().numpy()[phase >= 0]
+        spread = lambda a: float(np.ptp(a)) / max(float(np.max(np.abs(a))), 1e-30) if len(a) else 0.0  # noqa: E731
+        self.pressure_solver = cfg.pressure_solver
+        if self.pressure_solver == "cg" and (spread(mass.astype(np.float64)) > 1e-6 or
+                                             spread(rho0.astype(np.float64)) > 1e-6):
+            log.warning("SolverDFSPH: pressure_solver 'cg' needs one particle mass and rest density; using 'jacobi'")
+            self.pressure_solver = "jacobi"
+        self.lib = _abi.load_library()
+        self.tables = _abi.layout_from_model(model, cfg)
+        self._params = _abi.params_from_config(cfg, self.tables.thinnest)
+        self._params.pressure_solver = 0 if self.pressure_solver == "cg" else 1
+        if self.viscosity == "none":
+            self._params.xsph_c = 0.0
+        self._wall_volume = torch.tensor(self.tables.wall_volume, device=self.device)
+        self._gravity = model.gravity.reshape(-1).contiguous()
+        m = _abi.DfsphModel()
+        m.mass, m.rest_density = _dptr(model.particle_mass), _dptr(model.sph.rest_density)
+        m.flags, m.phase, m.world = _dptr(model.particle_flags), _dptr(model.sph.phase), _dptr(model.particle_world)
+        m.gravity, m.wall_volume = _dptr(self._gravity), _dptr(self._wall_volume)
+        self._model_struct = m
+        layout = self.tables.struct(model.particle_count)
+        h = ctypes.c_void_p()
+        _abi.check(self.lib.dfsph_create(ctypes.byref(self._params), ctypes.byref(layout), ctypes.byref(m),
+                                         ctypes.byref(h)), self.lib, "SolverDFSPH")
+        self._h = h
+        self._flags0 = model.particle_flags.clone()
+        self._phase0 = model.sph.phase.clone()
+        W = model.world_count
+        self._mass_in = np.zeros(W)
+        self._mass_out = np.zeros(W)
+        self._escaped = np.zeros(W, np.int32)
+        log.info("SolverDFSPH: %d particles in %d worlds, r=%g m, h=%g m, %d shapes, %d wall particles, solver=%s",
+                 model.particle_count, W, cfg.particle_radius, cfg.h, model.shape_count, self.tables.wall_count,
+                 self.pressure_solver)
+
+    # ------------------------------------------------------------------------------------------------- checks
+    @staticmethod
+    def _refuse_unbuilt(cfg: DFSPHConfig) -> None:
+        unbuilt = [
+            (cfg.surface_tension != "none", "surface_tension"),
+            (cfg.ambient_pressure is not None, "ambient_pressure"),
+            (cfg.reneighbor_skin is not None, "reneighbor_skin"),
+            (cfg.boundary_viscosity is not None and cfg.boundary_viscosity > 0.0, "boundary_viscosity"),
+            (cfg.viscosity == "weiler2018", "viscosity='weiler2018'"),
+        ]
+        for bad, name in unbuilt:
+            if bad:
+                raise ValueError(f"SolverDFSPH: {name} is not built in this DFSPH implementation")
+
+    @staticmethod
+    def _check_layout(model: Model) -> None:
+        """layout.py:47-72: SPH particles in a world, worlds contiguous."""
+        if model.particle_count == 0:
+            return
+        world = model.particle_world.cpu().numpy()
+        phase = model.sph.phase.cpu().numpy()
+        if np.any(world[phase >= 0] < 0):
+            raise ValueError("SPH particles must belong to a world (particle_world >= 0)")
+        if np.any(np.diff(world) < 0):
+            raise ValueError("particles must be contiguous per world (Model.particle_world sorted)")
+
+    def _check_capacity(self, capacity_per_world: int | None) -> None:
+        if capacity_per_world is None:
+            return
+        ws = self.model.world_start_np
+        phase = self.model.sph.phase.cpu().numpy()
+        for w in range(self.model.world_count):
+            have = int(np.count_nonzero(phase[ws[w]:ws[w + 1]] >= 0))
+            if have < capacity_per_world:
+                raise CapacityError(f"SolverDFSPH: world {w} has {have} SPH particle slots, "
+                                    f"capacity_per_world={capacity_per_world}")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self.lib.dfsph_destroy(h)
+            self._h = None
+
+    # ------------------------------------------------------------------------------------------------- step
+    def _call(self, fn, *args, what: str = "SolverDFSPH") -> None:
+        _abi.check(fn(self._h, *args), self.lib, what)
+
+    @staticmethod
+    def _checked_state(state: State, n: int) -> None:
+        for name in ("particle_q", "particle_qd"):
+            t = getattr(state, name)
+            if t.dtype != torch.float32 or not t.is_contiguous() or tuple(t.shape) != (n, 3):
+                raise ValueError(f"SolverDFSPH: state.{name} must be a contiguous float32 [{n}, 3] tensor")
+
+    def step(self, state_in: State, state_out: State, control=None, contacts=None, dt: float = 0.0, *,
+             observables=None) -> None:
+        """One DFSPH step from ``state_in`` into ``state_out`` (the same object allowed), solver.py:218-268."""
+        del control, contacts, observables
+        if not dt > 0.0:
+            raise ValueError(f"SolverDFSPH.step: dt must be > 0, got {dt}")
+        n = self.model.particle_count
+        if n == 0:
+            return
+        self._checked_state(state_in, n)
+        self._checked_state(state_out, n)
+        if state_out is not state_in:
+            state_out.particle_q.copy_(state_in.particle_q)
+            state_out.particle_qd.copy_(state_in.particle_qd)
+        f = state_in.particle_f
+        self._call(self.lib.dfsph_step, _dptr(state_out.particle_q), _dptr(state_out.particle_qd), _dptr(f),
+                   float(dt), _stream_ptr(self.device), what="SolverDFSPH.step")
+
+    def set_check_interval(self, every: int) -> None:
+        """Iterations between host convergence checks of the pressure solves (autotuned; results unchanged)."""
+        self._call(self.lib.dfsph_set_check_interval, int(every))
+
+    def set_warm_start(self, kappa_div: torch.Tensor | None, kappa_den: torch.Tensor | None) -> None:
+        """Learned warm start: initial kappa [n] of each solve (None: the reference rule). Tensors must stay alive
+        while set."""
+        self._warm = (kappa_div, kappa_den)
+        self._call(self.lib.dfsph_set_warm_start_kappa, _dptr(kappa_div), _dptr(kappa_den))
+
+    # ------------------------------------------------------------------------------------------------- reports
+    def buffer(self, which: int, dtype, shape) -> np.ndarray:
+        """Host copy of a solver buffer (tests, diagnostics; syncs)."""
+        out = np.empty(shape, dtype=dtype)
+        self._call(self.lib.dfsph_read_buffer, which, out.ctypes.data_as(ctypes.c_void_p), out.nbytes)
+        return out
+
+    def _raw_report(self) -> dict[str, np.ndarray]:
+        W = self.model.world_count
+        arrs = {k: np.zeros(W, dt) for k, dt in (
+            ("density_err", np.float32), ("divergence_err", np.float32), ("iters_density", np.int32),
+            ("iters_divergence", np.int32), ("neighbor_overflow", np.int32), ("max_speed", np.float32),
+            ("density_err_max", np.float32), ("density_tol", np.float32), ("projected", np.int32),
+            ("swept", np.int32), ("nonfinite", np.int32), ("n_active", np.float64))}
+        rep = _abi.DfsphReport()
+        ct = {np.float32: ctypes.c_float, np.int32: ctypes.c_int32, np.float64: ctypes.c_double}
+        for k, a in arrs.items():
+            setattr(rep, k, a.ctypes.data_as(ctypes.POINTER(ct[a.dtype.type])))
+        self._call(self.lib.dfsph_report, ctypes.byref(rep), _stream_ptr(self.device))
+        return arrs
+
+    def report(self) -> SPHReport:
+        """Per-world report of the last step (runtime.py:136-147); the step's sync point."""
+        r = self._raw_report()
+        mi = self.config.max_iterations
+        mass = self._active_mass()
+        return SPHReport(
+            density_err=r["density_err"].astype(np.float64), divergence_err=r["divergence_err"].astype(np.float64),
+            iters_density=r["iters_density"], iters_divergence=r["iters_divergence"], mass_active=mass,
+            mass_in=self._mass_in.copy(), mass_out=self._mass_out.copy(), escaped=self._escaped.copy(),
+            neighbor_overflow=r["neighbor_overflow"], max_speed=r["max_speed"].astype(np.float64),
+            unconverged=(r["iters_density"] >= mi) | (r["iters_divergence"] >= mi),
+            density_tol=r["density_tol"].astype(np.float64))
+
+    def diagnostics(self) -> dict[str, np.ndarray]:
+        """Extra per-world counters (runtime.py:155-163)."""
+        r = self._raw_report()
+        return {"density_err_max": r["density_err_max"].astype(np.float64), "projected": r["projected"],
+                "nonfinite": r["nonfinite"], "swept": r["swept"], "n_active": r["n_active"]}
+
+    def _active_mass(self) -> np.ndarray:
+        m = self.model
+        act = ((m.particle_flags & int(ParticleFlags.ACTIVE)) != 0) & (m.sph.phase >= PHASE_LIQUID)
+        out = torch.zeros(m.world_count, dtype=torch.float64, device=self.device)
+        out.index_add_(0, m.particle_world[act].long(), m.particle_mass[act].double())
+        return out.cpu().numpy()
+
+    def mass_ledger(self) -> dict[str, np.ndarray]:
+        """Per-world mass ledger [kg] (runtime.py:129-134): active + out == initial + in."""
+        return {"initial": self._active_mass(), "in": self._mass_in.copy(), "out": self._mass_out.copy(),
+                "active": self._active_mass()}
+
+    def shape_impulse(self) -> np.ndarray:
+        """Cumulative fluid impulse on each shape since the last reset [N s], (S, 3) float64 (solver.py:399-401)."""
+        S = max(self.model.shape_count, 1)
+        raw = self.buffer(_abi.BUF_SHAPE_METER, np.int64, (S, 3))
+        return raw.astype(np.float64) / float(2**50)
+
+    def reset_shape_meters(self) -> None:
+        self._call(self.lib.dfsph_zero_stats, _stream_ptr(self.device))
+
+    # ------------------------------------------------------------------------------------------------- helpers
+    def prime(self, state: State, dt: float, tol: float = 1.0e-6, max_iterations: int = 2000) -> int:
+        """Give a fluid at rest its hydrostatic pressure before the first step (prime.py:53-115); returns the
+        iterations used."""
+        if self.model.particle_count == 0:
+            return 0
+        guess = torch.tensor(hydrostatic_guess(self.model, state, self.config.spacing), device=self.device)
+        it = ctypes.c_int32(0)
+        self._call(self.lib.dfsph_prime, _dptr(state.particle_q), _dptr(state.particle_qd), _dptr(guess), float(dt),
+                   float(tol), int(max_iterations), ctypes.byref(it), _stream_ptr(self.device), what="prime")
+        return int(it.value)
+
+    def density_ratio(self, state: State) -> tuple[np.ndarray, np.ndarray]:
+        """Density ratio of every particle and the active-liquid mask, from fresh lists and slots (solver.py:386-393)."""
+        n = self.model.particle_count
+        if n == 0:
+            return np.zeros(0), np.zeros(0, dtype=bool)
+        st = _stream_ptr(self.device)
+        self._call(self.lib.dfsph_build_neighbors, _dptr(state.particle_q), st)
+        self._call(self.lib.dfsph_boundary_slots, _dptr(state.particle_q), _dptr(state.particle_qd), st)
+        self._call(self.lib.dfsph_density_alpha, _dptr(state.particle_q), st)
+        ratio = self.buffer(_abi.BUF_RATIO, np.float32, (n,))
+        active = self.buffer(_abi.BUF_ACTIVE, np.float32, (n,))
+        return ratio.copy(), active > 0.0
+
+    def reset(self, state: State, world_mask: np.ndarray | None = None, flags: int | None = None) -> None:
+        """Masked worlds back to their build-time flags and phase, warm starts cleared (solver.py:353-362)."""
+        del state, flags
+        m = self.model
+        W = m.world_count
+        mask = np.ones(W, bool) if world_mask is None else np.asarray(world_mask, bool)[:W]
+        sel = torch.tensor(mask, device=self.device)[m.particle_world.long().clamp(min=0)] & (m.particle_world >= 0)
+        m.particle_flags[sel] = self._flags0[sel]
+        m.sph.phase[sel] = self._phase0[sel]
+        n = m.particle_count
+        for which in (_abi.BUF_KV, _abi.BUF_KD):
+            k = self.buffer(which, np.float32, (max(n, 1),))
+            k[:n][sel.cpu().numpy()] = 0.0
+            self._call(self.lib.dfsph_write_buffer, which, k.ctypes.data_as(ctypes.c_void_p), k.nbytes)
+        self._mass_in[mask] = 0.0
+        self._mass_out[mask] = 0.0
+        self._escaped[mask] = 0
+
+    def choose_dt(self, control_dt: float) -> float:
+        """CFL step from the last step's max speed, snapped onto the D4 grid (solver.py:424-426)."""
+        v = float(np.max(self._raw_report()["max_speed"])) if self.model.world_count else 0.0
+        return snap_down(stable_dt(self.config, v), control_dt)
+
+    def alloy_stable_dt(self, state: State, contacts, out: torch.Tensor) -> None:
+        """Per-world CFL step from the state's liquid speeds (runtime.py _stable_dt_kernel)."""
+        del contacts
+        m, cfg = self.model, self.config
+        fl = ((m.particle_flags & int(ParticleFlags.ACTIVE)) != 0) & (m.sph.phase >= PHASE_LIQUID)
+        sp = torch.linalg.vector_norm(state.particle_qd, dim=1)
+        sp = torch.where(torch.isfinite(sp), sp, torch.full_like(sp, 1.0e30))
+        vmax = torch.zeros(m.world_count, device=self.device).scatter_reduce(
+            0, m.particle_world.long().clamp(min=0), torch.where(fl, sp, torch.zeros_like(sp)), "amax")
+        dt = torch.clamp(cfg.cfl * cfg.spacing / vmax.clamp(min=1e-30), cfg.dt_min, cfg.dt_max)
+        out.copy_(torch.where(vmax > 0, dt, torch.full_like(dt, cfg.dt_max)).to(out.dtype))
+
+    def alloy_energy(self, state: State, z_ref: torch.Tensor, out: torch.Tensor) -> None:
+        """KE + gravity PE about (0, 0, z_ref[w]) per world [J] (runtime.py _energy_kernel)."""
+        m = self.model
+        fl = ((m.particle_flags & int(ParticleFlags.ACTIVE)) != 0) & (m.sph.phase >= PHASE_LIQUID)
+        w = m.particle_world.long().clamp(min=0)
+        v = state.particle_qd.double()
+        x = state.particle_q.double().clone()
+        x[:, 2] -= z_ref.double()[w]
+        g = m.gravity.double()[w]
+        e = m.particle_mass.double() * (0.5 * (v * v).sum(1) - (g * x).sum(1))
+        e = torch.where(fl, e, torch.zeros_like(e))
+        out.copy_(torch.zeros(m.world_count, dtype=torch.float64, device=self.device).index_add_(0, w, e)
+                  .to(out.dtype))
+
+    def alloy_capacity_report(self) -> Mapping[str, int]:
+        return {"neighbor_overflow": int(self._raw_report()["neighbor_overflow"].sum()), "emitter_overflow": 0}
+
+    def alloy_state_arrays(self, state: State | None = None) -> dict[str, np.ndarray]:
+        """Arrays a snapshot must restore bitwise for this solver to replay (coupling_hooks.py:115-126)."""
+        n = max(self.model.particle_count, 1)
+        return {"sph.kappa_v": self.buffer(_abi.BUF_KV, np.float32, (n,)),
+                "sph.kappa_d": self.buffer(_abi.BUF_KD, np.float32, (n,)),
+                "sph.phase": self.model.sph.phase.cpu().numpy().copy(),
+                "sph.flags": self.model.particle_flags.cpu().numpy().copy()}
+
+    def alloy_restore_state_arrays(self, arrays: Mapping[str, np.ndarray]) -> None:
+        for key, which in (("sph.kappa_v", _abi.BUF_KV), ("sph.kappa_d", _abi.BUF_KD)):
+            a = np.ascontiguousarray(arrays[key], dtype=np.float32)
+            self._call(self.lib.dfsph_write_buffer, which, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+        self.model.sph.phase.copy_(torch.tensor(arrays["sph.phase"], device=self.device))
+        self.model.particle_flags.copy_(torch.tensor(arrays["sph.flags"], device=self.device))
+
+    def alloy_state_worlds(self) -> dict[str, np.ndarray]:
+        """Item -> world maps of the state arrays (coupling_hooks.py:128-138)."""
+        w = self.model.particle_world.cpu().numpy()
+        return {"sph.kappa_v": w, "sph.kappa_d": w, "sph.phase": w, "sph.flags": w}

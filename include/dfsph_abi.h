@@ -189,6 +189,8 @@ int32_t dfsph_density_alpha(void* handle, const float* q, void* stream);        
 int32_t dfsph_solve(void* handle, int32_t mode, float* q, float* qd, double dt, int32_t fixed_iters, void* stream);
 int32_t dfsph_nonpressure(void* handle, const float* q, float* qd, const float* f, double dt, void* stream); /* integrate.py:150-178 */
 int32_t dfsph_advect_project(void* handle, float* q, float* qd, double dt, void* stream);  /* integrate.py:181-309 + speed */
+/* Mean-velocity sums of the resting-jitter rule at the step's start state (solve_driver.py:100-106). */
+int32_t dfsph_flow_speed(void* handle, const float* qd, void* stream);
 /* Hydrostatic prime (prime.py:53-115): guess [n] k = p/rho0 (host-computed, device pointer). Returns
  * iterations used through *iters. */
 int32_t dfsph_prime(void* handle, float* q, float* qd, const float* guess, double dt, double tol,
@@ -198,6 +200,12 @@ int32_t dfsph_prime(void* handle, float* q, float* qd, const float* guess, doubl
  * of both solves in place of solve_init's warm start (pressure.py:254, 261). weights: device bf16 buffer in
  * the layout of dfsph_mlp_layout; NULL switches the learned start off. */
 int32_t dfsph_set_warm_start_model(void* handle, const void* weights, void* stream);
+/* Learned warm start given as initial kappa fields [n] (device; NULL = the reference warm-start rule) of the
+ * divergence and density solves, read by solve_init (pressure.py:254, 261) in place of warm x stored kappa; the
+ * solve still iterates to the same tolerance. */
+int32_t dfsph_set_warm_start_kappa(void* handle, const float* kappa_div, const float* kappa_den);
+/* Host convergence check every `every` iterations (1..64); converged worlds' blocks exit at once in between. */
+int32_t dfsph_set_check_interval(void* handle, int32_t every);
 
 /* Buffers */
 int32_t dfsph_get_buffer(void* handle, int32_t which, void** ptr, int64_t* n_bytes);
