@@ -93,7 +93,9 @@ struct Ctx {
     float* s_nbl;
     float* s_imp;
     // per particle
-    float *ratio, *alpha, *neff, *active, *reg, *src, *kappa, *acc, *err, *lower, *kv, *kd, *vtmp;
+    float *ratio, *alpha, *neff, *active, *reg, *src, *kappa, *err, *lower, *kv, *kd, *vtmp;
+    float4* q4;    // (x, y, z, mass) per particle, refreshed from q at every neighbour build: one 16 B gather
+    float4* acc4;  // pressure acceleration (x, y, z, 0)
     float *pd, *z, *wv, *ap, *corr;
     int* act;
     // per world
@@ -115,6 +117,8 @@ __device__ __forceinline__ bool is_fluid(const Ctx& c, int i) {
     return (c.flags[i] & ACTIVE) != 0 && int(c.phase[i]) >= 1;
 }
 __device__ __forceinline__ f3 Q(const Ctx& c, int i) { return ld3(c.q, i); }
+__device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+__device__ __forceinline__ f3 A4(const Ctx& c, int i) { return xyz(c.acc4[i]); }
 __device__ __forceinline__ f3 V(const Ctx& c, int i) { return ld3(c.qd, i); }
 
 __device__ __forceinline__ int nbr_j(const Ctx& c, int base, int i, int k) {
@@ -122,6 +126,38 @@ __device__ __forceinline__ int nbr_j(const Ctx& c, int base, int i, int k) {
     if (c.wide) j += int(c.nbr_hi[int64_t(k) * c.n + i]) << 16;
     return j;
 }
+
+template <bool WIDE>
+__device__ __forceinline__ int nbr_at(const Ctx& c, int base, int i, int k) {
+    int j = base + int(c.nbr[int64_t(k) * c.n + i]);
+    if (WIDE) j += int(c.nbr_hi[int64_t(k) * c.n + i]) << 16;
+    return j;
+}
+
+// Neighbour loop with staged loads: NB_U row entries, then the NB_U neighbour records (load(j)), then the
+// arithmetic body(j, record) in ascending k, exactly the sequential order. The loads of a batch are in flight
+// together instead of one dependent index -> record round trip per neighbour.
+constexpr int NB_U = 8;
+template <bool WIDE, class T, class LoadF, class BodyF>
+__device__ __forceinline__ void for_nbrs(const Ctx& c, int i, int cnt, int base, LoadF load, BodyF body) {
+    for (int k0 = 0; k0 < cnt; k0 += NB_U) {
+        int j[NB_U];
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u) j[u] = nbr_at<WIDE>(c, base, i, min(k0 + u, cnt - 1));
+        T d[NB_U];
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u) d[u] = load(j[u]);
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u)
+            if (k0 + u < cnt) body(j[u], d[u]);
+    }
+}
+
+#define LAUNCH_W(kernel, grid, block, ...)                                        \
+    do {                                                                          \
+        if (c.wide) kernel<true><<<(grid), (block), 0, st>>>(__VA_ARGS__);        \
+        else kernel<false><<<(grid), (block), 0, st>>>(__VA_ARGS__);             \
+    } while (0)
 
 // ----------------------------------------------------------------------------------------------- reductions
 // Fixed 256-wide tree: a[t] += a[t + s] for s = 128, 64 (LDS) then 32 .. 1 (wavefront shuffles; the same pairs).
@@ -306,11 +342,13 @@ __global__ void __launch_bounds__(256) k_groups_hist(Ctx c) {
     if ((fl & ACTIVE) != 0 && int(c.phase[i]) >= 1) g = c.world[i];
     else if (c.NW > 0 && c.wvol[i] > 0.0f && (fl & ACTIVE) != 0 && c.world[i] >= 0) g = -2 - c.world[i];
     c.group[i] = g;
+    f3 xi = Q(c, i);
+    c.q4[i] = make_float4(xi.x, xi.y, xi.z, c.mass[i]);
     int k = -1;
     if (g != -1) {
         int gi = g >= 0 ? g : c.W + (-2 - g);
         int cx, cy, cz;
-        k = cell_of(c, gi, Q(c, i), cx, cy, cz);
+        k = cell_of(c, gi, xi, cx, cy, cz);
         atomicAdd(c.cell_count + k, 1);
     }
     c.key[i] = k;
@@ -421,7 +459,7 @@ __global__ void __launch_bounds__(BUILD_BLOCK) k_build_lists(Ctx c) {
     int* L = row + threadIdx.x;
     const int cap = c.cap;
     int base = c.ws[g];
-    f3 xi = Q(c, i);
+    f3 xi = xyz(c.q4[i]);
     float h2 = c.h * c.h;
     int cx, cy, cz;
     cell_of(c, g, xi, cx, cy, cz);
@@ -440,7 +478,7 @@ __global__ void __launch_bounds__(BUILD_BLOCK) k_build_lists(Ctx c) {
             for (int e = e0; e < e1; ++e) {
                 int j = c.items[e];
                 if (j == i) continue;
-                f3 dd = Q(c, j) - xi;
+                f3 dd = xyz(c.q4[j]) - xi;
                 if (dot(dd, dd) < h2) {
                     total += 1;
                     int jl = j - base;
@@ -487,7 +525,7 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv)
                 int cc = c.cell_base[g] + (Z * d[1] + Y) * d[0] + X;
                 for (int e = c.cell_start[cc]; e < c.cell_start[cc + 1]; ++e) {
                     int j = c.items[e];
-                    f3 dd = x - Q(c, j);
+                    f3 dd = x - xyz(c.q4[j]);
                     float wj = c.wvol[j] * kernel_w(length(dd), c.h, c.kind);
                     if (wj > 0.0f) {
                         col += wj;
@@ -617,7 +655,7 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
         return;
     }
     int w = c.world[i];
-    f3 x = Q(c, i);
+    f3 x = xyz(c.q4[i]);
     int nn = 0;
     for (int e = c.sh_start[w]; e < c.sh_start[w + 1]; ++e) {
         int s = c.sh_ids[e];
@@ -735,6 +773,7 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
     bool valid = i < c.ch_end[b_];
 
 // pressure.py:144-193, plus the chunk partial of `active` (solver.py:241)
+template <bool WIDE>
 __global__ void __launch_bounds__(256) k_density_alpha(Ctx c) {
     CHUNK_PROLOGUE
     double act_v = 0.0;
@@ -746,23 +785,25 @@ __global__ void __launch_bounds__(256) k_density_alpha(Ctx c) {
             c.active[i] = 0.0f;
             c.reg[i] = 0.0f;
         } else {
-            f3 xi = Q(c, i);
-            float mi = c.mass[i];
+            float4 qi = c.q4[i];
+            f3 xi = xyz(qi);
+            float mi = qi.w;
             float inv_r0 = 1.0f / c.rho0[i];
             float rho = mi * kernel_w(0.0f, c.h, c.kind);
             f3 G{0.0f, 0.0f, 0.0f};
             float sq = 0.0f;
             int cnt = c.count[i];
             int base = c.ws[c.world[i]];
-            for (int k = 0; k < cnt; ++k) {
-                int j = nbr_j(c, base, i, k);
-                f3 rv = xi - Q(c, j);
-                float mj = c.mass[j];
-                rho += mj * kernel_w(length(rv), c.h, c.kind);
-                f3 g = kernel_grad(rv, c.h, c.kind);
-                G = G + (mj * inv_r0) * g;
-                sq += (mi * mj * inv_r0 * inv_r0) * dot(g, g);
-            }
+            for_nbrs<WIDE, float4>(
+                c, i, cnt, base, [&](int j) { return c.q4[j]; },
+                [&](int, float4 qj) {
+                    f3 rv = xi - xyz(qj);
+                    float mj = qj.w;
+                    rho += mj * kernel_w(length(rv), c.h, c.kind);
+                    f3 g = kernel_grad(rv, c.h, c.kind);
+                    G = G + (mj * inv_r0) * g;
+                    sq += (mi * mj * inv_r0 * inv_r0) * dot(g, g);
+                });
             float psi = 0.0f, lam = 0.0f;
             int64_t b4 = 4 * int64_t(i);
             for (int b = 0; b < 4; ++b) {
@@ -788,17 +829,23 @@ __global__ void __launch_bounds__(256) k_density_alpha(Ctx c) {
 }
 
 // ----------------------------------------------------------------------------------------------- solves
+struct QV {
+    float4 q;
+    f3 v;
+};
+template <bool WIDE>
 __device__ float density_rate(const Ctx& c, int i) {
-    f3 xi = Q(c, i), vi = V(c, i);
+    f3 xi = xyz(c.q4[i]), vi = V(c, i);
     float inv_r0 = 1.0f / c.rho0[i];
     float rate = 0.0f;
     int cnt = c.count[i];
     int base = c.ws[c.world[i]];
-    for (int k = 0; k < cnt; ++k) {
-        int j = nbr_j(c, base, i, k);
-        f3 g = kernel_grad(xi - Q(c, j), c.h, c.kind);
-        rate += (c.mass[j] * inv_r0) * dot(vi - V(c, j), g);
-    }
+    for_nbrs<WIDE, QV>(
+        c, i, cnt, base, [&](int j) { return QV{c.q4[j], V(c, j)}; },
+        [&](int, QV d) {
+            f3 g = kernel_grad(xi - xyz(d.q), c.h, c.kind);
+            rate += (d.q.w * inv_r0) * dot(vi - d.v, g);
+        });
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
         if (c.s_shape[b4 + b] >= 0) rate += dot(vi - ld3(c.s_vb, b4 + b), ld3(c.s_gpsi, b4 + b));
@@ -806,6 +853,7 @@ __device__ float density_rate(const Ctx& c, int i) {
 }
 
 // pressure.py:214-261; block (0, 0) clears the unconverged-world count for begin_solve
+template <bool WIDE>
 __global__ void __launch_bounds__(256) k_solve_init(Ctx c, int mode, float dt, float warm, float warm_den,
                                                     float relief, const float* store, const float* learned) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *c.n_unconv = 0;
@@ -817,7 +865,7 @@ __global__ void __launch_bounds__(256) k_solve_init(Ctx c, int mode, float dt, f
         c.lower[i] = 0.0f;
         return;
     }
-    float rate = density_rate(c, i);
+    float rate = density_rate<WIDE>(c, i);
     if (mode == 0) {
         float kdv = 0.0f;
         if (c.two_sided) kdv = fmx(warm_den * fmn(c.kd[i], 2.5e-4f) / (dt * dt), 0.0f);
@@ -858,17 +906,23 @@ __global__ void k_begin_solve(Ctx c) {
 }
 
 // pressure.py:264-277
+struct QKR {
+    float4 q;
+    float k, r;
+};
+template <bool WIDE>
 __device__ f3 pressure_accel(const Ctx& c, const float* kap, int i) {
-    f3 xi = Q(c, i);
+    f3 xi = xyz(c.q4[i]);
     float ki = kap[i] / c.rho0[i];
     f3 a{0.0f, 0.0f, 0.0f};
     int cnt = c.count[i];
     int base = c.ws[c.world[i]];
-    for (int k = 0; k < cnt; ++k) {
-        int j = nbr_j(c, base, i, k);
-        f3 g = kernel_grad(xi - Q(c, j), c.h, c.kind);
-        a = a - (c.mass[j] * (ki + kap[j] / c.rho0[j])) * g;
-    }
+    for_nbrs<WIDE, QKR>(
+        c, i, cnt, base, [&](int j) { return QKR{c.q4[j], kap[j], c.rho0[j]}; },
+        [&](int, QKR d) {
+            f3 g = kernel_grad(xi - xyz(d.q), c.h, c.kind);
+            a = a - (d.q.w * (ki + d.k / d.r)) * g;
+        });
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
         if (c.s_shape[b4 + b] >= 0) a = a - kap[i] * ld3(c.s_gpsi, b4 + b);
@@ -876,17 +930,22 @@ __device__ f3 pressure_accel(const Ctx& c, const float* kap, int i) {
 }
 
 // pressure.py:294-310
+struct QA {
+    float4 q, a;
+};
+template <bool WIDE>
 __device__ float accel_divergence(const Ctx& c, int i) {
-    f3 xi = Q(c, i), ai = ld3(c.acc, i);
+    f3 xi = xyz(c.q4[i]), ai = A4(c, i);
     float inv_r0 = 1.0f / c.rho0[i];
     float ar = 0.0f;
     int cnt = c.count[i];
     int base = c.ws[c.world[i]];
-    for (int k = 0; k < cnt; ++k) {
-        int j = nbr_j(c, base, i, k);
-        f3 g = kernel_grad(xi - Q(c, j), c.h, c.kind);
-        ar += (c.mass[j] * inv_r0) * dot(ai - ld3(c.acc, j), g);
-    }
+    for_nbrs<WIDE, QA>(
+        c, i, cnt, base, [&](int j) { return QA{c.q4[j], c.acc4[j]}; },
+        [&](int, QA d) {
+            f3 g = kernel_grad(xi - xyz(d.q), c.h, c.kind);
+            ar += (d.q.w * inv_r0) * dot(ai - xyz(d.a), g);
+        });
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
         if (c.s_shape[b4 + b] >= 0) ar += dot(ai, ld3(c.s_gpsi, b4 + b));
@@ -899,17 +958,20 @@ __device__ __forceinline__ bool block_skip(const Ctx& c, int w, int gated) {
 }
 
 // accel_kernel (pressure.py:280-291) / cg_accel_kernel (krylov.py:77-87)
+template <bool WIDE>
 __global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, int gated) {
     CHUNK_PROLOGUE
     if (block_skip(c, c.ch_world[b_], gated) || !valid) return;
     if (!is_fluid(c, i)) {
-        st3(c.acc, i, f3{0.0f, 0.0f, 0.0f});
+        c.acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
-    st3(c.acc, i, pressure_accel(c, vec, i));
+    f3 a = pressure_accel<WIDE>(c, vec, i);
+    c.acc4[i] = make_float4(a.x, a.y, a.z, 0.0f);
 }
 
 // cg_apply_kernel (krylov.py:90-109); with `reduce` the chunk partial of pair.x = vec_i (A vec)_i
+template <bool WIDE>
 __global__ void __launch_bounds__(256) k_apply(Ctx c, int mode, float dt, const float* vec, float* out, int gated,
                                                int bias, int accum, int reduce) {
     CHUNK_PROLOGUE
@@ -920,7 +982,7 @@ __global__ void __launch_bounds__(256) k_apply(Ctx c, int mode, float dt, const 
             out[i] = 0.0f;
         } else {
             float sc = mode == 0 ? dt : dt * dt;
-            float v = -sc * (accel_divergence(c, i) - c.reg[i] * vec[i]);
+            float v = -sc * (accel_divergence<WIDE>(c, i) - c.reg[i] * vec[i]);
             px = double(vec[i] * v);
             if (bias) v -= (mode == 0 ? c.src[i] : c.src[i] - 1.0f);
             if (accum) v += out[i];
@@ -1030,6 +1092,7 @@ __global__ void __launch_bounds__(256) k_cg_direction(Ctx c) {
 
 // jacobi_kernel (pressure.py:313-352) + the chunk partial of (error, -, max error); block (0,0) clears the
 // unconverged count the world epilogue recounts
+template <bool WIDE>
 __global__ void __launch_bounds__(256) k_jacobi(Ctx c, int mode, float dt) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *c.n_unconv = 0;
     CHUNK_PROLOGUE
@@ -1040,7 +1103,7 @@ __global__ void __launch_bounds__(256) k_jacobi(Ctx c, int mode, float dt) {
         if (!is_fluid(c, i)) {
             c.err[i] = 0.0f;
         } else {
-            float ar = accel_divergence(c, i) - c.reg[i] * c.kappa[i];
+            float ar = accel_divergence<WIDE>(c, i) - c.reg[i] * c.kappa[i];
             float s, scale;
             if (mode == 0) {
                 ar *= dt;
@@ -1073,6 +1136,7 @@ __global__ void k_zero_unconv(Ctx c) { *c.n_unconv = 0; }
 // contact_limit_kernel + apply_pressure_kernel + lone_friction_kernel (pressure.py:393-480) for one particle: a
 // lone particle (count 0) is nobody's neighbour, so limiting its kappa before the neighbours' pressure
 // accelerations read kappa changes nothing for them.
+template <bool WIDE>
 __global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, float* store) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.n || !is_fluid(c, i)) return;
@@ -1095,7 +1159,7 @@ __global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, f
         }
     }
     // apply_pressure_kernel
-    f3 a = pressure_accel(c, c.kappa, i);
+    f3 a = pressure_accel<WIDE>(c, c.kappa, i);
     f3 v = V(c, i) + dt * a;
     float kap = c.kappa[i];
     float mk = c.mass[i] * kap * dt;
@@ -1206,6 +1270,12 @@ __global__ void k_tol_scale(Ctx c, float dt, float speed, float ratio, float tol
 
 // ----------------------------------------------------------------------------------------------- non-pressure
 // nonpressure_kernel (integrate.py:150-178)
+struct XS {
+    float4 q;
+    f3 v;
+    float ratio, rho0;
+};
+template <bool WIDE>
 __global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.n) return;
@@ -1219,19 +1289,20 @@ __global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
     if (c.f) a = a + ld3(c.f, i) / c.mass[i];
     f3 v = vi + dt * a;
     if (c.xsph > 0.0f) {
-        f3 xi = Q(c, i);
+        f3 xi = xyz(c.q4[i]);
         f3 cr{0.0f, 0.0f, 0.0f};
         float rho_i = c.ratio[i] * c.rho0[i];
         int cnt = c.count[i];
         int base = c.ws[w];
-        for (int k = 0; k < cnt; ++k) {
-            int j = nbr_j(c, base, i, k);
-            float rho_bar = 0.5f * (rho_i + c.ratio[j] * c.rho0[j]);
-            if (rho_bar > 0.0f) {
-                float wk = kernel_w(length(xi - Q(c, j)), c.h, c.kind);
-                cr = cr + (c.mass[j] / rho_bar * wk) * (V(c, j) - vi);
-            }
-        }
+        for_nbrs<WIDE, XS>(
+            c, i, cnt, base, [&](int j) { return XS{c.q4[j], V(c, j), c.ratio[j], c.rho0[j]}; },
+            [&](int, XS d) {
+                float rho_bar = 0.5f * (rho_i + d.ratio * d.rho0);
+                if (rho_bar > 0.0f) {
+                    float wk = kernel_w(length(xi - xyz(d.q)), c.h, c.kind);
+                    cr = cr + (d.q.w / rho_bar * wk) * (d.v - vi);
+                }
+            });
         v = v + c.xsph * cr;
     }
     st3(c.vtmp, i, v);
@@ -1326,111 +1397,181 @@ __device__ __forceinline__ void atomic_max_nonneg(float* p, float v) {
     atomicMax(reinterpret_cast<int*>(p), __float_as_int(v));
 }
 
+// block-wide sum / max of per-thread values (blockDim 256), result in every thread
+__device__ __forceinline__ int block_sum_i32(int v) {
+    __shared__ int red[4];
+    for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int r = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ float block_max_f32(float v) {
+    __shared__ float red[4];
+    for (int s = 32; s >= 1; s >>= 1) v = fmx(v, __shfl_xor(v, s, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = fmx(fmx(red[0], red[1]), fmx(red[2], red[3]));
+    __syncthreads();
+    return r;
+}
+
+constexpr int LDS_SHAPES = 32;  // a world's shapes accumulated in LDS before one global atomic per shape and block
+
 // shape meters (bodies.py:37-60, static shapes), advect_kernel, particle-wall advance, project_kernel and
-// speed_kernel (integrate.py:181-326) for one particle: each reads only the particle's own state and slots.
+// speed_kernel (integrate.py:181-326). Each particle reads only its own state and slots; per-world counters and the
+// max speed are reduced in the block (a block is one world's chunk) and the shape meters in LDS, so global atomics
+// are one per block instead of one per particle (int64 sums and maxima: the result is order independent).
 __global__ void __launch_bounds__(256) k_advect_project(Ctx c, float dt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    int64_t b4 = 4 * int64_t(i);
-    if (!is_fluid(c, i)) {
-        if (c.NW > 0 && c.wvol[i] > 0.0f) st3(c.q, i, Q(c, i) + dt * V(c, i));
-        return;
-    }
-    for (int k = 0; k < 4; ++k) {  // bodies.accumulate_kernel meters (two-way)
-        int s = c.s_shape[b4 + k];
-        if (s < 0 || s >= c.S) continue;
-        f3 J = ld3(c.s_imp, b4 + k);
-        if (J.x == 0.0f && J.y == 0.0f && J.z == 0.0f) continue;
-        const double sc = 1125899906842624.0;
-        atomicAdd(c.meter + 3 * s, (unsigned long long)(long long)llround(double(J.x) * sc));
-        atomicAdd(c.meter + 3 * s + 1, (unsigned long long)(long long)llround(double(J.y) * sc));
-        atomicAdd(c.meter + 3 * s + 2, (unsigned long long)(long long)llround(double(J.z) * sc));
-    }
-    f3 v = V(c, i);
-    f3 x = Q(c, i) + dt * v;
-    int moved = 0;
-    int w = c.world[i];
-    if (length(dt * v) > c.sweep_len) {
-        f3 l;
-        int s = sweep_entry(c, w, x - dt * v, x, l);
-        if (s >= 0) {
-            Xf X = load_xf(c.sh_x + 7 * s);
-            float phi;
-            f3 gl;
-            shape_sdf(c.sh_geo[s], ld3(c.sh_scale, s), l, phi, gl);
-            f3 nn = normalize(xf_vector(X, gl));
-            x = xf_point(X, l) + fmx(c.eps - phi, 0.0f) * nn;
-            f3 vb{0.0f, 0.0f, 0.0f};
-            float vn = dot(v - vb, nn);
-            if (vn < 0.0f) {
-                v = v - vn * nn;
-                v = wall_friction(c, v, vb, nn, vn, s);
-            }
-            moved = 1;
-            atomicAdd(c.st_swept + w, 1);
-        }
-    }
-    int busy = 1, npass = 0;
-    while (busy != 0 && npass < 4) {
-        busy = 0;
-        npass += 1;
-        for (int b = 0; b < 4; ++b) {
-            int s = c.s_shape[b4 + b];
-            if (s == c.S && c.NW > 0) {  // pw_project (particle_walls.py:122-128)
-                f3 vb = ld3(c.s_vb, b4 + b), nb = ld3(c.s_nbl, b4 + b), xb = ld3(c.s_xbl, b4 + b);
-                float dd = dot(x - (xb + dt * vb), nb);
-                if (dd < c.eps) {
-                    x = x + (c.eps - dd) * nb;
-                    float vn = dot(v - vb, nb);
-                    if (vn < 0.0f) v = v - vn * nb;
-                    moved = 1;
-                    busy = 1;
+    CHUNK_PROLOGUE
+    int w = c.ch_world[b_];
+    __shared__ unsigned long long meter_lds[3 * LDS_SHAPES];
+    int s0 = c.sh_start[w], ns = c.sh_start[w + 1] - s0;
+    bool lds_meters = ns <= LDS_SHAPES;
+    for (int t = threadIdx.x; t < 3 * LDS_SHAPES; t += blockDim.x) meter_lds[t] = 0ull;
+    __syncthreads();
+    int n_proj = 0, n_swept = 0, n_nonfinite = 0;
+    float spmax = 0.0f;
+    if (valid) {
+        int64_t b4 = 4 * int64_t(i);
+        if (!is_fluid(c, i)) {
+            if (c.NW > 0 && c.wvol[i] > 0.0f) st3(c.q, i, Q(c, i) + dt * V(c, i));
+        } else {
+            for (int k = 0; k < 4; ++k) {  // bodies.accumulate_kernel meters (two-way)
+                int s = c.s_shape[b4 + k];
+                if (s < 0 || s >= c.S) continue;
+                f3 J = ld3(c.s_imp, b4 + k);
+                if (J.x == 0.0f && J.y == 0.0f && J.z == 0.0f) continue;
+                const double sc = 1125899906842624.0;
+                unsigned long long v[3] = {(unsigned long long)(long long)llround(double(J.x) * sc),
+                                           (unsigned long long)(long long)llround(double(J.y) * sc),
+                                           (unsigned long long)(long long)llround(double(J.z) * sc)};
+                int e = -1;
+                if (lds_meters)
+                    for (int t = 0; t < ns; ++t)
+                        if (c.sh_ids[s0 + t] == s) {
+                            e = t;
+                            break;
+                        }
+                for (int d = 0; d < 3; ++d) {
+                    if (e >= 0) atomicAdd(meter_lds + 3 * e + d, v[d]);
+                    else atomicAdd(c.meter + 3 * s + d, v[d]);
                 }
-            } else if (s >= 0 && s < c.S) {
-                Xf X = load_xf(c.sh_x + 7 * s);
-                f3 xl = xf_inv_point(X, x);
-                float phi;
-                f3 gl;
-                shape_sdf(c.sh_geo[s], ld3(c.sh_scale, s), xl, phi, gl);
-                if (phi < c.eps) {
+            }
+            f3 v = V(c, i);
+            f3 x = Q(c, i) + dt * v;
+            int moved = 0;
+            if (length(dt * v) > c.sweep_len) {
+                f3 l;
+                int s = sweep_entry(c, w, x - dt * v, x, l);
+                if (s >= 0) {
+                    Xf X = load_xf(c.sh_x + 7 * s);
+                    float phi;
+                    f3 gl;
+                    shape_sdf(c.sh_geo[s], ld3(c.sh_scale, s), l, phi, gl);
                     f3 nn = normalize(xf_vector(X, gl));
-                    float push = c.eps - phi;
-                    if (c.s_phi[b4 + b] >= 0.0f) {
-                        f3 ns = xf_vector(X, ld3(c.s_nbl, b4 + b));
-                        float push_s = c.eps - dot(ns, x - xf_point(X, ld3(c.s_xbl, b4 + b)));
-                        if (push_s > 0.0f) {
-                            nn = ns;
-                            push = push_s;
-                        }
+                    x = xf_point(X, l) + fmx(c.eps - phi, 0.0f) * nn;
+                    f3 vb{0.0f, 0.0f, 0.0f};
+                    float vn = dot(v - vb, nn);
+                    if (vn < 0.0f) {
+                        v = v - vn * nn;
+                        v = wall_friction(c, v, vb, nn, vn, s);
                     }
-                    if (push > 0.0f) {
-                        x = x + push * nn;
-                        f3 vb{0.0f, 0.0f, 0.0f};
-                        float vn = dot(v - vb, nn);
-                        if (vn < 0.0f) {
-                            v = v - vn * nn;
-                            v = wall_friction(c, v, vb, nn, vn, s);
+                    moved = 1;
+                    n_swept = 1;
+                }
+            }
+            int busy = 1, npass = 0;
+            while (busy != 0 && npass < 4) {
+                busy = 0;
+                npass += 1;
+                for (int b = 0; b < 4; ++b) {
+                    int s = c.s_shape[b4 + b];
+                    if (s == c.S && c.NW > 0) {  // pw_project (particle_walls.py:122-128)
+                        f3 vb = ld3(c.s_vb, b4 + b), nb = ld3(c.s_nbl, b4 + b), xb = ld3(c.s_xbl, b4 + b);
+                        float dd = dot(x - (xb + dt * vb), nb);
+                        if (dd < c.eps) {
+                            x = x + (c.eps - dd) * nb;
+                            float vn = dot(v - vb, nb);
+                            if (vn < 0.0f) v = v - vn * nb;
+                            moved = 1;
+                            busy = 1;
                         }
-                        moved = 1;
-                        busy = 1;
+                    } else if (s >= 0 && s < c.S) {
+                        Xf X = load_xf(c.sh_x + 7 * s);
+                        f3 xl = xf_inv_point(X, x);
+                        float phi;
+                        f3 gl;
+                        shape_sdf(c.sh_geo[s], ld3(c.sh_scale, s), xl, phi, gl);
+                        if (phi < c.eps) {
+                            f3 nn = normalize(xf_vector(X, gl));
+                            float push = c.eps - phi;
+                            if (c.s_phi[b4 + b] >= 0.0f) {
+                                f3 ns_ = xf_vector(X, ld3(c.s_nbl, b4 + b));
+                                float push_s = c.eps - dot(ns_, x - xf_point(X, ld3(c.s_xbl, b4 + b)));
+                                if (push_s > 0.0f) {
+                                    nn = ns_;
+                                    push = push_s;
+                                }
+                            }
+                            if (push > 0.0f) {
+                                x = x + push * nn;
+                                f3 vb{0.0f, 0.0f, 0.0f};
+                                float vn = dot(v - vb, nn);
+                                if (vn < 0.0f) {
+                                    v = v - vn * nn;
+                                    v = wall_friction(c, v, vb, nn, vn, s);
+                                }
+                                moved = 1;
+                                busy = 1;
+                            }
+                        }
                     }
                 }
             }
+            st3(c.q, i, x);
+            if (moved) {
+                st3(c.qd, i, v);
+                n_proj = 1;
+            }
+            float sp = length(v);
+            if (isfinite(sp) && isfinite(x.x + x.y + x.z)) {
+                spmax = sp;
+            } else {
+                n_nonfinite = 1;
+                spmax = 1.0e30f;
+            }
         }
     }
-    st3(c.q, i, x);
-    if (moved) {
-        st3(c.qd, i, v);
-        atomicAdd(c.st_projected + w, 1);
-    }
-    float sp = length(v);
-    if (isfinite(sp) && isfinite(x.x + x.y + x.z)) {
-        atomic_max_nonneg(c.st_max_speed + w, sp);
-    } else {
-        atomicAdd(c.st_nonfinite + w, 1);
-        atomic_max_nonneg(c.st_max_speed + w, 1.0e30f);
+    __syncthreads();
+    if (lds_meters)
+        for (int t = threadIdx.x; t < 3 * ns; t += blockDim.x) {
+            unsigned long long v = meter_lds[t];
+            if (v != 0ull) atomicAdd(c.meter + 3 * c.sh_ids[s0 + t / 3] + t % 3, v);
+        }
+    n_proj = block_sum_i32(n_proj);
+    n_swept = block_sum_i32(n_swept);
+    n_nonfinite = block_sum_i32(n_nonfinite);
+    spmax = block_max_f32(spmax);
+    if (threadIdx.x == 0) {
+        if (n_proj) atomicAdd(c.st_projected + w, n_proj);
+        if (n_swept) atomicAdd(c.st_swept + w, n_swept);
+        if (n_nonfinite) atomicAdd(c.st_nonfinite + w, n_nonfinite);
+        atomic_max_nonneg(c.st_max_speed + w, spmax);
     }
 }
+
+// ----------------------------------------------------------------------------------------------- profiling
+// Optional HIP-event timing of the main kernels on the launch stream (bench.py's live roofline figure).
+enum KernelId { KID_ACCEL = 0, KID_APPLY, KID_BUILD, KID_DENSITY, KID_SLOTS, KID_NONPRESSURE, KID_ADVECT,
+                KID_SOLVE_INIT, KID_POST, KID_COUNT };
+constexpr int PROF_CAP = 4096;
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> a, b;  // [KID_COUNT * PROF_CAP]
+    int used[KID_COUNT] = {};
+};
 
 // ----------------------------------------------------------------------------------------------- handle
 struct Handle {
@@ -1445,7 +1586,24 @@ struct Handle {
     float* learned_den = nullptr;
     int check_every = 1;
     int wall_first = 0;  // first cell of the wall grids
+    Prof prof;
 };
+
+struct ProfScope {
+    Handle* H;
+    int kid, slot;
+    hipStream_t st;
+    ProfScope(Handle* h, int k, hipStream_t s) : H(h), kid(k), slot(-1), st(s) {
+        if (H->prof.on && H->prof.used[k] < PROF_CAP) {
+            slot = k * PROF_CAP + H->prof.used[k]++;
+            (void)hipEventRecord(H->prof.a[slot], st);
+        }
+    }
+    ~ProfScope() {
+        if (slot >= 0) (void)hipEventRecord(H->prof.b[slot], st);
+    }
+};
+#define PROF(kid) ProfScope prof_scope_##kid(H, kid, st)
 
 template <class T>
 int dalloc(Handle* H, T** p, int64_t count) {
@@ -1487,7 +1645,10 @@ int phase_neighbors(Handle* H, hipStream_t st) {
     k_scan_apply<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part, c.cell_start, c.cell_fill);
     k_scatter<<<blocks_for(c.n), 256, 0, st>>>(c);
     if (c.NW > 0) k_sort_cells<<<blocks_for(c.n_cells - H->wall_first), 256, 0, st>>>(c, H->wall_first, c.n_cells);
-    k_build_lists<<<blocks_for(c.n, BUILD_BLOCK), BUILD_BLOCK, 0, st>>>(c);
+    {
+        PROF(KID_BUILD);
+        k_build_lists<<<blocks_for(c.n, BUILD_BLOCK), BUILD_BLOCK, 0, st>>>(c);
+    }
     LAUNCH_CHECK();
     return DFSPH_OK;
 }
@@ -1495,7 +1656,10 @@ int phase_neighbors(Handle* H, hipStream_t st) {
 int phase_slots(Handle* H, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
-    k_slots<<<blocks_for(c.n), 256, 0, st>>>(c);
+    {
+        PROF(KID_SLOTS);
+        k_slots<<<blocks_for(c.n), 256, 0, st>>>(c);
+    }
     LAUNCH_CHECK();
     return DFSPH_OK;
 }
@@ -1503,7 +1667,10 @@ int phase_slots(Handle* H, hipStream_t st) {
 int phase_density(Handle* H, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
-    k_density_alpha<<<H->n_chunks, 256, 0, st>>>(c);
+    {
+        PROF(KID_DENSITY);
+        LAUNCH_W(k_density_alpha, H->n_chunks, 256, c);
+    }
     k_world<<<c.W, 256, 0, st>>>(c, W_NACTIVE, StopSpec{});
     LAUNCH_CHECK();
     return DFSPH_OK;
@@ -1556,8 +1723,11 @@ int phase_solve(Handle* H, int mode, double dt_h, int fixed_iters, hipStream_t s
     float relief = 1.0e30f;
     if (!div && P.compression_relief_speed > 0.0) relief = float(P.compression_relief_speed * dt_h * P.lattice_grad);
     const float* learned = div ? H->learned_div : H->learned_den;
-    k_solve_init<<<blocks_for(c.n), 256, 0, st>>>(c, mode, dt, float(warm_h), float(P.warm_start_density), relief,
-                                                  store, learned);
+    {
+        PROF(KID_SOLVE_INIT);
+        LAUNCH_W(k_solve_init, blocks_for(c.n), 256, c, mode, dt, float(warm_h), float(P.warm_start_density), relief,
+                 store, learned);
+    }
     k_begin_solve<<<blocks_for(c.W), 256, 0, st>>>(c);
     StopSpec sp;
     sp.tol = float(tolh);
@@ -1570,8 +1740,8 @@ int phase_solve(Handle* H, int mode, double dt_h, int fixed_iters, hipStream_t s
     int nu = 0;
     int guard = 0;
     if (P.pressure_solver == 0) {
-        k_accel<<<H->n_chunks, 256, 0, st>>>(c, c.kappa, 0);
-        k_apply<<<H->n_chunks, 256, 0, st>>>(c, mode, dt, c.kappa, c.wv, 0, 1, 0, 0);
+        LAUNCH_W(k_accel, H->n_chunks, 256, c, c.kappa, 0);
+        LAUNCH_W(k_apply, H->n_chunks, 256, c, mode, dt, c.kappa, c.wv, 0, 1, 0, 0);
         k_cg_init<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
         k_world<<<c.W, 256, 0, st>>>(c, W_CG_INIT, sp);
         LAUNCH_CHECK();
@@ -1579,12 +1749,18 @@ int phase_solve(Handle* H, int mode, double dt_h, int fixed_iters, hipStream_t s
         if (rc) return rc;
         while (nu != 0) {
             for (int rep = 0; rep < H->check_every; ++rep) {
-                k_accel<<<H->n_chunks, 256, 0, st>>>(c, c.pd, 0);
-                k_apply<<<H->n_chunks, 256, 0, st>>>(c, mode, dt, c.pd, c.ap, 0, 0, 0, 1);
+                {
+                    PROF(KID_ACCEL);
+                    LAUNCH_W(k_accel, H->n_chunks, 256, c, c.pd, 0);
+                }
+                {
+                    PROF(KID_APPLY);
+                    LAUNCH_W(k_apply, H->n_chunks, 256, c, mode, dt, c.pd, c.ap, 0, 0, 0, 1);
+                }
                 k_world<<<c.W, 256, 0, st>>>(c, W_CG_PAP, sp);
                 k_cg_step<<<H->n_chunks, 256, 0, st>>>(c);
-                k_accel<<<H->n_chunks, 256, 0, st>>>(c, c.corr, 1);
-                k_apply<<<H->n_chunks, 256, 0, st>>>(c, mode, dt, c.corr, c.wv, 1, 0, 1, 0);
+                LAUNCH_W(k_accel, H->n_chunks, 256, c, c.corr, 1);
+                LAUNCH_W(k_apply, H->n_chunks, 256, c, mode, dt, c.corr, c.wv, 1, 0, 1, 0);
                 k_cg_finish<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
                 k_zero_unconv<<<1, 1, 0, st>>>(c);
                 k_world<<<c.W, 256, 0, st>>>(c, W_CG_BETA, sp);
@@ -1600,8 +1776,8 @@ int phase_solve(Handle* H, int mode, double dt_h, int fixed_iters, hipStream_t s
         if (rc) return rc;
         while (nu != 0) {
             for (int rep = 0; rep < H->check_every; ++rep) {
-                k_accel<<<H->n_chunks, 256, 0, st>>>(c, c.kappa, 0);
-                k_jacobi<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
+                LAUNCH_W(k_accel, H->n_chunks, 256, c, c.kappa, 0);
+                LAUNCH_W(k_jacobi, H->n_chunks, 256, c, mode, dt);
                 k_world<<<c.W, 256, 0, st>>>(c, W_JACOBI, sp);
             }
             LAUNCH_CHECK();
@@ -1610,7 +1786,10 @@ int phase_solve(Handle* H, int mode, double dt_h, int fixed_iters, hipStream_t s
             if (++guard > 100000) return fail(DFSPH_EDEVICE, "pressure solve did not terminate");
         }
     }
-    k_post_solve<<<blocks_for(c.n), 256, 0, st>>>(c, mode, dt, store);
+    {
+        PROF(KID_POST);
+        LAUNCH_W(k_post_solve, blocks_for(c.n), 256, c, mode, dt, store);
+    }
     k_record<<<blocks_for(c.W), 256, 0, st>>>(c, mode, float(tolh));
     LAUNCH_CHECK();
     return DFSPH_OK;
@@ -1619,7 +1798,10 @@ int phase_solve(Handle* H, int mode, double dt_h, int fixed_iters, hipStream_t s
 int phase_nonpressure(Handle* H, double dt, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
-    k_nonpressure<<<blocks_for(c.n), 256, 0, st>>>(c, float(dt));
+    {
+        PROF(KID_NONPRESSURE);
+        LAUNCH_W(k_nonpressure, blocks_for(c.n), 256, c, float(dt));
+    }
     k_copy_f32<<<blocks_for(3 * int64_t(c.n)), 256, 0, st>>>(c.qd, c.vtmp, 3 * int64_t(c.n));
     LAUNCH_CHECK();
     return DFSPH_OK;
@@ -1629,7 +1811,10 @@ int phase_advect(Handle* H, double dt, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
     k_fill_f32<<<blocks_for(c.W), 256, 0, st>>>(c.st_max_speed, 0.0f, c.W);
-    k_advect_project<<<blocks_for(c.n), 256, 0, st>>>(c, float(dt));
+    {
+        PROF(KID_ADVECT);
+        k_advect_project<<<H->n_chunks, 256, 0, st>>>(c, float(dt));
+    }
     LAUNCH_CHECK();
     return DFSPH_OK;
 }
@@ -1782,7 +1967,8 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         for (float** p : {&c.ratio, &c.alpha, &c.neff, &c.active, &c.reg, &c.src, &c.kappa, &c.err, &c.lower, &c.kv,
                           &c.kd, &c.pd, &c.z, &c.wv, &c.ap, &c.corr})
             TRY(dalloc(H, p, n));
-        TRY(dalloc(H, &c.acc, 3 * int64_t(n)));
+        TRY(dalloc(H, &c.acc4, n));
+        TRY(dalloc(H, &c.q4, n));
         TRY(dalloc(H, &c.vtmp, 3 * int64_t(n)));
         TRY(dalloc(H, &c.act, n));
         for (int** p : {&c.converged, &c.iters, &c.restart, &c.st_overflow, &c.st_projected, &c.st_swept,
@@ -1821,6 +2007,8 @@ int32_t dfsph_destroy(void* h) {
     if (!h) return DFSPH_OK;
     Handle* H = as_handle(h);
     (void)hipDeviceSynchronize();
+    for (auto* v : {&H->prof.a, &H->prof.b})
+        for (hipEvent_t e : *v) (void)hipEventDestroy(e);
     for (void* p : H->owned) (void)hipFree(p);
     if (H->h_unconv) (void)hipHostFree(H->h_unconv);
     delete H;
@@ -1961,6 +2149,41 @@ int32_t dfsph_set_check_interval(void* h, int32_t every) {
     return DFSPH_OK;
 }
 
+// per-kernel HIP-event timing (kernel ids: 0 accel, 1 apply, 2 build lists, 3 density, 4 slots, 5 non-pressure,
+// 6 advect/project, 7 solve init, 8 post-solve); enabling resets the counters
+int32_t dfsph_set_profiling(void* h, int32_t on) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    Handle* H = as_handle(h);
+    if (on && H->prof.a.empty()) {
+        H->prof.a.resize(KID_COUNT * PROF_CAP);
+        H->prof.b.resize(KID_COUNT * PROF_CAP);
+        for (int k = 0; k < KID_COUNT * PROF_CAP; ++k) {
+            HIPCHK(hipEventCreate(&H->prof.a[k]));
+            HIPCHK(hipEventCreate(&H->prof.b[k]));
+        }
+    }
+    H->prof.on = on != 0;
+    for (int k = 0; k < KID_COUNT; ++k) H->prof.used[k] = 0;
+    return DFSPH_OK;
+}
+
+// count and summed duration [ms] of kernel `kid`'s timed launches since profiling was enabled (syncs)
+int32_t dfsph_profile_read(void* h, int32_t kid, int32_t* count, double* total_ms) {
+    if (!h || !count || !total_ms) return fail(DFSPH_EINVAL, "null argument");
+    if (kid < 0 || kid >= KID_COUNT) return fail(DFSPH_EINVAL, "unknown kernel id");
+    Handle* H = as_handle(h);
+    HIPCHK(hipDeviceSynchronize());
+    double t = 0.0;
+    for (int k = 0; k < H->prof.used[kid]; ++k) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, H->prof.a[kid * PROF_CAP + k], H->prof.b[kid * PROF_CAP + k]));
+        t += ms;
+    }
+    *count = H->prof.used[kid];
+    *total_ms = t;
+    return DFSPH_OK;
+}
+
 int32_t dfsph_get_buffer(void* h, int32_t which, void** ptr, int64_t* nb) {
     if (!h || !ptr || !nb) return fail(DFSPH_EINVAL, "null argument");
     Handle* H = as_handle(h);
@@ -1993,7 +2216,7 @@ int32_t dfsph_get_buffer(void* h, int32_t which, void** ptr, int64_t* nb) {
         case DFSPH_BUF_REG: B(c.reg, 4 * n)
         case DFSPH_BUF_SRC: B(c.src, 4 * n)
         case DFSPH_BUF_KAPPA: B(c.kappa, 4 * n)
-        case DFSPH_BUF_ACC: B(c.acc, 12 * n)
+        case DFSPH_BUF_ACC: B(c.acc4, 16 * n)
         case DFSPH_BUF_ERR: B(c.err, 4 * n)
         case DFSPH_BUF_LOWER: B(c.lower, 4 * n)
         case DFSPH_BUF_KV: B(c.kv, 4 * n)

@@ -30,6 +30,12 @@ DFSPH_OK, DFSPH_EINVAL, DFSPH_ECAPACITY, DFSPH_EDEVICE = 0, -1, -2, -3
  BUF_ERR_MAX, BUF_N_ACTIVE, BUF_TOL_SCALE, BUF_OVERFLOW, BUF_MAX_SPEED, BUF_ITERS_DEN, BUF_ITERS_DIV, BUF_DEN_ERR,
  BUF_DIV_ERR, BUF_SHAPE_METER, BUF_FEATURES) = range(40)
 
+# kernel ids of dfsph_profile_read
+(KID_ACCEL, KID_APPLY, KID_BUILD, KID_DENSITY, KID_SLOTS, KID_NONPRESSURE, KID_ADVECT, KID_SOLVE_INIT,
+ KID_POST) = range(9)
+KERNEL_NAMES = ("k_accel", "k_apply", "k_build_lists", "k_density_alpha", "k_slots", "k_nonpressure",
+                "k_advect_project", "k_solve_init", "k_post_solve")
+
 PROJECT_EPS = 1.0e-3  # wall projection skin in particle radii (solver.py:483)
 SWEEP = 0.5           # integrate.py:38
 THIN_SWEEP = 0.5      # integrate.py:39
@@ -294,6 +300,8 @@ def load_library() -> ctypes.CDLL:
         "dfsph_set_warm_start_kappa": ([vp, vp, vp], i32),
         "dfsph_set_check_interval": ([vp, i32], i32),
         "dfsph_get_buffer": ([vp, i32, P(vp), P(i64)], i32),
+        "dfsph_set_profiling": ([vp, i32], i32),
+        "dfsph_profile_read": ([vp, i32, P(i32), P(dbl)], i32),
         "dfsph_read_buffer": ([vp, i32, vp, i64], i32),
         "dfsph_write_buffer": ([vp, i32, vp, i64], i32),
         "dfsph_zero_stats": ([vp, vp], i32),

@@ -258,11 +258,16 @@ class SolverDFSPH:
                 "nonfinite": r["nonfinite"], "swept": r["swept"], "n_active": r["n_active"]}
 
     def _active_mass(self) -> np.ndarray:
+        """Active liquid mass per world [kg]: a float64 prefix sum over the contiguous world ranges (no atomics)."""
         m = self.model
+        if m.particle_count == 0:
+            return np.zeros(m.world_count)
         act = ((m.particle_flags & int(ParticleFlags.ACTIVE)) != 0) & (m.sph.phase >= PHASE_LIQUID)
-        out = torch.zeros(m.world_count, dtype=torch.float64, device=self.device)
-        out.index_add_(0, m.particle_world[act].long(), m.particle_mass[act].double())
-        return out.cpu().numpy()
+        cum = torch.cumsum(torch.where(act, m.particle_mass.double(), torch.zeros((), dtype=torch.float64,
+                                                                                  device=self.device)), 0).cpu()
+        cum = np.concatenate([[0.0], cum.numpy()])
+        ws = m.world_start_np
+        return cum[ws[1:]] - cum[ws[:-1]]
 
     def mass_ledger(self) -> dict[str, np.ndarray]:
         """Per-world mass ledger [kg] (runtime.py:129-134): active + out == initial + in."""

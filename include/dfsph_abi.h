@@ -207,6 +207,13 @@ int32_t dfsph_set_warm_start_kappa(void* handle, const float* kappa_div, const f
 /* Host convergence check every `every` iterations (1..64); converged worlds' blocks exit at once in between. */
 int32_t dfsph_set_check_interval(void* handle, int32_t every);
 
+/* Per-kernel HIP-event timing on the launch stream: kernel ids 0 pressure acceleration, 1 operator apply,
+ * 2 neighbour rows, 3 density/alpha, 4 boundary slots, 5 non-pressure, 6 advect/project, 7 solve init,
+ * 8 post-solve. No counterpart in the reference (its timing is wall clock around graph replays,
+ * alloy/testing/sph.py:334-364); for measurement only. */
+int32_t dfsph_set_profiling(void* handle, int32_t on);
+int32_t dfsph_profile_read(void* handle, int32_t kernel_id, int32_t* count, double* total_ms);
+
 /* Buffers */
 int32_t dfsph_get_buffer(void* handle, int32_t which, void** ptr, int64_t* n_bytes);
 int32_t dfsph_read_buffer(void* handle, int32_t which, void* host_dst, int64_t n_bytes);   /* syncs */

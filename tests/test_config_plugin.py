@@ -1,0 +1,81 @@
+"""Configuration, time-step helpers and the plug-in boundary (tests/solvers/sph/test_contract.py:28-136 of the
+reference: capability, factory kwargs, closed forms of stable_dt / snap_down, config validation)."""
+
+from __future__ import annotations
+
+import dataclasses
+
+import pytest
+
+from alloy_amd.core.registry import get_solver, solvers_for_route
+from alloy_amd.sph import plugin
+from alloy_amd.sph.config import DFSPHConfig, snap_down, stable_dt
+from alloy_amd.sph.scenes import dam_break
+
+
+def test_defaults_and_derived():
+    cfg = DFSPHConfig()
+    assert cfg.h == pytest.approx(8e-3) and cfg.spacing == pytest.approx(4e-3)
+    assert cfg.max_density_error == 1e-3 and cfg.pressure_solver == "cg" and cfg.neighbor_capacity == 64
+    fast = DFSPHConfig.fast(particle_radius=0.005)
+    assert fast.rest_jitter_speed is None and fast.cfl == 0.4 and fast.particle_radius == 0.005
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("particle_radius", 0.0, "particle_radius must be > 0"),
+    ("support_factor", 1.5, "support_factor"),
+    ("min_iterations", 0, "min_iterations"),
+    ("jacobi_omega", 0.8, "jacobi_omega"),
+    ("pressure_solver", "sor", "pressure_solver"),
+    ("rest_jitter_flow_ratio", 1.5, "rest_jitter_flow_ratio"),
+    ("diagonal_floor", 1.0, "diagonal_floor"),
+])
+def test_validate_rejects(field, value, msg):
+    cfg = dataclasses.replace(DFSPHConfig(), **{field: value})
+    with pytest.raises(ValueError, match=msg):
+        cfg.validate()
+
+
+def test_stable_dt_and_snap_down_closed_forms():
+    cfg = DFSPHConfig(particle_radius=1e-3)
+    assert stable_dt(cfg, 0.0) == cfg.dt_max
+    assert stable_dt(cfg, float("inf")) == cfg.dt_min
+    assert stable_dt(cfg, 1.0) == pytest.approx(0.2 * 2e-3)
+    assert stable_dt(cfg, 1e6) == cfg.dt_min
+    assert snap_down(1e-3, 0.02) == pytest.approx(0.02 / 32)
+    assert snap_down(0.02, 0.02) == 0.02
+    assert snap_down(1e-9, 0.02) == pytest.approx(0.02 / 1024)
+
+
+def test_capability_registration():
+    cap = get_solver("sph")
+    assert cap is plugin.CAPABILITY and "liquid_sph" in cap.routes
+    assert cap in solvers_for_route("liquid_sph")
+    assert cap.uniform_worlds is False and cap.coupling and cap.determinism == "native"
+    plugin.register()  # idempotent
+
+
+def test_factory_rejects_unknown_kwargs():
+    sc = dam_break(4, 0.1)
+    with pytest.raises(TypeError, match="unknown kwargs"):
+        plugin._factory(sc.model, {"nonsense": 1})
+
+
+def test_solver_has_no_cpu_path():
+    """The product path runs only on a HIP device: a CPU model is an error, not a fallback."""
+    from alloy_amd.sph.solver import SolverDFSPH
+
+    sc = dam_break(4, 0.1)
+    with pytest.raises(RuntimeError, match="HIP device"):
+        SolverDFSPH(sc.model, sc.config)
+
+
+@pytest.mark.parametrize("field,value", [("surface_tension", "csf"), ("ambient_pressure", 100.0),
+                                         ("reneighbor_skin", 0.1), ("viscosity", "weiler2018")])
+def test_unbuilt_subsystems_are_refused(field, value):
+    from alloy_amd.sph.solver import SolverDFSPH
+
+    sc = dam_break(4, 0.1)
+    cfg = dataclasses.replace(sc.config, **{field: value})
+    with pytest.raises(ValueError, match="not built"):
+        SolverDFSPH(sc.model, cfg)
