@@ -19,6 +19,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "../../include/dfsph_abi.h"
@@ -95,6 +97,7 @@ struct Ctx {
     // per particle
     float *ratio, *alpha, *neff, *active, *reg, *src, *kappa, *err, *lower, *kv, *kd, *vtmp;
     float4* q4;    // (x, y, z, mass) per particle, refreshed from q at every neighbour build: one 16 B gather
+    float4* q4s;   // (x, y, z, id bits) in cell order: the list build streams candidates from it
     float4* acc4;  // pressure acceleration (x, y, z, 0)
     float *pd, *z, *wv, *ap, *corr;
     int* act;
@@ -223,6 +226,7 @@ __device__ __forceinline__ bool stop_test(const Ctx& c, int w, int it, float mea
 // the world's control logic in thread 0 (reduce.py:100-164 + pressure.py:370-390 / krylov.py:215-240).
 __global__ void __launch_bounds__(256) k_world(Ctx c, int mode, StopSpec sp) {
     int w = blockIdx.x;
+    if (mode == W_CG_PAP && w == 0 && threadIdx.x == 0) *c.n_unconv = 0;  // recounted by W_CG_BETA
     if ((mode == W_CG_PAP || mode == W_CG_BETA || mode == W_JACOBI) && c.converged[w] != 0) return;
     __shared__ double lds[CHUNK];
     double a = 0.0, b = 0.0, m = -3.4e38;
@@ -426,6 +430,8 @@ __global__ void __launch_bounds__(256) k_scatter(Ctx c) {
     if (k < 0) return;
     int pos = atomicAdd(c.cell_fill + k, 1);
     c.items[pos] = i;
+    float4 p = c.q4[i];
+    c.q4s[pos] = make_float4(p.x, p.y, p.z, __int_as_float(i));
 }
 
 // wall cells: ascending ids within each cell (the fixed visit order of the colour sums)
@@ -473,27 +479,34 @@ __global__ void __launch_bounds__(BUILD_BLOCK) k_build_lists(Ctx c) {
             if (Y < 0 || Y >= d[1]) continue;
             int x0 = max(cx - 1, 0), x1 = min(cx + 1, d[0] - 1);
             int cb = c.cell_base[g] + (Z * d[1] + Y) * d[0];
-            // cells x0..x1 of this row are contiguous in the sorted item array
+            // cells x0..x1 of this row are contiguous in cell order; candidates stream from q4s in batches of 8
             int e0 = c.cell_start[cb + x0], e1 = c.cell_start[cb + x1 + 1];
-            for (int e = e0; e < e1; ++e) {
-                int j = c.items[e];
-                if (j == i) continue;
-                f3 dd = xyz(c.q4[j]) - xi;
-                if (dot(dd, dd) < h2) {
-                    total += 1;
-                    int jl = j - base;
-                    int pos = nn, k = nn - 1;
-                    while (k >= 0) {
-                        int v = L[k * BUILD_BLOCK];
-                        if (v < jl) break;
-                        pos = k;
-                        k -= 1;
+            for (int e = e0; e < e1; e += 8) {
+                float4 pc[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) pc[u] = c.q4s[min(e + u, e1 - 1)];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (e + u >= e1) continue;
+                    int j = __float_as_int(pc[u].w);
+                    if (j == i) continue;
+                    f3 dd = xyz(pc[u]) - xi;
+                    if (dot(dd, dd) < h2) {
+                        total += 1;
+                        int jl = j - base;
+                        int pos = nn, k = nn - 1;
+                        while (k >= 0) {
+                            int v = L[k * BUILD_BLOCK];
+                            if (v < jl) break;
+                            pos = k;
+                            k -= 1;
+                        }
+                        if (pos >= cap) continue;
+                        int lastk = nn >= cap ? cap - 1 : nn;
+                        for (k = lastk; k > pos; --k) L[k * BUILD_BLOCK] = L[(k - 1) * BUILD_BLOCK];
+                        L[pos * BUILD_BLOCK] = jl;
+                        if (nn < cap) nn += 1;
                     }
-                    if (pos >= cap) continue;
-                    int lastk = nn >= cap ? cap - 1 : nn;
-                    for (k = lastk; k > pos; --k) L[k * BUILD_BLOCK] = L[(k - 1) * BUILD_BLOCK];
-                    L[pos * BUILD_BLOCK] = jl;
-                    if (nn < cap) nn += 1;
                 }
             }
         }
@@ -1040,7 +1053,7 @@ __global__ void __launch_bounds__(256) k_cg_step(Ctx c) {
             cc = c.lower[i] - k;
             k = c.lower[i];
             c.act[i] = 0;
-            atomicAdd(c.restart + w, 1);
+            c.restart[w] = 1;  // a flag: only restart != 0 is ever tested (krylov.py:70-74, 231)
         }
         c.kappa[i] = k;
     }
@@ -1062,7 +1075,7 @@ __global__ void __launch_bounds__(256) k_cg_finish(Ctx c, int mode, float dt) {
             float wvv = c.wv[i];
             if (c.act[i] == 0 && wvv < 0.0f) {
                 c.act[i] = 1;
-                atomicAdd(c.restart + w, 1);
+                c.restart[w] = 1;
             }
             float zi = 0.0f;
             if (c.act[i] != 0) zi = -wvv * c.alpha[i] / sc;
@@ -1574,12 +1587,23 @@ struct Prof {
 };
 
 // ----------------------------------------------------------------------------------------------- handle
+struct GraphKey {
+    int kind, mode, every;
+    uint32_t dt_bits;
+    const void *q, *qd, *f, *ld, *ln;
+    bool operator<(const GraphKey& o) const {
+        return std::tie(kind, mode, every, dt_bits, q, qd, f, ld, ln) <
+               std::tie(o.kind, o.mode, o.every, o.dt_bits, o.q, o.qd, o.f, o.ld, o.ln);
+    }
+};
+
 struct Handle {
     DfsphParams P;
     Ctx c;
     int n_chunks = 0, n_scan_blocks = 0;
     int64_t nbr_bytes = 0, nbr_hi_bytes = 0;
-    int* h_unconv = nullptr;  // pinned
+    int* h_unconv = nullptr;  // pinned [2]: unconverged-world counts of the last two iteration batches
+    hipEvent_t ev_unconv[2] = {nullptr, nullptr};
     std::vector<void*> owned;
     float* qd_saved = nullptr;
     float* learned_div = nullptr;
@@ -1587,6 +1611,9 @@ struct Handle {
     int check_every = 1;
     int wall_first = 0;  // first cell of the wall grids
     Prof prof;
+    bool use_graphs = true;
+    hipStream_t cap_stream = nullptr;
+    std::map<GraphKey, hipGraphExec_t> graphs;
 };
 
 struct ProfScope {
@@ -1687,10 +1714,18 @@ int phase_flow(Handle* H, hipStream_t st) {
     return DFSPH_OK;
 }
 
-int read_unconv(Handle* H, hipStream_t st, int* out) {
-    HIPCHK(hipMemcpyAsync(H->h_unconv, H->c.n_unconv, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    *out = *H->h_unconv;
+// The device-side loop replacing wp.capture_while: after each batch of iterations the unconverged-world count is
+// copied to pinned slot s and an event recorded; the host enqueues the next batch before it waits on that event,
+// so the GPU never idles on the host. The batch enqueued after the last unconverged world converged is a no-op
+// (every kernel of a converged world exits at its first instruction).
+int post_unconv(Handle* H, hipStream_t st, int slot) {
+    HIPCHK(hipMemcpyAsync(H->h_unconv + slot, H->c.n_unconv, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(H->ev_unconv[slot], st));
+    return DFSPH_OK;
+}
+int wait_unconv(Handle* H, int slot, int* out) {
+    HIPCHK(hipEventSynchronize(H->ev_unconv[slot]));
+    *out = H->h_unconv[slot];
     return DFSPH_OK;
 }
 
@@ -1699,100 +1734,180 @@ struct SolveOverride {
     int max_it = -1;
 };
 
-// solve_driver.py:121-162 (+ krylov.py:289-318 / solve_driver.py:171-186 loops)
-int phase_solve(Handle* H, int mode, double dt_h, int fixed_iters, hipStream_t st, const SolveOverride& ov = {}) {
-    Ctx c = H->c;  // by value: overrides stay local to this solve
+// Everything one pressure solve launches with (solve_driver.py:121-162), resolved on the host once.
+struct SolveSetup {
+    Ctx c;  // by value: overrides stay local to this solve
+    int mode;
+    float dt, warm, relief;
+    float* store;
+    const float* learned;
+    StopSpec sp;
+    double tolh;
+};
+
+SolveSetup solve_setup(Handle* H, int mode, double dt_h, int fixed_iters, const SolveOverride& ov) {
+    SolveSetup S;
+    S.c = H->c;
     const DfsphParams& P = H->P;
-    if (c.n == 0) return DFSPH_OK;
     bool div = mode == 0;
-    float dt = float(dt_h);
-    float* store = div ? c.kv : c.kd;
-    double tolh = div ? P.max_divergence_error : P.max_density_error;
-    if (ov.tol >= 0.0) tolh = ov.tol;
-    if (ov.min_nb >= 0.0) c.min_nb = float(ov.min_nb);
+    S.mode = mode;
+    S.dt = float(dt_h);
+    S.store = div ? S.c.kv : S.c.kd;
+    S.tolh = div ? P.max_divergence_error : P.max_density_error;
+    if (ov.tol >= 0.0) S.tolh = ov.tol;
+    if (ov.min_nb >= 0.0) S.c.min_nb = float(ov.min_nb);
     double err_factor = ov.err_factor >= 0.0 ? ov.err_factor : P.max_error_factor;
-    if (!div && P.rest_jitter_speed > 0.0) {  // _density_tolerance (solve_driver.py:108-119)
+    double warm_h = div ? P.warm_start_divergence : P.warm_start_density;
+    if (ov.warm >= 0.0) warm_h = ov.warm;
+    S.warm = float(warm_h);
+    S.relief = 1.0e30f;
+    if (!div && P.compression_relief_speed > 0.0)
+        S.relief = float(P.compression_relief_speed * dt_h * P.lattice_grad);
+    S.learned = div ? H->learned_div : H->learned_den;
+    S.sp.tol = float(S.tolh);
+    S.sp.tol_max = float(S.tolh * err_factor);
+    S.sp.min_it = P.min_iterations;
+    S.sp.max_it = ov.max_it > 0 ? ov.max_it : P.max_iterations;
+    S.sp.use_scale = div ? 0 : 1;
+    if (fixed_iters >= 0) S.sp.min_it = S.sp.max_it = std::max(fixed_iters, 1);
+    return S;
+}
+
+// tolerance scale, source / warm start, begin_solve and (CG) the initial residual
+int enq_solve_head(Handle* H, const SolveSetup& S, hipStream_t st) {
+    const Ctx& c = S.c;
+    const DfsphParams& P = H->P;
+    if (S.mode == 1 && P.rest_jitter_speed > 0.0) {  // _density_tolerance (solve_driver.py:108-119)
         k_head_terms<<<H->n_chunks, 256, 0, st>>>(c);
         k_world<<<c.W, 256, 0, st>>>(c, W_HEAD, StopSpec{});
-        k_tol_scale<<<blocks_for(c.W), 256, 0, st>>>(c, dt, float(P.rest_jitter_speed),
+        k_tol_scale<<<blocks_for(c.W), 256, 0, st>>>(c, S.dt, float(P.rest_jitter_speed),
                                                       float(P.rest_jitter_flow_ratio), float(P.max_density_error),
                                                       float(std::min(P.min_density_error, P.max_density_error)));
     }
-    double warm_h = div ? P.warm_start_divergence : P.warm_start_density;
-    if (ov.warm >= 0.0) warm_h = ov.warm;
-    float relief = 1.0e30f;
-    if (!div && P.compression_relief_speed > 0.0) relief = float(P.compression_relief_speed * dt_h * P.lattice_grad);
-    const float* learned = div ? H->learned_div : H->learned_den;
     {
         PROF(KID_SOLVE_INIT);
-        LAUNCH_W(k_solve_init, blocks_for(c.n), 256, c, mode, dt, float(warm_h), float(P.warm_start_density), relief,
-                 store, learned);
+        LAUNCH_W(k_solve_init, blocks_for(c.n), 256, c, S.mode, S.dt, S.warm, float(P.warm_start_density), S.relief,
+                 S.store, S.learned);
     }
     k_begin_solve<<<blocks_for(c.W), 256, 0, st>>>(c);
-    StopSpec sp;
-    sp.tol = float(tolh);
-    sp.tol_max = float(tolh * err_factor);
-    sp.min_it = P.min_iterations;
-    sp.max_it = ov.max_it > 0 ? ov.max_it : P.max_iterations;
-    sp.use_scale = div ? 0 : 1;
-    if (fixed_iters >= 0) sp.min_it = sp.max_it = std::max(fixed_iters, 1);
-    LAUNCH_CHECK();
-    int nu = 0;
-    int guard = 0;
     if (P.pressure_solver == 0) {
         LAUNCH_W(k_accel, H->n_chunks, 256, c, c.kappa, 0);
-        LAUNCH_W(k_apply, H->n_chunks, 256, c, mode, dt, c.kappa, c.wv, 0, 1, 0, 0);
-        k_cg_init<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
-        k_world<<<c.W, 256, 0, st>>>(c, W_CG_INIT, sp);
-        LAUNCH_CHECK();
-        int rc = read_unconv(H, st, &nu);
-        if (rc) return rc;
-        while (nu != 0) {
-            for (int rep = 0; rep < H->check_every; ++rep) {
-                {
-                    PROF(KID_ACCEL);
-                    LAUNCH_W(k_accel, H->n_chunks, 256, c, c.pd, 0);
-                }
-                {
-                    PROF(KID_APPLY);
-                    LAUNCH_W(k_apply, H->n_chunks, 256, c, mode, dt, c.pd, c.ap, 0, 0, 0, 1);
-                }
-                k_world<<<c.W, 256, 0, st>>>(c, W_CG_PAP, sp);
-                k_cg_step<<<H->n_chunks, 256, 0, st>>>(c);
-                LAUNCH_W(k_accel, H->n_chunks, 256, c, c.corr, 1);
-                LAUNCH_W(k_apply, H->n_chunks, 256, c, mode, dt, c.corr, c.wv, 1, 0, 1, 0);
-                k_cg_finish<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
-                k_zero_unconv<<<1, 1, 0, st>>>(c);
-                k_world<<<c.W, 256, 0, st>>>(c, W_CG_BETA, sp);
-                k_cg_direction<<<H->n_chunks, 256, 0, st>>>(c);
-            }
-            LAUNCH_CHECK();
-            rc = read_unconv(H, st, &nu);
-            if (rc) return rc;
-            if (++guard > 100000) return fail(DFSPH_EDEVICE, "pressure solve did not terminate");
-        }
-    } else {
-        int rc = read_unconv(H, st, &nu);
-        if (rc) return rc;
-        while (nu != 0) {
-            for (int rep = 0; rep < H->check_every; ++rep) {
-                LAUNCH_W(k_accel, H->n_chunks, 256, c, c.kappa, 0);
-                LAUNCH_W(k_jacobi, H->n_chunks, 256, c, mode, dt);
-                k_world<<<c.W, 256, 0, st>>>(c, W_JACOBI, sp);
-            }
-            LAUNCH_CHECK();
-            rc = read_unconv(H, st, &nu);
-            if (rc) return rc;
-            if (++guard > 100000) return fail(DFSPH_EDEVICE, "pressure solve did not terminate");
-        }
+        LAUNCH_W(k_apply, H->n_chunks, 256, c, S.mode, S.dt, c.kappa, c.wv, 0, 1, 0, 0);
+        k_cg_init<<<H->n_chunks, 256, 0, st>>>(c, S.mode, S.dt);
+        k_world<<<c.W, 256, 0, st>>>(c, W_CG_INIT, S.sp);
     }
-    {
-        PROF(KID_POST);
-        LAUNCH_W(k_post_solve, blocks_for(c.n), 256, c, mode, dt, store);
-    }
-    k_record<<<blocks_for(c.W), 256, 0, st>>>(c, mode, float(tolh));
     LAUNCH_CHECK();
     return DFSPH_OK;
+}
+
+// check_every iterations (krylov.py:298-313 / solve_driver.py:175-184); converged worlds' blocks exit at once
+int enq_solve_batch(Handle* H, const SolveSetup& S, hipStream_t st) {
+    const Ctx& c = S.c;
+    int mode = S.mode;
+    float dt = S.dt;
+    for (int rep = 0; rep < H->check_every; ++rep) {
+        if (H->P.pressure_solver == 0) {
+            {
+                PROF(KID_ACCEL);
+                LAUNCH_W(k_accel, H->n_chunks, 256, c, c.pd, 0);
+            }
+            {
+                PROF(KID_APPLY);
+                LAUNCH_W(k_apply, H->n_chunks, 256, c, mode, dt, c.pd, c.ap, 0, 0, 0, 1);
+            }
+            k_world<<<c.W, 256, 0, st>>>(c, W_CG_PAP, S.sp);
+            k_cg_step<<<H->n_chunks, 256, 0, st>>>(c);
+            LAUNCH_W(k_accel, H->n_chunks, 256, c, c.corr, 1);
+            LAUNCH_W(k_apply, H->n_chunks, 256, c, mode, dt, c.corr, c.wv, 1, 0, 1, 0);
+            k_cg_finish<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
+            k_world<<<c.W, 256, 0, st>>>(c, W_CG_BETA, S.sp);
+            k_cg_direction<<<H->n_chunks, 256, 0, st>>>(c);
+        } else {
+            LAUNCH_W(k_accel, H->n_chunks, 256, c, c.kappa, 0);
+            LAUNCH_W(k_jacobi, H->n_chunks, 256, c, mode, dt);
+            k_world<<<c.W, 256, 0, st>>>(c, W_JACOBI, S.sp);
+        }
+    }
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+// contact limit, apply pressure, lone friction; stats (solve_driver.py:151-162)
+int enq_solve_tail(Handle* H, const SolveSetup& S, hipStream_t st) {
+    const Ctx& c = S.c;
+    {
+        PROF(KID_POST);
+        LAUNCH_W(k_post_solve, blocks_for(c.n), 256, c, S.mode, S.dt, S.store);
+    }
+    k_record<<<blocks_for(c.W), 256, 0, st>>>(c, S.mode, float(S.tolh));
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+// A captured segment of the step: enqueued once into a hipGraph on the handle's private capture stream, then
+// replayed on the caller's stream. Kernel arguments (state pointers, dt, iteration batch) are part of the key.
+template <class F>
+int run_segment(Handle* H, hipStream_t st, int kind, int mode, float dt, F enqueue) {
+    if (!H->use_graphs || H->prof.on) return enqueue(st);
+    uint32_t bits;
+    std::memcpy(&bits, &dt, 4);
+    GraphKey key{kind, mode, H->check_every, bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den};
+    auto it = H->graphs.find(key);
+    if (it == H->graphs.end()) {
+        if (!H->cap_stream) HIPCHK(hipStreamCreateWithFlags(&H->cap_stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamBeginCapture(H->cap_stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue(H->cap_stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(H->cap_stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(DFSPH_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+        hipGraphExec_t ex = nullptr;
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(DFSPH_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+        it = H->graphs.emplace(key, ex).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, st));
+    return DFSPH_OK;
+}
+
+enum SegKind { SEG_PRE = 0, SEG_HEAD, SEG_BATCH, SEG_TAIL_DIV, SEG_TAIL_DEN, SEG_NONPRESSURE };
+
+// one pressure solve with the device-side convergence loop; `tail_extra` runs after the solve's own tail (in the
+// same captured segment)
+template <class Extra>
+int run_solve(Handle* H, const SolveSetup& S, hipStream_t st, int tail_kind, Extra tail_extra) {
+    int rc;
+    if ((rc = run_segment(H, st, SEG_HEAD, S.mode, S.dt, [&](hipStream_t s) { return enq_solve_head(H, S, s); })))
+        return rc;
+    if ((rc = post_unconv(H, st, 0))) return rc;
+    int nu = 0;
+    for (int batch = 0;; ++batch) {
+        if ((rc = run_segment(H, st, SEG_BATCH, S.mode, S.dt,
+                              [&](hipStream_t s) { return enq_solve_batch(H, S, s); })))
+            return rc;
+        if ((rc = post_unconv(H, st, (batch + 1) & 1))) return rc;
+        if ((rc = wait_unconv(H, batch & 1, &nu))) return rc;
+        if (nu == 0) break;
+        if (batch > 100000) return fail(DFSPH_EDEVICE, "pressure solve did not terminate");
+    }
+    return run_segment(H, st, tail_kind, S.mode, S.dt, [&](hipStream_t s) {
+        int r = enq_solve_tail(H, S, s);
+        return r ? r : tail_extra(s);
+    });
+}
+
+int phase_solve(Handle* H, int mode, double dt_h, int fixed_iters, hipStream_t st, const SolveOverride& ov = {}) {
+    if (H->c.n == 0) return DFSPH_OK;
+    SolveSetup S = solve_setup(H, mode, dt_h, fixed_iters, ov);
+    bool graphs = H->use_graphs;
+    H->use_graphs = false;  // phase entry points and prime run eagerly
+    int rc = run_solve(H, S, st, mode == 0 ? SEG_TAIL_DIV : SEG_TAIL_DEN, [](hipStream_t) { return 0; });
+    H->use_graphs = graphs;
+    return rc;
 }
 
 int phase_nonpressure(Handle* H, double dt, hipStream_t st) {
@@ -1969,6 +2084,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
             TRY(dalloc(H, p, n));
         TRY(dalloc(H, &c.acc4, n));
         TRY(dalloc(H, &c.q4, n));
+        TRY(dalloc(H, &c.q4s, n));
         TRY(dalloc(H, &c.vtmp, 3 * int64_t(n)));
         TRY(dalloc(H, &c.act, n));
         for (int** p : {&c.converged, &c.iters, &c.restart, &c.st_overflow, &c.st_projected, &c.st_swept,
@@ -1988,11 +2104,16 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
             HIPCHK(hipMemcpy(c.tol_scale, ones.data(), W * sizeof(float), hipMemcpyHostToDevice));
         }
         void* pinned = nullptr;
-        if (hipHostMalloc(&pinned, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc(&pinned, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
             rc = fail(DFSPH_EDEVICE, "hipHostMalloc failed");
             goto fail;
         }
         H->h_unconv = static_cast<int*>(pinned);
+        for (int k = 0; k < 2; ++k)
+            if (hipEventCreateWithFlags(&H->ev_unconv[k], hipEventDisableTiming) != hipSuccess) {
+                rc = fail(DFSPH_EDEVICE, "hipEventCreate failed");
+                goto fail;
+            }
     }
     *out = H;
     return DFSPH_OK;
@@ -2009,8 +2130,12 @@ int32_t dfsph_destroy(void* h) {
     (void)hipDeviceSynchronize();
     for (auto* v : {&H->prof.a, &H->prof.b})
         for (hipEvent_t e : *v) (void)hipEventDestroy(e);
+    for (auto& kv : H->graphs) (void)hipGraphExecDestroy(kv.second);
+    if (H->cap_stream) (void)hipStreamDestroy(H->cap_stream);
     for (void* p : H->owned) (void)hipFree(p);
     if (H->h_unconv) (void)hipHostFree(H->h_unconv);
+    for (hipEvent_t e : H->ev_unconv)
+        if (e) (void)hipEventDestroy(e);
     delete H;
     return DFSPH_OK;
 }
@@ -2075,15 +2200,32 @@ int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void
     Handle* H = as_handle(h);
     hipStream_t st = static_cast<hipStream_t>(stream);
     bind_state(H, q, qd, f);
-    int rc;
-    if ((rc = phase_neighbors(H, st))) return rc;
-    if ((rc = phase_slots(H, st))) return rc;
-    if ((rc = phase_density(H, st))) return rc;
-    if ((rc = phase_flow(H, st))) return rc;
-    if (H->P.divergence_solve && (rc = phase_solve(H, 0, dt, -1, st))) return rc;
-    if ((rc = phase_nonpressure(H, dt, st))) return rc;
-    if ((rc = phase_solve(H, 1, dt, -1, st))) return rc;
-    return phase_advect(H, dt, st);
+    if (H->c.n == 0) return DFSPH_OK;
+    float dtf = float(dt);
+    int rc = run_segment(H, st, SEG_PRE, 0, dtf, [&](hipStream_t s) {
+        int r;
+        if ((r = phase_neighbors(H, s))) return r;
+        if ((r = phase_slots(H, s))) return r;
+        if ((r = phase_density(H, s))) return r;
+        return phase_flow(H, s);
+    });
+    if (rc) return rc;
+    if (H->P.divergence_solve) {
+        SolveSetup S0 = solve_setup(H, 0, dt, -1, SolveOverride{});
+        rc = run_solve(H, S0, st, SEG_TAIL_DIV, [&](hipStream_t s) { return phase_nonpressure(H, dt, s); });
+    } else {
+        rc = run_segment(H, st, SEG_NONPRESSURE, 0, dtf, [&](hipStream_t s) { return phase_nonpressure(H, dt, s); });
+    }
+    if (rc) return rc;
+    SolveSetup S1 = solve_setup(H, 1, dt, -1, SolveOverride{});
+    return run_solve(H, S1, st, SEG_TAIL_DEN, [&](hipStream_t s) { return phase_advect(H, dt, s); });
+}
+
+// capture the step's segments in hipGraphs (default on; off runs every launch eagerly)
+int32_t dfsph_set_graphs(void* h, int32_t on) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    as_handle(h)->use_graphs = on != 0;
+    return DFSPH_OK;
 }
 
 // prime.py:53-115 (guess: k = p / rho0 per particle, device)

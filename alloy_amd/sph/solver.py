@@ -211,6 +211,10 @@ class SolverDFSPH:
         """Iterations between host convergence checks of the pressure solves (autotuned; results unchanged)."""
         self._call(self.lib.dfsph_set_check_interval, int(every))
 
+    def set_graphs(self, on: bool) -> None:
+        """Replay the step's segments as captured hipGraphs (default) or launch every kernel eagerly."""
+        self._call(self.lib.dfsph_set_graphs, int(bool(on)))
+
     def set_warm_start(self, kappa_div: torch.Tensor | None, kappa_den: torch.Tensor | None) -> None:
         """Learned warm start: initial kappa [n] of each solve (None: the reference rule). Tensors must stay alive
         while set."""

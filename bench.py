@@ -102,6 +102,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="dam_break_125k", choices=sorted(WORKLOADS))
     ap.add_argument("--check-every", type=int, default=1, help="iterations between host convergence checks")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel eagerly")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
@@ -120,6 +121,7 @@ def main() -> int:
     sc = scenes.dam_break(wl["n_side"], wl["box"], wl["seed"], device="cuda")
     solver = sc.make_solver()
     solver.set_check_interval(args.check_every)
+    solver.set_graphs(not args.no_graphs)
     state = sc.model.state()
     n_fluid = sc.fluid_counts[0]
     n_all = sc.model.particle_count
@@ -129,11 +131,9 @@ def main() -> int:
     torch.cuda.synchronize()
     _log("[bench] warm-up done")
     lib, h = solver.lib, solver._h
-    _abi.check(lib.dfsph_set_profiling(h, 1), lib)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     den, it_den, it_div, unconv = [], [], [], 0
-    counts_sum, slots_sum, samples = 0.0, 0.0, 0
     torch.cuda.synchronize()
     e0.record(stream)
     for k in range(args.steps):
@@ -148,6 +148,13 @@ def main() -> int:
     torch.cuda.synchronize()
     total_ms = e0.elapsed_time(e1)
     _log(f"[bench] timed {args.steps} steps in {total_ms:.1f} ms")
+    # per-kernel durations: HIP events around every launch of the main kernels on the launch stream, over a second
+    # window of the same run (event records need eager launches; the kernels are the same)
+    prof_steps = max(min(args.steps, 50), 1)
+    _abi.check(lib.dfsph_set_profiling(h, 1), lib)
+    for _ in range(prof_steps):
+        solver.step(state, state, None, None, dt)
+    torch.cuda.synchronize()
     kstats = {}
     for kid, name in enumerate(_abi.KERNEL_NAMES):
         c, t = ctypes.c_int32(), ctypes.c_double()
@@ -164,6 +171,7 @@ def main() -> int:
     bytes_launch = algorithmic_bytes(dom, n_fluid, n_all - n_fluid, sum_count, n_slots)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     value = n_fluid * args.steps / (total_ms * 1e-3)
+    steps_total = args.steps + prof_steps
     out = {
         "metric": "particle-steps/s at 0.1% density error, learned+batched vs plain DFSPH, 1 GPU",
         "value": value,
@@ -190,8 +198,9 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(dom, args.workload),
                      "alg_bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms, "launches": nl,
                      "mean_neighbors": sum_count / max(n_fluid, 1), "filled_slots": n_slots},
-        "kernel_ms_total": {k: round(v[1], 3) for k, v in kstats.items()},
-        "kernel_launches": {k: v[0] for k, v in kstats.items()},
+        "kernel_ms_per_step": {k: round(v[1] / prof_steps, 4) for k, v in kstats.items()},
+        "kernel_launches_per_step": {k: round(v[0] / prof_steps, 2) for k, v in kstats.items()},
+        "profiled_steps": prof_steps,
     }
     if not args.no_fp64:
         torch.cuda.synchronize()

@@ -206,6 +206,9 @@ int32_t dfsph_set_warm_start_model(void* handle, const void* weights, void* stre
 int32_t dfsph_set_warm_start_kappa(void* handle, const float* kappa_div, const float* kappa_den);
 /* Host convergence check every `every` iterations (1..64); converged worlds' blocks exit at once in between. */
 int32_t dfsph_set_check_interval(void* handle, int32_t every);
+/* dfsph_step replays its segments (pre-solve, solve head, iteration batch, solve tail) as hipGraphs captured once
+ * per (dt, state pointers, batch size); 0 launches every kernel eagerly. Results are identical either way. */
+int32_t dfsph_set_graphs(void* handle, int32_t on);
 
 /* Per-kernel HIP-event timing on the launch stream: kernel ids 0 pressure acceleration, 1 operator apply,
  * 2 neighbour rows, 3 density/alpha, 4 boundary slots, 5 non-pressure, 6 advect/project, 7 solve init,
