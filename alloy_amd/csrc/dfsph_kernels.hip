@@ -334,7 +334,7 @@ __device__ __forceinline__ int cell_of(const Ctx& c, int g, f3 x, int& cx, int& 
     cx = cell_coord(x.x, c.g_org[3 * w], c.cell, d[0]);
     cy = cell_coord(x.y, c.g_org[3 * w + 1], c.cell, d[1]);
     cz = cell_coord(x.z, c.g_org[3 * w + 2], c.cell, d[2]);
-    return c.cell_base[g] + (cz * d[1] + cy) * d[0] + cx;
+    return c.cell_base[g] + (cx * d[1] + cy) * d[2] + cz;  // z fastest: z-rows of cells are contiguous
 }
 
 // neighbors.py:36-51 group per particle, then its cell key and the cell histogram
@@ -434,19 +434,23 @@ __global__ void __launch_bounds__(256) k_scatter(Ctx c) {
     c.q4s[pos] = make_float4(p.x, p.y, p.z, __int_as_float(i));
 }
 
-// wall cells: ascending ids within each cell (the fixed visit order of the colour sums)
+// ascending ids within each cell: the fixed visit order of the colour sums (wall cells) and a nearly sorted
+// arrival order for the list build (fluid cells)
 __global__ void __launch_bounds__(256) k_sort_cells(Ctx c, int first, int last) {
     int cc = first + blockIdx.x * blockDim.x + threadIdx.x;
     if (cc >= last) return;
     int b = c.cell_start[cc], e = c.cell_start[cc + 1];
     for (int a = b + 1; a < e; ++a) {
         int v = c.items[a];
+        float4 pv = c.q4s[a];
         int k = a - 1;
         while (k >= b && c.items[k] > v) {
             c.items[k + 1] = c.items[k];
+            c.q4s[k + 1] = c.q4s[k];
             --k;
         }
         c.items[k + 1] = v;
+        c.q4s[k + 1] = pv;
     }
 }
 
@@ -471,16 +475,18 @@ __global__ void __launch_bounds__(BUILD_BLOCK) k_build_lists(Ctx c) {
     cell_of(c, g, xi, cx, cy, cz);
     const int* d = c.g_dims + 3 * g;
     int nn = 0, total = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        int Z = cz + dz;
-        if (Z < 0 || Z >= d[2]) continue;
+    // x outer, y, then the contiguous z-row of 3 cells, ids ascending within a cell: for lattice-ordered ids
+    // (x slowest) candidates arrive nearly sorted and most insertions append. Candidates stream from q4s (cell
+    // order) in batches of 8.
+    for (int dx = -1; dx <= 1; ++dx) {
+        int X = cx + dx;
+        if (X < 0 || X >= d[0]) continue;
         for (int dy = -1; dy <= 1; ++dy) {
             int Y = cy + dy;
             if (Y < 0 || Y >= d[1]) continue;
-            int x0 = max(cx - 1, 0), x1 = min(cx + 1, d[0] - 1);
-            int cb = c.cell_base[g] + (Z * d[1] + Y) * d[0];
-            // cells x0..x1 of this row are contiguous in cell order; candidates stream from q4s in batches of 8
-            int e0 = c.cell_start[cb + x0], e1 = c.cell_start[cb + x1 + 1];
+            int z0 = max(cz - 1, 0), z1 = min(cz + 1, d[2] - 1);
+            int cb = c.cell_base[g] + (X * d[1] + Y) * d[2];
+            int e0 = c.cell_start[cb + z0], e1 = c.cell_start[cb + z1 + 1];
             for (int e = e0; e < e1; e += 8) {
                 float4 pc[8];
 #pragma unroll
@@ -535,7 +541,7 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv)
             for (int dx = -1; dx <= 1; ++dx) {
                 int X = cx + dx, Y = cy + dy, Z = cz + dz;
                 if (X < 0 || Y < 0 || Z < 0 || X >= d[0] || Y >= d[1] || Z >= d[2]) continue;
-                int cc = c.cell_base[g] + (Z * d[1] + Y) * d[0] + X;
+                int cc = c.cell_base[g] + (X * d[1] + Y) * d[2] + Z;
                 for (int e = c.cell_start[cc]; e < c.cell_start[cc + 1]; ++e) {
                     int j = c.items[e];
                     f3 dd = x - xyz(c.q4[j]);
@@ -1671,7 +1677,7 @@ int phase_neighbors(Handle* H, hipStream_t st) {
     k_scan_top<<<1, 256, 0, st>>>(c.scan_part, H->n_scan_blocks, c.cell_start, c.n_cells);
     k_scan_apply<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part, c.cell_start, c.cell_fill);
     k_scatter<<<blocks_for(c.n), 256, 0, st>>>(c);
-    if (c.NW > 0) k_sort_cells<<<blocks_for(c.n_cells - H->wall_first), 256, 0, st>>>(c, H->wall_first, c.n_cells);
+    k_sort_cells<<<blocks_for(c.n_cells), 256, 0, st>>>(c, 0, c.n_cells);  // ascending ids in every cell
     {
         PROF(KID_BUILD);
         k_build_lists<<<blocks_for(c.n, BUILD_BLOCK), BUILD_BLOCK, 0, st>>>(c);

@@ -472,7 +472,7 @@ struct Oracle {
         cx = cell_coord(x.x, g_org[3 * w], d[0]);
         cy = cell_coord(x.y, g_org[3 * w + 1], d[1]);
         cz = cell_coord(x.z, g_org[3 * w + 2], d[2]);
-        return cell_base[g] + (cz * d[1] + cy) * d[0] + cx;
+        return cell_base[g] + (cx * d[1] + cy) * d[2] + cz;
     }
 
     // neighbors.py:36-51
@@ -524,7 +524,7 @@ struct Oracle {
                     for (int dx = -1; dx <= 1; ++dx) {
                         int x = cx + dx, y = cy + dy, zz = cz + dz;
                         if (x < 0 || y < 0 || zz < 0 || x >= d[0] || y >= d[1] || zz >= d[2]) continue;
-                        int c = cell_base[g] + (zz * d[1] + y) * d[0] + x;
+                        int c = cell_base[g] + (x * d[1] + y) * d[2] + zz;
                         for (int e = cell_start[c]; e < cell_start[c + 1]; ++e) {
                             int j = cell_items[e];
                             if (j == i) continue;
@@ -561,7 +561,7 @@ struct Oracle {
                 for (int dx = -1; dx <= 1; ++dx) {
                     int X = cx + dx, Y = cy + dy, Z = cz + dz;
                     if (X < 0 || Y < 0 || Z < 0 || X >= d[0] || Y >= d[1] || Z >= d[2]) continue;
-                    int cc = cell_base[g] + (Z * d[1] + Y) * d[0] + X;
+                    int cc = cell_base[g] + (X * d[1] + Y) * d[2] + Z;
                     for (int e = cell_start[cc]; e < cell_start[cc + 1]; ++e) {
                         int j = cell_items[e];
                         V3<T> dd = x - qv(j);
