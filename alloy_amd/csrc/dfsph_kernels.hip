@@ -28,6 +28,8 @@
 
 using namespace dfsph;
 
+void dfsph_launch_mlp_forward(const float* params, const float* X, int n, float* Y, hipStream_t st);
+
 namespace {
 
 thread_local std::string g_err;
@@ -99,6 +101,11 @@ struct Ctx {
     float4* q4;    // (x, y, z, mass) per particle, refreshed from q at every neighbour build: one 16 B gather
     float4* q4s;   // (x, y, z, id bits) in cell order: the list build streams candidates from it
     float4* acc4;  // pressure acceleration (x, y, z, 0)
+    float* feat;     // [n][16] learned warm-start features (13 used)
+    float* mlp_out;  // [n][2] network outputs
+    float* warm_k;   // [2][n] learned initial kappa of the divergence / density solve
+    float kscale;    // |g| h: the kappa scale of the features and targets [m^2/s^2]
+    float lat_diag;  // interior rest-lattice DFSPH diagonal [1/m^2]
     float *pd, *z, *wv, *ap, *corr;
     int* act;
     // per world
@@ -1581,6 +1588,81 @@ __global__ void __launch_bounds__(256) k_advect_project(Ctx c, float dt) {
     }
 }
 
+// ----------------------------------------------------------------------------------------------- learned warm start
+// Features of a liquid particle at the start of the step (13 of 16 columns): neighbour count / n_full, 100 (r - 1),
+// 100 dt Dr/Dt, asinh(kappa_v,prev / ks), asinh(kappa_d,prev / ks), h grad r (3), sum psi, h |grad r|, alpha D,
+// n_eff / n_full, h grad r . g_hat; ks = |g| h. Non-liquid rows are zero.
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_features(Ctx c, float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    float* F = c.feat + 16 * int64_t(i);
+    if (!is_fluid(c, i)) {
+        for (int k = 0; k < 16; ++k) F[k] = 0.0f;
+        return;
+    }
+    float4 qi = c.q4[i];
+    f3 xi = xyz(qi), vi = V(c, i);
+    float inv_r0 = 1.0f / c.rho0[i];
+    f3 G{0.0f, 0.0f, 0.0f};
+    float rate = 0.0f;
+    int cnt = c.count[i];
+    int base = c.ws[c.world[i]];
+    for_nbrs<WIDE, QV>(
+        c, i, cnt, base, [&](int j) { return QV{c.q4[j], V(c, j)}; },
+        [&](int, QV d) {
+            f3 g = kernel_grad(xi - xyz(d.q), c.h, c.kind);
+            G = G + (d.q.w * inv_r0) * g;
+            rate += (d.q.w * inv_r0) * dot(vi - d.v, g);
+        });
+    float psi = 0.0f;
+    int64_t b4 = 4 * int64_t(i);
+    for (int b = 0; b < 4; ++b)
+        if (c.s_shape[b4 + b] >= 0) {
+            psi += c.s_psi[b4 + b];
+            G = G + ld3(c.s_gpsi, b4 + b);
+            rate += dot(vi - ld3(c.s_vb, b4 + b), ld3(c.s_gpsi, b4 + b));
+        }
+    f3 gw = ld3(c.gravity, c.world[i]);
+    float gl = length(gw);
+    f3 gh = gl > 0.0f ? gw / gl : f3{0.0f, 0.0f, -1.0f};
+    float ks = c.kscale;
+    F[0] = float(cnt) / c.n_full;
+    F[1] = 100.0f * (c.ratio[i] - 1.0f);
+    F[2] = 100.0f * dt * rate;
+    F[3] = asinhf(c.kv[i] / dt / ks);
+    F[4] = asinhf(c.kd[i] / (dt * dt) / ks);
+    F[5] = c.h * G.x;
+    F[6] = c.h * G.y;
+    F[7] = c.h * G.z;
+    F[8] = psi;
+    F[9] = c.h * length(G);
+    F[10] = c.alpha[i] * c.lat_diag;
+    F[11] = c.neff[i] / c.n_full;
+    F[12] = c.h * dot(G, gh);
+    F[13] = F[14] = F[15] = 0.0f;
+}
+
+// network outputs -> initial kappa of both solves, within the reference warm start's clamps (pressure.py:254, 261)
+__global__ void k_warm_from_mlp(Ctx c, float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    float y0 = c.mlp_out[2 * int64_t(i)], y1 = c.mlp_out[2 * int64_t(i) + 1];
+    float kv = c.kscale * sinhf(y0), kd = c.kscale * sinhf(y1);
+    if (!isfinite(kv)) kv = 0.0f;
+    if (!isfinite(kd)) kd = 0.0f;
+    c.warm_k[i] = fmn(kv, 0.5f / dt);
+    c.warm_k[int64_t(c.n) + i] = fmx(fmn(kd, 2.5e-4f / (dt * dt)), 0.0f);
+}
+
+// training targets of the step just taken: asinh(converged kappa / ks) of both solves, [n][2]
+__global__ void k_targets(Ctx c, float dt, float* T) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    T[2 * int64_t(i)] = asinhf(c.kv[i] / dt / c.kscale);
+    T[2 * int64_t(i) + 1] = asinhf(c.kd[i] / (dt * dt) / c.kscale);
+}
+
 // ----------------------------------------------------------------------------------------------- profiling
 // Optional HIP-event timing of the main kernels on the launch stream (bench.py's live roofline figure).
 enum KernelId { KID_ACCEL = 0, KID_APPLY, KID_BUILD, KID_DENSITY, KID_SLOTS, KID_NONPRESSURE, KID_ADVECT,
@@ -1596,10 +1678,10 @@ struct Prof {
 struct GraphKey {
     int kind, mode, every;
     uint32_t dt_bits;
-    const void *q, *qd, *f, *ld, *ln;
+    const void *q, *qd, *f, *ld, *ln, *mlp, *lf, *lt;
     bool operator<(const GraphKey& o) const {
-        return std::tie(kind, mode, every, dt_bits, q, qd, f, ld, ln) <
-               std::tie(o.kind, o.mode, o.every, o.dt_bits, o.q, o.qd, o.f, o.ld, o.ln);
+        return std::tie(kind, mode, every, dt_bits, q, qd, f, ld, ln, mlp, lf, lt) <
+               std::tie(o.kind, o.mode, o.every, o.dt_bits, o.q, o.qd, o.f, o.ld, o.ln, o.mlp, o.lf, o.lt);
     }
 };
 
@@ -1617,7 +1699,10 @@ struct Handle {
     int check_every = 1;
     int wall_first = 0;  // first cell of the wall grids
     Prof prof;
-    bool use_graphs = true;
+    bool use_graphs = false;  // measured: eager launches are faster than graph replay for this step (bench.py --graphs)
+    const float* mlp_params = nullptr;  // learned warm start on when set
+    float* log_feat = nullptr;          // when set, the step copies its features [n][16] here ...
+    float* log_targets = nullptr;       // ... and its converged-kappa targets [n][2] here
     hipStream_t cap_stream = nullptr;
     std::map<GraphKey, hipGraphExec_t> graphs;
 };
@@ -1857,7 +1942,8 @@ int run_segment(Handle* H, hipStream_t st, int kind, int mode, float dt, F enque
     if (!H->use_graphs || H->prof.on) return enqueue(st);
     uint32_t bits;
     std::memcpy(&bits, &dt, 4);
-    GraphKey key{kind, mode, H->check_every, bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den};
+    GraphKey key{kind, mode, H->check_every, bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
+                 H->mlp_params, H->log_feat, H->log_targets};
     auto it = H->graphs.find(key);
     if (it == H->graphs.end()) {
         if (!H->cap_stream) HIPCHK(hipStreamCreateWithFlags(&H->cap_stream, hipStreamNonBlocking));
@@ -2003,6 +2089,16 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
     c.reach = float(P->reach);
     c.eps = float(P->project_eps);
     c.sweep_len = float(P->sweep_len);
+    {
+        double gmax = 0.0;  // |g| of the first world with gravity sets the kappa scale (9.81 when none)
+        std::vector<float> gh(3 * size_t(W));
+        if (hipMemcpy(gh.data(), M->gravity, gh.size() * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess)
+            for (int w = 0; w < W && gmax == 0.0; ++w)
+                gmax = std::sqrt(double(gh[3 * w]) * gh[3 * w] + double(gh[3 * w + 1]) * gh[3 * w + 1] +
+                                 double(gh[3 * w + 2]) * gh[3 * w + 2]);
+        c.kscale = float((gmax > 0.0 ? gmax : 9.81) * P->h);
+        c.lat_diag = float(P->lattice_grad * P->lattice_grad);
+    }
     c.mass = M->mass;
     c.rho0 = M->rest_density;
     c.flags = M->flags;
@@ -2091,6 +2187,9 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         TRY(dalloc(H, &c.acc4, n));
         TRY(dalloc(H, &c.q4, n));
         TRY(dalloc(H, &c.q4s, n));
+        TRY(dalloc(H, &c.feat, 16 * int64_t(n)));
+        TRY(dalloc(H, &c.mlp_out, 2 * int64_t(n)));
+        TRY(dalloc(H, &c.warm_k, 2 * int64_t(n)));
         TRY(dalloc(H, &c.vtmp, 3 * int64_t(n)));
         TRY(dalloc(H, &c.act, n));
         for (int** p : {&c.converged, &c.iters, &c.restart, &c.st_overflow, &c.st_projected, &c.st_swept,
@@ -2213,7 +2312,19 @@ int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void
         if ((r = phase_neighbors(H, s))) return r;
         if ((r = phase_slots(H, s))) return r;
         if ((r = phase_density(H, s))) return r;
-        return phase_flow(H, s);
+        if ((r = phase_flow(H, s))) return r;
+        if (H->mlp_params || H->log_feat) {  // learned warm start: features -> MLP -> initial kappa
+            Ctx& c = H->c;
+            LAUNCH_W(k_features, blocks_for(c.n), 256, c, dtf);
+            if (H->mlp_params) {
+                dfsph_launch_mlp_forward(H->mlp_params, c.feat, c.n, c.mlp_out, s);
+                k_warm_from_mlp<<<blocks_for(c.n), 256, 0, s>>>(c, dtf);
+            }
+            if (H->log_feat)
+                k_copy_f32<<<blocks_for(16 * int64_t(c.n)), 256, 0, s>>>(H->log_feat, c.feat, 16 * int64_t(c.n));
+            LAUNCH_CHECK();
+        }
+        return 0;
     });
     if (rc) return rc;
     if (H->P.divergence_solve) {
@@ -2224,7 +2335,11 @@ int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void
     }
     if (rc) return rc;
     SolveSetup S1 = solve_setup(H, 1, dt, -1, SolveOverride{});
-    return run_solve(H, S1, st, SEG_TAIL_DEN, [&](hipStream_t s) { return phase_advect(H, dt, s); });
+    rc = run_solve(H, S1, st, SEG_TAIL_DEN, [&](hipStream_t s) {
+        if (H->log_targets) k_targets<<<blocks_for(H->c.n), 256, 0, s>>>(H->c, dtf, H->log_targets);
+        return phase_advect(H, dt, s);
+    });
+    return rc;
 }
 
 // capture the step's segments in hipGraphs (default on; off runs every launch eagerly)
@@ -2274,10 +2389,22 @@ int32_t dfsph_prime(void* h, float* q, float* qd, const float* guess, double dt,
 }
 
 int32_t dfsph_set_warm_start_model(void* h, const void* weights, void* stream) {
-    (void)h;
-    (void)weights;
     (void)stream;
-    return fail(DFSPH_EINVAL, "learned warm start: use dfsph_set_warm_start_kappa");
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    Handle* H = as_handle(h);
+    H->mlp_params = static_cast<const float*>(weights);
+    H->learned_div = weights ? H->c.warm_k : nullptr;
+    H->learned_den = weights ? H->c.warm_k + H->c.n : nullptr;
+    return DFSPH_OK;
+}
+
+// training-data logging of dfsph_step: features [n][16] and targets [n][2] (device; NULL stops logging)
+int32_t dfsph_set_logging(void* h, float* features, float* targets) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    Handle* H = as_handle(h);
+    H->log_feat = features;
+    H->log_targets = targets;
+    return DFSPH_OK;
 }
 
 // learned warm start as given kappa fields [n] (device, NULL = the reference rule) for the divergence and the
@@ -2383,6 +2510,7 @@ int32_t dfsph_get_buffer(void* h, int32_t which, void** ptr, int64_t* nb) {
         case DFSPH_BUF_DEN_ERR: B(c.st_den_err, 4 * W)
         case DFSPH_BUF_DIV_ERR: B(c.st_div_err, 4 * W)
         case DFSPH_BUF_SHAPE_METER: B(c.meter, 24 * int64_t(std::max(c.S, 1)))
+        case DFSPH_BUF_FEATURES: B(c.feat, 64 * n)
         default: return fail(DFSPH_EINVAL, "unknown buffer id");
     }
 #undef B

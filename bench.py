@@ -102,7 +102,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="dam_break_125k", choices=sorted(WORKLOADS))
     ap.add_argument("--check-every", type=int, default=1, help="iterations between host convergence checks")
-    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel eagerly")
+    ap.add_argument("--graphs", action="store_true", help="replay the step's segments as hipGraphs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
@@ -121,7 +121,7 @@ def main() -> int:
     sc = scenes.dam_break(wl["n_side"], wl["box"], wl["seed"], device="cuda")
     solver = sc.make_solver()
     solver.set_check_interval(args.check_every)
-    solver.set_graphs(not args.no_graphs)
+    solver.set_graphs(args.graphs)
     state = sc.model.state()
     n_fluid = sc.fluid_counts[0]
     n_all = sc.model.particle_count

@@ -217,6 +217,21 @@ int32_t dfsph_set_graphs(void* handle, int32_t on);
 int32_t dfsph_set_profiling(void* handle, int32_t on);
 int32_t dfsph_profile_read(void* handle, int32_t kernel_id, int32_t* count, double* total_ms);
 
+/* Learned warm start network (13 -> 64 -> 64 -> 64 -> 2, SiLU; float32 master parameters, bf16 on the MFMA units).
+ * No counterpart in the reference: these train and evaluate the model that dfsph_set_warm_start_model installs.
+ * All pointers are device pointers; X [n][16] float32 (13 features used), Y/T [n][2]. */
+int32_t dfsph_mlp_param_count(void);
+int32_t dfsph_mlp_forward(const float* params, const float* X, int32_t n, float* Y, void* stream);
+/* gradient of sum((Y - T)^2) / (2 n_total) into grad [P] and the squared-error sum into grad[P];
+ * scratch: float [ceil(n / 16) * (P + 1)] per-wave partials, reduced in a fixed order (deterministic). */
+int32_t dfsph_mlp_grad(const float* params, const float* X, const float* T, int32_t n, int32_t n_total,
+                       float* scratch, float* grad, void* stream);
+int32_t dfsph_mlp_adam(float* params, const float* grad, float* m, float* v, double lr, double beta1,
+                       double beta2, double eps, int32_t t, void* stream);
+/* Training-data logging of dfsph_step: the step's start-of-step features [n][16] and the converged kappa of its
+ * two solves as targets asinh(kappa / (|g| h)) [n][2] (device pointers; NULL stops logging). */
+int32_t dfsph_set_logging(void* handle, float* features, float* targets);
+
 /* Buffers */
 int32_t dfsph_get_buffer(void* handle, int32_t which, void** ptr, int64_t* n_bytes);
 int32_t dfsph_read_buffer(void* handle, int32_t which, void* host_dst, int64_t n_bytes);   /* syncs */
