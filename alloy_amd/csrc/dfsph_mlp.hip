@@ -338,10 +338,16 @@ int32_t dfsph_mlp_forward(const float* params, const float* X, int32_t n, float*
     return hipGetLastError() == hipSuccess ? DFSPH_OK : DFSPH_EDEVICE;
 }
 
-// scratch: float [ceil(n / 16) * (N_PARAMS + 1)]; grad: float [N_PARAMS + 1] (last: squared-error sum)
+// one partial per wave: 4 waves per 64-sample block, each N_PARAMS gradient floats + its squared-error sum
+int64_t dfsph_mlp_grad_scratch_floats(int32_t n) {
+    return n <= 0 ? 0 : int64_t(4) * ((int64_t(n) + 63) / 64) * PART_STRIDE;
+}
+
+// grad: float [N_PARAMS + 1] (last: squared-error sum)
 int32_t dfsph_mlp_grad(const float* params, const float* X, const float* T, int32_t n, int32_t n_total, float* scratch,
-                       float* grad, void* stream) {
+                       int64_t scratch_floats, float* grad, void* stream) {
     if (!params || !X || !T || !scratch || !grad || n <= 0 || n_total <= 0) return DFSPH_EINVAL;
+    if (scratch_floats < dfsph_mlp_grad_scratch_floats(n)) return DFSPH_EINVAL;
     hipStream_t st = static_cast<hipStream_t>(stream);
     int blocks = (n + 63) / 64;
     k_mlp_grad<<<blocks, 256, 0, st>>>(params, X, T, n, 1.0f / (2.0f * float(n_total)), scratch);

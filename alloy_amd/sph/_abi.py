@@ -302,7 +302,8 @@ def load_library() -> ctypes.CDLL:
         "dfsph_set_graphs": ([vp, i32], i32),
         "dfsph_mlp_param_count": ([], i32),
         "dfsph_mlp_forward": ([vp, vp, i32, vp, vp], i32),
-        "dfsph_mlp_grad": ([vp, vp, vp, i32, i32, vp, vp, vp], i32),
+        "dfsph_mlp_grad_scratch_floats": ([i32], i64),
+        "dfsph_mlp_grad": ([vp, vp, vp, i32, i32, vp, i64, vp, vp], i32),
         "dfsph_mlp_adam": ([vp, vp, vp, vp, dbl, dbl, dbl, dbl, i32, vp], i32),
         "dfsph_set_logging": ([vp, vp, vp], i32),
         "dfsph_get_buffer": ([vp, i32, P(vp), P(i64)], i32),

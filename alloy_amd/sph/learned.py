@@ -64,6 +64,11 @@ class WarmStartMLP:
         parts = [W1, np.zeros(H), W2, np.zeros(H), W3, np.zeros(H), W4, np.zeros(16)]
         return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
 
+    def _scratch(self, n: int) -> torch.Tensor:
+        """Per-wave gradient partials for n samples, sized by the library."""
+        return torch.empty(int(self.lib.dfsph_mlp_grad_scratch_floats(int(n))), dtype=torch.float32,
+                           device=self.device)
+
     def forward(self, X: torch.Tensor) -> torch.Tensor:
         """Y [n, 2] = MLP(X [n, 16])."""
         X = X.contiguous()
@@ -76,10 +81,10 @@ class WarmStartMLP:
     def grad(self, X: torch.Tensor, T: torch.Tensor, n_total: int | None = None) -> tuple[torch.Tensor, float]:
         """(gradient [n_params], squared-error sum) of sum((y - T)^2) / (2 n_total)."""
         n = int(X.shape[0])
-        scratch = torch.empty(((n + 15) // 16) * (self.n_params + 1), dtype=torch.float32, device=self.device)
+        scratch = self._scratch(n)
         g = torch.empty(self.n_params + 1, dtype=torch.float32, device=self.device)
         rc = self.lib.dfsph_mlp_grad(_vp(self.params), _vp(X.contiguous()), _vp(T.contiguous()), n, int(n_total or n),
-                                     _vp(scratch), _vp(g), _stream())
+                                     _vp(scratch), scratch.numel(), _vp(g), _stream())
         if rc != 0:
             raise RuntimeError(f"dfsph_mlp_grad failed ({rc})")
         return g[: self.n_params], float(g[self.n_params].item())
@@ -89,14 +94,15 @@ class WarmStartMLP:
         """Adam on random minibatches; returns the minibatch mean squared errors."""
         gen = torch.Generator(device=self.device).manual_seed(seed)
         n = int(X.shape[0])
-        scratch = torch.empty(((batch + 15) // 16) * (self.n_params + 1), dtype=torch.float32, device=self.device)
+        scratch = self._scratch(batch)
         g = torch.empty(self.n_params + 1, dtype=torch.float32, device=self.device)
         losses = []
         st = _stream()
         for k in range(steps):
             idx = torch.randint(0, n, (batch,), device=self.device, generator=gen)
             xb, tb = X.index_select(0, idx).contiguous(), T.index_select(0, idx).contiguous()
-            rc = self.lib.dfsph_mlp_grad(_vp(self.params), _vp(xb), _vp(tb), batch, batch, _vp(scratch), _vp(g), st)
+            rc = self.lib.dfsph_mlp_grad(_vp(self.params), _vp(xb), _vp(tb), batch, batch, _vp(scratch),
+                                         scratch.numel(), _vp(g), st)
             if rc != 0:
                 raise RuntimeError(f"dfsph_mlp_grad failed ({rc})")
             self.t += 1

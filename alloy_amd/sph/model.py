@@ -105,6 +105,7 @@ class ModelBuilder:
         self.current_world = -1
         self.world_count = 0
         self._sph_registered = False
+        self._count = 0
 
     # ---------------------------------------------------------------------------------------- worlds
     def begin_world(self, gravity: tuple[float, float, float] | None = None) -> int:
@@ -125,7 +126,7 @@ class ModelBuilder:
 
     @property
     def particle_count(self) -> int:
-        return len(self._mass)
+        return self._count
 
     # ---------------------------------------------------------------------------------------- content
     def add_particles(self, pos, vel, mass, radius, flags, rest_density=1000.0, viscosity=1.0e-3,
@@ -134,18 +135,18 @@ class ModelBuilder:
             raise ValueError("ModelBuilder: particles must be added inside a world")
         pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
         n = len(pos)
-        vel = np.broadcast_to(np.asarray(vel, dtype=np.float64), (n, 3))
         first = self.particle_count
-        bc = lambda v: list(np.broadcast_to(np.asarray(v), (n,)))  # noqa: E731
-        self._q.extend(pos.tolist())
-        self._qd.extend(vel.tolist())
-        self._mass.extend(bc(mass))
-        self._radius.extend(bc(radius))
-        self._flags.extend(bc(flags))
-        self._world.extend([self.current_world] * n)
-        self._rho0.extend(bc(rest_density))
-        self._mu.extend(bc(viscosity))
-        self._phase.extend(bc(phase))
+        bc = lambda v, dt: np.broadcast_to(np.asarray(v, dtype=dt), (n,)).copy()  # noqa: E731
+        self._q.append(pos)
+        self._qd.append(np.broadcast_to(np.asarray(vel, dtype=np.float64), (n, 3)).copy())
+        self._mass.append(bc(mass, np.float64))
+        self._radius.append(bc(radius, np.float64))
+        self._flags.append(bc(flags, np.int32))
+        self._world.append(np.full(n, self.current_world, np.int32))
+        self._rho0.append(bc(rest_density, np.float64))
+        self._mu.append(bc(viscosity, np.float64))
+        self._phase.append(bc(phase, np.int8))
+        self._count += n
         return range(first, first + n)
 
     def add_shape(self, geo: int, center, half, quat=(0.0, 0.0, 0.0, 1.0), body: int = -1, mu: float = 0.5,
@@ -172,14 +173,13 @@ class ModelBuilder:
         dev = torch.device(device)
         n = self.particle_count
         W = max(self.world_count, 1)
-        world = np.asarray(self._world, dtype=np.int32)
+        cat = lambda parts, dt, shape=(0,): np.concatenate(parts).astype(dt) if parts else np.zeros(shape, dt)  # noqa: E731
+        world = cat(self._world, np.int32)
         if n and np.any(np.diff(world) < 0):
             raise ValueError("particles must be contiguous per world")
         ws = np.zeros(W + 1, dtype=np.int64)
-        for w in range(W):
-            ws[w + 1] = ws[w] + int(np.count_nonzero(world == w))
-        f32 = lambda a, shp=None: torch.tensor(np.asarray(a, dtype=np.float32).reshape(shp) if shp else  # noqa: E731
-                                               np.asarray(a, dtype=np.float32), device=dev)
+        ws[1:] = np.cumsum(np.bincount(world, minlength=W)[:W])
+        f32 = lambda a: torch.tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev)  # noqa: E731
         grav = np.asarray(self._world_gravity or [self.gravity], dtype=np.float32).reshape(-1, 3)
         S = len(self._shapes)
         sh = self._shapes
@@ -191,16 +191,16 @@ class ModelBuilder:
             device=dev,
             particle_count=n,
             world_count=W,
-            particle_q=f32(self._q if n else np.zeros((0, 3)), (-1, 3)),
-            particle_qd=f32(self._qd if n else np.zeros((0, 3)), (-1, 3)),
-            particle_mass=f32(self._mass),
-            particle_radius=f32(self._radius),
-            particle_flags=torch.tensor(np.asarray(self._flags, np.int32), device=dev),
+            particle_q=f32(cat(self._q, np.float64, (0, 3)).reshape(-1, 3)),
+            particle_qd=f32(cat(self._qd, np.float64, (0, 3)).reshape(-1, 3)),
+            particle_mass=f32(cat(self._mass, np.float64)),
+            particle_radius=f32(cat(self._radius, np.float64)),
+            particle_flags=torch.tensor(cat(self._flags, np.int32), device=dev),
             particle_world=torch.tensor(world, device=dev),
             particle_world_start=torch.tensor(ws.astype(np.int32), device=dev),
             gravity=torch.tensor(grav, device=dev),
-            sph=SPHAttributes(rest_density=f32(self._rho0), viscosity=f32(self._mu),
-                              phase=torch.tensor(np.asarray(self._phase, np.int8), device=dev)),
+            sph=SPHAttributes(rest_density=f32(cat(self._rho0, np.float64)), viscosity=f32(cat(self._mu, np.float64)),
+                              phase=torch.tensor(cat(self._phase, np.int8), device=dev)),
             shape_count=S,
             shape_type=np.asarray([d["geo"] for d in sh], np.int32),
             shape_scale=np.asarray([d["half"] for d in sh], np.float32).reshape(S, 3),

@@ -31,10 +31,30 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
-    "dam_break_125k": dict(n_side=50, box=1.0, seed=1),
-    "dam_break_8k": dict(n_side=20, box=0.4, seed=0),
-    "dam_break_1m": dict(n_side=100, box=2.0, seed=3),
+    "dam_break_125k": dict(kind="dam_break", n_side=50, box=1.0, seed=1),
+    "dam_break_8k": dict(kind="dam_break", n_side=20, box=0.4, seed=0),
+    "dam_break_1m": dict(kind="dam_break", n_side=100, box=2.0, seed=3),
+    "bowl_worlds_512": dict(kind="bowl_worlds", worlds=512, seed=2),
 }
+
+
+def make_scene(wl: dict, device: str):
+    from alloy_amd.sph import scenes
+
+    if wl["kind"] == "dam_break":
+        return scenes.dam_break(wl["n_side"], wl["box"], wl["seed"], device=device)
+    return scenes.bowl_worlds(wl["worlds"], seed=wl["seed"], device=device)
+
+
+def describe(wl: dict, sc) -> str:
+    n = sum(sc.fluid_counts)
+    if wl["kind"] == "dam_break":
+        return (f"plain DFSPH dam break, {n} particles ({wl['n_side']}^3 lattice, spacing 0.01 m, jitter 1e-4 m, seed "
+                f"{wl['seed']}) in a {wl['box']} m SDF box tank, dt 1 ms, 0.1 % density and divergence tolerance, "
+                "projected CG, 1 world")
+    return (f"batched DFSPH, {wl['worlds']} worlds in one pack (box tanks of side U(0.3, 0.6) m, a static particle-wall "
+            f"bowl of radius 0.08 m each, U(2000, 20000) liquid particles each, seed {wl['seed']}): {n} liquid "
+            f"particles, {sc.model.particle_count - n} wall particles, dt 1 ms, 0.1 % tolerances, projected CG")
 
 
 def _log(msg: str) -> None:
@@ -59,7 +79,7 @@ def cpu_baseline(workload: dict, steps: int, warmup: int) -> dict:
     from alloy_amd.sph import scenes
     from oracle.oracle import OracleDFSPH
 
-    sc = scenes.dam_break(workload["n_side"], workload["box"], workload["seed"])
+    sc = make_scene(workload, "cpu")
     o = OracleDFSPH(sc.model, sc.config, 32)
     for _ in range(warmup):
         o.step(scenes.DT)
@@ -67,7 +87,7 @@ def cpu_baseline(workload: dict, steps: int, warmup: int) -> dict:
     for _ in range(steps):
         o.step(scenes.DT)
     dt = time.perf_counter() - t
-    n = sc.fluid_counts[0]
+    n = sum(sc.fluid_counts)
     return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
             "sample": f"CPU oracle (float32, serial C++ -O3) on the same scene: {n} particles, {warmup} warm-up + "
                       f"{steps} timed steps ({dt:.1f} s)"}
@@ -118,12 +138,13 @@ def main() -> int:
 
     wl = WORKLOADS[args.workload]
     _log(f"[bench] workload {args.workload} {wl} steps={args.steps} warmup={args.warmup}")
-    sc = scenes.dam_break(wl["n_side"], wl["box"], wl["seed"], device="cuda")
+    sc = make_scene(wl, "cuda")
     solver = sc.make_solver()
     solver.set_check_interval(args.check_every)
     solver.set_graphs(args.graphs)
     state = sc.model.state()
-    n_fluid = sc.fluid_counts[0]
+    n_fluid = sum(sc.fluid_counts)
+    W = sc.model.world_count
     n_all = sc.model.particle_count
     dt = scenes.DT
     for _ in range(args.warmup):
@@ -140,10 +161,10 @@ def main() -> int:
         solver.step(state, state, None, None, dt)
         if k % 10 == 9:
             r = solver.report()
-            den.append(float(r.density_err[0]))
-            it_den.append(int(r.iters_density[0]))
-            it_div.append(int(r.iters_divergence[0]))
-            unconv += int(r.unconverged[0])
+            den.append(float(np.mean(r.density_err)))
+            it_den.append(float(np.mean(r.iters_density)))
+            it_div.append(float(np.mean(r.iters_divergence)))
+            unconv += int(np.sum(r.unconverged))
     e1.record(stream)
     torch.cuda.synchronize()
     total_ms = e0.elapsed_time(e1)
@@ -185,10 +206,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"plain DFSPH dam break, {n_fluid} particles ({wl['n_side']}^3 lattice, spacing "
-                               f"0.01 m, jitter 1e-4 m, seed {wl['seed']}) in a {wl['box']} m SDF box tank, "
-                               "dt 1 ms, 0.1 % density and divergence tolerance, projected CG, 1 world",
-                   "particles": n_fluid, "worlds": 1, "solver": "plain (single pack, no learned warm start)"},
+        "config": {"workload": describe(wl, sc), "particles": n_fluid, "worlds": W,
+                   "solver": "plain (no learned warm start)"},
         "density_err_mean": float(np.mean(den)) if den else None,
         "density_err_max_sampled": float(np.max(den)) if den else None,
         "iters_density_mean": float(np.mean(it_den)) if it_den else None,

@@ -222,10 +222,14 @@ int32_t dfsph_profile_read(void* handle, int32_t kernel_id, int32_t* count, doub
  * All pointers are device pointers; X [n][16] float32 (13 features used), Y/T [n][2]. */
 int32_t dfsph_mlp_param_count(void);
 int32_t dfsph_mlp_forward(const float* params, const float* X, int32_t n, float* Y, void* stream);
+/* Floats of scratch dfsph_mlp_grad needs for n samples: one (P + 1)-float partial per 16-sample wave of each
+ * 64-sample block, 4 * ceil(n / 64) * (P + 1). */
+int64_t dfsph_mlp_grad_scratch_floats(int32_t n);
 /* gradient of sum((Y - T)^2) / (2 n_total) into grad [P] and the squared-error sum into grad[P];
- * scratch: float [ceil(n / 16) * (P + 1)] per-wave partials, reduced in a fixed order (deterministic). */
+ * scratch: scratch_floats >= dfsph_mlp_grad_scratch_floats(n) floats of per-wave partials, reduced in a fixed
+ * order (deterministic); a smaller scratch is rejected with DFSPH_EINVAL. */
 int32_t dfsph_mlp_grad(const float* params, const float* X, const float* T, int32_t n, int32_t n_total,
-                       float* scratch, float* grad, void* stream);
+                       float* scratch, int64_t scratch_floats, float* grad, void* stream);
 int32_t dfsph_mlp_adam(float* params, const float* grad, float* m, float* v, double lr, double beta1,
                        double beta2, double eps, int32_t t, void* stream);
 /* Training-data logging of dfsph_step: the step's start-of-step features [n][16] and the converged kappa of its

@@ -72,3 +72,17 @@ def test_errors_map_to_python_exceptions():
     assert rc == _abi.DFSPH_EINVAL
     with pytest.raises(ValueError, match="null argument"):
         _abi.check(rc, lib, "create")
+
+
+def test_mlp_grad_scratch_size_and_rejection():
+    """The gradient's scratch covers one partial per 16-sample wave of every 64-sample block, and a smaller scratch
+    is refused before anything is launched (the pointers below are never dereferenced)."""
+    lib = _abi.load_library()
+    P = lib.dfsph_mlp_param_count()
+    for n, waves in [(1, 4), (64, 4), (65, 8), (300, 20), (4096, 256)]:
+        assert lib.dfsph_mlp_grad_scratch_floats(n) == waves * (P + 1)
+    assert lib.dfsph_mlp_grad_scratch_floats(0) == 0
+    fake = ctypes.c_void_p(0x1000)
+    need = lib.dfsph_mlp_grad_scratch_floats(300)
+    rc = lib.dfsph_mlp_grad(fake, fake, fake, 300, 300, fake, need - 1, fake, None)
+    assert rc == _abi.DFSPH_EINVAL
