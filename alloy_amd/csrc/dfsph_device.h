@@ -46,6 +46,57 @@ __device__ __forceinline__ void st3(float* p, int64_t k, f3 v) {
 
 constexpr float PI_F = 3.14159265358979323846f;
 
+// The smoothing kernel's launch constants, built once per thread from the same float expressions the per-call
+// forms below evaluate (h^3, the kind's normalisation, the 1e-9 h cut-off), so both give identical bits.
+struct KC {
+    float h, rmin, knorm;
+    int kind;
+};
+__device__ __forceinline__ KC make_kc(float h, int kind) {
+    float h3 = h * h * h;
+    KC K;
+    K.h = h;
+    K.kind = kind;
+    K.rmin = 1.0e-9f * h;
+    K.knorm = kind == 1 ? (21.0f / (2.0f * PI_F * h3)) : 8.0f / (PI_F * h3);
+    return K;
+}
+
+// kernels.py:41-55
+__device__ __forceinline__ float kernel_w(float r, const KC& K) {
+    float q = r / K.h;
+    if (q >= 1.0f) return 0.0f;
+    if (K.kind == 1) {
+        float s = 1.0f - q;
+        return K.knorm * s * s * s * s * (1.0f + 4.0f * q);
+    }
+    if (q <= 0.5f) return K.knorm * (6.0f * q * q * q - 6.0f * q * q + 1.0f);
+    float s = 1.0f - q;
+    return K.knorm * 2.0f * s * s * s;
+}
+
+// kernels.py:58-77 as a scale: grad W(rv) = rv * dW/dq / (r h), with 0 standing for the zero vector of the cut-off
+// cases (q >= 1 or r <= 1e-9 h). dW/dq vanishes inside (0, 1) nowhere, so a zero scale occurs only there.
+__device__ __forceinline__ float kernel_grad_scale(f3 rv, const KC& K) {
+    float r = length(rv);
+    float q = r / K.h;
+    if (q >= 1.0f || r <= K.rmin) return 0.0f;
+    float dwdq;
+    if (K.kind == 1) {
+        float s = 1.0f - q;
+        dwdq = K.knorm * (-20.0f * q * s * s * s);
+    } else if (q <= 0.5f) {
+        dwdq = K.knorm * (18.0f * q * q - 12.0f * q);
+    } else {
+        float s = 1.0f - q;
+        dwdq = K.knorm * (-6.0f * s * s);
+    }
+    return dwdq / (r * K.h);
+}
+// the gradient from its scale: the same products kernel_grad forms, the exact zero vector for a zero scale
+__device__ __forceinline__ f3 pair_grad(f3 rv, float s) { return s == 0.0f ? f3{0.0f, 0.0f, 0.0f} : rv * s; }
+__device__ __forceinline__ f3 kernel_grad(f3 rv, const KC& K) { return pair_grad(rv, kernel_grad_scale(rv, K)); }
+
 // kernels.py:41-55
 __device__ __forceinline__ float kernel_w(float r, float h, int kind) {
     float q = r / h;

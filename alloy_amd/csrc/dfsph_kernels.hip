@@ -13,6 +13,7 @@
 // partials in fixed order and runs the world's control logic (check_kernel / cg_beta_kernel). The sums are thus
 // bitwise independent of batching and of scheduling.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include <algorithm>
 #include <cmath>
@@ -48,6 +49,7 @@ int fail(int code, const std::string& msg) {
 // Everything a kernel reads: parameters as float32 kernel arguments and device pointers.
 struct Ctx {
     int n, W, S, NW, cap, wide, kind, n_cells, friction, curvature, two_sided;
+    int pair_cache;  // DFSPH_PAIR_CACHE_* bits: which gather passes read pgrad instead of recomputing the scale
     float h, cell, r, n_full, tot_min, min_nb, omega, xsph, v_sep;
     float offset, amplitude, taper0, taper1, reach, eps, sweep_len;
     // model (caller-owned)
@@ -84,6 +86,7 @@ struct Ctx {
     // lists
     uint16_t* nbr;
     uint8_t* nbr_hi;
+    float* pgrad;  // [cap][n] the step's gradient scale kernel_grad_scale(x_i - x_j) of every listed pair
     int* count;
     int* group;
     // slots
@@ -107,6 +110,7 @@ struct Ctx {
     float kscale;    // |g| h: the kappa scale of the features and targets [m^2/s^2]
     float lat_diag;  // interior rest-lattice DFSPH diagonal [1/m^2]
     float *pd, *z, *wv, *ap, *corr;
+    float *kr, *pdr, *corrr;  // kappa / rho0, pd / rho0, corr / rho0: the quotients pressure_accel gathers
     int* act;
     // per world
     int *converged, *iters, *restart;
@@ -144,22 +148,64 @@ __device__ __forceinline__ int nbr_at(const Ctx& c, int base, int i, int k) {
     return j;
 }
 
-// Neighbour loop with staged loads: NB_U row entries, then the NB_U neighbour records (load(j)), then the
-// arithmetic body(j, record) in ascending k, exactly the sequential order. The loads of a batch are in flight
-// together instead of one dependent index -> record round trip per neighbour.
+// Neighbour loops with staged loads: NB_U row entries, then the NB_U neighbour records (load(j)), then the
+// arithmetic in ascending k, exactly the sequential order. The loads of a batch are in flight together instead of
+// one dependent index -> record round trip per neighbour. Row k is addressed as a wave-uniform row base plus the
+// lane's i; slots past the lane's count (still inside the allocated rows) read a stale entry that is replaced by
+// i, so every gathered record is in bounds, and their bodies are skipped.
 constexpr int NB_U = 8;
+template <class T>
+__device__ __forceinline__ float4 load_pos(const T& d) { return d.q; }
+__device__ __forceinline__ float4 load_pos(float4 d) { return d; }
+template <bool WIDE>
+__device__ __forceinline__ void nbr_batch(const Ctx& c, int i, int cnt, int base, int k0, int (&j)[NB_U]) {
+#pragma unroll
+    for (int u = 0; u < NB_U; ++u) {
+        int k = k0 + u, v = i;
+        if (k < c.cap) {
+            int jj = base + int((c.nbr + int64_t(k) * c.n)[i]);
+            if (WIDE) jj += int((c.nbr_hi + int64_t(k) * c.n)[i]) << 16;
+            v = k < cnt ? jj : i;
+        }
+        j[u] = v;
+    }
+}
+// body(k, j, record)
 template <bool WIDE, class T, class LoadF, class BodyF>
 __device__ __forceinline__ void for_nbrs(const Ctx& c, int i, int cnt, int base, LoadF load, BodyF body) {
     for (int k0 = 0; k0 < cnt; k0 += NB_U) {
         int j[NB_U];
-#pragma unroll
-        for (int u = 0; u < NB_U; ++u) j[u] = nbr_at<WIDE>(c, base, i, min(k0 + u, cnt - 1));
+        nbr_batch<WIDE>(c, i, cnt, base, k0, j);
         T d[NB_U];
 #pragma unroll
         for (int u = 0; u < NB_U; ++u) d[u] = load(j[u]);
 #pragma unroll
         for (int u = 0; u < NB_U; ++u)
-            if (k0 + u < cnt) body(j[u], d[u]);
+            if (k0 + u < cnt) body(k0 + u, j[u], d[u]);
+    }
+}
+// body(j, record, grad W_ij): with `cached` the gradient comes from the pair's scale stored by the density pass
+// (pair_grad), otherwise it is evaluated (kernel_grad); both give the same bits, so the choice is a tuning option.
+template <bool WIDE, class T, class LoadF, class BodyF>
+__device__ __forceinline__ void for_nbrs_g(const Ctx& c, int i, f3 xi, int cnt, int base, bool cached, LoadF load,
+                                           BodyF body) {
+    const KC K = make_kc(c.h, c.kind);
+    for (int k0 = 0; k0 < cnt; k0 += NB_U) {
+        int j[NB_U];
+        nbr_batch<WIDE>(c, i, cnt, base, k0, j);
+        T d[NB_U];
+        float sc[NB_U];
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u) {
+            d[u] = load(j[u]);
+            sc[u] = cached && k0 + u < c.cap ? (c.pgrad + int64_t(k0 + u) * c.n)[i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u) {
+            if (k0 + u >= cnt) continue;
+            f3 rv = xi - xyz(load_pos(d[u]));
+            body(j[u], d[u], cached ? pair_grad(rv, sc[u]) : kernel_grad(rv, K));
+        }
     }
 }
 
@@ -168,6 +214,19 @@ __device__ __forceinline__ void for_nbrs(const Ctx& c, int i, int cnt, int base,
         if (c.wide) kernel<true><<<(grid), (block), 0, st>>>(__VA_ARGS__);        \
         else kernel<false><<<(grid), (block), 0, st>>>(__VA_ARGS__);             \
     } while (0)
+
+// ----------------------------------------------------------------------------------------------- XCD mapping
+// The dispatcher hands workgroup b to XCD b % 8. Renumbering so that XCD x runs the contiguous logical range
+// [x q + min(x, r), ...) (G = 8 q + r) keeps the particles whose neighbours a block gathers inside one XCD's L2
+// instead of spreading consecutive chunks over all eight. A bijection of [0, G): results do not depend on it.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_block(int b, int G) {
+    int q = G / N_XCD, r = G % N_XCD, x = b % N_XCD;
+    return x * q + min(x, r) + b / N_XCD;
+}
+__device__ __forceinline__ int xcd_particle() {
+    return xcd_block(int(blockIdx.x), int(gridDim.x)) * int(blockDim.x) + int(threadIdx.x);
+}
 
 // ----------------------------------------------------------------------------------------------- reductions
 // Fixed 256-wide tree: a[t] += a[t + s] for s = 128, 64 (LDS) then 32 .. 1 (wavefront shuffles; the same pairs).
@@ -210,7 +269,7 @@ __device__ __forceinline__ void chunk_reduce(const Ctx& c, double x, double y, d
     double sy = tree_sum(y, lds);
     double mx = tree_max(m, lds);
     if (threadIdx.x == 0) {
-        double* p = c.part + 3 * int64_t(blockIdx.x);
+        double* p = c.part + 3 * int64_t(xcd_block(int(blockIdx.x), int(gridDim.x)));
         p[0] = sx;
         p[1] = sy;
         p[2] = mx;
@@ -462,18 +521,22 @@ __global__ void __launch_bounds__(256) k_sort_cells(Ctx c, int first, int last) 
 }
 
 // neighbors.py:160-195: rows of the lowest `cap` world-local ids within h (fp32 dot(d, d) < h^2), ascending;
-// overflow counted per world. Each thread keeps its row in LDS (column per thread: conflict-free).
+// overflow counted per world. Each thread keeps its row in LDS (column per thread: conflict-free), as uint16
+// world-local ids unless a world exceeds 65,536 particles (WIDE: 32-bit entries), so the narrow build holds
+// twice the waves per CU.
 constexpr int BUILD_BLOCK = 128;
+template <bool WIDE>
 __global__ void __launch_bounds__(BUILD_BLOCK) k_build_lists(Ctx c) {
-    __shared__ int row[64 * BUILD_BLOCK];
-    int i = blockIdx.x * BUILD_BLOCK + threadIdx.x;
+    using RowT = typename std::conditional<WIDE, int, uint16_t>::type;
+    __shared__ RowT row[64 * BUILD_BLOCK];
+    int i = xcd_particle();
     if (i >= c.n) return;
     int g = c.group[i];
     if (g < 0) {
         c.count[i] = 0;
         return;
     }
-    int* L = row + threadIdx.x;
+    RowT* L = row + threadIdx.x;
     const int cap = c.cap;
     int base = c.ws[g];
     f3 xi = xyz(c.q4[i]);
@@ -517,7 +580,7 @@ __global__ void __launch_bounds__(BUILD_BLOCK) k_build_lists(Ctx c) {
                         if (pos >= cap) continue;
                         int lastk = nn >= cap ? cap - 1 : nn;
                         for (k = lastk; k > pos; --k) L[k * BUILD_BLOCK] = L[(k - 1) * BUILD_BLOCK];
-                        L[pos * BUILD_BLOCK] = jl;
+                        L[pos * BUILD_BLOCK] = RowT(jl);
                         if (nn < cap) nn += 1;
                     }
                 }
@@ -527,7 +590,7 @@ __global__ void __launch_bounds__(BUILD_BLOCK) k_build_lists(Ctx c) {
     for (int k = 0; k < nn; ++k) {
         int v = L[k * BUILD_BLOCK];
         c.nbr[int64_t(k) * c.n + i] = uint16_t(v & 65535);
-        if (c.wide) c.nbr_hi[int64_t(k) * c.n + i] = uint8_t(v >> 16);
+        if (WIDE) c.nbr_hi[int64_t(k) * c.n + i] = uint8_t(v >> 16);
     }
     if (total > cap) atomicAdd(c.st_overflow + g, 1);
     c.count[i] = nn;
@@ -536,6 +599,7 @@ __global__ void __launch_bounds__(BUILD_BLOCK) k_build_lists(Ctx c) {
 // ----------------------------------------------------------------------------------------------- particle walls
 // particle_walls.py:65-84: colour, grad colour and mean velocity of world w's wall particles at x
 __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv) {
+    const KC K = make_kc(c.h, c.kind);
     col = 0.0f;
     gc = f3{0.0f, 0.0f, 0.0f};
     mv = f3{0.0f, 0.0f, 0.0f};
@@ -552,10 +616,10 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv)
                 for (int e = c.cell_start[cc]; e < c.cell_start[cc + 1]; ++e) {
                     int j = c.items[e];
                     f3 dd = x - xyz(c.q4[j]);
-                    float wj = c.wvol[j] * kernel_w(length(dd), c.h, c.kind);
+                    float wj = c.wvol[j] * kernel_w(length(dd), K);
                     if (wj > 0.0f) {
                         col += wj;
-                        gc = gc + c.wvol[j] * kernel_grad(dd, c.h, c.kind);
+                        gc = gc + c.wvol[j] * kernel_grad(dd, K);
                         mv = mv + wj * V(c, j);
                     }
                 }
@@ -662,7 +726,7 @@ __device__ __forceinline__ void taper(const Ctx& c, float& v0, float& v1, float 
 
 // boundary.py:190-238 (find) and 241-396 (fill, inclusion-exclusion) for one fluid particle
 __global__ void __launch_bounds__(256) k_slots(Ctx c) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_particle();
     if (i >= c.n) return;
     SlotRow R;
     for (int k = 0; k < 4; ++k) {
@@ -793,14 +857,15 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
 }
 
 // ----------------------------------------------------------------------------------------------- density / alpha
-#define CHUNK_PROLOGUE                    \
-    int b_ = blockIdx.x;                  \
+#define CHUNK_PROLOGUE                                     \
+    int b_ = xcd_block(int(blockIdx.x), int(gridDim.x));   \
     int i = c.ch_begin[b_] + threadIdx.x; \
     bool valid = i < c.ch_end[b_];
 
 // pressure.py:144-193, plus the chunk partial of `active` (solver.py:241)
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_density_alpha(Ctx c) {
+    const KC K = make_kc(c.h, c.kind);
     CHUNK_PROLOGUE
     double act_v = 0.0;
     if (valid) {
@@ -815,18 +880,20 @@ __global__ void __launch_bounds__(256) k_density_alpha(Ctx c) {
             f3 xi = xyz(qi);
             float mi = qi.w;
             float inv_r0 = 1.0f / c.rho0[i];
-            float rho = mi * kernel_w(0.0f, c.h, c.kind);
+            float rho = mi * kernel_w(0.0f, K);
             f3 G{0.0f, 0.0f, 0.0f};
             float sq = 0.0f;
             int cnt = c.count[i];
             int base = c.ws[c.world[i]];
             for_nbrs<WIDE, float4>(
                 c, i, cnt, base, [&](int j) { return c.q4[j]; },
-                [&](int, float4 qj) {
+                [&](int k, int, float4 qj) {
                     f3 rv = xi - xyz(qj);
                     float mj = qj.w;
-                    rho += mj * kernel_w(length(rv), c.h, c.kind);
-                    f3 g = kernel_grad(rv, c.h, c.kind);
+                    rho += mj * kernel_w(length(rv), K);
+                    float sc = kernel_grad_scale(rv, K);
+                    if (c.pair_cache) c.pgrad[int64_t(k) * c.n + i] = sc;
+                    f3 g = pair_grad(rv, sc);
                     G = G + (mj * inv_r0) * g;
                     sq += (mi * mj * inv_r0 * inv_r0) * dot(g, g);
                 });
@@ -861,17 +928,15 @@ struct QV {
 };
 template <bool WIDE>
 __device__ float density_rate(const Ctx& c, int i) {
+    const KC K = make_kc(c.h, c.kind);
     f3 xi = xyz(c.q4[i]), vi = V(c, i);
     float inv_r0 = 1.0f / c.rho0[i];
     float rate = 0.0f;
     int cnt = c.count[i];
     int base = c.ws[c.world[i]];
-    for_nbrs<WIDE, QV>(
-        c, i, cnt, base, [&](int j) { return QV{c.q4[j], V(c, j)}; },
-        [&](int, QV d) {
-            f3 g = kernel_grad(xi - xyz(d.q), c.h, c.kind);
-            rate += (d.q.w * inv_r0) * dot(vi - d.v, g);
-        });
+    for_nbrs_g<WIDE, QV>(c, i, xi, cnt, base, (c.pair_cache & DFSPH_PAIR_CACHE_OTHER) != 0,
+                         [&](int j) { return QV{c.q4[j], V(c, j)}; },
+                         [&](int, QV d, f3 g) { rate += (d.q.w * inv_r0) * dot(vi - d.v, g); });
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
         if (c.s_shape[b4 + b] >= 0) rate += dot(vi - ld3(c.s_vb, b4 + b), ld3(c.s_gpsi, b4 + b));
@@ -883,11 +948,12 @@ template <bool WIDE>
 __global__ void __launch_bounds__(256) k_solve_init(Ctx c, int mode, float dt, float warm, float warm_den,
                                                     float relief, const float* store, const float* learned) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *c.n_unconv = 0;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_particle();
     if (i >= c.n) return;
     if (!is_fluid(c, i)) {
         c.src[i] = 0.0f;
         c.kappa[i] = 0.0f;
+        c.kr[i] = 0.0f;
         c.lower[i] = 0.0f;
         return;
     }
@@ -914,6 +980,7 @@ __global__ void __launch_bounds__(256) k_solve_init(Ctx c, int mode, float dt, f
         c.kappa[i] = warm * fmn(store[i], 2.5e-4f) / (dt * dt);
         if (learned) c.kappa[i] = fmx(learned[i], 0.0f);
     }
+    c.kr[i] = c.kappa[i] / c.rho0[i];
 }
 
 // pressure.py:355-367
@@ -934,21 +1001,20 @@ __global__ void k_begin_solve(Ctx c) {
 // pressure.py:264-277
 struct QKR {
     float4 q;
-    float k, r;
+    float kr;
 };
+// kapr = kap / rho0 per particle, written by the kernel that produced kap (the same division, gathered once)
 template <bool WIDE>
-__device__ f3 pressure_accel(const Ctx& c, const float* kap, int i) {
+__device__ f3 pressure_accel(const Ctx& c, const float* kap, const float* kapr, int i) {
+    const KC K = make_kc(c.h, c.kind);
     f3 xi = xyz(c.q4[i]);
     float ki = kap[i] / c.rho0[i];
     f3 a{0.0f, 0.0f, 0.0f};
     int cnt = c.count[i];
     int base = c.ws[c.world[i]];
-    for_nbrs<WIDE, QKR>(
-        c, i, cnt, base, [&](int j) { return QKR{c.q4[j], kap[j], c.rho0[j]}; },
-        [&](int, QKR d) {
-            f3 g = kernel_grad(xi - xyz(d.q), c.h, c.kind);
-            a = a - (d.q.w * (ki + d.k / d.r)) * g;
-        });
+    for_nbrs_g<WIDE, QKR>(c, i, xi, cnt, base, (c.pair_cache & DFSPH_PAIR_CACHE_ACCEL) != 0,
+                          [&](int j) { return QKR{c.q4[j], kapr[j]}; },
+                          [&](int, QKR d, f3 g) { a = a - (d.q.w * (ki + d.kr)) * g; });
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
         if (c.s_shape[b4 + b] >= 0) a = a - kap[i] * ld3(c.s_gpsi, b4 + b);
@@ -961,17 +1027,15 @@ struct QA {
 };
 template <bool WIDE>
 __device__ float accel_divergence(const Ctx& c, int i) {
+    const KC K = make_kc(c.h, c.kind);
     f3 xi = xyz(c.q4[i]), ai = A4(c, i);
     float inv_r0 = 1.0f / c.rho0[i];
     float ar = 0.0f;
     int cnt = c.count[i];
     int base = c.ws[c.world[i]];
-    for_nbrs<WIDE, QA>(
-        c, i, cnt, base, [&](int j) { return QA{c.q4[j], c.acc4[j]}; },
-        [&](int, QA d) {
-            f3 g = kernel_grad(xi - xyz(d.q), c.h, c.kind);
-            ar += (d.q.w * inv_r0) * dot(ai - xyz(d.a), g);
-        });
+    for_nbrs_g<WIDE, QA>(c, i, xi, cnt, base, (c.pair_cache & DFSPH_PAIR_CACHE_APPLY) != 0,
+                         [&](int j) { return QA{c.q4[j], c.acc4[j]}; },
+                         [&](int, QA d, f3 g) { ar += (d.q.w * inv_r0) * dot(ai - xyz(d.a), g); });
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
         if (c.s_shape[b4 + b] >= 0) ar += dot(ai, ld3(c.s_gpsi, b4 + b));
@@ -985,14 +1049,14 @@ __device__ __forceinline__ bool block_skip(const Ctx& c, int w, int gated) {
 
 // accel_kernel (pressure.py:280-291) / cg_accel_kernel (krylov.py:77-87)
 template <bool WIDE>
-__global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, int gated) {
+__global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, const float* vecr, int gated) {
     CHUNK_PROLOGUE
     if (block_skip(c, c.ch_world[b_], gated) || !valid) return;
     if (!is_fluid(c, i)) {
         c.acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
-    f3 a = pressure_accel<WIDE>(c, vec, i);
+    f3 a = pressure_accel<WIDE>(c, vec, vecr, i);
     c.acc4[i] = make_float4(a.x, a.y, a.z, 0.0f);
 }
 
@@ -1028,6 +1092,7 @@ __global__ void __launch_bounds__(256) k_cg_init(Ctx c, int mode, float dt) {
             c.act[i] = 0;
             c.z[i] = 0.0f;
             c.pd[i] = 0.0f;
+            c.pdr[i] = 0.0f;
         } else {
             float sc = mode == 0 ? dt : dt * dt;
             int a = 0;
@@ -1040,6 +1105,7 @@ __global__ void __launch_bounds__(256) k_cg_init(Ctx c, int mode, float dt) {
             c.act[i] = a;
             c.z[i] = zi;
             c.pd[i] = zi;
+            c.pdr[i] = zi / c.rho0[i];
             px = double(-wi * zi);
         }
     }
@@ -1053,6 +1119,7 @@ __global__ void __launch_bounds__(256) k_cg_step(Ctx c) {
     if (c.converged[w] != 0 || !valid) return;
     if (!is_fluid(c, i)) {
         c.corr[i] = 0.0f;
+        c.corrr[i] = 0.0f;
         return;
     }
     double pap = c.sums[2 * w];
@@ -1069,8 +1136,10 @@ __global__ void __launch_bounds__(256) k_cg_step(Ctx c) {
             c.restart[w] = 1;  // a flag: only restart != 0 is ever tested (krylov.py:70-74, 231)
         }
         c.kappa[i] = k;
+        c.kr[i] = k / c.rho0[i];
     }
     c.corr[i] = cc;
+    c.corrr[i] = cc / c.rho0[i];
     c.wv[i] = c.wv[i] + a * c.ap[i];
 }
 
@@ -1108,12 +1177,18 @@ __global__ void __launch_bounds__(256) k_cg_finish(Ctx c, int mode, float dt) {
 }
 
 // cg_direction_kernel (krylov.py:243-255)
-__global__ void __launch_bounds__(256) k_cg_direction(Ctx c) {
+// flag_out (pinned host memory, may be null): block 0 stores the unconverged-world count that k_world's stop test
+// just produced, so the host learns it from this batch's completion event without a separate copy
+__global__ void __launch_bounds__(256) k_cg_direction(Ctx c, int* flag_out) {
     CHUNK_PROLOGUE
+    if (flag_out && blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(flag_out, *c.n_unconv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     int w = c.ch_world[b_];
     if (c.converged[w] != 0 || !valid || !is_fluid(c, i)) return;
-    if (c.act[i] != 0) c.pd[i] = c.z[i] + float(c.stepb[w]) * c.pd[i];
-    else c.pd[i] = 0.0f;
+    float p = 0.0f;
+    if (c.act[i] != 0) p = c.z[i] + float(c.stepb[w]) * c.pd[i];
+    c.pd[i] = p;
+    c.pdr[i] = p / c.rho0[i];
 }
 
 // jacobi_kernel (pressure.py:313-352) + the chunk partial of (error, -, max error); block (0,0) clears the
@@ -1144,6 +1219,7 @@ __global__ void __launch_bounds__(256) k_jacobi(Ctx c, int mode, float dt) {
             float lb = c.lower[i];
             float k_new = fmx(c.kappa[i] - c.omega * rr * c.alpha[i] * scale, lb);
             c.kappa[i] = k_new;
+            c.kr[i] = k_new / c.rho0[i];
             float e = fmx(-rr, 0.0f);
             if (k_new > lb) e = fabsf(rr);
             if (mode == 0) {
@@ -1164,7 +1240,7 @@ __global__ void k_zero_unconv(Ctx c) { *c.n_unconv = 0; }
 // accelerations read kappa changes nothing for them.
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, float* store) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_particle();
     if (i >= c.n || !is_fluid(c, i)) return;
     int64_t b4 = 4 * int64_t(i);
     bool lone = c.count[i] == 0;
@@ -1185,7 +1261,7 @@ __global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, f
         }
     }
     // apply_pressure_kernel
-    f3 a = pressure_accel<WIDE>(c, c.kappa, i);
+    f3 a = pressure_accel<WIDE>(c, c.kappa, c.kr, i);
     f3 v = V(c, i) + dt * a;
     float kap = c.kappa[i];
     float mk = c.mass[i] * kap * dt;
@@ -1303,7 +1379,8 @@ struct XS {
 };
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const KC K = make_kc(c.h, c.kind);
+    int i = xcd_particle();
     if (i >= c.n) return;
     f3 vi = V(c, i);
     if (!is_fluid(c, i)) {
@@ -1322,10 +1399,10 @@ __global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
         int base = c.ws[w];
         for_nbrs<WIDE, XS>(
             c, i, cnt, base, [&](int j) { return XS{c.q4[j], V(c, j), c.ratio[j], c.rho0[j]}; },
-            [&](int, XS d) {
+            [&](int, int, XS d) {
                 float rho_bar = 0.5f * (rho_i + d.ratio * d.rho0);
                 if (rho_bar > 0.0f) {
-                    float wk = kernel_w(length(xi - xyz(d.q)), c.h, c.kind);
+                    float wk = kernel_w(length(xi - xyz(d.q)), K);
                     cr = cr + (d.q.w / rho_bar * wk) * (d.v - vi);
                 }
             });
@@ -1594,7 +1671,8 @@ __global__ void __launch_bounds__(256) k_advect_project(Ctx c, float dt) {
 // n_eff / n_full, h grad r . g_hat; ks = |g| h. Non-liquid rows are zero.
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_features(Ctx c, float dt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const KC K = make_kc(c.h, c.kind);
+    int i = xcd_particle();
     if (i >= c.n) return;
     float* F = c.feat + 16 * int64_t(i);
     if (!is_fluid(c, i)) {
@@ -1608,13 +1686,12 @@ __global__ void __launch_bounds__(256) k_features(Ctx c, float dt) {
     float rate = 0.0f;
     int cnt = c.count[i];
     int base = c.ws[c.world[i]];
-    for_nbrs<WIDE, QV>(
-        c, i, cnt, base, [&](int j) { return QV{c.q4[j], V(c, j)}; },
-        [&](int, QV d) {
-            f3 g = kernel_grad(xi - xyz(d.q), c.h, c.kind);
-            G = G + (d.q.w * inv_r0) * g;
-            rate += (d.q.w * inv_r0) * dot(vi - d.v, g);
-        });
+    for_nbrs_g<WIDE, QV>(c, i, xi, cnt, base, (c.pair_cache & DFSPH_PAIR_CACHE_OTHER) != 0,
+                         [&](int j) { return QV{c.q4[j], V(c, j)}; },
+                         [&](int, QV d, f3 g) {
+                             G = G + (d.q.w * inv_r0) * g;
+                             rate += (d.q.w * inv_r0) * dot(vi - d.v, g);
+                         });
     float psi = 0.0f;
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
@@ -1676,12 +1753,13 @@ struct Prof {
 
 // ----------------------------------------------------------------------------------------------- handle
 struct GraphKey {
-    int kind, mode, every;
+    int kind, mode, every, pair_cache;
     uint32_t dt_bits;
     const void *q, *qd, *f, *ld, *ln, *mlp, *lf, *lt;
     bool operator<(const GraphKey& o) const {
-        return std::tie(kind, mode, every, dt_bits, q, qd, f, ld, ln, mlp, lf, lt) <
-               std::tie(o.kind, o.mode, o.every, o.dt_bits, o.q, o.qd, o.f, o.ld, o.ln, o.mlp, o.lf, o.lt);
+        return std::tie(kind, mode, every, pair_cache, dt_bits, q, qd, f, ld, ln, mlp, lf, lt) <
+               std::tie(o.kind, o.mode, o.every, o.pair_cache, o.dt_bits, o.q, o.qd, o.f, o.ld, o.ln, o.mlp, o.lf,
+                        o.lt);
     }
 };
 
@@ -1691,6 +1769,7 @@ struct Handle {
     int n_chunks = 0, n_scan_blocks = 0;
     int64_t nbr_bytes = 0, nbr_hi_bytes = 0;
     int* h_unconv = nullptr;  // pinned [2]: unconverged-world counts of the last two iteration batches
+    int* d_unconv = nullptr;  // the same memory as the device addresses it
     hipEvent_t ev_unconv[2] = {nullptr, nullptr};
     std::vector<void*> owned;
     float* qd_saved = nullptr;
@@ -1765,7 +1844,7 @@ int phase_neighbors(Handle* H, hipStream_t st) {
     k_sort_cells<<<blocks_for(c.n_cells), 256, 0, st>>>(c, 0, c.n_cells);  // ascending ids in every cell
     {
         PROF(KID_BUILD);
-        k_build_lists<<<blocks_for(c.n, BUILD_BLOCK), BUILD_BLOCK, 0, st>>>(c);
+        LAUNCH_W(k_build_lists, blocks_for(c.n, BUILD_BLOCK), BUILD_BLOCK, c);
     }
     LAUNCH_CHECK();
     return DFSPH_OK;
@@ -1805,12 +1884,14 @@ int phase_flow(Handle* H, hipStream_t st) {
     return DFSPH_OK;
 }
 
-// The device-side loop replacing wp.capture_while: after each batch of iterations the unconverged-world count is
-// copied to pinned slot s and an event recorded; the host enqueues the next batch before it waits on that event,
-// so the GPU never idles on the host. The batch enqueued after the last unconverged world converged is a no-op
-// (every kernel of a converged world exits at its first instruction).
-int post_unconv(Handle* H, hipStream_t st, int slot) {
-    HIPCHK(hipMemcpyAsync(H->h_unconv + slot, H->c.n_unconv, sizeof(int), hipMemcpyDeviceToHost, st));
+// The device-side loop replacing wp.capture_while: batch b leaves the unconverged-world count in pinned slot b & 1
+// (stored by its last kernel, or copied where the batch is a replayed graph or a Jacobi batch) and records that
+// slot's event; the host enqueues batch b + 1 before it waits on batch b's event, so the GPU never idles on the
+// host. The batch enqueued after the last unconverged world converged is a no-op (every kernel of a converged world
+// exits at its first instruction).
+int post_unconv(Handle* H, hipStream_t st, int slot, bool stored) {
+    if (!stored)
+        HIPCHK(hipMemcpyAsync(H->h_unconv + slot, H->c.n_unconv, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(H->ev_unconv[slot], st));
     return DFSPH_OK;
 }
@@ -1882,7 +1963,7 @@ int enq_solve_head(Handle* H, const SolveSetup& S, hipStream_t st) {
     }
     k_begin_solve<<<blocks_for(c.W), 256, 0, st>>>(c);
     if (P.pressure_solver == 0) {
-        LAUNCH_W(k_accel, H->n_chunks, 256, c, c.kappa, 0);
+        LAUNCH_W(k_accel, H->n_chunks, 256, c, c.kappa, c.kr, 0);
         LAUNCH_W(k_apply, H->n_chunks, 256, c, S.mode, S.dt, c.kappa, c.wv, 0, 1, 0, 0);
         k_cg_init<<<H->n_chunks, 256, 0, st>>>(c, S.mode, S.dt);
         k_world<<<c.W, 256, 0, st>>>(c, W_CG_INIT, S.sp);
@@ -1892,7 +1973,7 @@ int enq_solve_head(Handle* H, const SolveSetup& S, hipStream_t st) {
 }
 
 // check_every iterations (krylov.py:298-313 / solve_driver.py:175-184); converged worlds' blocks exit at once
-int enq_solve_batch(Handle* H, const SolveSetup& S, hipStream_t st) {
+int enq_solve_batch(Handle* H, const SolveSetup& S, hipStream_t st, int* flag_out) {
     const Ctx& c = S.c;
     int mode = S.mode;
     float dt = S.dt;
@@ -1900,7 +1981,7 @@ int enq_solve_batch(Handle* H, const SolveSetup& S, hipStream_t st) {
         if (H->P.pressure_solver == 0) {
             {
                 PROF(KID_ACCEL);
-                LAUNCH_W(k_accel, H->n_chunks, 256, c, c.pd, 0);
+                LAUNCH_W(k_accel, H->n_chunks, 256, c, c.pd, c.pdr, 0);
             }
             {
                 PROF(KID_APPLY);
@@ -1908,13 +1989,13 @@ int enq_solve_batch(Handle* H, const SolveSetup& S, hipStream_t st) {
             }
             k_world<<<c.W, 256, 0, st>>>(c, W_CG_PAP, S.sp);
             k_cg_step<<<H->n_chunks, 256, 0, st>>>(c);
-            LAUNCH_W(k_accel, H->n_chunks, 256, c, c.corr, 1);
+            LAUNCH_W(k_accel, H->n_chunks, 256, c, c.corr, c.corrr, 1);
             LAUNCH_W(k_apply, H->n_chunks, 256, c, mode, dt, c.corr, c.wv, 1, 0, 1, 0);
             k_cg_finish<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
             k_world<<<c.W, 256, 0, st>>>(c, W_CG_BETA, S.sp);
-            k_cg_direction<<<H->n_chunks, 256, 0, st>>>(c);
+            k_cg_direction<<<H->n_chunks, 256, 0, st>>>(c, rep == H->check_every - 1 ? flag_out : nullptr);
         } else {
-            LAUNCH_W(k_accel, H->n_chunks, 256, c, c.kappa, 0);
+            LAUNCH_W(k_accel, H->n_chunks, 256, c, c.kappa, c.kr, 0);
             LAUNCH_W(k_jacobi, H->n_chunks, 256, c, mode, dt);
             k_world<<<c.W, 256, 0, st>>>(c, W_JACOBI, S.sp);
         }
@@ -1942,7 +2023,7 @@ int run_segment(Handle* H, hipStream_t st, int kind, int mode, float dt, F enque
     if (!H->use_graphs || H->prof.on) return enqueue(st);
     uint32_t bits;
     std::memcpy(&bits, &dt, 4);
-    GraphKey key{kind, mode, H->check_every, bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
+    GraphKey key{kind, mode, H->check_every, H->c.pair_cache, bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
                  H->mlp_params, H->log_feat, H->log_targets};
     auto it = H->graphs.find(key);
     if (it == H->graphs.end()) {
@@ -1975,14 +2056,18 @@ int run_solve(Handle* H, const SolveSetup& S, hipStream_t st, int tail_kind, Ext
     int rc;
     if ((rc = run_segment(H, st, SEG_HEAD, S.mode, S.dt, [&](hipStream_t s) { return enq_solve_head(H, S, s); })))
         return rc;
-    if ((rc = post_unconv(H, st, 0))) return rc;
+    // a captured batch cannot alternate its slot pointer, so replayed graphs and Jacobi batches copy the count
+    bool direct = H->P.pressure_solver == 0 && (!H->use_graphs || H->prof.on);
     int nu = 0;
     for (int batch = 0;; ++batch) {
+        int slot = batch & 1;
+        int* fl = direct ? H->d_unconv + slot : nullptr;
         if ((rc = run_segment(H, st, SEG_BATCH, S.mode, S.dt,
-                              [&](hipStream_t s) { return enq_solve_batch(H, S, s); })))
+                              [&](hipStream_t s) { return enq_solve_batch(H, S, s, fl); })))
             return rc;
-        if ((rc = post_unconv(H, st, (batch + 1) & 1))) return rc;
-        if ((rc = wait_unconv(H, batch & 1, &nu))) return rc;
+        if ((rc = post_unconv(H, st, slot, direct))) return rc;
+        if (batch == 0) continue;
+        if ((rc = wait_unconv(H, slot ^ 1, &nu))) return rc;
         if (nu == 0) break;
         if (batch > 100000) return fail(DFSPH_EDEVICE, "pressure solve did not terminate");
     }
@@ -2174,6 +2259,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         H->nbr_hi_bytes = c.wide ? int64_t(c.cap) * std::max(n, 1) : 1;
         TRY(dalloc(H, &c.nbr, int64_t(c.cap) * std::max(n, 1)));
         TRY(dalloc(H, &c.nbr_hi, c.wide ? int64_t(c.cap) * std::max(n, 1) : 1));
+        TRY(dalloc(H, &c.pgrad, int64_t(c.cap) * std::max(n, 1)));
         TRY(dalloc(H, &c.count, n));
         TRY(dalloc(H, &c.group, n));
         // slots
@@ -2182,7 +2268,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         TRY(dalloc(H, &c.s_phi, 4 * int64_t(n)));
         for (float** p : {&c.s_gpsi, &c.s_xb, &c.s_vb, &c.s_xbl, &c.s_nbl, &c.s_imp}) TRY(dalloc(H, p, 12 * int64_t(n)));
         for (float** p : {&c.ratio, &c.alpha, &c.neff, &c.active, &c.reg, &c.src, &c.kappa, &c.err, &c.lower, &c.kv,
-                          &c.kd, &c.pd, &c.z, &c.wv, &c.ap, &c.corr})
+                          &c.kd, &c.pd, &c.z, &c.wv, &c.ap, &c.corr, &c.kr, &c.pdr, &c.corrr})
             TRY(dalloc(H, p, n));
         TRY(dalloc(H, &c.acc4, n));
         TRY(dalloc(H, &c.q4, n));
@@ -2209,11 +2295,18 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
             HIPCHK(hipMemcpy(c.tol_scale, ones.data(), W * sizeof(float), hipMemcpyHostToDevice));
         }
         void* pinned = nullptr;
-        if (hipHostMalloc(&pinned, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc(&pinned, 2 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
             rc = fail(DFSPH_EDEVICE, "hipHostMalloc failed");
             goto fail;
         }
         H->h_unconv = static_cast<int*>(pinned);
+        H->h_unconv[0] = H->h_unconv[1] = 1;
+        void* dview = nullptr;
+        if (hipHostGetDevicePointer(&dview, pinned, 0) != hipSuccess) {
+            rc = fail(DFSPH_EDEVICE, "hipHostGetDevicePointer failed");
+            goto fail;
+        }
+        H->d_unconv = static_cast<int*>(dview);
         for (int k = 0; k < 2; ++k)
             if (hipEventCreateWithFlags(&H->ev_unconv[k], hipEventDisableTiming) != hipSuccess) {
                 rc = fail(DFSPH_EDEVICE, "hipEventCreate failed");
@@ -2422,6 +2515,42 @@ int32_t dfsph_set_check_interval(void* h, int32_t every) {
     if (every < 1 || every > 64) return fail(DFSPH_EINVAL, "check interval must be in [1, 64]");
     as_handle(h)->check_every = every;
     return DFSPH_OK;
+}
+
+// tuning options (none changes a result bit)
+int32_t dfsph_set_option(void* h, int32_t option, int32_t value) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    Handle* H = as_handle(h);
+    switch (option) {
+        case DFSPH_OPT_CHECK_INTERVAL:
+            return dfsph_set_check_interval(h, value);
+        case DFSPH_OPT_GRAPHS:
+            return dfsph_set_graphs(h, value);
+        case DFSPH_OPT_PAIR_CACHE:
+            if (value < 0 || value > DFSPH_PAIR_CACHE_ALL)
+                return fail(DFSPH_EINVAL, "pair cache: a mask of DFSPH_PAIR_CACHE_* bits");
+            H->c.pair_cache = value;
+            return DFSPH_OK;
+        default:
+            return fail(DFSPH_EINVAL, "unknown option");
+    }
+}
+int32_t dfsph_get_option(void* h, int32_t option, int32_t* value) {
+    if (!h || !value) return fail(DFSPH_EINVAL, "null argument");
+    Handle* H = as_handle(h);
+    switch (option) {
+        case DFSPH_OPT_CHECK_INTERVAL:
+            *value = H->check_every;
+            return DFSPH_OK;
+        case DFSPH_OPT_GRAPHS:
+            *value = H->use_graphs ? 1 : 0;
+            return DFSPH_OK;
+        case DFSPH_OPT_PAIR_CACHE:
+            *value = H->c.pair_cache;
+            return DFSPH_OK;
+        default:
+            return fail(DFSPH_EINVAL, "unknown option");
+    }
 }
 
 // per-kernel HIP-event timing (kernel ids: 0 accel, 1 apply, 2 build lists, 3 density, 4 slots, 5 non-pressure,

@@ -31,6 +31,10 @@ DFSPH_OK, DFSPH_EINVAL, DFSPH_ECAPACITY, DFSPH_EDEVICE = 0, -1, -2, -3
  BUF_DIV_ERR, BUF_SHAPE_METER, BUF_FEATURES) = range(40)
 
 # kernel ids of dfsph_profile_read
+# tuning options (dfsph_set_option); none changes a result bit
+OPT_CHECK_INTERVAL, OPT_GRAPHS, OPT_PAIR_CACHE = 1, 2, 3
+PAIR_CACHE_ACCEL, PAIR_CACHE_APPLY, PAIR_CACHE_OTHER, PAIR_CACHE_ALL = 1, 2, 4, 7
+
 (KID_ACCEL, KID_APPLY, KID_BUILD, KID_DENSITY, KID_SLOTS, KID_NONPRESSURE, KID_ADVECT, KID_SOLVE_INIT,
  KID_POST) = range(9)
 KERNEL_NAMES = ("k_accel", "k_apply", "k_build_lists", "k_density_alpha", "k_slots", "k_nonpressure",
@@ -300,6 +304,8 @@ def load_library() -> ctypes.CDLL:
         "dfsph_set_warm_start_kappa": ([vp, vp, vp], i32),
         "dfsph_set_check_interval": ([vp, i32], i32),
         "dfsph_set_graphs": ([vp, i32], i32),
+        "dfsph_set_option": ([vp, i32, i32], i32),
+        "dfsph_get_option": ([vp, i32, P(i32)], i32),
         "dfsph_mlp_param_count": ([], i32),
         "dfsph_mlp_forward": ([vp, vp, i32, vp, vp], i32),
         "dfsph_mlp_grad_scratch_floats": ([i32], i64),

@@ -211,6 +211,15 @@ class SolverDFSPH:
         """Iterations between host convergence checks of the pressure solves (autotuned; results unchanged)."""
         self._call(self.lib.dfsph_set_check_interval, int(every))
 
+    def set_option(self, option: int, value: int) -> None:
+        """A tuning option of the native step (``_abi.OPT_*``); results are bit-identical for every value."""
+        self._call(self.lib.dfsph_set_option, int(option), int(value))
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int32()
+        self._call(self.lib.dfsph_get_option, int(option), ctypes.byref(v))
+        return int(v.value)
+
     def set_graphs(self, on: bool) -> None:
         """Replay the step's segments as captured hipGraphs (default) or launch every kernel eagerly."""
         self._call(self.lib.dfsph_set_graphs, int(bool(on)))

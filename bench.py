@@ -123,9 +123,13 @@ def main() -> int:
     ap.add_argument("--workload", default="dam_break_125k", choices=sorted(WORKLOADS))
     ap.add_argument("--check-every", type=int, default=1, help="iterations between host convergence checks")
     ap.add_argument("--graphs", action="store_true", help="replay the step's segments as hipGraphs")
+    ap.add_argument("--pair-cache", type=int, default=None, help="DFSPH_PAIR_CACHE_* mask (default: the library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB (3 launches) and exit: the known-size "
+                         "kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (worlds shard across "
@@ -133,6 +137,16 @@ def main() -> int:
         return 2
 
     import torch
+
+    if args.copy_probe_mib > 0:
+        nbytes = args.copy_probe_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "launches": 3}))
+        return 0
 
     from alloy_amd.sph import _abi, scenes
 
@@ -142,6 +156,8 @@ def main() -> int:
     solver = sc.make_solver()
     solver.set_check_interval(args.check_every)
     solver.set_graphs(args.graphs)
+    if args.pair_cache is not None:
+        solver.set_option(_abi.OPT_PAIR_CACHE, args.pair_cache)
     state = sc.model.state()
     n_fluid = sum(sc.fluid_counts)
     W = sc.model.world_count

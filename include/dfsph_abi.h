@@ -206,6 +206,23 @@ int32_t dfsph_set_warm_start_model(void* handle, const void* weights, void* stre
 int32_t dfsph_set_warm_start_kappa(void* handle, const float* kappa_div, const float* kappa_den);
 /* Host convergence check every `every` iterations (1..64); converged worlds' blocks exit at once in between. */
 int32_t dfsph_set_check_interval(void* handle, int32_t every);
+/* Tuning options: every value gives bit-identical results; they only trade time. */
+enum DfsphOption {
+    DFSPH_OPT_CHECK_INTERVAL = 1, /* as dfsph_set_check_interval */
+    DFSPH_OPT_GRAPHS = 2,         /* as dfsph_set_graphs */
+    DFSPH_OPT_PAIR_CACHE = 3      /* mask of DFSPH_PAIR_CACHE_*: gather passes that read the density pass's stored
+                                     kernel-gradient scale of each listed pair instead of re-evaluating it (4 B of
+                                     memory traffic per pair traded for a square root and two divisions) */
+};
+enum DfsphPairCache {
+    DFSPH_PAIR_CACHE_ACCEL = 1, /* pressure acceleration (k_accel, post-solve) */
+    DFSPH_PAIR_CACHE_APPLY = 2, /* operator apply (divergence of the acceleration) */
+    DFSPH_PAIR_CACHE_OTHER = 4, /* solve source, learned-start features */
+    DFSPH_PAIR_CACHE_ALL = 7
+};
+/* No counterpart in the reference (Warp picks launch parameters itself); used by the autotuner. */
+int32_t dfsph_set_option(void* handle, int32_t option, int32_t value);
+int32_t dfsph_get_option(void* handle, int32_t option, int32_t* value);
 /* dfsph_step replays its segments (pre-solve, solve head, iteration batch, solve tail) as hipGraphs captured once
  * per (dt, state pointers, batch size); 0 launches every kernel eagerly. Results are identical either way. */
 int32_t dfsph_set_graphs(void* handle, int32_t on);
