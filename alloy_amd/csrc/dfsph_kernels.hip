@@ -12,6 +12,7 @@
 // in float64 into a chunk partial. A per-world epilogue (one block per world) tree-reduces that world's chunk
 // partials in fixed order and runs the world's control logic (check_kernel / cg_beta_kernel). The sums are thus
 // bitwise independent of batching and of scheduling.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <type_traits>
 
@@ -20,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <deque>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -50,6 +52,7 @@ int fail(int code, const std::string& msg) {
 struct Ctx {
     int n, W, S, NW, cap, wide, kind, n_cells, friction, curvature, two_sided;
     int pair_cache;  // DFSPH_PAIR_CACHE_* bits: which gather passes read pgrad instead of recomputing the scale
+    int n_chunks;
     float h, cell, r, n_full, tot_min, min_nb, omega, xsph, v_sep;
     float offset, amplitude, taper0, taper1, reach, eps, sweep_len;
     // model (caller-owned)
@@ -121,6 +124,7 @@ struct Ctx {
     unsigned long long* meter;
     double* part;
     int* n_unconv;
+    unsigned long long* prof_bytes;  // [2][n_chunks][4] per-wave byte counts while profiling, else null
     // state (bound per call)
     float* q;
     float* qd;
@@ -214,6 +218,23 @@ __device__ __forceinline__ void for_nbrs_g(const Ctx& c, int i, f3 xi, int cnt, 
         if (c.wide) kernel<true><<<(grid), (block), 0, st>>>(__VA_ARGS__);        \
         else kernel<false><<<(grid), (block), 0, st>>>(__VA_ARGS__);             \
     } while (0)
+
+// kernel ids of the per-launch profiling (dfsph_set_profiling / dfsph_profile_read)
+enum KernelId { KID_ACCEL = 0, KID_APPLY, KID_BUILD, KID_DENSITY, KID_SLOTS, KID_NONPRESSURE, KID_ADVECT,
+                KID_SOLVE_INIT, KID_POST, KID_COUNT };
+
+// Profiling only (c.prof_bytes set): add the algorithmic bytes of this wave's particles to the wave's own slot of
+// kernel kid's counters ([2][4 n_chunks]: k_accel, k_apply), summed on the host. One plain add per wave and no
+// shared address: a single counter would take one atomic per wave from every block and serialise the launch.
+// Called where every lane of a block that did work is still active.
+__device__ __forceinline__ void account_bytes(const Ctx& c, int kid, int chunk, unsigned v) {
+    unsigned long long s = v;
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if ((threadIdx.x & 63) == 0) {
+        unsigned long long* slot = c.prof_bytes + (int64_t(kid) * c.n_chunks + chunk) * 4 + (threadIdx.x >> 6);
+        *slot += s;
+    }
+}
 
 // ----------------------------------------------------------------------------------------------- XCD mapping
 // The dispatcher hands workgroup b to XCD b % 8. Renumbering so that XCD x runs the contiguous logical range
@@ -1042,6 +1063,22 @@ __device__ float accel_divergence(const Ctx& c, int i) {
     return ar;
 }
 
+// Algorithmic (compulsory) bytes of one particle in a pressure-acceleration / operator-apply launch: each
+// per-particle array the pass reads or writes, once -- a neighbour's position, kappa quotient or acceleration is
+// counted in its owner's read -- plus the row entries (2 B, 3 B in 24-bit worlds) and 12 B per filled slot
+// gradient. k_accel, liquid: q4 16, vec 4, vec / rho0 4, rho0 4, count 4, flags 4, phase 1, world 4, slot shapes 16,
+// acc written 12; other particles: flags, phase, acc written (17). k_apply, liquid: q4 16, acc 12, rho0 4, count 4,
+// flags 4, phase 1, world 4, reg 4, vec 4, out written 4, slot shapes 16 (+4 source with the bias, +4 out read
+// when accumulating); other particles: flags, phase, out written (9). Padding and caches are not counted.
+template <bool WIDE>
+__device__ __forceinline__ unsigned pass_bytes(const Ctx& c, int i, unsigned liquid, unsigned other) {
+    if (!is_fluid(c, i)) return other;
+    int64_t b4 = 4 * int64_t(i);
+    unsigned ns = 0;
+    for (int b = 0; b < 4; ++b) ns += c.s_shape[b4 + b] >= 0 ? 1u : 0u;
+    return liquid + (WIDE ? 3u : 2u) * unsigned(c.count[i]) + 12u * ns;
+}
+
 __device__ __forceinline__ bool block_skip(const Ctx& c, int w, int gated) {
     if (c.converged[w] != 0) return true;
     return gated != 0 && c.restart[w] == 0;
@@ -1051,7 +1088,9 @@ __device__ __forceinline__ bool block_skip(const Ctx& c, int w, int gated) {
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, const float* vecr, int gated) {
     CHUNK_PROLOGUE
-    if (block_skip(c, c.ch_world[b_], gated) || !valid) return;
+    if (block_skip(c, c.ch_world[b_], gated)) return;
+    if (c.prof_bytes) account_bytes(c, KID_ACCEL, b_, valid ? pass_bytes<WIDE>(c, i, 69u, 17u) : 0u);
+    if (!valid) return;
     if (!is_fluid(c, i)) {
         c.acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
@@ -1066,6 +1105,9 @@ __global__ void __launch_bounds__(256) k_apply(Ctx c, int mode, float dt, const 
                                                int bias, int accum, int reduce) {
     CHUNK_PROLOGUE
     if (block_skip(c, c.ch_world[b_], gated)) return;
+    if (c.prof_bytes)
+        account_bytes(c, KID_APPLY, b_,
+                      valid ? pass_bytes<WIDE>(c, i, 77u + (bias ? 4u : 0u) + (accum ? 4u : 0u), 9u) : 0u);
     double px = 0.0;
     if (valid) {
         if (!is_fluid(c, i)) {
@@ -1741,14 +1783,24 @@ __global__ void k_targets(Ctx c, float dt, float* T) {
 }
 
 // ----------------------------------------------------------------------------------------------- profiling
-// Optional HIP-event timing of the main kernels on the launch stream (bench.py's live roofline figure).
-enum KernelId { KID_ACCEL = 0, KID_APPLY, KID_BUILD, KID_DENSITY, KID_SLOTS, KID_NONPRESSURE, KID_ADVECT,
-                KID_SOLVE_INIT, KID_POST, KID_COUNT };
-constexpr int PROF_CAP = 4096;
+// Optional per-launch timing of the main kernels: the start / stop events ride in the kernel's own dispatch
+// (hipExtLaunchKernelGGL), so a duration is the kernel's execution alone, as rocprofv3's kernel trace reports it,
+// never the gap before a starved launch. Finished pairs are summed and recycled at the next step, which keeps the
+// event pool bounded over any number of steps.
 struct Prof {
     bool on = false;
-    std::vector<hipEvent_t> a, b;  // [KID_COUNT * PROF_CAP]
-    int used[KID_COUNT] = {};
+    uint32_t mask = 0;  // kernel ids timed
+    int sample = 1;     // time a pseudo-random 1 / sample of those launches (host cost of a timed launch)
+    uint64_t launches[KID_COUNT] = {};
+    struct Rec {
+        int kid;
+        hipEvent_t a, b;
+    };
+    std::deque<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    double total_ms[KID_COUNT] = {};
+    int count[KID_COUNT] = {};
+    unsigned long long* d_bytes = nullptr;  // [2][n_chunks][4] algorithmic bytes counted by k_accel / k_apply
 };
 
 // ----------------------------------------------------------------------------------------------- handle
@@ -1786,21 +1838,73 @@ struct Handle {
     std::map<GraphKey, hipGraphExec_t> graphs;
 };
 
-struct ProfScope {
-    Handle* H;
-    int kid, slot;
-    hipStream_t st;
-    ProfScope(Handle* h, int k, hipStream_t s) : H(h), kid(k), slot(-1), st(s) {
-        if (H->prof.on && H->prof.used[k] < PROF_CAP) {
-            slot = k * PROF_CAP + H->prof.used[k]++;
-            (void)hipEventRecord(H->prof.a[slot], st);
+// an event pair (and, for k_accel / k_apply, the byte counters) for a timed launch of kernel kid, or false when
+// this launch is not timed; the 1 / sample choice hashes the launch count, so it does not alias with the fixed
+// launch pattern of a solver iteration
+bool prof_begin(Handle* H, int kid, hipEvent_t* a, hipEvent_t* b, unsigned long long** bytes) {
+    Prof& P = H->prof;
+    if (!P.on || !((P.mask >> kid) & 1u)) return false;
+    uint64_t z = (P.launches[kid]++ + (uint64_t(kid) << 48)) * 0x9E3779B97F4A7C15ull;  // splitmix64 finaliser
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    if (P.sample > 1 && z % uint64_t(P.sample) != 0) return false;
+    hipEvent_t e[2];
+    for (hipEvent_t& x : e) {
+        if (!P.pool.empty()) {
+            x = P.pool.back();
+            P.pool.pop_back();
+        } else if (hipEventCreateWithFlags(&x, hipEventDisableSystemFence) != hipSuccess) {
+            // timing-only events: no system-scope fence (an L2 writeback at every timed launch would slow the
+            // kernels being timed, and the ones after them)
+            return false;
         }
     }
-    ~ProfScope() {
-        if (slot >= 0) (void)hipEventRecord(H->prof.b[slot], st);
+    *a = e[0];
+    *b = e[1];
+    *bytes = (kid == KID_ACCEL || kid == KID_APPLY) ? P.d_bytes : nullptr;
+    P.pending.push_back(Prof::Rec{kid, e[0], e[1]});
+    return true;
+}
+// sum the finished timed launches (all of them with `wait`), returning their events to the pool
+int prof_harvest(Handle* H, bool wait) {
+    Prof& P = H->prof;
+    while (!P.pending.empty()) {
+        Prof::Rec r = P.pending.front();
+        if (wait) {
+            HIPCHK(hipEventSynchronize(r.b));
+        } else if (hipEventQuery(r.b) != hipSuccess) {
+            break;
+        }
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        P.total_ms[r.kid] += ms;
+        P.count[r.kid] += 1;
+        P.pool.push_back(r.a);
+        P.pool.push_back(r.b);
+        P.pending.pop_front();
     }
-};
-#define PROF(kid) ProfScope prof_scope_##kid(H, kid, st)
+    return DFSPH_OK;
+}
+// launch `kernel` (first argument the Ctx `ctx`), timed as kernel id `kid` when profiling samples it: the timed
+// launch carries its events in its dispatch and a Ctx copy whose prof_bytes points at the byte counters
+#define LAUNCH_T(kid, kernel, grid, block, ctx, ...)                                                       \
+    do {                                                                                                   \
+        hipEvent_t ea_, eb_;                                                                               \
+        unsigned long long* pb_ = nullptr;                                                                 \
+        if (prof_begin(H, kid, &ea_, &eb_, &pb_)) {                                                        \
+            Ctx ct_ = (ctx);                                                                               \
+            ct_.prof_bytes = pb_;                                                                          \
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, st, ea_, eb_, 0u, ct_, ##__VA_ARGS__); \
+        } else {                                                                                           \
+            kernel<<<(grid), (block), 0, st>>>((ctx), ##__VA_ARGS__);                                      \
+        }                                                                                                  \
+    } while (0)
+#define LAUNCH_WT(kid, kernel, grid, block, ctx, ...)                                    \
+    do {                                                                                 \
+        if ((ctx).wide) LAUNCH_T(kid, kernel<true>, grid, block, ctx, ##__VA_ARGS__);    \
+        else LAUNCH_T(kid, kernel<false>, grid, block, ctx, ##__VA_ARGS__);              \
+    } while (0)
 
 template <class T>
 int dalloc(Handle* H, T** p, int64_t count) {
@@ -1842,10 +1946,7 @@ int phase_neighbors(Handle* H, hipStream_t st) {
     k_scan_apply<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part, c.cell_start, c.cell_fill);
     k_scatter<<<blocks_for(c.n), 256, 0, st>>>(c);
     k_sort_cells<<<blocks_for(c.n_cells), 256, 0, st>>>(c, 0, c.n_cells);  // ascending ids in every cell
-    {
-        PROF(KID_BUILD);
-        LAUNCH_W(k_build_lists, blocks_for(c.n, BUILD_BLOCK), BUILD_BLOCK, c);
-    }
+    LAUNCH_WT(KID_BUILD, k_build_lists, blocks_for(c.n, BUILD_BLOCK), BUILD_BLOCK, c);
     LAUNCH_CHECK();
     return DFSPH_OK;
 }
@@ -1853,10 +1954,7 @@ int phase_neighbors(Handle* H, hipStream_t st) {
 int phase_slots(Handle* H, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
-    {
-        PROF(KID_SLOTS);
-        k_slots<<<blocks_for(c.n), 256, 0, st>>>(c);
-    }
+    LAUNCH_T(KID_SLOTS, k_slots, blocks_for(c.n), 256, c);
     LAUNCH_CHECK();
     return DFSPH_OK;
 }
@@ -1864,10 +1962,7 @@ int phase_slots(Handle* H, hipStream_t st) {
 int phase_density(Handle* H, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
-    {
-        PROF(KID_DENSITY);
-        LAUNCH_W(k_density_alpha, H->n_chunks, 256, c);
-    }
+    LAUNCH_WT(KID_DENSITY, k_density_alpha, H->n_chunks, 256, c);
     k_world<<<c.W, 256, 0, st>>>(c, W_NACTIVE, StopSpec{});
     LAUNCH_CHECK();
     return DFSPH_OK;
@@ -1956,15 +2051,12 @@ int enq_solve_head(Handle* H, const SolveSetup& S, hipStream_t st) {
                                                       float(P.rest_jitter_flow_ratio), float(P.max_density_error),
                                                       float(std::min(P.min_density_error, P.max_density_error)));
     }
-    {
-        PROF(KID_SOLVE_INIT);
-        LAUNCH_W(k_solve_init, blocks_for(c.n), 256, c, S.mode, S.dt, S.warm, float(P.warm_start_density), S.relief,
-                 S.store, S.learned);
-    }
+    LAUNCH_WT(KID_SOLVE_INIT, k_solve_init, blocks_for(c.n), 256, c, S.mode, S.dt, S.warm,
+              float(P.warm_start_density), S.relief, (const float*)S.store, S.learned);
     k_begin_solve<<<blocks_for(c.W), 256, 0, st>>>(c);
     if (P.pressure_solver == 0) {
-        LAUNCH_W(k_accel, H->n_chunks, 256, c, c.kappa, c.kr, 0);
-        LAUNCH_W(k_apply, H->n_chunks, 256, c, S.mode, S.dt, c.kappa, c.wv, 0, 1, 0, 0);
+        LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.kappa, c.kr, 0);
+        LAUNCH_WT(KID_APPLY, k_apply, H->n_chunks, 256, c, S.mode, S.dt, c.kappa, c.wv, 0, 1, 0, 0);
         k_cg_init<<<H->n_chunks, 256, 0, st>>>(c, S.mode, S.dt);
         k_world<<<c.W, 256, 0, st>>>(c, W_CG_INIT, S.sp);
     }
@@ -1979,23 +2071,17 @@ int enq_solve_batch(Handle* H, const SolveSetup& S, hipStream_t st, int* flag_ou
     float dt = S.dt;
     for (int rep = 0; rep < H->check_every; ++rep) {
         if (H->P.pressure_solver == 0) {
-            {
-                PROF(KID_ACCEL);
-                LAUNCH_W(k_accel, H->n_chunks, 256, c, c.pd, c.pdr, 0);
-            }
-            {
-                PROF(KID_APPLY);
-                LAUNCH_W(k_apply, H->n_chunks, 256, c, mode, dt, c.pd, c.ap, 0, 0, 0, 1);
-            }
+            LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.pd, c.pdr, 0);
+            LAUNCH_WT(KID_APPLY, k_apply, H->n_chunks, 256, c, mode, dt, c.pd, c.ap, 0, 0, 0, 1);
             k_world<<<c.W, 256, 0, st>>>(c, W_CG_PAP, S.sp);
             k_cg_step<<<H->n_chunks, 256, 0, st>>>(c);
-            LAUNCH_W(k_accel, H->n_chunks, 256, c, c.corr, c.corrr, 1);
-            LAUNCH_W(k_apply, H->n_chunks, 256, c, mode, dt, c.corr, c.wv, 1, 0, 1, 0);
+            LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.corr, c.corrr, 1);
+            LAUNCH_WT(KID_APPLY, k_apply, H->n_chunks, 256, c, mode, dt, c.corr, c.wv, 1, 0, 1, 0);
             k_cg_finish<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
             k_world<<<c.W, 256, 0, st>>>(c, W_CG_BETA, S.sp);
             k_cg_direction<<<H->n_chunks, 256, 0, st>>>(c, rep == H->check_every - 1 ? flag_out : nullptr);
         } else {
-            LAUNCH_W(k_accel, H->n_chunks, 256, c, c.kappa, c.kr, 0);
+            LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.kappa, c.kr, 0);
             LAUNCH_W(k_jacobi, H->n_chunks, 256, c, mode, dt);
             k_world<<<c.W, 256, 0, st>>>(c, W_JACOBI, S.sp);
         }
@@ -2007,10 +2093,7 @@ int enq_solve_batch(Handle* H, const SolveSetup& S, hipStream_t st, int* flag_ou
 // contact limit, apply pressure, lone friction; stats (solve_driver.py:151-162)
 int enq_solve_tail(Handle* H, const SolveSetup& S, hipStream_t st) {
     const Ctx& c = S.c;
-    {
-        PROF(KID_POST);
-        LAUNCH_W(k_post_solve, blocks_for(c.n), 256, c, S.mode, S.dt, S.store);
-    }
+    LAUNCH_WT(KID_POST, k_post_solve, blocks_for(c.n), 256, c, S.mode, S.dt, S.store);
     k_record<<<blocks_for(c.W), 256, 0, st>>>(c, S.mode, float(S.tolh));
     LAUNCH_CHECK();
     return DFSPH_OK;
@@ -2090,10 +2173,7 @@ int phase_solve(Handle* H, int mode, double dt_h, int fixed_iters, hipStream_t s
 int phase_nonpressure(Handle* H, double dt, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
-    {
-        PROF(KID_NONPRESSURE);
-        LAUNCH_W(k_nonpressure, blocks_for(c.n), 256, c, float(dt));
-    }
+    LAUNCH_WT(KID_NONPRESSURE, k_nonpressure, blocks_for(c.n), 256, c, float(dt));
     k_copy_f32<<<blocks_for(3 * int64_t(c.n)), 256, 0, st>>>(c.qd, c.vtmp, 3 * int64_t(c.n));
     LAUNCH_CHECK();
     return DFSPH_OK;
@@ -2103,10 +2183,7 @@ int phase_advect(Handle* H, double dt, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
     k_fill_f32<<<blocks_for(c.W), 256, 0, st>>>(c.st_max_speed, 0.0f, c.W);
-    {
-        PROF(KID_ADVECT);
-        k_advect_project<<<H->n_chunks, 256, 0, st>>>(c, float(dt));
-    }
+    LAUNCH_T(KID_ADVECT, k_advect_project, H->n_chunks, 256, c, float(dt));
     LAUNCH_CHECK();
     return DFSPH_OK;
 }
@@ -2212,6 +2289,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
             wc.push_back(int(cb.size()));
         }
         H->n_chunks = int(cb.size());
+        c.n_chunks = H->n_chunks;
         TRY(dupload(H, (int**)&c.ch_begin, cb.data(), int64_t(cb.size())));
         TRY(dupload(H, (int**)&c.ch_end, ce.data(), int64_t(ce.size())));
         TRY(dupload(H, (int**)&c.ch_world, cw.data(), int64_t(cw.size())));
@@ -2326,8 +2404,8 @@ int32_t dfsph_destroy(void* h) {
     if (!h) return DFSPH_OK;
     Handle* H = as_handle(h);
     (void)hipDeviceSynchronize();
-    for (auto* v : {&H->prof.a, &H->prof.b})
-        for (hipEvent_t e : *v) (void)hipEventDestroy(e);
+    (void)prof_harvest(H, true);
+    for (hipEvent_t e : H->prof.pool) (void)hipEventDestroy(e);
     for (auto& kv : H->graphs) (void)hipGraphExecDestroy(kv.second);
     if (H->cap_stream) (void)hipStreamDestroy(H->cap_stream);
     for (void* p : H->owned) (void)hipFree(p);
@@ -2399,6 +2477,7 @@ int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void
     hipStream_t st = static_cast<hipStream_t>(stream);
     bind_state(H, q, qd, f);
     if (H->c.n == 0) return DFSPH_OK;
+    if (H->prof.on && prof_harvest(H, false)) return DFSPH_EDEVICE;
     float dtf = float(dt);
     int rc = run_segment(H, st, SEG_PRE, 0, dtf, [&](hipStream_t s) {
         int r;
@@ -2553,38 +2632,59 @@ int32_t dfsph_get_option(void* h, int32_t option, int32_t* value) {
     }
 }
 
-// per-kernel HIP-event timing (kernel ids: 0 accel, 1 apply, 2 build lists, 3 density, 4 slots, 5 non-pressure,
-// 6 advect/project, 7 solve init, 8 post-solve); enabling resets the counters
-int32_t dfsph_set_profiling(void* h, int32_t on) {
+// per-launch timing of a 1 / sample pseudo-random subset of the launches of the kernels in `mask` (bit k: kernel
+// id k; 0 stops): enabling resets the totals
+int32_t dfsph_set_profiling(void* h, int32_t mask, int32_t sample) {
     if (!h) return fail(DFSPH_EINVAL, "null handle");
+    if (sample < 1) return fail(DFSPH_EINVAL, "profiling sample must be >= 1");
     Handle* H = as_handle(h);
-    if (on && H->prof.a.empty()) {
-        H->prof.a.resize(KID_COUNT * PROF_CAP);
-        H->prof.b.resize(KID_COUNT * PROF_CAP);
-        for (int k = 0; k < KID_COUNT * PROF_CAP; ++k) {
-            HIPCHK(hipEventCreate(&H->prof.a[k]));
-            HIPCHK(hipEventCreate(&H->prof.b[k]));
+    Prof& P = H->prof;
+    int rc = prof_harvest(H, true);
+    if (rc) return rc;
+    const size_t nb = size_t(2) * size_t(std::max(H->n_chunks, 1)) * 4 * sizeof(unsigned long long);
+    if (mask && !P.d_bytes) {
+        void* q = nullptr;
+        HIPCHK(hipMalloc(&q, nb));
+        H->owned.push_back(q);
+        P.d_bytes = static_cast<unsigned long long*>(q);
+    }
+    if (mask) {
+        HIPCHK(hipMemset(P.d_bytes, 0, nb));
+        for (int k = 0; k < KID_COUNT; ++k) {
+            P.total_ms[k] = 0.0;
+            P.count[k] = 0;
+            P.launches[k] = 0;
         }
     }
-    H->prof.on = on != 0;
-    for (int k = 0; k < KID_COUNT; ++k) H->prof.used[k] = 0;
+    P.on = mask != 0;
+    P.mask = uint32_t(mask);
+    P.sample = sample;
     return DFSPH_OK;
 }
 
-// count and summed duration [ms] of kernel `kid`'s timed launches since profiling was enabled (syncs)
-int32_t dfsph_profile_read(void* h, int32_t kid, int32_t* count, double* total_ms) {
+// launches, summed duration [ms] and counted algorithmic bytes of kernel `kid` since profiling was enabled (syncs;
+// bytes are counted by k_accel and k_apply only, 0 for the others)
+int32_t dfsph_profile_read(void* h, int32_t kid, int32_t* count, double* total_ms, double* alg_bytes) {
     if (!h || !count || !total_ms) return fail(DFSPH_EINVAL, "null argument");
     if (kid < 0 || kid >= KID_COUNT) return fail(DFSPH_EINVAL, "unknown kernel id");
     Handle* H = as_handle(h);
     HIPCHK(hipDeviceSynchronize());
-    double t = 0.0;
-    for (int k = 0; k < H->prof.used[kid]; ++k) {
-        float ms = 0.0f;
-        HIPCHK(hipEventElapsedTime(&ms, H->prof.a[kid * PROF_CAP + k], H->prof.b[kid * PROF_CAP + k]));
-        t += ms;
+    int rc = prof_harvest(H, true);
+    if (rc) return rc;
+    *count = H->prof.count[kid];
+    *total_ms = H->prof.total_ms[kid];
+    if (alg_bytes) {
+        *alg_bytes = 0.0;
+        if (H->prof.d_bytes && (kid == KID_ACCEL || kid == KID_APPLY)) {
+            size_t m = size_t(std::max(H->n_chunks, 1)) * 4;
+            std::vector<unsigned long long> v(m);
+            HIPCHK(hipMemcpy(v.data(), H->prof.d_bytes + size_t(kid) * m, m * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost));
+            unsigned long long b = 0;
+            for (unsigned long long x : v) b += x;
+            *alg_bytes = double(b);
+        }
     }
-    *count = H->prof.used[kid];
-    *total_ms = t;
     return DFSPH_OK;
 }
 

@@ -20,7 +20,7 @@ from . import kernels as kn
 from .config import DFSPHConfig, full_neighbor_count
 from .model import GeoType, Model, ParticleFlags
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 DFSPH_OK, DFSPH_EINVAL, DFSPH_ECAPACITY, DFSPH_EDEVICE = 0, -1, -2, -3
 
@@ -313,8 +313,8 @@ def load_library() -> ctypes.CDLL:
         "dfsph_mlp_adam": ([vp, vp, vp, vp, dbl, dbl, dbl, dbl, i32, vp], i32),
         "dfsph_set_logging": ([vp, vp, vp], i32),
         "dfsph_get_buffer": ([vp, i32, P(vp), P(i64)], i32),
-        "dfsph_set_profiling": ([vp, i32], i32),
-        "dfsph_profile_read": ([vp, i32, P(i32), P(dbl)], i32),
+        "dfsph_set_profiling": ([vp, i32, i32], i32),
+        "dfsph_profile_read": ([vp, i32, P(i32), P(dbl), P(dbl)], i32),
         "dfsph_read_buffer": ([vp, i32, vp, i64], i32),
         "dfsph_write_buffer": ([vp, i32, vp, i64], i32),
         "dfsph_zero_stats": ([vp, vp], i32),

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import os
 
 import numpy as np
 import torch
@@ -24,6 +25,8 @@ from . import _abi
 
 N_FEATURES = 13
 FEAT_STRIDE = 16
+# the network trained by ``python -m alloy_amd.sph.train_warm_start`` (see that module for the data it learns from)
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "warm_start_mlp.npy")
 
 
 def _vp(t: torch.Tensor | None):
@@ -114,6 +117,19 @@ class WarmStartMLP:
             if log_every and (k % log_every == 0 or k == steps - 1):
                 losses.append(float(g[self.n_params].item()) / (2 * batch))
         return losses
+
+    def save(self, path: str) -> None:
+        """The float32 master parameters as a .npy file (numpy's format, no pickling)."""
+        np.save(path, self.params.detach().cpu().numpy().astype(np.float32), allow_pickle=False)
+
+    @classmethod
+    def load(cls, path: str, device="cuda") -> "WarmStartMLP":
+        net = cls(device)
+        p = np.load(path, allow_pickle=False)
+        if p.shape != (net.n_params,) or p.dtype != np.float32:
+            raise ValueError(f"{path}: expected float32[{net.n_params}] parameters, got {p.dtype}{list(p.shape)}")
+        net.params.copy_(torch.from_numpy(p).to(net.device))
+        return net
 
     def attach(self, solver) -> None:
         """Use this network as ``solver``'s warm start (the parameter tensor must stay alive)."""

@@ -78,15 +78,26 @@ def water_column(nx: int = 6, ny: int = 6, nz: int = 20, *, worlds: int = 1, fil
 
 
 def bowl_worlds(worlds: int = 512, seed: int = 2, n_min: int = 2000, n_max: int = 20000, side=(0.3, 0.6),
-                bowl_radius: float = 0.08, cfg: DFSPHConfig | None = None, device: str = "cpu") -> Scene:
+                bowl_radius: float = 0.08, cfg: DFSPHConfig | None = None, device: str = "cpu",
+                subset=None) -> Scene:
+    """``worlds`` box tanks of side U(side) with a two-layer particle-wall bowl each and a lattice block of
+    U[n_min, n_max] liquid particles (full layers of the block's footprint, the top layer partial) above it, ids in
+    the lattice order of build.py:26-31 (x slowest). ``subset``: build only these world indices, each exactly as in
+    the full pack (the seeded draws of every world are consumed either way)."""
+    from .build import add_fluid_points, lattice_points
+
     cfg = cfg or bench_config()
     rng = np.random.default_rng(seed)
     sp = cfg.spacing
+    keep_w = None if subset is None else set(int(w) for w in subset)
     b = ModelBuilder(gravity=(0.0, 0.0, -9.81))
     los, his, counts = [], [], []
-    for _w in range(worlds):
+    for w in range(worlds):
         L = float(rng.uniform(*side))
         N = int(rng.integers(n_min, n_max + 1))
+        jitter_seed = rng.integers(1 << 31)
+        if keep_w is not None and w not in keep_w:
+            continue
         nxy = max(1, int(np.floor(0.9 * L / sp)))
         nz = int(np.ceil(N / (nxy * nxy)))
         rim = 2.0 * sp + bowl_radius
@@ -97,14 +108,10 @@ def bowl_worlds(worlds: int = 512, seed: int = 2, n_min: int = 2000, n_max: int 
         add_box_tank(b, lo, hi, wall=cfg.h)
         add_wall_particles(b, cfg, bowl_points((0.5 * L, 0.5 * L, rim), bowl_radius, sp, layers=2))
         off = 0.5 * (L - nxy * sp)
-        pts_lo = (off, off, z0)
-        rng_j = np.random.default_rng(rng.integers(1 << 31))
-        # N particles: full layers of nxy^2, the top layer partial
-        from .build import add_fluid_points, lattice_points
-
-        pts = lattice_points(pts_lo, (nxy, nxy, nz), sp)
-        pts = pts[np.lexsort((pts[:, 1], pts[:, 0], pts[:, 2]))][:N]
-        pts = pts + rng_j.uniform(-1e-4, 1e-4, pts.shape)
+        pts = lattice_points((off, off, z0), (nxy, nxy, nz), sp)
+        pts = pts[np.lexsort((pts[:, 1], pts[:, 0], pts[:, 2]))][:N]  # the lowest N lattice sites, layer by layer
+        pts = pts[np.lexsort((pts[:, 2], pts[:, 1], pts[:, 0]))]       # back to x-slowest lattice order
+        pts = pts + np.random.default_rng(jitter_seed).uniform(-1e-4, 1e-4, pts.shape)
         add_fluid_points(b, cfg, pts)
         b.end_world()
         los.append(lo)

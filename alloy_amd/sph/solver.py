@@ -131,6 +131,7 @@ class SolverDFSPH:
         self._mass_in = np.zeros(W)
         self._mass_out = np.zeros(W)
         self._escaped = np.zeros(W, np.int32)
+        self._ws_dev = torch.as_tensor(model.world_start_np, dtype=torch.int64, device=self.device)
         log.info("SolverDFSPH: %d particles in %d worlds, r=%g m, h=%g m, %d shapes, %d wall particles, solver=%s",
                  model.particle_count, W, cfg.particle_radius, cfg.h, model.shape_count, self.tables.wall_count,
                  self.pressure_solver)
@@ -276,11 +277,10 @@ class SolverDFSPH:
         if m.particle_count == 0:
             return np.zeros(m.world_count)
         act = ((m.particle_flags & int(ParticleFlags.ACTIVE)) != 0) & (m.sph.phase >= PHASE_LIQUID)
-        cum = torch.cumsum(torch.where(act, m.particle_mass.double(), torch.zeros((), dtype=torch.float64,
-                                                                                  device=self.device)), 0).cpu()
-        cum = np.concatenate([[0.0], cum.numpy()])
-        ws = m.world_start_np
-        return cum[ws[1:]] - cum[ws[:-1]]
+        w = torch.where(act, m.particle_mass.double(), torch.zeros((), dtype=torch.float64, device=self.device))
+        cum = torch.cat([torch.zeros(1, dtype=torch.float64, device=self.device), torch.cumsum(w, 0)])
+        at = cum.index_select(0, self._ws_dev).cpu().numpy()  # W + 1 values cross to the host, not n
+        return at[1:] - at[:-1]
 
     def mass_ledger(self) -> dict[str, np.ndarray]:
         """Per-world mass ledger [kg] (runtime.py:129-134): active + out == initial + in."""

@@ -1,17 +1,23 @@
-"""Benchmark of the DFSPH liquid step on one MI355X: particle-steps/s at the 0.1 % density-error tolerance.
+"""Benchmark of the DFSPH liquid step on one MI355X: particle-steps/s at the 0.1 % density-error tolerance, the
+batched learned-warm-start solver against plain single-world DFSPH.
 
-Default workload: the 125,000-particle dam break (50^3 lattice block, spacing 0.01 m, jitter U(+-1e-4) m from seed
-1, in one corner of a 1.0 m SDF box tank), plain DFSPH, fp32 physics, dt 1 ms, 500 timed steps after 20 warm-up
-steps. Inputs are resident in HBM when the timed region starts; the timed region is bracketed by a device
-synchronisation and timed with HIP events on the solver's stream. One JSON line is printed.
+Default workload: 512 batched worlds in one pack, each a box tank of side U(0.3, 0.6) m with a static two-layer
+particle-wall bowl (hemisphere, radius 0.08 m) and a seeded lattice block of U{2,000..20,000} liquid particles
+(5,672,431 in all, plus 549,888 wall particles), fp32 physics, dt 1 ms, 300 timed steps after 20 warm-up steps,
+every world one pressure solve per phase launch. The learned warm start (the committed trained network) predicts
+both solves' initial kappa when its weights are present.
 
-The line also carries
-- "roofline": the dominant kernel's algorithmic bytes per launch / its average launch duration, both measured live
-  over the timed region with HIP events around each launch on the launch stream; "traffic" is the HBM bytes per
-  launch from rocprofv3 FETCH_SIZE + WRITE_SIZE counters (profiles/), or null when no counter run is recorded;
-- "cpu_baseline": the CPU oracle (the same algorithm, float32, one core) timed on this host on a bounded sample of
-  the same workload;
-- "fp64_density_err": mean max(rho/rho0 - 1, 0) over the liquid of the final state, evaluated by the FP64 oracle.
+`value` is the batched (+learned) run's liquid particle-steps per second: inputs resident in HBM, the timed region
+bracketed by device synchronisations and timed with HIP events on the solver's stream. The line also carries
+- "plain": the same worlds stepped as plain single-world DFSPH (one solver, i.e. one launch per phase, per world,
+  no learned start), on an even sample of the worlds, and `speedup_vs_plain`;
+- "roofline": the dominant kernel's algorithmic bytes, counted by the kernels themselves over the timed region,
+  divided by its summed launch durations (HIP events inside each launch's dispatch, so they agree with
+  rocprofv3's kernel trace); "traffic" is the HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs
+  (profiles/), or null without one;
+- "cpu_baseline": the CPU oracle (the same algorithm, float32, one core) on a bounded sample of the workload;
+- "fp64_density_err": mean max(rho/rho0 - 1, 0) of the final state's liquid, evaluated by the FP64 oracle.
+`--workload dam_break_125k` (the 125,000-particle dam break) and the others measure a single world.
 """
 
 from __future__ import annotations
@@ -19,7 +25,6 @@ from __future__ import annotations
 import argparse
 import ctypes
 import json
-import math
 import os
 import sys
 import time
@@ -30,56 +35,133 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s at 0.1% density error, learned+batched vs plain DFSPH, 1 GPU"
 WORKLOADS = {
+    "bowl_worlds_512": dict(kind="bowl_worlds", worlds=512, seed=2),
     "dam_break_125k": dict(kind="dam_break", n_side=50, box=1.0, seed=1),
     "dam_break_8k": dict(kind="dam_break", n_side=20, box=0.4, seed=0),
     "dam_break_1m": dict(kind="dam_break", n_side=100, box=2.0, seed=3),
-    "bowl_worlds_512": dict(kind="bowl_worlds", worlds=512, seed=2),
 }
 
 
-def make_scene(wl: dict, device: str):
+def make_scene(wl: dict, device: str, subset=None):
     from alloy_amd.sph import scenes
 
     if wl["kind"] == "dam_break":
         return scenes.dam_break(wl["n_side"], wl["box"], wl["seed"], device=device)
-    return scenes.bowl_worlds(wl["worlds"], seed=wl["seed"], device=device)
+    return scenes.bowl_worlds(wl["worlds"], seed=wl["seed"], device=device, subset=subset)
 
 
 def describe(wl: dict, sc) -> str:
     n = sum(sc.fluid_counts)
     if wl["kind"] == "dam_break":
-        return (f"plain DFSPH dam break, {n} particles ({wl['n_side']}^3 lattice, spacing 0.01 m, jitter 1e-4 m, seed "
-                f"{wl['seed']}) in a {wl['box']} m SDF box tank, dt 1 ms, 0.1 % density and divergence tolerance, "
-                "projected CG, 1 world")
-    return (f"batched DFSPH, {wl['worlds']} worlds in one pack (box tanks of side U(0.3, 0.6) m, a static particle-wall "
-            f"bowl of radius 0.08 m each, U(2000, 20000) liquid particles each, seed {wl['seed']}): {n} liquid "
-            f"particles, {sc.model.particle_count - n} wall particles, dt 1 ms, 0.1 % tolerances, projected CG")
+        return (f"dam break, {n} particles ({wl['n_side']}^3 lattice, spacing 0.01 m, jitter 1e-4 m, seed {wl['seed']}) "
+                f"in a {wl['box']} m SDF box tank, dt 1 ms, 0.1 % density and divergence tolerance, projected CG, "
+                "1 world")
+    return (f"{wl['worlds']} batched worlds (box tanks of side U(0.3, 0.6) m, a static particle-wall bowl of radius "
+            f"0.08 m each, U(2000, 20000) liquid particles each, seed {wl['seed']}): {n} liquid particles, "
+            f"{sc.model.particle_count - n} wall particles, dt 1 ms, 0.1 % tolerances, projected CG")
 
 
 def _log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(kernel: str, n_fluid: int, n_other: int, sum_count: int, n_slots: int) -> float:
-    """Compulsory bytes of one launch over the step's particles: every per-particle array the kernel reads or
-    writes, once; neighbour attributes counted in their owner's read; neighbour entries 2 B (uint16 rows); slot
-    gradients 12 B per filled slot, the slot-shape row 16 B per particle; non-liquid particles read their flags and
-    phase (and write zeros)."""
-    if kernel == "k_apply":
-        # q 12, acc 12, mass 4, rho0 4, count 4, flags 4, phase 1, world 4, reg 4, vec 4, out 4, slot shapes 16
-        return n_fluid * 77.0 + 2.0 * sum_count + 12.0 * n_slots + n_other * 9.0
-    if kernel == "k_accel":
-        # q 12, vec 4, rho0 4, mass 4, count 4, flags 4, phase 1, world 4, slot shapes 16, acc out 12
-        return n_fluid * 65.0 + 2.0 * sum_count + 12.0 * n_slots + n_other * 17.0
-    raise KeyError(kernel)
+def sample_worlds(W: int, k: int) -> list[int]:
+    return sorted(set(np.linspace(0, W - 1, min(k, W)).round().astype(int).tolist()))
 
 
-def cpu_baseline(workload: dict, steps: int, warmup: int) -> dict:
+def load_learned(mode: str, device: str):
+    """The committed trained warm-start network, or None (mode auto: when its weights exist)."""
+    from alloy_amd.sph.learned import DEFAULT_WEIGHTS, WarmStartMLP
+
+    if mode == "off" or (mode == "auto" and not os.path.exists(DEFAULT_WEIGHTS)):
+        return None
+    return WarmStartMLP.load(DEFAULT_WEIGHTS, device)
+
+
+def timed_run(solver, state, steps: int, dt: float, prof_mask: int = 0, sample: int = 1) -> dict:
+    """`steps` steps timed with HIP events on the current stream, a report (the step's only host sync point) every
+    10 steps; over the same region, per-launch timing of a pseudo-random 1 / `sample` of the launches of the kernels
+    in `prof_mask` (a timed launch costs the host more than a plain one; sampling keeps that out of the step time)."""
+    import torch
+
+    from alloy_amd.sph import _abi
+
+    lib, h = solver.lib, solver._h
+    stream = torch.cuda.current_stream()
+    if prof_mask:
+        _abi.check(lib.dfsph_set_profiling(h, prof_mask, sample), lib)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    den, it_den, it_div, unconv = [], [], [], 0
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for k in range(steps):
+        solver.step(state, state, None, None, dt)
+        if k % 10 == 9:
+            r = solver.report()
+            den.append(float(np.mean(r.density_err)))
+            it_den.append(float(np.mean(r.iters_density)))
+            it_div.append(float(np.mean(r.iters_divergence)))
+            unconv += int(np.sum(r.unconverged))
+    e1.record(stream)
+    torch.cuda.synchronize()
+    out = {"total_ms": e0.elapsed_time(e1), "density_err_mean": float(np.mean(den)) if den else None,
+           "density_err_max_sampled": float(np.max(den)) if den else None,
+           "iters_density_mean": float(np.mean(it_den)) if it_den else None,
+           "iters_divergence_mean": float(np.mean(it_div)) if it_div else None, "unconverged_sampled_steps": unconv}
+    if prof_mask:
+        ks = {}
+        for kid, name in enumerate(_abi.KERNEL_NAMES):
+            if not (prof_mask >> kid) & 1:
+                continue
+            c, t, b = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+            _abi.check(lib.dfsph_profile_read(h, kid, ctypes.byref(c), ctypes.byref(t), ctypes.byref(b)), lib)
+            ks[name] = (int(c.value), float(t.value), float(b.value))
+        _abi.check(lib.dfsph_set_profiling(h, 0, 1), lib)
+        out["kernels"] = ks
+    return out
+
+
+def plain_sample(wl: dict, worlds: list[int], steps: int, warmup: int, dt: float, pair_cache) -> dict:
+    """The sampled worlds as plain DFSPH: one single-world solver each (every phase one launch over that world), no
+    learned start, run one after another for the same warm-up and step counts."""
+    import torch
+
+    from alloy_amd.sph import _abi
+
+    n_tot, ms_tot, den, its = 0, 0.0, [], []
+    for w in worlds:
+        sc = make_scene(wl, "cuda", subset=[w])
+        solver = sc.make_solver()
+        if pair_cache is not None:
+            solver.set_option(_abi.OPT_PAIR_CACHE, pair_cache)
+        state = sc.model.state()
+        for _ in range(warmup):
+            solver.step(state, state, None, None, dt)
+        r = timed_run(solver, state, steps, dt)
+        n_tot += sc.fluid_counts[0] * steps
+        ms_tot += r["total_ms"]
+        den.append(r["density_err_mean"])
+        its.append((r["iters_density_mean"], r["iters_divergence_mean"]))
+        del solver, state, sc
+        torch.cuda.empty_cache()
+    return {"value": n_tot / (ms_tot * 1e-3), "unit": "particle-steps/s", "worlds_sampled": len(worlds),
+            "world_ids": worlds, "steps": steps, "warmup": warmup, "ms_per_world_step": ms_tot / (steps * len(worlds)),
+            "density_err_mean": float(np.mean([d for d in den if d is not None])),
+            "iters_density_mean": float(np.mean([i[0] for i in its])),
+            "iters_divergence_mean": float(np.mean([i[1] for i in its])),
+            "solver": "plain DFSPH, one single-world solver per world stepped in turn, no learned warm start"}
+
+
+def cpu_baseline(wl: dict, steps: int, warmup: int, sample_w: int) -> dict:
+    """The CPU oracle (float32, serial C++) on a bounded sample: the whole scene for single-world workloads, an
+    even sample of `sample_w` worlds (each exactly as in the pack) for the batched one."""
     from alloy_amd.sph import scenes
     from oracle.oracle import OracleDFSPH
 
-    sc = make_scene(workload, "cpu")
+    subset = sample_worlds(wl["worlds"], sample_w) if wl["kind"] == "bowl_worlds" else None
+    sc = make_scene(wl, "cpu", subset=subset)
     o = OracleDFSPH(sc.model, sc.config, 32)
     for _ in range(warmup):
         o.step(scenes.DT)
@@ -88,9 +170,10 @@ def cpu_baseline(workload: dict, steps: int, warmup: int) -> dict:
         o.step(scenes.DT)
     dt = time.perf_counter() - t
     n = sum(sc.fluid_counts)
+    what = f"{len(subset)} of the {wl['worlds']} worlds ({n} liquid particles)" if subset else f"{n} particles"
     return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"CPU oracle (float32, serial C++ -O3) on the same scene: {n} particles, {warmup} warm-up + "
-                      f"{steps} timed steps ({dt:.1f} s)"}
+            "sample": f"CPU oracle (float32, serial C++ -O3, one core) on {what} of the same workload from its "
+                      f"initial state: {warmup} warm-up + {steps} timed steps ({dt:.1f} s)"}
 
 
 def fp64_density_error(scene, q: np.ndarray, qd: np.ndarray) -> float:
@@ -118,15 +201,24 @@ def traffic_from_profiles(kernel: str, workload: str):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="dam_break_125k", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="bowl_worlds_512", choices=sorted(WORKLOADS))
+    ap.add_argument("--learned", default="auto", choices=["auto", "on", "off"],
+                    help="learned warm start (auto: on when the trained weights are present)")
+    ap.add_argument("--plain-worlds", type=int, default=16, help="worlds sampled for the plain single-world run")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--check-every", type=int, default=1, help="iterations between host convergence checks")
     ap.add_argument("--graphs", action="store_true", help="replay the step's segments as hipGraphs")
     ap.add_argument("--pair-cache", type=int, default=None, help="DFSPH_PAIR_CACHE_* mask (default: the library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-worlds", type=int, default=4, help="worlds of the batched pack in the CPU sample")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--roofline-sample", type=int, default=16,
+                    help="time one in this many k_accel / k_apply launches of the timed region (pseudo-random)")
+    ap.add_argument("--breakdown-steps", type=int, default=30,
+                    help="steps after the timed region with every kernel timed (per-kernel table)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (3 launches) and exit: the known-size "
                          "kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings")
@@ -151,6 +243,7 @@ def main() -> int:
     from alloy_amd.sph import _abi, scenes
 
     wl = WORKLOADS[args.workload]
+    dt = scenes.DT
     _log(f"[bench] workload {args.workload} {wl} steps={args.steps} warmup={args.warmup}")
     sc = make_scene(wl, "cuda")
     solver = sc.make_solver()
@@ -158,59 +251,29 @@ def main() -> int:
     solver.set_graphs(args.graphs)
     if args.pair_cache is not None:
         solver.set_option(_abi.OPT_PAIR_CACHE, args.pair_cache)
+    net = load_learned(args.learned, "cuda")
+    if net is not None:
+        net.attach(solver)
     state = sc.model.state()
     n_fluid = sum(sc.fluid_counts)
     W = sc.model.world_count
-    n_all = sc.model.particle_count
-    dt = scenes.DT
     for _ in range(args.warmup):
         solver.step(state, state, None, None, dt)
     torch.cuda.synchronize()
     _log("[bench] warm-up done")
-    lib, h = solver.lib, solver._h
-    stream = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    den, it_den, it_div, unconv = [], [], [], 0
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for k in range(args.steps):
-        solver.step(state, state, None, None, dt)
-        if k % 10 == 9:
-            r = solver.report()
-            den.append(float(np.mean(r.density_err)))
-            it_den.append(float(np.mean(r.iters_density)))
-            it_div.append(float(np.mean(r.iters_divergence)))
-            unconv += int(np.sum(r.unconverged))
-    e1.record(stream)
-    torch.cuda.synchronize()
-    total_ms = e0.elapsed_time(e1)
+    roof_mask = (1 << _abi.KID_ACCEL) | (1 << _abi.KID_APPLY)
+    run = timed_run(solver, state, args.steps, dt, prof_mask=roof_mask, sample=args.roofline_sample)
+    total_ms = run["total_ms"]
     _log(f"[bench] timed {args.steps} steps in {total_ms:.1f} ms")
-    # per-kernel durations: HIP events around every launch of the main kernels on the launch stream, over a second
-    # window of the same run (event records need eager launches; the kernels are the same)
-    prof_steps = max(min(args.steps, 50), 1)
-    _abi.check(lib.dfsph_set_profiling(h, 1), lib)
-    for _ in range(prof_steps):
-        solver.step(state, state, None, None, dt)
-    torch.cuda.synchronize()
-    kstats = {}
-    for kid, name in enumerate(_abi.KERNEL_NAMES):
-        c, t = ctypes.c_int32(), ctypes.c_double()
-        _abi.check(lib.dfsph_profile_read(h, kid, ctypes.byref(c), ctypes.byref(t)), lib)
-        kstats[name] = (int(c.value), float(t.value))
-    _abi.check(lib.dfsph_set_profiling(h, 0), lib)
-    # per-launch algorithmic bytes from the final state's lists and slots (K and filled slots drift slowly)
-    cnt = solver.buffer(_abi.BUF_COUNT, np.int32, (n_all,))
-    shp = solver.buffer(_abi.BUF_SLOT_SHAPE, np.int32, (n_all, 4))
-    sum_count, n_slots = int(cnt.sum()), int((shp >= 0).sum())
-    dom = max(("k_apply", "k_accel"), key=lambda k: kstats[k][1])
-    nl, tl = kstats[dom]
-    avg_ms = tl / max(nl, 1)
-    bytes_launch = algorithmic_bytes(dom, n_fluid, n_all - n_fluid, sum_count, n_slots)
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    ks = run["kernels"]
+    dom = max(ks, key=lambda k: ks[k][1])
+    nl, tl, bl = ks[dom]
+    achieved = bl / (tl * 1e-3) / 1e9
     value = n_fluid * args.steps / (total_ms * 1e-3)
-    steps_total = args.steps + prof_steps
+    solver_desc = ("batched, one launch per phase over all worlds" if W > 1 else "single world") + (
+        ", learned warm start (trained network)" if net is not None else ", plain warm start")
     out = {
-        "metric": "particle-steps/s at 0.1% density error, learned+batched vs plain DFSPH, 1 GPU",
+        "metric": METRIC,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -222,29 +285,40 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": describe(wl, sc), "particles": n_fluid, "worlds": W,
-                   "solver": "plain (no learned warm start)"},
-        "density_err_mean": float(np.mean(den)) if den else None,
-        "density_err_max_sampled": float(np.max(den)) if den else None,
-        "iters_density_mean": float(np.mean(it_den)) if it_den else None,
-        "iters_divergence_mean": float(np.mean(it_div)) if it_div else None,
-        "unconverged_sampled_steps": unconv,
+        "config": {"workload": describe(wl, sc), "particles": n_fluid, "worlds": W, "solver": solver_desc},
+        "density_err_mean": run["density_err_mean"],
+        "density_err_max_sampled": run["density_err_max_sampled"],
+        "iters_density_mean": run["iters_density_mean"],
+        "iters_divergence_mean": run["iters_divergence_mean"],
+        "unconverged_sampled_steps": run["unconverged_sampled_steps"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(dom, args.workload),
-                     "alg_bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms, "launches": nl,
-                     "mean_neighbors": sum_count / max(n_fluid, 1), "filled_slots": n_slots},
-        "kernel_ms_per_step": {k: round(v[1] / prof_steps, 4) for k, v in kstats.items()},
-        "kernel_launches_per_step": {k: round(v[0] / prof_steps, 2) for k, v in kstats.items()},
-        "profiled_steps": prof_steps,
+                     "alg_bytes_per_launch": bl / max(nl, 1), "avg_launch_ms": tl / max(nl, 1), "launches_timed": nl,
+                     "window": f"the timed region, 1 in {args.roofline_sample} launches (pseudo-random)"},
+        "kernel_ms_per_step_timed": {k: round(v[1] / args.steps, 4) for k, v in ks.items()},
     }
+    if args.breakdown_steps > 0:
+        bd = timed_run(solver, state, args.breakdown_steps, dt, prof_mask=(1 << len(_abi.KERNEL_NAMES)) - 1)
+        out["kernel_ms_per_step"] = {k: round(v[1] / args.breakdown_steps, 4) for k, v in bd["kernels"].items()}
+        out["kernel_launches_per_step"] = {k: round(v[0] / args.breakdown_steps, 2) for k, v in bd["kernels"].items()}
+        out["breakdown_steps"] = args.breakdown_steps
     if not args.no_fp64:
         torch.cuda.synchronize()
+        _log("[bench] FP64 density error of the final state")
         out["fp64_density_err"] = fp64_density_error(sc, state.particle_q.cpu().numpy(),
                                                      state.particle_qd.cpu().numpy())
         _log(f"[bench] fp64 density error {out['fp64_density_err']:.3e}")
+    del solver, state
+    torch.cuda.empty_cache()
+    if wl["kind"] == "bowl_worlds" and not args.no_plain:
+        _log(f"[bench] plain single-world DFSPH on {args.plain_worlds} sampled worlds")
+        plain = plain_sample(wl, sample_worlds(W, args.plain_worlds), args.steps, args.warmup, dt, args.pair_cache)
+        out["plain"] = plain
+        out["speedup_vs_plain"] = value / plain["value"]
+        _log(f"[bench] plain {plain['value']:.3e} particle-steps/s, speedup {out['speedup_vs_plain']:.2f}")
     if not args.no_cpu_baseline:
         _log("[bench] timing the CPU oracle")
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps, 2)
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps, 2, args.cpu_worlds)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define DFSPH_ABI_VERSION 1
+#define DFSPH_ABI_VERSION 2
 
 #define DFSPH_OK 0
 #define DFSPH_EINVAL (-1)
@@ -227,12 +227,16 @@ int32_t dfsph_get_option(void* handle, int32_t option, int32_t* value);
  * per (dt, state pointers, batch size); 0 launches every kernel eagerly. Results are identical either way. */
 int32_t dfsph_set_graphs(void* handle, int32_t on);
 
-/* Per-kernel HIP-event timing on the launch stream: kernel ids 0 pressure acceleration, 1 operator apply,
- * 2 neighbour rows, 3 density/alpha, 4 boundary slots, 5 non-pressure, 6 advect/project, 7 solve init,
- * 8 post-solve. No counterpart in the reference (its timing is wall clock around graph replays,
- * alloy/testing/sph.py:334-364); for measurement only. */
-int32_t dfsph_set_profiling(void* handle, int32_t on);
-int32_t dfsph_profile_read(void* handle, int32_t kernel_id, int32_t* count, double* total_ms);
+/* Per-launch timing of the kernels selected by `mask` (bit k = kernel id k: 0 pressure acceleration, 1 operator
+ * apply, 2 neighbour rows, 3 density/alpha, 4 boundary slots, 5 non-pressure, 6 advect/project, 7 solve init,
+ * 8 post-solve; 0 stops), of a pseudo-random 1 / `sample` of their launches (a timed launch costs the host more).
+ * The events are part of each timed kernel's dispatch and skip the system-scope fence, so durations agree with a
+ * profiler's kernel trace; timed k_accel and k_apply launches also count the algorithmic bytes of the particles
+ * they process. No counterpart in the reference (its timing is wall clock around graph replays,
+ * alloy/testing/sph.py:334-364); for measurement only. Enabling resets the totals. */
+int32_t dfsph_set_profiling(void* handle, int32_t mask, int32_t sample);
+/* launches, summed duration [ms] and algorithmic bytes (may be NULL) of kernel id `kernel_id` (synchronizes) */
+int32_t dfsph_profile_read(void* handle, int32_t kernel_id, int32_t* count, double* total_ms, double* alg_bytes);
 
 /* Learned warm start network (13 -> 64 -> 64 -> 64 -> 2, SiLU; float32 master parameters, bf16 on the MFMA units).
  * No counterpart in the reference: these train and evaluate the model that dfsph_set_warm_start_model installs.
