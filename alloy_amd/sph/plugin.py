@@ -1,4 +1,4 @@
-"""Registers SolverDFSPH as the ``sph`` solver capability (alloy/solvers/sph/plugin.py:36-145).
+"""Registers SolverDFSPH as the ``sph`` solver capability (alloy/solvers/sph/plugin.py:92-114).
 
 The factory takes ``config`` (a DFSPHConfig) or config fields as kwargs plus ``capacity_per_world``; unknown kwargs
 raise TypeError. The health adapter reports, per world, the last step's total pressure iterations and its mean
@@ -21,7 +21,8 @@ _CONFIG_FIELDS = frozenset(f.name for f in dataclasses.fields(DFSPHConfig))
 
 
 def _factory(view: Any, kwargs: Mapping[str, Any]):
-    """Entry factory (plugin.py:64-93): ``config`` or Config fields, plus capacity_per_world."""
+    """Entry factory (plugin.py:36-65): ``config`` or Config fields, plus capacity_per_world; unknown kwargs raise
+    TypeError. The reference's automatic escape box and cloth absorption (plugin.py:48-64) are outside this path."""
     from .solver import SolverDFSPH
 
     kw = dict(kwargs)
@@ -38,7 +39,7 @@ def _factory(view: Any, kwargs: Mapping[str, Any]):
 
 
 class _SPHEmitter:
-    """Emitter protocol adapter (plugin.py:96-100): scene objects become liquid blocks."""
+    """Emitter protocol adapter (plugin.py:68-73): scene objects become liquid blocks."""
 
     def emit(self, builder: Any, obj: Any, decision: Any) -> None:
         from .build import add_fluid_block
@@ -47,7 +48,7 @@ class _SPHEmitter:
 
 
 def _health_adapter(solver, sample: HealthSample) -> None:
-    """iters = iters_den + iters_div, residual = den_err per world (plugin.py:103-117)."""
+    """iters = iters_den + iters_div, residual = den_err per world (plugin.py:76-89)."""
     r = solver.report()
     sample.iters[:] = np.asarray(r.iters_density) + np.asarray(r.iters_divergence)
     sample.residual[:] = np.asarray(r.density_err)
