@@ -139,12 +139,6 @@ __device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
 __device__ __forceinline__ f3 A4(const Ctx& c, int i) { return xyz(c.acc4[i]); }
 __device__ __forceinline__ f3 V(const Ctx& c, int i) { return ld3(c.qd, i); }
 
-__device__ __forceinline__ int nbr_j(const Ctx& c, int base, int i, int k) {
-    int j = base + int(c.nbr[int64_t(k) * c.n + i]);
-    if (c.wide) j += int(c.nbr_hi[int64_t(k) * c.n + i]) << 16;
-    return j;
-}
-
 template <bool WIDE>
 __device__ __forceinline__ int nbr_at(const Ctx& c, int base, int i, int k) {
     int j = base + int(c.nbr[int64_t(k) * c.n + i]);
@@ -160,7 +154,7 @@ __device__ __forceinline__ int nbr_at(const Ctx& c, int base, int i, int k) {
 constexpr int NB_U = 8;
 template <class T>
 __device__ __forceinline__ float4 load_pos(const T& d) { return d.q; }
-__device__ __forceinline__ float4 load_pos(float4 d) { return d; }
+[[maybe_unused]] __device__ __forceinline__ float4 load_pos(float4 d) { return d; }
 template <bool WIDE>
 __device__ __forceinline__ void nbr_batch(const Ctx& c, int i, int cnt, int base, int k0, int (&j)[NB_U]) {
 #pragma unroll
@@ -949,7 +943,6 @@ struct QV {
 };
 template <bool WIDE>
 __device__ float density_rate(const Ctx& c, int i) {
-    const KC K = make_kc(c.h, c.kind);
     f3 xi = xyz(c.q4[i]), vi = V(c, i);
     float inv_r0 = 1.0f / c.rho0[i];
     float rate = 0.0f;
@@ -1027,7 +1020,6 @@ struct QKR {
 // kapr = kap / rho0 per particle, written by the kernel that produced kap (the same division, gathered once)
 template <bool WIDE>
 __device__ f3 pressure_accel(const Ctx& c, const float* kap, const float* kapr, int i) {
-    const KC K = make_kc(c.h, c.kind);
     f3 xi = xyz(c.q4[i]);
     float ki = kap[i] / c.rho0[i];
     f3 a{0.0f, 0.0f, 0.0f};
@@ -1048,7 +1040,6 @@ struct QA {
 };
 template <bool WIDE>
 __device__ float accel_divergence(const Ctx& c, int i) {
-    const KC K = make_kc(c.h, c.kind);
     f3 xi = xyz(c.q4[i]), ai = A4(c, i);
     float inv_r0 = 1.0f / c.rho0[i];
     float ar = 0.0f;
@@ -1274,8 +1265,6 @@ __global__ void __launch_bounds__(256) k_jacobi(Ctx c, int mode, float dt) {
     }
     chunk_reduce(c, e64, 0.0, valid ? e64 : -3.4e38);
 }
-
-__global__ void k_zero_unconv(Ctx c) { *c.n_unconv = 0; }
 
 // contact_limit_kernel + apply_pressure_kernel + lone_friction_kernel (pressure.py:393-480) for one particle: a
 // lone particle (count 0) is nobody's neighbour, so limiting its kappa before the neighbours' pressure
@@ -1713,7 +1702,6 @@ __global__ void __launch_bounds__(256) k_advect_project(Ctx c, float dt) {
 // n_eff / n_full, h grad r . g_hat; ks = |g| h. Non-liquid rows are zero.
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_features(Ctx c, float dt) {
-    const KC K = make_kc(c.h, c.kind);
     int i = xcd_particle();
     if (i >= c.n) return;
     float* F = c.feat + 16 * int64_t(i);
