@@ -134,10 +134,12 @@ class WarmStartMLP:
     def attach(self, solver) -> None:
         """Use this network as ``solver``'s warm start (the parameter tensor must stay alive)."""
         _abi.check(solver.lib.dfsph_set_warm_start_model(solver._h, _vp(self.params), None), solver.lib)
+        solver._warm_model = self.params
 
     @staticmethod
     def detach(solver) -> None:
         _abi.check(solver.lib.dfsph_set_warm_start_model(solver._h, None, None), solver.lib)
+        solver._warm_model = None
 
 
 def collect(solver, state, steps: int, dt: float, per_step: int | None = None, seed: int = 0,

@@ -23,6 +23,15 @@ class ParticleFlags(IntEnum):
     PROXY = 1 << 1
 
 
+class ModelFlags(IntEnum):
+    """Newton's notify_model_changed flags (alloy/_vendor/newton/_src/sim/enums.py:8-38), the ones this path reads."""
+
+    BODY_PROPERTIES = 1 << 2
+    BODY_INERTIAL_PROPERTIES = 1 << 3
+    SHAPE_PROPERTIES = 1 << 4
+    MODEL_PROPERTIES = 1 << 5
+
+
 class ShapeFlags(IntEnum):
     VISIBLE = 1 << 0
     COLLIDE_SHAPES = 1 << 1

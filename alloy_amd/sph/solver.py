@@ -23,7 +23,7 @@ import torch
 from . import _abi
 from .config import (PHASE_LIQUID, DFSPHConfig, register_custom_attributes, resolve_viscosity, snap_down,
                      stable_dt)
-from .model import Model, ParticleFlags, State
+from .model import Model, ModelFlags, ParticleFlags, State
 from .types import SPHReport
 
 log = logging.getLogger("alloy.sph")
@@ -108,13 +108,32 @@ class SolverDFSPH:
             log.warning("SolverDFSPH: pressure_solver 'cg' needs one particle mass and rest density; using 'jacobi'")
             self.pressure_solver = "jacobi"
         self.lib = _abi.load_library()
+        self._h = None
+        self._warm_model = None  # learned warm-start parameters attached to the native handle, if any
+        self._create_native()
+        self._flags0 = model.particle_flags.clone()
+        self._phase0 = model.sph.phase.clone()
+        W = model.world_count
+        self._mass_in = np.zeros(W)
+        self._mass_out = np.zeros(W)
+        self._escaped = np.zeros(W, np.int32)
+        self._ws_dev = torch.as_tensor(model.world_start_np, dtype=torch.int64, device=self.device)
+        self._mass_initial = self._active_mass()  # runtime.py:132-134 init_mass
+        log.info("SolverDFSPH: %d particles in %d worlds, r=%g m, h=%g m, %d shapes, %d wall particles, solver=%s",
+                 model.particle_count, W, cfg.particle_radius, cfg.h, model.shape_count, self.tables.wall_count,
+                 self.pressure_solver)
+
+    # ------------------------------------------------------------------------------------------------- checks
+    def _create_native(self) -> None:
+        """The native handle for the model's current shapes and layout (solver.py:127-168 allocates the same)."""
+        model, cfg = self.model, self.config
         self.tables = _abi.layout_from_model(model, cfg)
         self._params = _abi.params_from_config(cfg, self.tables.thinnest)
         self._params.pressure_solver = 0 if self.pressure_solver == "cg" else 1
         if self.viscosity == "none":
             self._params.xsph_c = 0.0
         self._wall_volume = torch.tensor(self.tables.wall_volume, device=self.device)
-        self._gravity = model.gravity.reshape(-1).contiguous()
+        self._gravity = model.gravity.reshape(-1).contiguous().clone()
         m = _abi.DfsphModel()
         m.mass, m.rest_density = _dptr(model.particle_mass), _dptr(model.sph.rest_density)
         m.flags, m.phase, m.world = _dptr(model.particle_flags), _dptr(model.sph.phase), _dptr(model.particle_world)
@@ -125,18 +144,7 @@ class SolverDFSPH:
         _abi.check(self.lib.dfsph_create(ctypes.byref(self._params), ctypes.byref(layout), ctypes.byref(m),
                                          ctypes.byref(h)), self.lib, "SolverDFSPH")
         self._h = h
-        self._flags0 = model.particle_flags.clone()
-        self._phase0 = model.sph.phase.clone()
-        W = model.world_count
-        self._mass_in = np.zeros(W)
-        self._mass_out = np.zeros(W)
-        self._escaped = np.zeros(W, np.int32)
-        self._ws_dev = torch.as_tensor(model.world_start_np, dtype=torch.int64, device=self.device)
-        log.info("SolverDFSPH: %d particles in %d worlds, r=%g m, h=%g m, %d shapes, %d wall particles, solver=%s",
-                 model.particle_count, W, cfg.particle_radius, cfg.h, model.shape_count, self.tables.wall_count,
-                 self.pressure_solver)
 
-    # ------------------------------------------------------------------------------------------------- checks
     @staticmethod
     def _refuse_unbuilt(cfg: DFSPHConfig) -> None:
         unbuilt = [
@@ -284,7 +292,7 @@ class SolverDFSPH:
 
     def mass_ledger(self) -> dict[str, np.ndarray]:
         """Per-world mass ledger [kg] (runtime.py:129-134): active + out == initial + in."""
-        return {"initial": self._active_mass(), "in": self._mass_in.copy(), "out": self._mass_out.copy(),
+        return {"initial": self._mass_initial.copy(), "in": self._mass_in.copy(), "out": self._mass_out.copy(),
                 "active": self._active_mass()}
 
     def shape_impulse(self) -> np.ndarray:
@@ -338,6 +346,7 @@ class SolverDFSPH:
         self._mass_in[mask] = 0.0
         self._mass_out[mask] = 0.0
         self._escaped[mask] = 0
+        self._mass_initial[mask] = self._active_mass()[mask]
 
     def choose_dt(self, control_dt: float) -> float:
         """CFL step from the last step's max speed, snapped onto the D4 grid (solver.py:424-426)."""
@@ -392,3 +401,70 @@ class SolverDFSPH:
         """Item -> world maps of the state arrays (coupling_hooks.py:128-138)."""
         w = self.model.particle_world.cpu().numpy()
         return {"sph.kappa_v": w, "sph.kappa_d": w, "sph.phase": w, "sph.flags": w}
+
+    def notify_model_changed(self, flags: int) -> None:
+        """solver.py:343-351. Shape properties (transform, scale, radius, friction): the native handle is rebuilt from
+        the model's shapes, keeping the warm starts, shape meters, tuning options and learned warm start. Model
+        properties (gravity): the per-world gravity is re-read. This solver owns no bodies."""
+        if flags & int(ModelFlags.MODEL_PROPERTIES):
+            self._gravity.copy_(self.model.gravity.reshape(-1))
+        if flags & int(ModelFlags.SHAPE_PROPERTIES):
+            saved = self.alloy_state_arrays(None)
+            S = self.model.shape_count
+            meters = self.buffer(_abi.BUF_SHAPE_METER, np.int64, (max(S, 1), 3)) if S else None
+            opts = {o: self.get_option(o) for o in (_abi.OPT_CHECK_INTERVAL, _abi.OPT_GRAPHS, _abi.OPT_PAIR_CACHE)}
+            self.lib.dfsph_destroy(self._h)
+            self._h = None
+            self._create_native()
+            self.alloy_restore_state_arrays(saved)
+            if meters is not None and self.model.shape_count == S:
+                self._call(self.lib.dfsph_write_buffer, _abi.BUF_SHAPE_METER, meters.ctypes.data_as(ctypes.c_void_p),
+                           meters.nbytes)
+            for o, v in opts.items():
+                self.set_option(o, v)
+            if self._warm_model is not None:
+                self._call(self.lib.dfsph_set_warm_start_model, _dptr(self._warm_model), None)
+
+    def alloy_shape_work(self, out: torch.Tensor) -> None:
+        """Work [J] each shape's boundary did on the liquid in the last step at the wall's point velocity, per model
+        shape (solver.py:407-412). The shapes this solver accepts are static, so the work is 0."""
+        out.zero_()
+
+    def retire(self, state: State, mask: torch.Tensor) -> None:
+        """Retire the masked liquid particles for good (solver.py:428-431): they leave the active set (flags cleared)
+        and their mass goes to the worlds' mass_out."""
+        del state
+        m = self.model
+        sel = (mask.to(torch.bool) & ((m.particle_flags & int(ParticleFlags.ACTIVE)) != 0)
+               & (m.sph.phase >= PHASE_LIQUID))
+        w = m.particle_world.long().clamp(min=0)
+        out = torch.zeros(m.world_count, dtype=torch.float64, device=self.device).index_add_(
+            0, w, torch.where(sel, m.particle_mass.double(), torch.zeros((), dtype=torch.float64, device=self.device)))
+        m.particle_flags[sel] = m.particle_flags[sel] & ~int(ParticleFlags.ACTIVE)
+        self._mass_out += out.cpu().numpy()
+
+    def _no_emitters(self, what: str):
+        raise NotImplementedError(f"SolverDFSPH.{what}: emitters, sinks and patch emitters (emitters.py) are not "
+                                  "built in this solver")
+
+    def add_emitter(self, world, region, half_extents, rate, velocity, pool=None) -> int:
+        self._no_emitters("add_emitter")
+
+    def set_emitter_enabled(self, e: int, on: bool) -> None:
+        self._no_emitters("set_emitter_enabled")
+
+    def add_patch_emitter(self, world_mask, body, local_points, velocity=(0.0, 0.0, 0.0)) -> int:
+        self._no_emitters("add_patch_emitter")
+
+    def trigger(self, pid: int, world_flags) -> None:
+        self._no_emitters("trigger")
+
+    def add_sink(self, world, region, half_extents) -> int:
+        self._no_emitters("add_sink")
+
+    @staticmethod
+    def emit(builder, obj, decision) -> None:
+        """Scene objects become liquid blocks (solver.py:475-479)."""
+        from .build import add_fluid_block
+
+        add_fluid_block(builder, decision.config, obj.lo, obj.counts)

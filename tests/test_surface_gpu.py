@@ -1,0 +1,88 @@
+"""The plug-in surface around the step (solver.py:343-479): shape-change notification, retiring particles, shape
+work and the emitter calls this build does not provide."""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from tests.conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a HIP device")]
+
+DT = 1.0e-3
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_shape_change_rebuilds_like_a_new_solver():
+    """After a wall moves, notify_model_changed(SHAPE_PROPERTIES) gives the same steps, bit for bit, as a solver
+    created on the changed model with the same state arrays restored."""
+    import torch
+
+    from alloy_amd.sph import scenes
+    from alloy_amd.sph.model import ModelFlags
+
+    sc = scenes.dam_break(8, 0.2, 0, device="cuda")
+    _log(f"[surface] shape change: {sc.model.particle_count} particles, {sc.model.shape_count} shapes")
+    a = sc.make_solver()
+    st_a = sc.model.state()
+    for _ in range(5):
+        a.step(st_a, st_a, None, None, DT)
+    sc.model.shape_transform[0, 0] += 0.01  # move one wall of the tank
+    a.notify_model_changed(int(ModelFlags.SHAPE_PROPERTIES))
+    b = sc.make_solver()
+    b.alloy_restore_state_arrays(a.alloy_state_arrays(None))
+    st_b = sc.model.state()
+    st_b.particle_q.copy_(st_a.particle_q)
+    st_b.particle_qd.copy_(st_a.particle_qd)
+    for _ in range(5):
+        a.step(st_a, st_a, None, None, DT)
+        b.step(st_b, st_b, None, None, DT)
+    torch.cuda.synchronize()
+    assert torch.equal(st_a.particle_q, st_b.particle_q) and torch.equal(st_a.particle_qd, st_b.particle_qd)
+
+
+def test_retire_moves_mass_to_the_ledger():
+    import torch
+
+    from alloy_amd.sph import scenes
+
+    sc = scenes.water_column(4, 4, 6, fills=[6, 4], device="cuda")
+    s = sc.make_solver()
+    st = sc.model.state()
+    s.step(st, st, None, None, DT)
+    before = s.mass_ledger()
+    mask = torch.zeros(sc.model.particle_count, dtype=torch.int32, device="cuda")
+    liquid = (sc.model.sph.phase >= 1).nonzero().reshape(-1)
+    mask[liquid[:10]] = 1
+    s.retire(st, mask)
+    after = s.mass_ledger()
+    _log(f"[surface] retire: active {before['active']} -> {after['active']}, out {after['out']}")
+    assert np.allclose(after["active"] + after["out"], after["initial"] + after["in"], rtol=0, atol=1e-12)
+    assert np.array_equal(after["initial"], before["initial"])
+    assert after["out"].sum() > 0.0
+    s.step(st, st, None, None, DT)  # the retired particles are simply absent from the next step
+    r = s.report()
+    assert np.all(np.isfinite(r.density_err))
+
+
+def test_shape_work_and_unbuilt_emitters():
+    import torch
+
+    from alloy_amd.sph import scenes
+
+    sc = scenes.dam_break(4, 0.1, 0, device="cuda")
+    s = sc.make_solver()
+    out = torch.full((sc.model.shape_count,), 7.0, dtype=torch.float64, device="cuda")
+    s.alloy_shape_work(out)
+    assert torch.count_nonzero(out) == 0
+    with pytest.raises(NotImplementedError):
+        s.add_emitter(0, None, None, 1.0, None)
+    with pytest.raises(NotImplementedError):
+        s.add_sink(0, None, None)
