@@ -111,6 +111,7 @@ struct Ctx {
     float* mlp_out;  // [n][2] network outputs
     float* warm_k;   // [2][n] learned initial kappa of the divergence / density solve
     float kscale;    // |g| h: the kappa scale of the features and targets [m^2/s^2]
+    float warm_v, warm_d;  // the reference warm-start factors of the divergence / density solve
     float lat_diag;  // interior rest-lattice DFSPH diagonal [1/m^2]
     float *pd, *z, *wv, *ap, *corr;
     float *kr, *pdr, *corrr;  // kappa / rho0, pd / rho0, corr / rho0: the quotients pressure_accel gathers
@@ -877,7 +878,99 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
     int i = c.ch_begin[b_] + threadIdx.x; \
     bool valid = i < c.ch_end[b_];
 
-// pressure.py:144-193, plus the chunk partial of `active` (solver.py:241)
+// pressure.py:144-193 for one particle (is_fluid), returning 1 for the chunk partial of `active` (solver.py:241).
+// With FEAT the same neighbour pass also gathers the neighbours' velocities and writes the particle's learned
+// warm-start features [16] (13 network inputs, then the rule's initial kappa of both solves), so the learned start
+// costs no pass of its own: neighbour count / n_full, 100 (r - 1), 100 dt Dr/Dt, asinh(kappa_v,prev / ks),
+// asinh(kappa_d,prev / ks), h grad r (3), sum psi, h |grad r|, alpha D, n_eff / n_full, h grad r . g_hat; ks = |g| h.
+struct QVd {
+    float4 q;
+    f3 v;
+};
+template <bool WIDE, bool FEAT>
+__device__ __forceinline__ void density_alpha_one(const Ctx& c, const KC& K, int i, float dt) {
+    float4 qi = c.q4[i];
+    f3 xi = xyz(qi);
+    float mi = qi.w;
+    float inv_r0 = 1.0f / c.rho0[i];
+    float rho = mi * kernel_w(0.0f, K);
+    f3 G{0.0f, 0.0f, 0.0f}, vi{0.0f, 0.0f, 0.0f};
+    float sq = 0.0f, rate = 0.0f;
+    if (FEAT) vi = V(c, i);
+    int cnt = c.count[i];
+    int base = c.ws[c.world[i]];
+    for_nbrs<WIDE, QVd>(
+        c, i, cnt, base, [&](int j) { return QVd{c.q4[j], FEAT ? V(c, j) : f3{0.0f, 0.0f, 0.0f}}; },
+        [&](int k, int, QVd d) {
+            f3 rv = xi - xyz(d.q);
+            float mj = d.q.w;
+            rho += mj * kernel_w(length(rv), K);
+            float sc = kernel_grad_scale(rv, K);
+            if (c.pair_cache) c.pgrad[int64_t(k) * c.n + i] = sc;
+            f3 g = pair_grad(rv, sc);
+            G = G + (mj * inv_r0) * g;
+            sq += (mi * mj * inv_r0 * inv_r0) * dot(g, g);
+            if (FEAT) rate += (mj * inv_r0) * dot(vi - d.v, g);
+        });
+    float psi = 0.0f, lam = 0.0f;
+    int64_t b4 = 4 * int64_t(i);
+    for (int b = 0; b < 4; ++b) {
+        if (c.s_shape[b4 + b] < 0) continue;
+        psi += c.s_psi[b4 + b];
+        f3 gb = ld3(c.s_gpsi, b4 + b);
+        G = G + gb;
+        if (FEAT) rate += dot(vi - ld3(c.s_vb, b4 + b), gb);
+        for (int cc = b + 1; cc < 4; ++cc)
+            if (c.s_shape[b4 + cc] >= 0) lam += fmx(-2.0f * dot(gb, ld3(c.s_gpsi, b4 + cc)), 0.0f);
+    }
+    float tot = dot(G, G) + sq + lam;
+    lam += fmx(c.tot_min - tot, 0.0f);
+    tot = fmx(tot, c.tot_min);
+    float ratio = rho * inv_r0 + psi;
+    float alpha = tot > 1.0e-12f ? 1.0f / tot : 0.0f;
+    float neff = float(cnt) + psi * c.n_full;
+    c.reg[i] = lam;
+    c.ratio[i] = ratio;
+    c.alpha[i] = alpha;
+    c.neff[i] = neff;
+    c.active[i] = 1.0f;
+    if (FEAT) {
+        float* F = c.feat + 16 * int64_t(i);
+        f3 gw = ld3(c.gravity, c.world[i]);
+        float gl = length(gw);
+        f3 gh = gl > 0.0f ? gw / gl : f3{0.0f, 0.0f, -1.0f};
+        float ks = c.kscale;
+        F[0] = float(cnt) / c.n_full;
+        F[1] = 100.0f * (ratio - 1.0f);
+        F[2] = 100.0f * dt * rate;
+        F[3] = asinhf(c.kv[i] / dt / ks);
+        F[4] = asinhf(c.kd[i] / (dt * dt) / ks);
+        F[5] = c.h * G.x;
+        F[6] = c.h * G.y;
+        F[7] = c.h * G.z;
+        F[8] = psi;
+        F[9] = c.h * length(G);
+        F[10] = alpha * c.lat_diag;
+        F[11] = neff / c.n_full;
+        F[12] = c.h * dot(G, gh);
+        // not network inputs: the reference rule's initial kappa of each solve (pressure.py:254, 261, before the
+        // lower bound), the baseline the network's output corrects
+        F[13] = asinhf(c.warm_v * fmn(c.kv[i], 0.5f) / dt / ks);
+        F[14] = asinhf(c.warm_d * fmn(c.kd[i], 2.5e-4f) / (dt * dt) / ks);
+        F[15] = 0.0f;
+    }
+}
+template <bool FEAT>
+__device__ __forceinline__ void density_alpha_other(const Ctx& c, int i) {
+    c.ratio[i] = 0.0f;
+    c.alpha[i] = 0.0f;
+    c.neff[i] = 0.0f;
+    c.active[i] = 0.0f;
+    c.reg[i] = 0.0f;
+    if (FEAT)
+        for (int k = 0; k < 16; ++k) c.feat[16 * int64_t(i) + k] = 0.0f;
+}
+
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_density_alpha(Ctx c) {
     const KC K = make_kc(c.h, c.kind);
@@ -885,51 +978,25 @@ __global__ void __launch_bounds__(256) k_density_alpha(Ctx c) {
     double act_v = 0.0;
     if (valid) {
         if (!is_fluid(c, i)) {
-            c.ratio[i] = 0.0f;
-            c.alpha[i] = 0.0f;
-            c.neff[i] = 0.0f;
-            c.active[i] = 0.0f;
-            c.reg[i] = 0.0f;
+            density_alpha_other<false>(c, i);
         } else {
-            float4 qi = c.q4[i];
-            f3 xi = xyz(qi);
-            float mi = qi.w;
-            float inv_r0 = 1.0f / c.rho0[i];
-            float rho = mi * kernel_w(0.0f, K);
-            f3 G{0.0f, 0.0f, 0.0f};
-            float sq = 0.0f;
-            int cnt = c.count[i];
-            int base = c.ws[c.world[i]];
-            for_nbrs<WIDE, float4>(
-                c, i, cnt, base, [&](int j) { return c.q4[j]; },
-                [&](int k, int, float4 qj) {
-                    f3 rv = xi - xyz(qj);
-                    float mj = qj.w;
-                    rho += mj * kernel_w(length(rv), K);
-                    float sc = kernel_grad_scale(rv, K);
-                    if (c.pair_cache) c.pgrad[int64_t(k) * c.n + i] = sc;
-                    f3 g = pair_grad(rv, sc);
-                    G = G + (mj * inv_r0) * g;
-                    sq += (mi * mj * inv_r0 * inv_r0) * dot(g, g);
-                });
-            float psi = 0.0f, lam = 0.0f;
-            int64_t b4 = 4 * int64_t(i);
-            for (int b = 0; b < 4; ++b) {
-                if (c.s_shape[b4 + b] < 0) continue;
-                psi += c.s_psi[b4 + b];
-                f3 gb = ld3(c.s_gpsi, b4 + b);
-                G = G + gb;
-                for (int cc = b + 1; cc < 4; ++cc)
-                    if (c.s_shape[b4 + cc] >= 0) lam += fmx(-2.0f * dot(gb, ld3(c.s_gpsi, b4 + cc)), 0.0f);
-            }
-            float tot = dot(G, G) + sq + lam;
-            lam += fmx(c.tot_min - tot, 0.0f);
-            tot = fmx(tot, c.tot_min);
-            c.reg[i] = lam;
-            c.ratio[i] = rho * inv_r0 + psi;
-            c.alpha[i] = tot > 1.0e-12f ? 1.0f / tot : 0.0f;
-            c.neff[i] = float(cnt) + psi * c.n_full;
-            c.active[i] = 1.0f;
+            density_alpha_one<WIDE, false>(c, K, i, 0.0f);
+            act_v = 1.0;
+        }
+    }
+    chunk_reduce(c, act_v, 0.0, valid ? act_v : -3.4e38);
+}
+// the same pass plus the learned warm start's features of the step (dt) in c.feat
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_density_features(Ctx c, float dt) {
+    const KC K = make_kc(c.h, c.kind);
+    CHUNK_PROLOGUE
+    double act_v = 0.0;
+    if (valid) {
+        if (!is_fluid(c, i)) {
+            density_alpha_other<true>(c, i);
+        } else {
+            density_alpha_one<WIDE, true>(c, K, i, dt);
             act_v = 1.0;
         }
     }
@@ -1697,77 +1764,28 @@ __global__ void __launch_bounds__(256) k_advect_project(Ctx c, float dt) {
 }
 
 // ----------------------------------------------------------------------------------------------- learned warm start
-// Features of a liquid particle at the start of the step (13 of 16 columns): neighbour count / n_full, 100 (r - 1),
-// 100 dt Dr/Dt, asinh(kappa_v,prev / ks), asinh(kappa_d,prev / ks), h grad r (3), sum psi, h |grad r|, alpha D,
-// n_eff / n_full, h grad r . g_hat; ks = |g| h. Non-liquid rows are zero.
-template <bool WIDE>
-__global__ void __launch_bounds__(256) k_features(Ctx c, float dt) {
-    int i = xcd_particle();
-    if (i >= c.n) return;
-    float* F = c.feat + 16 * int64_t(i);
-    if (!is_fluid(c, i)) {
-        for (int k = 0; k < 16; ++k) F[k] = 0.0f;
-        return;
-    }
-    float4 qi = c.q4[i];
-    f3 xi = xyz(qi), vi = V(c, i);
-    float inv_r0 = 1.0f / c.rho0[i];
-    f3 G{0.0f, 0.0f, 0.0f};
-    float rate = 0.0f;
-    int cnt = c.count[i];
-    int base = c.ws[c.world[i]];
-    for_nbrs_g<WIDE, QV>(c, i, xi, cnt, base, (c.pair_cache & DFSPH_PAIR_CACHE_OTHER) != 0,
-                         [&](int j) { return QV{c.q4[j], V(c, j)}; },
-                         [&](int, QV d, f3 g) {
-                             G = G + (d.q.w * inv_r0) * g;
-                             rate += (d.q.w * inv_r0) * dot(vi - d.v, g);
-                         });
-    float psi = 0.0f;
-    int64_t b4 = 4 * int64_t(i);
-    for (int b = 0; b < 4; ++b)
-        if (c.s_shape[b4 + b] >= 0) {
-            psi += c.s_psi[b4 + b];
-            G = G + ld3(c.s_gpsi, b4 + b);
-            rate += dot(vi - ld3(c.s_vb, b4 + b), ld3(c.s_gpsi, b4 + b));
-        }
-    f3 gw = ld3(c.gravity, c.world[i]);
-    float gl = length(gw);
-    f3 gh = gl > 0.0f ? gw / gl : f3{0.0f, 0.0f, -1.0f};
-    float ks = c.kscale;
-    F[0] = float(cnt) / c.n_full;
-    F[1] = 100.0f * (c.ratio[i] - 1.0f);
-    F[2] = 100.0f * dt * rate;
-    F[3] = asinhf(c.kv[i] / dt / ks);
-    F[4] = asinhf(c.kd[i] / (dt * dt) / ks);
-    F[5] = c.h * G.x;
-    F[6] = c.h * G.y;
-    F[7] = c.h * G.z;
-    F[8] = psi;
-    F[9] = c.h * length(G);
-    F[10] = c.alpha[i] * c.lat_diag;
-    F[11] = c.neff[i] / c.n_full;
-    F[12] = c.h * dot(G, gh);
-    F[13] = F[14] = F[15] = 0.0f;
-}
-
-// network outputs -> initial kappa of both solves, within the reference warm start's clamps (pressure.py:254, 261)
+// The network predicts the correction y to the rule in asinh(kappa / ks) units: kappa0 = ks sinh(rule + y), so a
+// zero output reproduces the reference's warm start.
 __global__ void k_warm_from_mlp(Ctx c, float dt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.n) return;
+    const float* F = c.feat + 16 * int64_t(i);
     float y0 = c.mlp_out[2 * int64_t(i)], y1 = c.mlp_out[2 * int64_t(i) + 1];
-    float kv = c.kscale * sinhf(y0), kd = c.kscale * sinhf(y1);
+    float kv = c.kscale * sinhf(F[13] + y0), kd = c.kscale * sinhf(F[14] + y1);
     if (!isfinite(kv)) kv = 0.0f;
     if (!isfinite(kd)) kd = 0.0f;
     c.warm_k[i] = fmn(kv, 0.5f / dt);
     c.warm_k[int64_t(c.n) + i] = fmx(fmn(kd, 2.5e-4f / (dt * dt)), 0.0f);
 }
 
-// training targets of the step just taken: asinh(converged kappa / ks) of both solves, [n][2]
+// training targets of the step just taken, [n][2]: asinh(converged kappa / ks) of both solves minus the rule's
+// initial value logged with the step's features (the correction the network learns)
 __global__ void k_targets(Ctx c, float dt, float* T) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.n) return;
-    T[2 * int64_t(i)] = asinhf(c.kv[i] / dt / c.kscale);
-    T[2 * int64_t(i) + 1] = asinhf(c.kd[i] / (dt * dt) / c.kscale);
+    const float* F = c.feat + 16 * int64_t(i);
+    T[2 * int64_t(i)] = asinhf(c.kv[i] / dt / c.kscale) - F[13];
+    T[2 * int64_t(i) + 1] = asinhf(c.kd[i] / (dt * dt) / c.kscale) - F[14];
 }
 
 // ----------------------------------------------------------------------------------------------- profiling
@@ -1820,6 +1838,7 @@ struct Handle {
     Prof prof;
     bool use_graphs = false;  // measured: eager launches are faster than graph replay for this step (bench.py --graphs)
     const float* mlp_params = nullptr;  // learned warm start on when set
+    int learned_solves = 3;             // DFSPH_OPT_LEARNED_SOLVES: bit 0 divergence, bit 1 density
     float* log_feat = nullptr;          // when set, the step copies its features [n][16] here ...
     float* log_targets = nullptr;       // ... and its converged-kappa targets [n][2] here
     hipStream_t cap_stream = nullptr;
@@ -1947,10 +1966,12 @@ int phase_slots(Handle* H, hipStream_t st) {
     return DFSPH_OK;
 }
 
-int phase_density(Handle* H, hipStream_t st) {
+// density and alpha; with `features` (the step's dt > 0) also the learned warm start's features
+int phase_density(Handle* H, hipStream_t st, float features_dt = 0.0f) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
-    LAUNCH_WT(KID_DENSITY, k_density_alpha, H->n_chunks, 256, c);
+    if (features_dt > 0.0f) LAUNCH_WT(KID_DENSITY, k_density_features, H->n_chunks, 256, c, features_dt);
+    else LAUNCH_WT(KID_DENSITY, k_density_alpha, H->n_chunks, 256, c);
     k_world<<<c.W, 256, 0, st>>>(c, W_NACTIVE, StopSpec{});
     LAUNCH_CHECK();
     return DFSPH_OK;
@@ -2182,6 +2203,13 @@ void bind_state(Handle* H, const float* q, const float* qd, const float* f) {
     H->c.f = f;
 }
 
+// the learned warm start feeds the solves DFSPH_OPT_LEARNED_SOLVES selects
+void apply_learned(Handle* H) {
+    bool on = H->mlp_params != nullptr;
+    H->learned_div = on && (H->learned_solves & 1) ? H->c.warm_k : nullptr;
+    H->learned_den = on && (H->learned_solves & 2) ? H->c.warm_k + H->c.n : nullptr;
+}
+
 }  // namespace
 
 // =============================================================================================== C-ABI
@@ -2230,6 +2258,8 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
     c.tot_min = float(P->tot_min);
     c.min_nb = float(P->min_neighbors_divergence);
     c.omega = float(P->jacobi_omega);
+    c.warm_v = float(P->warm_start_divergence);
+    c.warm_d = float(P->warm_start_density);
     c.xsph = float(P->xsph_c);
     c.v_sep = float(P->contact_separation_speed);
     c.offset = float(P->wall_offset);
@@ -2471,11 +2501,11 @@ int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void
         int r;
         if ((r = phase_neighbors(H, s))) return r;
         if ((r = phase_slots(H, s))) return r;
-        if ((r = phase_density(H, s))) return r;
+        const bool learned = H->mlp_params || H->log_feat;
+        if ((r = phase_density(H, s, learned ? dtf : 0.0f))) return r;
         if ((r = phase_flow(H, s))) return r;
-        if (H->mlp_params || H->log_feat) {  // learned warm start: features -> MLP -> initial kappa
+        if (learned) {  // learned warm start: features (written by the density pass) -> MLP -> initial kappa
             Ctx& c = H->c;
-            LAUNCH_W(k_features, blocks_for(c.n), 256, c, dtf);
             if (H->mlp_params) {
                 dfsph_launch_mlp_forward(H->mlp_params, c.feat, c.n, c.mlp_out, s);
                 k_warm_from_mlp<<<blocks_for(c.n), 256, 0, s>>>(c, dtf);
@@ -2553,8 +2583,7 @@ int32_t dfsph_set_warm_start_model(void* h, const void* weights, void* stream) {
     if (!h) return fail(DFSPH_EINVAL, "null handle");
     Handle* H = as_handle(h);
     H->mlp_params = static_cast<const float*>(weights);
-    H->learned_div = weights ? H->c.warm_k : nullptr;
-    H->learned_den = weights ? H->c.warm_k + H->c.n : nullptr;
+    apply_learned(H);
     return DFSPH_OK;
 }
 
@@ -2584,7 +2613,7 @@ int32_t dfsph_set_check_interval(void* h, int32_t every) {
     return DFSPH_OK;
 }
 
-// tuning options (none changes a result bit)
+// tuning options (none changes a result bit, except which solves the learned warm start feeds)
 int32_t dfsph_set_option(void* h, int32_t option, int32_t value) {
     if (!h) return fail(DFSPH_EINVAL, "null handle");
     Handle* H = as_handle(h);
@@ -2597,6 +2626,11 @@ int32_t dfsph_set_option(void* h, int32_t option, int32_t value) {
             if (value < 0 || value > DFSPH_PAIR_CACHE_ALL)
                 return fail(DFSPH_EINVAL, "pair cache: a mask of DFSPH_PAIR_CACHE_* bits");
             H->c.pair_cache = value;
+            return DFSPH_OK;
+        case DFSPH_OPT_LEARNED_SOLVES:
+            if (value < 0 || value > 3) return fail(DFSPH_EINVAL, "learned solves: bit 0 divergence, bit 1 density");
+            H->learned_solves = value;
+            apply_learned(H);
             return DFSPH_OK;
         default:
             return fail(DFSPH_EINVAL, "unknown option");
@@ -2614,6 +2648,9 @@ int32_t dfsph_get_option(void* h, int32_t option, int32_t* value) {
             return DFSPH_OK;
         case DFSPH_OPT_PAIR_CACHE:
             *value = H->c.pair_cache;
+            return DFSPH_OK;
+        case DFSPH_OPT_LEARNED_SOLVES:
+            *value = H->learned_solves;
             return DFSPH_OK;
         default:
             return fail(DFSPH_EINVAL, "unknown option");

@@ -13,6 +13,8 @@
 //   D (M x N): D[4 (l / 16) + i][l % 16]
 // One wave owns 16 samples (rows); a 256-thread block owns 64.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include <cmath>
@@ -23,6 +25,7 @@
 namespace {
 
 constexpr int MLP_IN = 16;   // 13 features, zero padded
+constexpr int MLP_IN_USED = 13;
 constexpr int MLP_H = 64;
 constexpr int MLP_OUT = 16;  // 2 outputs, zero padded
 constexpr int OFF_W1 = 0;
@@ -35,38 +38,52 @@ constexpr int OFF_W4 = OFF_B3 + MLP_H;
 constexpr int OFF_B4 = OFF_W4 + MLP_OUT * MLP_H;
 constexpr int N_PARAMS = OFF_B4 + MLP_OUT;  // 10448
 constexpr int PART_STRIDE = N_PARAMS + 1;   // per-wave gradient partial + its squared-error sum
-// bf16 weight staging in LDS: W1 | W2 | W3 | W4
-constexpr int SW1 = 0, SW2 = SW1 + MLP_H * MLP_IN, SW3 = SW2 + MLP_H * MLP_H, SW4 = SW3 + MLP_H * MLP_H;
-constexpr int SW_TOTAL = SW4 + MLP_OUT * MLP_H;  // 10240
+// LDS row strides (bf16 elements) of 16- and 64-wide rows, padded so that the 16 rows one MFMA fragment load
+// touches start on distinct bank pairs (10 and 34 words): an unpadded 64-wide row (32 words) puts all 16 on one bank
+constexpr int LS_IN = 20, LS_H = 68;
+// bf16 weight staging in LDS: W1 [64][LS_IN] | W2 [64][LS_H] | W3 [64][LS_H] | W4 [16][LS_H]
+constexpr int SW1 = 0, SW2 = SW1 + MLP_H * LS_IN, SW3 = SW2 + MLP_H * LS_H, SW4 = SW3 + MLP_H * LS_H;
+constexpr int SW_TOTAL = SW4 + MLP_OUT * LS_H;
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+// float -> bf16, round to nearest even: gfx950's v_cvt_pk_bf16_f32
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f)); }
 
 __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// sigmoid by the hardware exp and reciprocal (about 1 ulp each): the activations are rounded to bf16 right after
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float silu_grad(float z) {
-    float s = 1.0f / (1.0f + __expf(-z));
+    float s = sigmoid(z);
     return s * (1.0f + z * (1.0f - s));
 }
 
 // stage the four weight matrices as bf16 in LDS (block-wide)
 __device__ void stage_weights(const float* __restrict__ p, uint16_t* sW) {
-    for (int t = threadIdx.x; t < SW_TOTAL; t += blockDim.x) {
-        int src;
-        if (t < SW2) src = OFF_W1 + t;
-        else if (t < SW3) src = OFF_W2 + (t - SW2);
-        else if (t < SW4) src = OFF_W3 + (t - SW3);
-        else src = OFF_W4 + (t - SW4);
-        sW[t] = f2bf(p[src]);
+    constexpr int N_W = MLP_H * MLP_IN + 2 * MLP_H * MLP_H + MLP_OUT * MLP_H;  // 10,240 weights
+    static_assert(SW4 + (MLP_OUT - 1) * LS_H + MLP_H <= SW_TOTAL, "staged W4 fits");
+    for (int t = threadIdx.x; t < N_W; t += blockDim.x) {
+        // t runs over the weights in parameter order; dst is its padded LDS slot
+        int dst;
+        if (t < MLP_H * MLP_IN) dst = SW1 + (t / MLP_IN) * LS_IN + t % MLP_IN;
+        else if (t < MLP_H * (MLP_IN + MLP_H)) {
+            int u = t - MLP_H * MLP_IN;
+            dst = SW2 + (u / MLP_H) * LS_H + u % MLP_H;
+        } else if (t < MLP_H * (MLP_IN + 2 * MLP_H)) {
+            int u = t - MLP_H * (MLP_IN + MLP_H);
+            dst = SW3 + (u / MLP_H) * LS_H + u % MLP_H;
+        } else {
+            int u = t - MLP_H * (MLP_IN + 2 * MLP_H);
+            dst = SW4 + (u / MLP_H) * LS_H + u % MLP_H;
+        }
+        int src = t < MLP_H * MLP_IN ? OFF_W1 + t
+                  : t < MLP_H * (MLP_IN + MLP_H) ? OFF_W2 + (t - MLP_H * MLP_IN)
+                  : t < MLP_H * (MLP_IN + 2 * MLP_H) ? OFF_W3 + (t - MLP_H * (MLP_IN + MLP_H))
+                                                     : OFF_W4 + (t - MLP_H * (MLP_IN + 2 * MLP_H));
+        sW[dst] = f2bf(p[src]);
     }
 }
 
@@ -112,13 +129,13 @@ __device__ __forceinline__ void store_d_bf16(uint16_t* T, int ld, int nt, int la
     T[(r0 + 3) * ld + col] = f2bf(v.w);
 }
 
-struct WaveTiles {
-    uint16_t a0[16 * MLP_IN];
-    uint16_t a1[16 * MLP_H];
-    uint16_t a2[16 * MLP_H];
-    uint16_t a3[16 * MLP_H];
-    uint16_t g[16 * MLP_H];   // backward: dZ of the current layer (bf16 operand)
-    uint16_t gy[16 * MLP_OUT];
+struct WaveTiles {  // row strides LS_IN / LS_H
+    uint16_t a0[16 * LS_IN];
+    uint16_t a1[16 * LS_H];
+    uint16_t a2[16 * LS_H];
+    uint16_t a3[16 * LS_H];
+    uint16_t g[16 * LS_H];   // backward: dZ of the current layer (bf16 operand)
+    uint16_t gy[16 * LS_IN];
 };
 
 // forward of one wave's 16 samples: z1..z3 kept in registers (D layout), activations in LDS; returns y (D layout)
@@ -127,69 +144,77 @@ __device__ f4 forward_wave(const float* __restrict__ p, const uint16_t* sW, Wave
     // layer 1: 16 -> 64
     for (int nt = 0; nt < 4; ++nt) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma(frag_rows(T.a0, MLP_IN, 0, lane), frag_wt(sW + SW1, MLP_IN, 0, nt, lane), acc);
+        acc = mfma(frag_rows(T.a0, LS_IN, 0, lane), frag_wt(sW + SW1, LS_IN, 0, nt, lane), acc);
         float b = p[OFF_B1 + 16 * nt + (lane & 15)];
         acc += b;
         z1[nt] = acc;
         f4 a = {silu(acc.x), silu(acc.y), silu(acc.z), silu(acc.w)};
-        store_d_bf16(T.a1, MLP_H, nt, lane, a);
+        store_d_bf16(T.a1, LS_H, nt, lane, a);
     }
     // layer 2
     for (int nt = 0; nt < 4; ++nt) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
-        for (int ks = 0; ks < 4; ++ks) acc = mfma(frag_rows(T.a1, MLP_H, ks, lane), frag_wt(sW + SW2, MLP_H, ks, nt, lane), acc);
+        for (int ks = 0; ks < 4; ++ks) acc = mfma(frag_rows(T.a1, LS_H, ks, lane), frag_wt(sW + SW2, LS_H, ks, nt, lane), acc);
         acc += p[OFF_B2 + 16 * nt + (lane & 15)];
         z2[nt] = acc;
         f4 a = {silu(acc.x), silu(acc.y), silu(acc.z), silu(acc.w)};
-        store_d_bf16(T.a2, MLP_H, nt, lane, a);
+        store_d_bf16(T.a2, LS_H, nt, lane, a);
     }
     // layer 3
     for (int nt = 0; nt < 4; ++nt) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
-        for (int ks = 0; ks < 4; ++ks) acc = mfma(frag_rows(T.a2, MLP_H, ks, lane), frag_wt(sW + SW3, MLP_H, ks, nt, lane), acc);
+        for (int ks = 0; ks < 4; ++ks) acc = mfma(frag_rows(T.a2, LS_H, ks, lane), frag_wt(sW + SW3, LS_H, ks, nt, lane), acc);
         acc += p[OFF_B3 + 16 * nt + (lane & 15)];
         z3[nt] = acc;
         f4 a = {silu(acc.x), silu(acc.y), silu(acc.z), silu(acc.w)};
-        store_d_bf16(T.a3, MLP_H, nt, lane, a);
+        store_d_bf16(T.a3, LS_H, nt, lane, a);
     }
     // layer 4: 64 -> 16 (2 used), linear
     f4 y = {0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < 4; ++ks) y = mfma(frag_rows(T.a3, MLP_H, ks, lane), frag_wt(sW + SW4, MLP_H, ks, 0, lane), y);
+    for (int ks = 0; ks < 4; ++ks) y = mfma(frag_rows(T.a3, LS_H, ks, lane), frag_wt(sW + SW4, LS_H, ks, 0, lane), y);
     y += p[OFF_B4 + (lane & 15)];
     return y;
 }
 
-// stage a wave's input rows [16][MLP_IN] as bf16 (rows past n are zero)
+// stage a wave's input rows [16][MLP_IN] as bf16: the first MLP_IN_USED columns of X (rows past n and the padding
+// columns, which carry other per-particle data, are zero)
 __device__ void load_inputs(const float* __restrict__ X, int n, int row0, uint16_t* a0, int lane) {
     for (int t = lane; t < 16 * MLP_IN; t += 64) {
         int r = t / MLP_IN, k = t % MLP_IN;
-        float v = (row0 + r < n) ? X[int64_t(row0 + r) * MLP_IN + k] : 0.0f;
-        a0[t] = f2bf(v);
+        float v = (row0 + r < n && k < MLP_IN_USED) ? X[int64_t(row0 + r) * MLP_IN + k] : 0.0f;
+        a0[r * LS_IN + k] = f2bf(v);
     }
 }
 
+// Y = MLP(X) for rows [0, n): the block stages the weights in LDS once and then walks 64-row tiles (16 per wave)
+// with a grid stride, so the 42 KB of weights are read once per block rather than once per 64 rows.
+constexpr int FWD_MAX_BLOCKS = 2048;
 __global__ void __launch_bounds__(256) k_mlp_forward(const float* __restrict__ p, const float* __restrict__ X, int n,
                                                      float* __restrict__ Y) {
     __shared__ uint16_t sW[SW_TOTAL];
     __shared__ WaveTiles tiles[4];
     stage_weights(p, sW);
-    __syncthreads();
     int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int row0 = blockIdx.x * 64 + 16 * w;
     WaveTiles& T = tiles[w];
-    load_inputs(X, n, row0, T.a0, lane);
-    __syncthreads();
-    f4 z1[4], z2[4], z3[4];
-    f4 y = forward_wave(p, sW, T, lane, z1, z2, z3);
-    int col = lane & 15;
-    if (col < 2) {
-        float v[4] = {y.x, y.y, y.z, y.w};
-        for (int i = 0; i < 4; ++i) {
-            int r = row0 + 4 * (lane >> 4) + i;
-            if (r < n) Y[int64_t(r) * 2 + col] = v[i];
+    const int n_tiles = (n + 63) / 64;
+    for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {  // block-uniform trip count
+        int row0 = tile * 64 + 16 * w;
+        __syncthreads();  // weights staged (first pass); the previous tile's LDS reads done
+        load_inputs(X, n, row0, T.a0, lane);
+        __syncthreads();
+        f4 z1[4], z2[4], z3[4];
+        f4 y = forward_wave(p, sW, T, lane, z1, z2, z3);
+        int col = lane & 15;
+        if (col < 2) {
+            float v[4] = {y.x, y.y, y.z, y.w};
+            for (int i = 0; i < 4; ++i) {
+                int r = row0 + 4 * (lane >> 4) + i;
+                if (r < n) Y[int64_t(r) * 2 + col] = v[i];
+            }
         }
     }
 }
+inline int fwd_blocks(int n) { return std::min((n + 63) / 64, FWD_MAX_BLOCKS); }
 
 // sum over the 16 rows of a D tile, per column (lanes 0..15 hold column totals after the shuffles)
 __device__ __forceinline__ float col_sum(f4 v) {
@@ -246,18 +271,18 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const float* __restrict__ p, c
     for (int s = 32; s >= 1; s >>= 1) se += __shfl_xor(se, s, 64);
     if (lane == 0) out[N_PARAMS] = se;
     f4 gY = {dy[0], dy[1], dy[2], dy[3]};
-    store_d_bf16(T.gy, MLP_OUT, 0, lane, gY);
+    store_d_bf16(T.gy, LS_IN, 0, lane, gY);
     {
         float s = col_sum(gY);
         if (lane < 16) out[OFF_B4 + lane] = s;
     }
     __syncthreads();
     // layer 4: dW4 = dY^T a3, da3 = dY W4
-    weight_grad(T.gy, MLP_OUT, 1, T.a3, MLP_H, 4, out + OFF_W4, MLP_H, lane);
+    weight_grad(T.gy, LS_IN, 1, T.a3, LS_H, 4, out + OFF_W4, MLP_H, lane);
     f4 dz[4];
     for (int nt = 0; nt < 4; ++nt) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma(frag_rows(T.gy, MLP_OUT, 0, lane), frag_w(sW + SW4, MLP_H, 0, nt, lane), acc);
+        acc = mfma(frag_rows(T.gy, LS_IN, 0, lane), frag_w(sW + SW4, LS_H, 0, nt, lane), acc);
         f4 z = z3[nt];
         dz[nt] = f4{acc.x * silu_grad(z.x), acc.y * silu_grad(z.y), acc.z * silu_grad(z.z), acc.w * silu_grad(z.w)};
     }
@@ -268,19 +293,19 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const float* __restrict__ p, c
     for (int L = 0; L < 3; ++L) {
         __syncthreads();
         for (int nt = 0; nt < 4; ++nt) {
-            store_d_bf16(T.g, MLP_H, nt, lane, dz[nt]);
+            store_d_bf16(T.g, LS_H, nt, lane, dz[nt]);
             float s = col_sum(dz[nt]);
             if (lane < 16) out[boff[L] + 16 * nt + lane] = s;
         }
         __syncthreads();
-        int in_dim = L == 2 ? MLP_IN : MLP_H;
-        weight_grad(T.g, MLP_H, 4, aprev[L], in_dim, in_dim / 16, out + woff[L], in_dim, lane);
+        int in_dim = L == 2 ? MLP_IN : MLP_H, in_ls = L == 2 ? LS_IN : LS_H;
+        weight_grad(T.g, LS_H, 4, aprev[L], in_ls, in_dim / 16, out + woff[L], in_dim, lane);
         if (L == 2) break;
         // da_prev = dZ W (W [64 out][64 in])
         for (int nt = 0; nt < 4; ++nt) {
             f4 acc = {0.f, 0.f, 0.f, 0.f};
             for (int ks = 0; ks < 4; ++ks)
-                acc = mfma(frag_rows(T.g, MLP_H, ks, lane), frag_w(sW + sw[L], MLP_H, ks, nt, lane), acc);
+                acc = mfma(frag_rows(T.g, LS_H, ks, lane), frag_w(sW + sw[L], LS_H, ks, nt, lane), acc);
             f4 z = zs[L][nt];
             dz[nt] = f4{acc.x * silu_grad(z.x), acc.y * silu_grad(z.y), acc.z * silu_grad(z.z), acc.w * silu_grad(z.w)};
         }
@@ -323,7 +348,7 @@ thread_local std::string g_mlp_err;
 
 // launched by the step (dfsph_kernels.hip) inside its captured segment
 void dfsph_launch_mlp_forward(const float* params, const float* X, int n, float* Y, hipStream_t st) {
-    if (n > 0) k_mlp_forward<<<(n + 63) / 64, 256, 0, st>>>(params, X, n, Y);
+    if (n > 0) k_mlp_forward<<<fwd_blocks(n), 256, 0, st>>>(params, X, n, Y);
 }
 
 extern "C" {
@@ -334,7 +359,7 @@ int32_t dfsph_mlp_param_count(void) { return N_PARAMS; }
 int32_t dfsph_mlp_forward(const float* params, const float* X, int32_t n, float* Y, void* stream) {
     if (!params || !X || !Y || n < 0) return DFSPH_EINVAL;
     if (n == 0) return DFSPH_OK;
-    k_mlp_forward<<<(n + 63) / 64, 256, 0, static_cast<hipStream_t>(stream)>>>(params, X, n, Y);
+    k_mlp_forward<<<fwd_blocks(n), 256, 0, static_cast<hipStream_t>(stream)>>>(params, X, n, Y);
     return hipGetLastError() == hipSuccess ? DFSPH_OK : DFSPH_EDEVICE;
 }
 

@@ -11,6 +11,8 @@ both solves' initial kappa when its weights are present.
 bracketed by device synchronisations and timed with HIP events on the solver's stream. The line also carries
 - "plain": the same worlds stepped as plain single-world DFSPH (one solver, i.e. one launch per phase, per world,
   no learned start), on an even sample of the worlds, and `speedup_vs_plain`;
+- "batched_rule_start": the same batched run with the reference's warm-start rule instead of the network, so the
+  learned start's own effect is on the line;
 - "roofline": the dominant kernel's algorithmic bytes, counted by the kernels themselves over the timed region,
   divided by its summed launch durations (HIP events inside each launch's dispatch, so they agree with
   rocprofv3's kernel trace); "traffic" is the HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs
@@ -72,12 +74,15 @@ def sample_worlds(W: int, k: int) -> list[int]:
 
 
 def load_learned(mode: str, device: str):
-    """The committed trained warm-start network, or None (mode auto: when its weights exist)."""
+    """The committed trained warm-start network and the solves its training run chose to feed, or (None, 0) (mode
+    auto: when its weights exist)."""
     from alloy_amd.sph.learned import DEFAULT_WEIGHTS, WarmStartMLP
 
     if mode == "off" or (mode == "auto" and not os.path.exists(DEFAULT_WEIGHTS)):
-        return None
-    return WarmStartMLP.load(DEFAULT_WEIGHTS, device)
+        return None, 0
+    rec = os.path.splitext(DEFAULT_WEIGHTS)[0] + ".json"
+    solves = json.load(open(rec)).get("learned_solves", 3) if os.path.exists(rec) else 3
+    return WarmStartMLP.load(DEFAULT_WEIGHTS, device), int(solves)
 
 
 def timed_run(solver, state, steps: int, dt: float, prof_mask: int = 0, sample: int = 1) -> dict:
@@ -208,6 +213,8 @@ def main() -> int:
                     help="learned warm start (auto: on when the trained weights are present)")
     ap.add_argument("--plain-worlds", type=int, default=16, help="worlds sampled for the plain single-world run")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-rule-compare", action="store_true",
+                    help="skip the batched run with the rule warm start (the learned start's own effect)")
     ap.add_argument("--check-every", type=int, default=1, help="iterations between host convergence checks")
     ap.add_argument("--graphs", action="store_true", help="replay the step's segments as hipGraphs")
     ap.add_argument("--pair-cache", type=int, default=None, help="DFSPH_PAIR_CACHE_* mask (default: the library's)")
@@ -251,9 +258,10 @@ def main() -> int:
     solver.set_graphs(args.graphs)
     if args.pair_cache is not None:
         solver.set_option(_abi.OPT_PAIR_CACHE, args.pair_cache)
-    net = load_learned(args.learned, "cuda")
+    net, solves = load_learned(args.learned, "cuda")
     if net is not None:
         net.attach(solver)
+        solver.set_option(_abi.OPT_LEARNED_SOLVES, solves)
     state = sc.model.state()
     n_fluid = sum(sc.fluid_counts)
     W = sc.model.world_count
@@ -271,7 +279,8 @@ def main() -> int:
     achieved = bl / (tl * 1e-3) / 1e9
     value = n_fluid * args.steps / (total_ms * 1e-3)
     solver_desc = ("batched, one launch per phase over all worlds" if W > 1 else "single world") + (
-        ", learned warm start (trained network)" if net is not None else ", plain warm start")
+        f", learned warm start (trained network; solves {'divergence+density' if solves == 3 else 'density' if solves == 2 else 'divergence'})"
+        if net is not None else ", plain warm start")
     out = {
         "metric": METRIC,
         "value": value,
@@ -310,6 +319,24 @@ def main() -> int:
         _log(f"[bench] fp64 density error {out['fp64_density_err']:.3e}")
     del solver, state
     torch.cuda.empty_cache()
+    if net is not None and not args.no_rule_compare:
+        _log("[bench] the same run with the rule warm start")
+        s2 = sc.make_solver()
+        s2.set_check_interval(args.check_every)
+        s2.set_graphs(args.graphs)
+        if args.pair_cache is not None:
+            s2.set_option(_abi.OPT_PAIR_CACHE, args.pair_cache)
+        st2 = sc.model.state()
+        for _ in range(args.warmup):
+            s2.step(st2, st2, None, None, dt)
+        r2 = timed_run(s2, st2, args.steps, dt)
+        out["batched_rule_start"] = {
+            "value": n_fluid * args.steps / (r2["total_ms"] * 1e-3), "ms_per_step": r2["total_ms"] / args.steps,
+            "iters_density_mean": r2["iters_density_mean"], "iters_divergence_mean": r2["iters_divergence_mean"],
+            "density_err_mean": r2["density_err_mean"]}
+        out["learned_vs_rule_start"] = value / out["batched_rule_start"]["value"]
+        del s2, st2
+        torch.cuda.empty_cache()
     if wl["kind"] == "bowl_worlds" and not args.no_plain:
         _log(f"[bench] plain single-world DFSPH on {args.plain_worlds} sampled worlds")
         plain = plain_sample(wl, sample_worlds(W, args.plain_worlds), args.steps, args.warmup, dt, args.pair_cache)

@@ -196,9 +196,10 @@ int32_t dfsph_flow_speed(void* handle, const float* qd, void* stream);
 int32_t dfsph_prime(void* handle, float* q, float* qd, const float* guess, double dt, double tol,
                     int32_t max_iterations, int32_t* iters, void* stream);
 
-/* Learned warm start: an MLP (13 -> 64 -> 64 -> 64 -> 2, SiLU, bf16 weights) predicting the initial kappa
- * of both solves in place of solve_init's warm start (pressure.py:254, 261). weights: device bf16 buffer in
- * the layout of dfsph_mlp_layout; NULL switches the learned start off. */
+/* Learned warm start: an MLP (13 -> 64 -> 64 -> 64 -> 2, SiLU, bf16 on the MFMA units) predicting, per particle,
+ * the correction y to solve_init's warm-start rule (pressure.py:254, 261): kappa0 = ks sinh(asinh(rule / ks) + y),
+ * ks = |g| h, within the rule's clamps. weights: the network's float32 parameters on the device (the layout of
+ * dfsph_mlp_param_count); NULL switches the learned start off. */
 int32_t dfsph_set_warm_start_model(void* handle, const void* weights, void* stream);
 /* Learned warm start given as initial kappa fields [n] (device; NULL = the reference warm-start rule) of the
  * divergence and density solves, read by solve_init (pressure.py:254, 261) in place of warm x stored kappa; the
@@ -206,13 +207,16 @@ int32_t dfsph_set_warm_start_model(void* handle, const void* weights, void* stre
 int32_t dfsph_set_warm_start_kappa(void* handle, const float* kappa_div, const float* kappa_den);
 /* Host convergence check every `every` iterations (1..64); converged worlds' blocks exit at once in between. */
 int32_t dfsph_set_check_interval(void* handle, int32_t every);
-/* Tuning options: every value gives bit-identical results; they only trade time. */
+/* Tuning options. Options 1-3 give bit-identical results and only trade time; option 4 chooses which pressure
+ * solves start from the learned warm start (both still iterate to the same tolerance). */
 enum DfsphOption {
     DFSPH_OPT_CHECK_INTERVAL = 1, /* as dfsph_set_check_interval */
     DFSPH_OPT_GRAPHS = 2,         /* as dfsph_set_graphs */
-    DFSPH_OPT_PAIR_CACHE = 3      /* mask of DFSPH_PAIR_CACHE_*: gather passes that read the density pass's stored
+    DFSPH_OPT_PAIR_CACHE = 3,     /* mask of DFSPH_PAIR_CACHE_*: gather passes that read the density pass's stored
                                      kernel-gradient scale of each listed pair instead of re-evaluating it (4 B of
                                      memory traffic per pair traded for a square root and two divisions) */
+    DFSPH_OPT_LEARNED_SOLVES = 4  /* with a warm-start model set: bit 0 the divergence solve, bit 1 the density solve
+                                     start from its prediction (default 3); the others keep the reference's rule */
 };
 enum DfsphPairCache {
     DFSPH_PAIR_CACHE_ACCEL = 1, /* pressure acceleration (k_accel, post-solve) */

@@ -44,8 +44,13 @@ def _silu_grad(z):
     return s * (1.0 + z * (1.0 - s))
 
 
+IN_USED = 13  # the feature rows carry other data in columns 13..15; the network reads columns 0..12
+
+
 def forward(p: np.ndarray, X: np.ndarray):
     W1, b1, W2, b2, W3, b3, W4, b4 = unpack(p)
+    X = np.array(X, np.float32, copy=True)
+    X[:, IN_USED:] = 0.0
     a0 = bf16(X)
     z1 = a0 @ bf16(W1).T + b1
     a1 = bf16(_silu(z1))
