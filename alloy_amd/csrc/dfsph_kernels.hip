@@ -52,6 +52,7 @@ int fail(int code, const std::string& msg) {
 struct Ctx {
     int n, W, S, NW, cap, wide, kind, n_cells, friction, curvature, two_sided;
     int pair_cache;  // DFSPH_PAIR_CACHE_* bits: which gather passes read pgrad instead of recomputing the scale
+    int sparse_corr; // DFSPH_OPT_SPARSE_CORRECTION: gated passes skip particles whose every term is an exact zero
     int n_chunks;
     float h, cell, r, n_full, tot_min, min_nb, omega, xsph, v_sep;
     float offset, amplitude, taper0, taper1, reach, eps, sweep_len;
@@ -115,6 +116,7 @@ struct Ctx {
     float lat_diag;  // interior rest-lattice DFSPH diagonal [1/m^2]
     float *pd, *z, *wv, *ap, *corr;
     float *kr, *pdr, *corrr;  // kappa / rho0, pd / rho0, corr / rho0: the quotients pressure_accel gathers
+    uint8_t* touched;         // gated accel pass: 1 where the acceleration was evaluated (not the exact-zero skip)
     int* act;
     // per world
     int *converged, *iters, *restart;
@@ -1079,6 +1081,23 @@ __global__ void k_begin_solve(Ctx c) {
     }
 }
 
+// true when p[j] != 0 for a listed neighbour j of i (stops at the first one)
+template <bool WIDE, class T>
+__device__ __forceinline__ bool any_nbr_nonzero(const Ctx& c, int i, int cnt, int base, const T* p) {
+    bool any = false;
+    for (int k0 = 0; k0 < cnt && !any; k0 += NB_U) {
+        int j[NB_U];
+        nbr_batch<WIDE>(c, i, cnt, base, k0, j);
+        T v[NB_U];
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u) v[u] = p[j[u]];
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u)
+            if (k0 + u < cnt && v[u] != T(0)) any = true;
+    }
+    return any;
+}
+
 // pressure.py:264-277
 struct QKR {
     float4 q;
@@ -1147,14 +1166,33 @@ template <bool WIDE>
 __global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, const float* vecr, int gated) {
     CHUNK_PROLOGUE
     if (block_skip(c, c.ch_world[b_], gated)) return;
-    if (c.prof_bytes) account_bytes(c, KID_ACCEL, b_, valid ? pass_bytes<WIDE>(c, i, 69u, 17u) : 0u);
-    if (!valid) return;
-    if (!is_fluid(c, i)) {
-        c.acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        return;
+    unsigned nb = 0;  // algorithmic bytes of this particle (profiling)
+    if (valid) {
+        if (!is_fluid(c, i)) {
+            c.acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            nb = 17u;
+        } else {
+            bool full = true;
+            if (gated && c.sparse_corr) {
+                // every term of pressure_accel is m_j (k_i + k_j) grad W or k_i grad psi: with vec_i = 0 (so k_i = 0)
+                // and every gathered k_j = 0 each term is +-0 and the sum stays exactly +0, which is what is stored
+                int cnt = c.count[i];
+                full = vec[i] != 0.0f || any_nbr_nonzero<WIDE>(c, i, cnt, c.ws[c.world[i]], vecr);
+                c.touched[i] = full ? 1 : 0;
+                if (!full) {
+                    c.acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    // flags 4, phase 1, vec 4, vec / rho0 4, count 4, world 4, touched 1, acc written 12
+                    nb = 34u + (WIDE ? 3u : 2u) * unsigned(cnt);
+                }
+            }
+            if (full) {
+                f3 a = pressure_accel<WIDE>(c, vec, vecr, i);
+                c.acc4[i] = make_float4(a.x, a.y, a.z, 0.0f);
+                nb = pass_bytes<WIDE>(c, i, 69u + (gated && c.sparse_corr ? 1u : 0u), 17u);
+            }
+        }
     }
-    f3 a = pressure_accel<WIDE>(c, vec, vecr, i);
-    c.acc4[i] = make_float4(a.x, a.y, a.z, 0.0f);
+    if (c.prof_bytes) account_bytes(c, KID_ACCEL, b_, nb);
 }
 
 // cg_apply_kernel (krylov.py:90-109); with `reduce` the chunk partial of pair.x = vec_i (A vec)_i
@@ -1163,22 +1201,34 @@ __global__ void __launch_bounds__(256) k_apply(Ctx c, int mode, float dt, const 
                                                int bias, int accum, int reduce) {
     CHUNK_PROLOGUE
     if (block_skip(c, c.ch_world[b_], gated)) return;
-    if (c.prof_bytes)
-        account_bytes(c, KID_APPLY, b_,
-                      valid ? pass_bytes<WIDE>(c, i, 77u + (bias ? 4u : 0u) + (accum ? 4u : 0u), 9u) : 0u);
+    unsigned nb = 0;  // algorithmic bytes of this particle (profiling)
     double px = 0.0;
     if (valid) {
         if (!is_fluid(c, i)) {
             out[i] = 0.0f;
+            nb = 9u;
         } else {
-            float sc = mode == 0 ? dt : dt * dt;
-            float v = -sc * (accel_divergence<WIDE>(c, i) - c.reg[i] * vec[i]);
-            px = double(vec[i] * v);
-            if (bias) v -= (mode == 0 ? c.src[i] : c.src[i] - 1.0f);
-            if (accum) v += out[i];
-            out[i] = v;
+            bool full = true;
+            if (gated && c.sparse_corr) {
+                // the gated pass accumulates A corr: where neither this particle nor a neighbour had its acceleration
+                // evaluated (all of them exact +0), the divergence is +0, v = -0 and out += v leaves out unchanged
+                int cnt = c.count[i];
+                full = c.touched[i] != 0 || any_nbr_nonzero<WIDE>(c, i, cnt, c.ws[c.world[i]], c.touched);
+                if (!full) nb = 14u + (WIDE ? 3u : 2u) * unsigned(cnt);  // flags, phase, touched, count, world
+            }
+            if (full) {
+                float sc = mode == 0 ? dt : dt * dt;
+                float v = -sc * (accel_divergence<WIDE>(c, i) - c.reg[i] * vec[i]);
+                px = double(vec[i] * v);
+                if (bias) v -= (mode == 0 ? c.src[i] : c.src[i] - 1.0f);
+                if (accum) v += out[i];
+                out[i] = v;
+                nb = pass_bytes<WIDE>(c, i, 77u + (bias ? 4u : 0u) + (accum ? 4u : 0u) +
+                                                (gated && c.sparse_corr ? 1u : 0u), 9u);
+            }
         }
     }
+    if (c.prof_bytes) account_bytes(c, KID_APPLY, b_, nb);
     if (reduce) chunk_reduce(c, px, 0.0, valid ? 0.0 : -3.4e38);
 }
 
@@ -2115,7 +2165,7 @@ int run_segment(Handle* H, hipStream_t st, int kind, int mode, float dt, F enque
     if (!H->use_graphs || H->prof.on) return enqueue(st);
     uint32_t bits;
     std::memcpy(&bits, &dt, 4);
-    GraphKey key{kind, mode, H->check_every, H->c.pair_cache, bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
+    GraphKey key{kind, mode, H->check_every, H->c.pair_cache | (H->c.sparse_corr << 8), bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
                  H->mlp_params, H->log_feat, H->log_targets};
     auto it = H->graphs.find(key);
     if (it == H->graphs.end()) {
@@ -2244,6 +2294,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
     std::memset(&c, 0, sizeof(Ctx));
     c.n = n;
     c.W = W;
+    c.sparse_corr = 1;
     c.S = S;
     c.NW = L->wall_particle_count;
     c.cap = P->neighbor_capacity;
@@ -2367,6 +2418,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
                           &c.kd, &c.pd, &c.z, &c.wv, &c.ap, &c.corr, &c.kr, &c.pdr, &c.corrr})
             TRY(dalloc(H, p, n));
         TRY(dalloc(H, &c.acc4, n));
+        TRY(dalloc(H, &c.touched, n));
         TRY(dalloc(H, &c.q4, n));
         TRY(dalloc(H, &c.q4s, n));
         TRY(dalloc(H, &c.feat, 16 * int64_t(n)));
@@ -2627,6 +2679,10 @@ int32_t dfsph_set_option(void* h, int32_t option, int32_t value) {
                 return fail(DFSPH_EINVAL, "pair cache: a mask of DFSPH_PAIR_CACHE_* bits");
             H->c.pair_cache = value;
             return DFSPH_OK;
+        case DFSPH_OPT_SPARSE_CORRECTION:
+            if (value < 0 || value > 1) return fail(DFSPH_EINVAL, "sparse correction: 0 or 1");
+            H->c.sparse_corr = value;
+            return DFSPH_OK;
         case DFSPH_OPT_LEARNED_SOLVES:
             if (value < 0 || value > 3) return fail(DFSPH_EINVAL, "learned solves: bit 0 divergence, bit 1 density");
             H->learned_solves = value;
@@ -2651,6 +2707,9 @@ int32_t dfsph_get_option(void* h, int32_t option, int32_t* value) {
             return DFSPH_OK;
         case DFSPH_OPT_LEARNED_SOLVES:
             *value = H->learned_solves;
+            return DFSPH_OK;
+        case DFSPH_OPT_SPARSE_CORRECTION:
+            *value = H->c.sparse_corr;
             return DFSPH_OK;
         default:
             return fail(DFSPH_EINVAL, "unknown option");

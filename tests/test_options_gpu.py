@@ -48,7 +48,8 @@ def test_options_are_bit_identical(scene_name):
     _log(f"[options] {scene_name}: {sc.model.particle_count} particles, {sc.model.world_count} worlds")
     q0, v0, it0 = _run(sc, 12, {})
     for opts in ({_abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ALL}, {_abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_APPLY},
-                 {_abi.OPT_CHECK_INTERVAL: 3}, {_abi.OPT_GRAPHS: 1, _abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ACCEL}):
+                 {_abi.OPT_CHECK_INTERVAL: 3}, {_abi.OPT_GRAPHS: 1, _abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ACCEL},
+                 {_abi.OPT_SPARSE_CORRECTION: 0}, {_abi.OPT_SPARSE_CORRECTION: 0, _abi.OPT_GRAPHS: 1}):
         q, v, it = _run(sc, 12, opts)
         _log(f"  {opts}: q equal {np.array_equal(q, q0)} qd equal {np.array_equal(v, v0)}")
         assert np.array_equal(q, q0) and np.array_equal(v, v0), opts
@@ -70,6 +71,9 @@ def test_option_abi_round_trip_and_errors():
         s.set_option(_abi.OPT_PAIR_CACHE, 8)
     with pytest.raises(ValueError):
         s.set_option(99, 1)
+    assert s.get_option(_abi.OPT_SPARSE_CORRECTION) == 1
+    s.set_option(_abi.OPT_SPARSE_CORRECTION, 0)
+    assert s.get_option(_abi.OPT_SPARSE_CORRECTION) == 0
 
 
 def test_profiled_bytes_match_the_host_formula():
@@ -80,6 +84,7 @@ def test_profiled_bytes_match_the_host_formula():
 
     sc = scenes.dam_break(20, 0.4, 0, device="cuda")
     s = sc.make_solver()
+    s.set_option(_abi.OPT_SPARSE_CORRECTION, 0)  # every launch of a world does full work (the formula's case)
     st = sc.model.state()
     for _ in range(3):
         s.step(st, st, None, None, DT)
