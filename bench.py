@@ -38,12 +38,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s at 0.1% density error, learned+batched vs plain DFSPH, 1 GPU"
-WORKLOADS = {
-    "bowl_worlds_512": dict(kind="bowl_worlds", worlds=512, seed=2),
-    "dam_break_125k": dict(kind="dam_break", n_side=50, box=1.0, seed=1),
-    "dam_break_8k": dict(kind="dam_break", n_side=20, box=0.4, seed=0),
-    "dam_break_1m": dict(kind="dam_break", n_side=100, box=2.0, seed=3),
+WORKLOADS = {  # steps: the timed steps each workload is specified with
+    "bowl_worlds_512": dict(kind="bowl_worlds", worlds=512, seed=2, steps=300),
+    "dam_break_125k": dict(kind="dam_break", n_side=50, box=1.0, seed=1, steps=500),
+    "dam_break_8k": dict(kind="dam_break", n_side=20, box=0.4, seed=0, steps=200),
+    "dam_break_1m": dict(kind="dam_break", n_side=100, box=2.0, seed=3, steps=200),
 }
+CPU_SAMPLE_PARTICLE_STEPS = 3.0e6  # about 20 s of the serial oracle
 
 
 def make_scene(wl: dict, device: str, subset=None):
@@ -167,6 +168,8 @@ def cpu_baseline(wl: dict, steps: int, warmup: int, sample_w: int) -> dict:
 
     subset = sample_worlds(wl["worlds"], sample_w) if wl["kind"] == "bowl_worlds" else None
     sc = make_scene(wl, "cpu", subset=subset)
+    if steps is None:
+        steps = int(max(2, min(40, CPU_SAMPLE_PARTICLE_STEPS // max(sum(sc.fluid_counts), 1))))
     o = OracleDFSPH(sc.model, sc.config, 32)
     for _ in range(warmup):
         o.step(scenes.DT)
@@ -206,7 +209,7 @@ def traffic_from_profiles(kernel: str, workload: str):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="bowl_worlds_512", choices=sorted(WORKLOADS))
     ap.add_argument("--learned", default="auto", choices=["auto", "on", "off"],
@@ -219,7 +222,8 @@ def main() -> int:
     ap.add_argument("--graphs", action="store_true", help="replay the step's segments as hipGraphs")
     ap.add_argument("--pair-cache", type=int, default=None, help="DFSPH_PAIR_CACHE_* mask (default: the library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="timed oracle steps (default: about 3e6 particle-steps of the sample)")
     ap.add_argument("--cpu-worlds", type=int, default=4, help="worlds of the batched pack in the CPU sample")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--roofline-sample", type=int, default=16,
@@ -250,6 +254,8 @@ def main() -> int:
     from alloy_amd.sph import _abi, scenes
 
     wl = WORKLOADS[args.workload]
+    if args.steps is None:
+        args.steps = wl["steps"]
     dt = scenes.DT
     _log(f"[bench] workload {args.workload} {wl} steps={args.steps} warmup={args.warmup}")
     sc = make_scene(wl, "cuda")
