@@ -542,11 +542,11 @@ __global__ void __launch_bounds__(256) k_sort_cells(Ctx c, int first, int last) 
 // overflow counted per world. Each thread keeps its row in LDS (column per thread: conflict-free), as uint16
 // world-local ids unless a world exceeds 65,536 particles (WIDE: 32-bit entries), so the narrow build holds
 // twice the waves per CU.
-constexpr int BUILD_BLOCK = 128;
-template <bool WIDE>
-__global__ void __launch_bounds__(BUILD_BLOCK) k_build_lists(Ctx c) {
+// BB: the block size (DFSPH_OPT_BUILD_BLOCK); the row of thread t is row[k * BB + t]
+template <bool WIDE, int BB>
+__global__ void __launch_bounds__(BB) k_build_lists(Ctx c) {
     using RowT = typename std::conditional<WIDE, int, uint16_t>::type;
-    __shared__ RowT row[64 * BUILD_BLOCK];
+    __shared__ RowT row[64 * BB];
     int i = xcd_particle();
     if (i >= c.n) return;
     int g = c.group[i];
@@ -590,15 +590,15 @@ __global__ void __launch_bounds__(BUILD_BLOCK) k_build_lists(Ctx c) {
                         int jl = j - base;
                         int pos = nn, k = nn - 1;
                         while (k >= 0) {
-                            int v = L[k * BUILD_BLOCK];
+                            int v = L[k * BB];
                             if (v < jl) break;
                             pos = k;
                             k -= 1;
                         }
                         if (pos >= cap) continue;
                         int lastk = nn >= cap ? cap - 1 : nn;
-                        for (k = lastk; k > pos; --k) L[k * BUILD_BLOCK] = L[(k - 1) * BUILD_BLOCK];
-                        L[pos * BUILD_BLOCK] = RowT(jl);
+                        for (k = lastk; k > pos; --k) L[k * BB] = L[(k - 1) * BB];
+                        L[pos * BB] = RowT(jl);
                         if (nn < cap) nn += 1;
                     }
                 }
@@ -606,7 +606,7 @@ __global__ void __launch_bounds__(BUILD_BLOCK) k_build_lists(Ctx c) {
         }
     }
     for (int k = 0; k < nn; ++k) {
-        int v = L[k * BUILD_BLOCK];
+        int v = L[k * BB];
         c.nbr[int64_t(k) * c.n + i] = uint16_t(v & 65535);
         if (WIDE) c.nbr_hi[int64_t(k) * c.n + i] = uint8_t(v >> 16);
     }
@@ -1889,6 +1889,7 @@ struct Handle {
     bool use_graphs = false;  // measured: eager launches are faster than graph replay for this step (bench.py --graphs)
     const float* mlp_params = nullptr;  // learned warm start on when set
     int learned_solves = 3;             // DFSPH_OPT_LEARNED_SOLVES: bit 0 divergence, bit 1 density
+    int build_block = 128;              // DFSPH_OPT_BUILD_BLOCK
     float* log_feat = nullptr;          // when set, the step copies its features [n][16] here ...
     float* log_targets = nullptr;       // ... and its converged-kappa targets [n][2] here
     hipStream_t cap_stream = nullptr;
@@ -1992,6 +1993,28 @@ inline int blocks_for(int64_t n, int b = 256) { return int(std::max<int64_t>((n 
 Handle* as_handle(void* h) { return static_cast<Handle*>(h); }
 
 // ----------------------------------------------------------------------------------------------- phases (host)
+template <bool WIDE, int BB>
+void launch_build_lists_bb(Handle* H, hipStream_t st) {
+    const Ctx& c = H->c;
+    auto kf = k_build_lists<WIDE, BB>;
+    hipEvent_t ea, eb;
+    unsigned long long* pb = nullptr;
+    int grid = blocks_for(c.n, BB);
+    if (prof_begin(H, KID_BUILD, &ea, &eb, &pb)) hipExtLaunchKernelGGL(kf, dim3(grid), dim3(BB), 0, st, ea, eb, 0u, c);
+    else hipLaunchKernelGGL(kf, dim3(grid), dim3(BB), 0, st, c);
+}
+// the list build with the tuned block size (64, 128 or 256; 256 only with uint16 rows: 64 KB of LDS otherwise)
+void launch_build_lists(Handle* H, hipStream_t st) {
+    if (H->c.wide) {
+        if (H->build_block == 64) launch_build_lists_bb<true, 64>(H, st);
+        else launch_build_lists_bb<true, 128>(H, st);
+    } else {
+        if (H->build_block == 64) launch_build_lists_bb<false, 64>(H, st);
+        else if (H->build_block == 256) launch_build_lists_bb<false, 256>(H, st);
+        else launch_build_lists_bb<false, 128>(H, st);
+    }
+}
+
 int phase_neighbors(Handle* H, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
@@ -2003,7 +2026,7 @@ int phase_neighbors(Handle* H, hipStream_t st) {
     k_scan_apply<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part, c.cell_start, c.cell_fill);
     k_scatter<<<blocks_for(c.n), 256, 0, st>>>(c);
     k_sort_cells<<<blocks_for(c.n_cells), 256, 0, st>>>(c, 0, c.n_cells);  // ascending ids in every cell
-    LAUNCH_WT(KID_BUILD, k_build_lists, blocks_for(c.n, BUILD_BLOCK), BUILD_BLOCK, c);
+    launch_build_lists(H, st);
     LAUNCH_CHECK();
     return DFSPH_OK;
 }
@@ -2165,7 +2188,7 @@ int run_segment(Handle* H, hipStream_t st, int kind, int mode, float dt, F enque
     if (!H->use_graphs || H->prof.on) return enqueue(st);
     uint32_t bits;
     std::memcpy(&bits, &dt, 4);
-    GraphKey key{kind, mode, H->check_every, H->c.pair_cache | (H->c.sparse_corr << 8), bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
+    GraphKey key{kind, mode, H->check_every, H->c.pair_cache | (H->c.sparse_corr << 8) | (H->build_block << 16), bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
                  H->mlp_params, H->log_feat, H->log_targets};
     auto it = H->graphs.find(key);
     if (it == H->graphs.end()) {
@@ -2679,6 +2702,12 @@ int32_t dfsph_set_option(void* h, int32_t option, int32_t value) {
                 return fail(DFSPH_EINVAL, "pair cache: a mask of DFSPH_PAIR_CACHE_* bits");
             H->c.pair_cache = value;
             return DFSPH_OK;
+        case DFSPH_OPT_BUILD_BLOCK:
+            if (value != 64 && value != 128 && value != 256)
+                return fail(DFSPH_EINVAL, "build block: 64, 128 or 256");
+            if (value == 256 && H->c.wide) return fail(DFSPH_EINVAL, "build block 256 needs uint16 rows (no world over 65,536)");
+            H->build_block = value;
+            return DFSPH_OK;
         case DFSPH_OPT_SPARSE_CORRECTION:
             if (value < 0 || value > 1) return fail(DFSPH_EINVAL, "sparse correction: 0 or 1");
             H->c.sparse_corr = value;
@@ -2710,6 +2739,9 @@ int32_t dfsph_get_option(void* h, int32_t option, int32_t* value) {
             return DFSPH_OK;
         case DFSPH_OPT_SPARSE_CORRECTION:
             *value = H->c.sparse_corr;
+            return DFSPH_OK;
+        case DFSPH_OPT_BUILD_BLOCK:
+            *value = H->build_block;
             return DFSPH_OK;
         default:
             return fail(DFSPH_EINVAL, "unknown option");

@@ -1,0 +1,102 @@
+"""Autotuner of the native step's launch options on the pack it will run.
+
+Every tuning option (DFSPH_OPT_*: the host convergence-check interval, i.e. the iteration schedule of the pressure
+solves, the neighbour-list block size, the pair-gradient cache, the sparse correction pass, hipGraph replay) gives
+bit-identical results, so the tuner can time them on the live simulation: it snapshots the state, the warm starts and
+the shape meters, times short step windows for each candidate (one option at a time, each from the snapshot,
+median of ``rounds``), keeps the fastest value, and restores the snapshot. The caller's simulation continues exactly
+as if no tuning had happened. The reference picks launch parameters inside Warp; this has no counterpart there.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from typing import Callable
+
+import numpy as np
+import torch
+
+from . import _abi
+
+# (option, candidate values) in the order they are tuned; the first value of each is the library default
+CANDIDATES = (
+    (_abi.OPT_CHECK_INTERVAL, (1, 2, 3)),
+    (_abi.OPT_BUILD_BLOCK, (128, 64, 256)),
+    (_abi.OPT_SPARSE_CORRECTION, (1, 0)),
+    (_abi.OPT_PAIR_CACHE, (0, _abi.PAIR_CACHE_APPLY, _abi.PAIR_CACHE_ALL)),
+    (_abi.OPT_GRAPHS, (0, 1)),
+)
+NAMES = {_abi.OPT_CHECK_INTERVAL: "check_interval", _abi.OPT_BUILD_BLOCK: "build_block",
+         _abi.OPT_SPARSE_CORRECTION: "sparse_correction", _abi.OPT_PAIR_CACHE: "pair_cache", _abi.OPT_GRAPHS: "graphs"}
+
+
+class _Snapshot:
+    def __init__(self, solver, state):
+        self.q = state.particle_q.clone()
+        self.qd = state.particle_qd.clone()
+        self.arrays = solver.alloy_state_arrays(None)
+        S = solver.model.shape_count
+        self.meters = solver.buffer(_abi.BUF_SHAPE_METER, np.int64, (S, 3)) if S else None
+
+    def restore(self, solver, state):
+        state.particle_q.copy_(self.q)
+        state.particle_qd.copy_(self.qd)
+        solver.alloy_restore_state_arrays(self.arrays)
+        if self.meters is not None:
+            import ctypes
+
+            m = np.ascontiguousarray(self.meters)
+            solver._call(solver.lib.dfsph_write_buffer, _abi.BUF_SHAPE_METER, m.ctypes.data_as(ctypes.c_void_p),
+                         m.nbytes)
+
+
+def _window_ms(solver, state, dt: float, warm: int, steps: int) -> float:
+    for _ in range(warm):
+        solver.step(state, state, None, None, dt)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        solver.step(state, state, None, None, dt)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def autotune(solver, state, dt: float, *, steps: int = 8, warm: int = 2, rounds: int = 2, candidates=CANDIDATES,
+             log: Callable[[str], None] | None = None) -> dict:
+    """Tune ``solver``'s options on ``state`` in place (the state and the solver come back unchanged) and return
+    {"options": {name: value}, "ms_per_step": {name: {value: median ms}}, "seconds": tuning time}."""
+    t0 = time.perf_counter()
+    snap = _Snapshot(solver, state)
+    current = {opt: solver.get_option(opt) for opt, _ in candidates}
+    table: dict[str, dict] = {}
+    for opt, values in candidates:
+        best_v, best_t, row = current[opt], math.inf, {}
+        for v in values:
+            times = []
+            for _ in range(rounds):
+                snap.restore(solver, state)
+                try:
+                    for o, val in current.items():
+                        solver.set_option(o, v if o == opt else val)
+                except ValueError:  # not applicable to this pack (e.g. 256-thread builds with 24-bit rows)
+                    times = []
+                    break
+                times.append(_window_ms(solver, state, dt, warm, steps))
+            if not times:
+                continue
+            row[v] = float(np.median(times))
+            if row[v] < best_t:
+                best_v, best_t = v, row[v]
+        current[opt] = best_v
+        table[NAMES.get(opt, str(opt))] = row
+        if log:
+            log(f"autotune {NAMES.get(opt, opt)}: " + ", ".join(f"{k} {t:.3f} ms" for k, t in row.items())
+                + f" -> {best_v}")
+    for o, val in current.items():
+        solver.set_option(o, val)
+    snap.restore(solver, state)
+    return {"options": {NAMES.get(o, str(o)): v for o, v in current.items()}, "ms_per_step": table,
+            "seconds": time.perf_counter() - t0}

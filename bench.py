@@ -45,6 +45,7 @@ WORKLOADS = {  # steps: the timed steps each workload is specified with
     "dam_break_1m": dict(kind="dam_break", n_side=100, box=2.0, seed=3, steps=200),
 }
 CPU_SAMPLE_PARTICLE_STEPS = 3.0e6  # about 20 s of the serial oracle
+OPTION_IDS = {"check_interval": 1, "graphs": 2, "pair_cache": 3, "sparse_correction": 5, "build_block": 6}
 
 
 def make_scene(wl: dict, device: str, subset=None):
@@ -219,6 +220,9 @@ def main() -> int:
     ap.add_argument("--no-rule-compare", action="store_true",
                     help="skip the batched run with the rule warm start (the learned start's own effect)")
     ap.add_argument("--check-every", type=int, default=1, help="iterations between host convergence checks")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (it runs after the warm-up, outside the timed region, and changes no "
+                         "result bit)")
     ap.add_argument("--graphs", action="store_true", help="replay the step's segments as hipGraphs")
     ap.add_argument("--pair-cache", type=int, default=None, help="DFSPH_PAIR_CACHE_* mask (default: the library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -275,6 +279,11 @@ def main() -> int:
         solver.step(state, state, None, None, dt)
     torch.cuda.synchronize()
     _log("[bench] warm-up done")
+    tuned = None
+    if not args.no_autotune:
+        from alloy_amd.sph.autotune import autotune
+
+        tuned = autotune(solver, state, dt, log=_log)
     roof_mask = (1 << _abi.KID_ACCEL) | (1 << _abi.KID_APPLY)
     run = timed_run(solver, state, args.steps, dt, prof_mask=roof_mask, sample=args.roofline_sample)
     total_ms = run["total_ms"]
@@ -311,6 +320,9 @@ def main() -> int:
                      "alg_bytes_per_launch": bl / max(nl, 1), "avg_launch_ms": tl / max(nl, 1), "launches_timed": nl,
                      "window": f"the timed region, 1 in {args.roofline_sample} launches (pseudo-random)"},
         "kernel_ms_per_step_timed": {k: round(v[1] / args.steps, 4) for k, v in ks.items()},
+        "tuning": {"autotuned": tuned is not None,
+                   "options": tuned["options"] if tuned else None,
+                   "seconds": round(tuned["seconds"], 2) if tuned else 0.0},
     }
     if args.breakdown_steps > 0:
         bd = timed_run(solver, state, args.breakdown_steps, dt, prof_mask=(1 << len(_abi.KERNEL_NAMES)) - 1)
@@ -332,6 +344,8 @@ def main() -> int:
         s2.set_graphs(args.graphs)
         if args.pair_cache is not None:
             s2.set_option(_abi.OPT_PAIR_CACHE, args.pair_cache)
+        for name, v in (tuned["options"].items() if tuned else ()):
+            s2.set_option(OPTION_IDS[name], v)
         st2 = sc.model.state()
         for _ in range(args.warmup):
             s2.step(st2, st2, None, None, dt)

@@ -207,7 +207,7 @@ int32_t dfsph_set_warm_start_model(void* handle, const void* weights, void* stre
 int32_t dfsph_set_warm_start_kappa(void* handle, const float* kappa_div, const float* kappa_den);
 /* Host convergence check every `every` iterations (1..64); converged worlds' blocks exit at once in between. */
 int32_t dfsph_set_check_interval(void* handle, int32_t every);
-/* Tuning options. Options 1-3 and 5 give bit-identical results and only trade time; option 4 chooses which
+/* Tuning options. Options 1-3, 5 and 6 give bit-identical results and only trade time; option 4 chooses which
  * pressure solves start from the learned warm start (both still iterate to the same tolerance). */
 enum DfsphOption {
     DFSPH_OPT_CHECK_INTERVAL = 1, /* as dfsph_set_check_interval */
@@ -217,9 +217,11 @@ enum DfsphOption {
                                      memory traffic per pair traded for a square root and two divisions) */
     DFSPH_OPT_LEARNED_SOLVES = 4, /* with a warm-start model set: bit 0 the divergence solve, bit 1 the density solve
                                      start from its prediction (default 3); the others keep the reference's rule */
-    DFSPH_OPT_SPARSE_CORRECTION = 5 /* 1 (default): the CG iteration's correction product (krylov.py:238, A corr
-                                       on clamped particles) evaluates only particles with a nonzero term and writes
-                                       the others' exact zero results; 0 evaluates every particle. Bit-identical. */
+    DFSPH_OPT_SPARSE_CORRECTION = 5, /* 1 (default): the CG iteration's correction product (krylov.py:238, A corr
+                                        on clamped particles) evaluates only particles with a nonzero term and writes
+                                        the others' exact zero results; 0 evaluates every particle. Bit-identical. */
+    DFSPH_OPT_BUILD_BLOCK = 6        /* threads per block of the neighbour-list build: 64, 128 (default) or 256
+                                        (256 only without 24-bit rows). Bit-identical. */
 };
 enum DfsphPairCache {
     DFSPH_PAIR_CACHE_ACCEL = 1, /* pressure acceleration (k_accel, post-solve) */

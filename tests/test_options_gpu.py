@@ -49,7 +49,8 @@ def test_options_are_bit_identical(scene_name):
     q0, v0, it0 = _run(sc, 12, {})
     for opts in ({_abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ALL}, {_abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_APPLY},
                  {_abi.OPT_CHECK_INTERVAL: 3}, {_abi.OPT_GRAPHS: 1, _abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ACCEL},
-                 {_abi.OPT_SPARSE_CORRECTION: 0}, {_abi.OPT_SPARSE_CORRECTION: 0, _abi.OPT_GRAPHS: 1}):
+                 {_abi.OPT_SPARSE_CORRECTION: 0}, {_abi.OPT_SPARSE_CORRECTION: 0, _abi.OPT_GRAPHS: 1},
+                 {_abi.OPT_BUILD_BLOCK: 64}, {_abi.OPT_BUILD_BLOCK: 256}):
         q, v, it = _run(sc, 12, opts)
         _log(f"  {opts}: q equal {np.array_equal(q, q0)} qd equal {np.array_equal(v, v0)}")
         assert np.array_equal(q, q0) and np.array_equal(v, v0), opts
@@ -111,3 +112,27 @@ def test_profiled_bytes_match_the_host_formula():
     assert abs(k - round(k)) < 1e-9 and 1 <= round(k) <= launches
     la, ma, ba = res[_abi.KID_APPLY]
     assert la >= 3 and ma > 0.0 and ba > 0.0
+
+
+def test_autotune_leaves_the_simulation_unchanged():
+    """The autotuner times candidate options on the live state and restores it: stepping on afterwards gives the
+    same bits as a solver that was never tuned."""
+    import torch
+
+    from alloy_amd.sph import scenes
+    from alloy_amd.sph.autotune import autotune
+
+    sc = scenes.bowl_pool(3, bowl_radius=0.05, side=0.2, device="cuda")
+    a, b = sc.make_solver(), sc.make_solver()
+    sa, sb = sc.model.state(), sc.model.state()
+    for _ in range(3):
+        a.step(sa, sa, None, None, DT)
+        b.step(sb, sb, None, None, DT)
+    res = autotune(a, sa, DT, steps=2, warm=1, rounds=1, log=_log)
+    _log(f"[autotune] {res['options']} in {res['seconds']:.1f} s")
+    assert set(res["options"]) == {"check_interval", "build_block", "sparse_correction", "pair_cache", "graphs"}
+    for _ in range(6):
+        a.step(sa, sa, None, None, DT)
+        b.step(sb, sb, None, None, DT)
+    torch.cuda.synchronize()
+    assert torch.equal(sa.particle_q, sb.particle_q) and torch.equal(sa.particle_qd, sb.particle_qd)
