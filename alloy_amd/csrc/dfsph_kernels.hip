@@ -83,6 +83,13 @@ struct Ctx {
     // grid
     int* cell_count;
     int* cell_start;
+    uint8_t* wall_near;  // per cell, from wall_first: 1 when a wall particle is in it or one of its 26 neighbours
+    int wall_first;
+    int speed_later;     // the emitter pass (after emission) computes the speed statistics instead of the advect pass
+    int one_way;         // DFSPHConfig.coupling == "one_way": the liquid does not push back on particle walls
+    const int* wcidx;    // [n] compact index of each particle-wall particle (ascending ids), -1 otherwise
+    long long* pw_exch;  // [NW*3] impulse of the exchange window per wall particle, 2^50 N s
+    long long* pw_work;  // [W] the walls' work on the liquid in the last step, 2^50 J
     int* cell_fill;
     int* items;
     int* key;
@@ -154,7 +161,10 @@ __device__ __forceinline__ int nbr_at(const Ctx& c, int base, int i, int k) {
 // one dependent index -> record round trip per neighbour. Row k is addressed as a wave-uniform row base plus the
 // lane's i; slots past the lane's count (still inside the allocated rows) read a stale entry that is replaced by
 // i, so every gathered record is in bounds, and their bodies are skipped.
-constexpr int NB_U = 8;
+#ifndef DFSPH_NB_U
+#define DFSPH_NB_U 4  // neighbour records in flight per lane (build-time tuning; 4 measured fastest of 1..16)
+#endif
+constexpr int NB_U = DFSPH_NB_U;
 template <class T>
 __device__ __forceinline__ float4 load_pos(const T& d) { return d.q; }
 [[maybe_unused]] __device__ __forceinline__ float4 load_pos(float4 d) { return d; }
@@ -538,10 +548,37 @@ __global__ void __launch_bounds__(256) k_sort_cells(Ctx c, int first, int last) 
     }
 }
 
+// Every wall particle marks the cells of its wall grid within one cell of its own (the block pw_sample walks), so a
+// point whose cell is unmarked has no wall particle in its walk and its colour sums are exactly zero.
+__global__ void __launch_bounds__(256) k_mark_wall_cells(Ctx c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    int g = c.group[i];
+    if (g > -2) return;  // not a wall particle
+    int w = -2 - g;
+    int cx, cy, cz;
+    cell_of(c, c.W + w, xyz(c.q4[i]), cx, cy, cz);
+    const int* d = c.g_dims + 3 * w;
+    int cb = c.cell_base[c.W + w];
+    for (int X = max(cx - 1, 0); X <= min(cx + 1, d[0] - 1); ++X)
+        for (int Y = max(cy - 1, 0); Y <= min(cy + 1, d[1] - 1); ++Y)
+            for (int Z = max(cz - 1, 0); Z <= min(cz + 1, d[2] - 1); ++Z)
+                c.wall_near[cb + (X * d[1] + Y) * d[2] + Z - c.wall_first] = 1;
+}
+__global__ void k_fill_u8(uint8_t* p, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0;
+}
+
 // neighbors.py:160-195: rows of the lowest `cap` world-local ids within h (fp32 dot(d, d) < h^2), ascending;
 // overflow counted per world. Each thread keeps its row in LDS (column per thread: conflict-free), as uint16
 // world-local ids unless a world exceeds 65,536 particles (WIDE: 32-bit entries), so the narrow build holds
 // twice the waves per CU.
+#ifndef DFSPH_BUILD_U
+#define DFSPH_BUILD_U 8  // candidate records in flight per lane in the list build (build-time tuning)
+#endif
+constexpr int BUILD_U = DFSPH_BUILD_U;
 // BB: the block size (DFSPH_OPT_BUILD_BLOCK); the row of thread t is row[k * BB + t]
 template <bool WIDE, int BB>
 __global__ void __launch_bounds__(BB) k_build_lists(Ctx c) {
@@ -565,7 +602,7 @@ __global__ void __launch_bounds__(BB) k_build_lists(Ctx c) {
     int nn = 0, total = 0;
     // x outer, y, then the contiguous z-row of 3 cells, ids ascending within a cell: for lattice-ordered ids
     // (x slowest) candidates arrive nearly sorted and most insertions append. Candidates stream from q4s (cell
-    // order) in batches of 8.
+    // order) in batches of BUILD_U.
     for (int dx = -1; dx <= 1; ++dx) {
         int X = cx + dx;
         if (X < 0 || X >= d[0]) continue;
@@ -575,12 +612,12 @@ __global__ void __launch_bounds__(BB) k_build_lists(Ctx c) {
             int z0 = max(cz - 1, 0), z1 = min(cz + 1, d[2] - 1);
             int cb = c.cell_base[g] + (X * d[1] + Y) * d[2];
             int e0 = c.cell_start[cb + z0], e1 = c.cell_start[cb + z1 + 1];
-            for (int e = e0; e < e1; e += 8) {
-                float4 pc[8];
+            for (int e = e0; e < e1; e += BUILD_U) {
+                float4 pc[BUILD_U];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) pc[u] = c.q4s[min(e + u, e1 - 1)];
+                for (int u = 0; u < BUILD_U; ++u) pc[u] = c.q4s[min(e + u, e1 - 1)];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < BUILD_U; ++u) {
                     if (e + u >= e1) continue;
                     int j = __float_as_int(pc[u].w);
                     if (j == i) continue;
@@ -623,7 +660,8 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv)
     mv = f3{0.0f, 0.0f, 0.0f};
     if (c.NW == 0) return;
     int g = c.W + w, cx, cy, cz;
-    cell_of(c, g, x, cx, cy, cz);
+    int own = cell_of(c, g, x, cx, cy, cz);
+    if (!c.wall_near[own - c.wall_first]) return;  // no wall particle in the 27 cells: the sums below stay 0
     const int* d = c.g_dims + 3 * w;
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
@@ -657,6 +695,66 @@ __device__ float colour_distance(const Ctx& c, float col) {
     return 0.5f * (lo + hi);
 }
 
+// particle_walls.py:138-177 _accumulate_kernel: each particle-wall slot's impulse J, split over the world's wall
+// particles by V_j W_ij / c_i in int64 at 2^50 (the rounding remainder to the largest share, so the parts sum exactly
+// to the slot's fixed-point impulse), and the walls' work -J.v_b
+__global__ void __launch_bounds__(256) k_pw_accumulate(Ctx c) {
+    int i = xcd_particle();
+    if (i >= c.n) return;
+    const KC K = make_kc(c.h, c.kind);
+    const double IMP = 1125899906842624.0;
+    int64_t b4 = 4 * int64_t(i);
+    for (int k = 0; k < 4; ++k) {
+        if (c.s_shape[b4 + k] != c.S) continue;
+        f3 J = ld3(c.s_imp, b4 + k);
+        if (J.x == 0.0f && J.y == 0.0f && J.z == 0.0f) continue;
+        int w = c.world[i];
+        atomicAdd(reinterpret_cast<unsigned long long*>(c.pw_work) + w,
+                  static_cast<unsigned long long>(llround(double(-dot(J, ld3(c.s_vb, b4 + k))) * IMP)));
+        if (c.one_way) continue;
+        f3 x = xyz(c.q4[i]);
+        float col;
+        f3 gc, mv;
+        pw_sample(c, x, w, col, gc, mv);
+        if (col <= 0.0f) continue;
+        long long left[3] = {llround(double(J.x) * IMP), llround(double(J.y) * IMP), llround(double(J.z) * IMP)};
+        int best = -1;
+        float sbest = 0.0f;
+        int g = c.W + w, cx, cy, cz;
+        cell_of(c, g, x, cx, cy, cz);
+        const int* d = c.g_dims + 3 * w;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int X = cx + dx, Y = cy + dy, Z = cz + dz;
+                    if (X < 0 || Y < 0 || Z < 0 || X >= d[0] || Y >= d[1] || Z >= d[2]) continue;
+                    int cc = c.cell_base[g] + (X * d[1] + Y) * d[2] + Z;
+                    for (int e = c.cell_start[cc]; e < c.cell_start[cc + 1]; ++e) {
+                        int j = c.items[e];
+                        float sj = c.wvol[j] * kernel_w(length(x - xyz(c.q4[j])), K) / col;
+                        if (sj > 0.0f) {
+                            int cj = c.wcidx[j];
+                            long long part[3] = {llround(double(J.x * sj) * IMP), llround(double(J.y * sj) * IMP),
+                                                 llround(double(J.z * sj) * IMP)};
+                            for (int q = 0; q < 3; ++q) {
+                                atomicAdd(reinterpret_cast<unsigned long long*>(c.pw_exch) + 3 * int64_t(cj) + q,
+                                          static_cast<unsigned long long>(part[q]));
+                                left[q] -= part[q];
+                            }
+                            if (sj > sbest) {
+                                sbest = sj;
+                                best = cj;
+                            }
+                        }
+                    }
+                }
+        if (best >= 0)
+            for (int q = 0; q < 3; ++q)
+                atomicAdd(reinterpret_cast<unsigned long long*>(c.pw_exch) + 3 * int64_t(best) + q,
+                          static_cast<unsigned long long>(left[q]));
+    }
+}
+
 // ----------------------------------------------------------------------------------------------- boundary slots
 struct SlotRow {
     int shape[4];
@@ -664,22 +762,43 @@ struct SlotRow {
     f3 nrm[4];
 };
 
-// boundary_slots.py:13-38 (slot normals are not shifted, as in the reference)
+// Element k of a 4-array with a run-time k, by selects over the constant indices, so the array stays in registers.
+template <typename T>
+__device__ __forceinline__ T pick4(const T (&a)[4], int k) {
+    T r = a[0];
+#pragma unroll
+    for (int m = 1; m < 4; ++m)
+        if (k == m) r = a[m];
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ void put4(T (&a)[4], int k, T v) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+        if (k == m) a[m] = v;
+}
+
+// boundary_slots.py:13-38 (slot normals are not shifted, as in the reference). The scan runs from slot nn - 1 down
+// and stops at the first phi <= the new one; the shifts run from the top, every index a constant.
 __device__ __forceinline__ int slot_insert(SlotRow& R, int nn, int s, float phi) {
-    int pos = nn, k = nn - 1;
-    while (k >= 0) {
-        if (R.phi[k] <= phi) break;
-        pos = k;
-        k -= 1;
-    }
+    int pos = nn;
+    bool scan = true;
+#pragma unroll
+    for (int k = 3; k >= 0; --k)
+        if (k < nn && scan) {
+            if (R.phi[k] <= phi) scan = false;
+            else pos = k;
+        }
     if (pos >= 4) return -1;
     int last = nn >= 4 ? 3 : nn;
-    for (k = last; k > pos; --k) {
-        R.shape[k] = R.shape[k - 1];
-        R.phi[k] = R.phi[k - 1];
-    }
-    R.shape[pos] = s;
-    R.phi[pos] = phi;
+#pragma unroll
+    for (int k = 3; k >= 1; --k)
+        if (k <= last && k > pos) {
+            R.shape[k] = R.shape[k - 1];
+            R.phi[k] = R.phi[k - 1];
+        }
+    put4(R.shape, pos, s);
+    put4(R.phi, pos, phi);
     return pos;
 }
 __device__ __forceinline__ void slot_copy(SlotRow& R, int src, int dst) {
@@ -692,8 +811,10 @@ __device__ int shape_insert(SlotRow& R, int nn, int s, float phi, f3 nrm, float 
     if (nn < 4) {
         int rr = slot_insert(R, nn, s, phi);
         if (rr >= 0) {
-            for (int k = nn; k > rr; --k) R.nrm[k] = R.nrm[k - 1];
-            R.nrm[rr] = nrm;
+#pragma unroll
+            for (int k = 3; k >= 1; --k)
+                if (k <= nn && k > rr) R.nrm[k] = R.nrm[k - 1];
+            put4(R.nrm, rr, nrm);
             return 1;
         }
         return 0;
@@ -719,15 +840,22 @@ __device__ int shape_insert(SlotRow& R, int nn, int s, float phi, f3 nrm, float 
     }
     if (ev < 0 && phi < R.phi[3]) ev = 3;
     if (ev < 0 || ev == 4) return 0;
-    for (int k = ev; k < 3; ++k) slot_copy(R, k + 1, k);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (k >= ev) slot_copy(R, k + 1, k);
     int pos = 3;
-    while (pos > 0 && R.phi[pos - 1] > phi) {
-        slot_copy(R, pos - 1, pos);
-        pos -= 1;
-    }
-    R.shape[pos] = s;
-    R.phi[pos] = phi;
-    R.nrm[pos] = nrm;
+    bool scan = true;
+#pragma unroll
+    for (int k = 3; k >= 1; --k)
+        if (scan && R.phi[k - 1] > phi) {
+            slot_copy(R, k - 1, k);
+            pos = k - 1;
+        } else {
+            scan = false;
+        }
+    put4(R.shape, pos, s);
+    put4(R.phi, pos, phi);
+    put4(R.nrm, pos, nrm);
     return 1;
 }
 
@@ -795,11 +923,11 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
     float psi[4];
     f3 gp[4];
     for (int k = 0; k < 4; ++k) {
-        int s = R.shape[k];
+        int s = pick4(R.shape, k);
         c.s_shape[b4 + k] = s;
         if (s < 0) {
-            psi[k] = 0.0f;
-            gp[k] = f3{0.0f, 0.0f, 0.0f};
+            put4(psi, k, 0.0f);
+            put4(gp, k, f3{0.0f, 0.0f, 0.0f});
             continue;
         }
         if (s == c.S && have_pw) {  // particle wall: its local plane (boundary.py:272-287)
@@ -814,8 +942,8 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
             v1 = v1 * c.amplitude;
             taper(c, v0, v1, phi);
             c.s_phi[b4 + k] = phi;
-            psi[k] = v0;
-            gp[k] = v1 * nrm;
+            put4(psi, k, v0);
+            put4(gp, k, v1 * nrm);
             st3(c.s_xb, b4 + k, xb);
             st3(c.s_vb, b4 + k, pmv);
             st3(c.s_xbl, b4 + k, xb);
@@ -844,17 +972,19 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
         v1 = v1 * c.amplitude;
         taper(c, v0, v1, phi);
         c.s_phi[b4 + k] = phi;
-        psi[k] = v0;
-        gp[k] = v1 * nrm;
+        put4(psi, k, v0);
+        put4(gp, k, v1 * nrm);
         st3(c.s_xb, b4 + k, x - phi * nrm);
         st3(c.s_vb, b4 + k, f3{0.0f, 0.0f, 0.0f});
     }
     // inclusion-exclusion (boundary.py:370-396); the gradients of earlier slots are read after their update
     float ps[4] = {psi[0], psi[1], psi[2], psi[3]};
+#pragma unroll
     for (int b = 0; b < 4; ++b) {
         if (R.shape[b] < 0) continue;
         f3 gb = gp[b];
         float ob = 0.0f, sb = 0.0f;
+#pragma unroll
         for (int cc = 0; cc < 4; ++cc) {
             if (cc == b || R.shape[cc] < 0) continue;
             f3 gcv = gp[cc];
@@ -1670,6 +1800,17 @@ __device__ __forceinline__ float block_max_f32(float v) {
 
 constexpr int LDS_SHAPES = 32;  // a world's shapes accumulated in LDS before one global atomic per shape and block
 
+// speed_kernel (integrate.py:312-326) for one fluid particle: its speed, or 1e30 and a non-finite count
+__device__ __forceinline__ void speed_stat(f3 v, f3 x, float& spmax, int& n_nonfinite) {
+    float sp = length(v);
+    if (isfinite(sp) && isfinite(x.x + x.y + x.z)) {
+        spmax = fmx(spmax, sp);
+    } else {
+        n_nonfinite += 1;
+        spmax = 1.0e30f;
+    }
+}
+
 // shape meters (bodies.py:37-60, static shapes), advect_kernel, particle-wall advance, project_kernel and
 // speed_kernel (integrate.py:181-326). Each particle reads only its own state and slots; per-world counters and the
 // max speed are reduced in the block (a block is one world's chunk) and the shape meters in LDS, so global atomics
@@ -1689,7 +1830,7 @@ __global__ void __launch_bounds__(256) k_advect_project(Ctx c, float dt) {
         if (!is_fluid(c, i)) {
             if (c.NW > 0 && c.wvol[i] > 0.0f) st3(c.q, i, Q(c, i) + dt * V(c, i));
         } else {
-            for (int k = 0; k < 4; ++k) {  // bodies.accumulate_kernel meters (two-way)
+            for (int k = 0; k < 4 && !c.one_way; ++k) {  // bodies.accumulate_kernel meters (two-way only)
                 int s = c.s_shape[b4 + k];
                 if (s < 0 || s >= c.S) continue;
                 f3 J = ld3(c.s_imp, b4 + k);
@@ -1786,13 +1927,7 @@ __global__ void __launch_bounds__(256) k_advect_project(Ctx c, float dt) {
                 st3(c.qd, i, v);
                 n_proj = 1;
             }
-            float sp = length(v);
-            if (isfinite(sp) && isfinite(x.x + x.y + x.z)) {
-                spmax = sp;
-            } else {
-                n_nonfinite = 1;
-                spmax = 1.0e30f;
-            }
+            if (!c.speed_later) speed_stat(v, x, spmax, n_nonfinite);
         }
     }
     __syncthreads();
@@ -1811,6 +1946,177 @@ __global__ void __launch_bounds__(256) k_advect_project(Ctx c, float dt) {
         if (n_nonfinite) atomicAdd(c.st_nonfinite + w, n_nonfinite);
         atomic_max_nonneg(c.st_max_speed + w, spmax);
     }
+}
+
+// ----------------------------------------------------------------------------------------------- emitters
+// Box emitters, sinks, the escape box and the int64 ledgers (emitters.py:32-255, runtime.py:197-409). The tables are
+// device copies of the host's emitter table; per-world ledgers are int64 fixed point, so their atomic sums do not
+// depend on the order of the additions.
+struct EmitCtx {
+    int E, has_bounds;
+    f3 lo, hi;                    // escape box (config.world_bounds)
+    const int* e_world;           // [E]
+    const float* e_xform;         // [E*7] region transform
+    const float* e_vel;           // [E*3]
+    const float* e_interval;      // [E] seconds per layer
+    const int* off_start;         // [E+1] CSR into off_local
+    const float* off_local;       // [*3] layer lattice in the region frame
+    const int* pool_start;        // [E+1] CSR into pool_ids
+    const int* pool_ids;          // spare ids in emission order
+    float* t_acc;                 // [E] layer clock
+    int* cursor;                  // [E] spares used
+    int* enabled;                 // [E]
+    const int* sink_start;        // [W+1] CSR of the sinks of each world
+    const float* s_xform;         // [*7]
+    const float* s_half;          // [*3]
+    long long *mass0, *mass_in, *mass_out, *mass_act, *flux_e, *flux_g;  // [W] fixed point
+    int *escaped, *emit_overflow;                                         // [W]
+};
+
+// kernels.py:142-144 to_fixed at 2^40 kg; bodies.py:31-32 _fx
+__device__ __forceinline__ long long to_fixed_scaled(double x, double scale) { return llround(x * scale); }
+__device__ __forceinline__ long long to_fixed(float m) { return llround(double(m) * 1099511627776.0); }
+__device__ __forceinline__ void add_i64(long long* p, long long v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
+}
+
+// emitters.py:82-89 flux_add: the energy m (v^2/2 - g.x) [2^50 J] and the slope m g_z [2^40 N] a particle brings in
+// (sign +1) or takes out (-1)
+__device__ __forceinline__ void flux_add(const EmitCtx& e, int w, float m, f3 v, f3 x, f3 g, float sign) {
+    float en = sign * m * (0.5f * dot(v, v) - dot(g, x));
+    add_i64(e.flux_e + w, to_fixed_scaled(double(en), 1125899906842624.0));
+    add_i64(e.flux_g + w, to_fixed_scaled(double(sign * m * g.z), 1099511627776.0));
+}
+
+// emitters.py:127-130 _in_box
+__device__ __forceinline__ bool in_box(const Xf& X, f3 half, f3 x) {
+    f3 xl = xf_inv_point(X, x);
+    return fabsf(xl.x) <= half.x && fabsf(xl.y) <= half.y && fabsf(xl.z) <= half.z;
+}
+
+// emitters.py:133-165 retire_kernel (sinks of the particle's world, then the escape box) and, for the particles that
+// stay, speed_kernel (integrate.py:312-326), which the reference runs after emission; emitted particles add their
+// own speed in k_emit. One block per chunk (one world): the speed statistics are reduced in the block.
+__global__ void __launch_bounds__(256) k_retire(Ctx c, EmitCtx e) {
+    CHUNK_PROLOGUE
+    int w = c.ch_world[b_];
+    int n_nonfinite = 0;
+    float spmax = 0.0f;
+    if (valid && is_fluid(c, i)) {
+        f3 x = Q(c, i);
+        int gone = 0;
+        for (int s = e.sink_start[w]; s < e.sink_start[w + 1]; ++s)
+            if (in_box(load_xf(e.s_xform + 7 * s), ld3(e.s_half, s), x)) gone = 1;
+        if (e.has_bounds != 0 && gone == 0) {
+            bool out = x.x < e.lo.x || x.y < e.lo.y || x.z < e.lo.z || x.x > e.hi.x || x.y > e.hi.y || x.z > e.hi.z;
+            if (out || !isfinite(x.x + x.y + x.z)) {
+                gone = 2;
+                atomicAdd(e.escaped + w, 1);
+            }
+        }
+        if (gone != 0) {
+            f3 v = V(c, i);
+            float m = c.mass[i];
+            if (isfinite(x.x + x.y + x.z + v.x + v.y + v.z)) flux_add(e, w, m, v, x, ld3(c.gravity, w), -1.0f);
+            c.flags[i] = c.flags[i] & ~ACTIVE;
+            c.phase[i] = int8_t(-2);  // retired: never reused by an emitter
+            st3(c.qd, i, f3{0.0f, 0.0f, 0.0f});
+            add_i64(e.mass_out + w, to_fixed(m));
+        } else {
+            speed_stat(V(c, i), x, spmax, n_nonfinite);
+        }
+    }
+    n_nonfinite = block_sum_i32(n_nonfinite);
+    spmax = block_max_f32(spmax);
+    if (threadIdx.x == 0) {
+        if (n_nonfinite) atomicAdd(c.st_nonfinite + w, n_nonfinite);
+        atomic_max_nonneg(c.st_max_speed + w, spmax);
+    }
+}
+
+// emitters.py:67-79 emit_clock_kernel, 92-117 emit_kernel and 120-124 advance_cursor_kernel for one emitter per
+// block: the clock, then a layer from the emitter's pool (whole spares in pool order; the ones past the pool count
+// as overflow), then the cursor. Emitted particles enter speed_kernel's statistics here.
+__global__ void __launch_bounds__(256) k_emit(Ctx c, EmitCtx e, float dt) {
+    const int em = blockIdx.x;
+    __shared__ int now;
+    if (threadIdx.x == 0) {
+        int go = 0;
+        if (e.enabled[em] != 0) {
+            float t = e.t_acc[em] + dt;
+            if (t >= e.e_interval[em]) {
+                t -= e.e_interval[em];
+                go = 1;
+            }
+            e.t_acc[em] = t;
+        }
+        now = go;
+    }
+    __syncthreads();
+    if (now == 0) return;
+    const int w = e.e_world[em];
+    const int o0 = e.off_start[em], nl = e.off_start[em + 1] - o0;
+    const int p0 = e.pool_start[em], psize = e.pool_start[em + 1] - p0;
+    const int cur = e.cursor[em];
+    const Xf X = load_xf(e.e_xform + 7 * em);
+    const f3 vel = ld3(e.e_vel, em);
+    const f3 g = ld3(c.gravity, w);
+    int over = 0, n_nonfinite = 0;
+    float spmax = 0.0f;
+    for (int r = threadIdx.x; r < nl; r += blockDim.x) {
+        int slot = cur + r;
+        if (slot >= psize) {
+            over += 1;
+            continue;
+        }
+        int i = e.pool_ids[p0 + slot];
+        f3 x = xf_point(X, ld3(e.off_local, o0 + r));
+        st3(c.q, i, x);
+        st3(c.qd, i, vel);
+        c.flags[i] = c.flags[i] | ACTIVE;
+        c.phase[i] = int8_t(1);
+        float m = c.mass[i];
+        add_i64(e.mass_in + w, to_fixed(m));
+        flux_add(e, w, m, vel, x, g, 1.0f);
+        if (c.speed_later) speed_stat(vel, x, spmax, n_nonfinite);
+    }
+    over = block_sum_i32(over);
+    n_nonfinite = block_sum_i32(n_nonfinite);
+    spmax = block_max_f32(spmax);  // every thread has passed its reads of the cursor
+    if (threadIdx.x == 0) {
+        if (over) atomicAdd(e.emit_overflow + w, over);
+        if (n_nonfinite) atomicAdd(c.st_nonfinite + w, n_nonfinite);
+        if (c.speed_later && spmax > 0.0f) atomic_max_nonneg(c.st_max_speed + w, spmax);
+        e.cursor[em] = min(cur + nl, psize);
+    }
+}
+
+// ports.py:66-93 _take_kernel (retire): the masked liquid particles leave for good, their mass and energy out of the
+// ledgers as at a sink
+__global__ void __launch_bounds__(256) k_retire_mask(Ctx c, EmitCtx e, const int* mask) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n || mask[i] == 0 || !is_fluid(c, i)) return;
+    int w = c.world[i];
+    f3 x = Q(c, i), v = V(c, i);
+    float m = c.mass[i];
+    if (isfinite(x.x + x.y + x.z + v.x + v.y + v.z)) flux_add(e, w, m, v, x, ld3(c.gravity, w), -1.0f);
+    c.flags[i] = c.flags[i] & ~ACTIVE;
+    st3(c.qd, i, f3{0.0f, 0.0f, 0.0f});
+    add_i64(e.mass_out + w, to_fixed(m));
+    c.phase[i] = int8_t(-2);
+}
+
+// emitters.py:168-172 active_mass_kernel: the fixed-point mass of each world's active liquid (block sums, then one
+// atomic per block)
+__global__ void __launch_bounds__(256) k_active_mass(Ctx c, long long* out) {
+    CHUNK_PROLOGUE
+    int w = c.ch_world[b_];
+    __shared__ unsigned long long acc;
+    if (threadIdx.x == 0) acc = 0ull;
+    __syncthreads();
+    if (valid && is_fluid(c, i)) atomicAdd(&acc, static_cast<unsigned long long>(to_fixed(c.mass[i])));
+    __syncthreads();
+    if (threadIdx.x == 0 && acc != 0ull) atomicAdd(reinterpret_cast<unsigned long long*>(out) + w, acc);
 }
 
 // ----------------------------------------------------------------------------------------------- learned warm start
@@ -1884,7 +2190,6 @@ struct Handle {
     float* learned_div = nullptr;
     float* learned_den = nullptr;
     int check_every = 1;
-    int wall_first = 0;  // first cell of the wall grids
     Prof prof;
     bool use_graphs = false;  // measured: eager launches are faster than graph replay for this step (bench.py --graphs)
     const float* mlp_params = nullptr;  // learned warm start on when set
@@ -1894,6 +2199,11 @@ struct Handle {
     float* log_targets = nullptr;       // ... and its converged-kappa targets [n][2] here
     hipStream_t cap_stream = nullptr;
     std::map<GraphKey, hipGraphExec_t> graphs;
+    EmitCtx em{};                      // emitter, sink and escape-box tables and the per-world ledgers
+    int emit_active = 0;               // any emitter, sink or escape box: the step ends with the emitter pass
+    int sink_count = 0;
+    std::vector<void*> emit_owned;     // the current tables (replaced by dfsph_set_emitters)
+    int n_wall = 0;                    // particles with a wall volume (the exchange window's rows)
 };
 
 // an event pair (and, for k_accel / k_apply, the byte counters) for a timed launch of kernel kid, or false when
@@ -2026,6 +2336,11 @@ int phase_neighbors(Handle* H, hipStream_t st) {
     k_scan_apply<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part, c.cell_start, c.cell_fill);
     k_scatter<<<blocks_for(c.n), 256, 0, st>>>(c);
     k_sort_cells<<<blocks_for(c.n_cells), 256, 0, st>>>(c, 0, c.n_cells);  // ascending ids in every cell
+    if (c.NW > 0) {
+        int64_t nw_cells = int64_t(c.n_cells) - c.wall_first;
+        k_fill_u8<<<std::min(blocks_for(nw_cells), 4096), 256, 0, st>>>(c.wall_near, nw_cells);
+        k_mark_wall_cells<<<blocks_for(c.n), 256, 0, st>>>(c);
+    }
     launch_build_lists(H, st);
     LAUNCH_CHECK();
     return DFSPH_OK;
@@ -2270,6 +2585,32 @@ int phase_advect(Handle* H, double dt, hipStream_t st) {
     return DFSPH_OK;
 }
 
+// particle_walls.py:246-259 begin (work zeroed) and accumulate: the walls' share of the step's slot impulses
+int phase_walls(Handle* H, hipStream_t st) {
+    Ctx& c = H->c;
+    if (c.n == 0 || c.NW == 0) return DFSPH_OK;
+    k_fill_i32<<<blocks_for(2 * int64_t(c.W)), 256, 0, st>>>(reinterpret_cast<int*>(c.pw_work), 0, 2 * int64_t(c.W));
+    k_pw_accumulate<<<blocks_for(c.n), 256, 0, st>>>(c);
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+// runtime.py:318-347 EmitterManager.step: sinks and the escape box, then the box emitters; the speed statistics of
+// the step's end state come from these passes (c.speed_later)
+int phase_emitters(Handle* H, double dt, hipStream_t st) {
+    Ctx& c = H->c;
+    if (c.n == 0 || !H->emit_active) return DFSPH_OK;
+    k_retire<<<H->n_chunks, 256, 0, st>>>(c, H->em);
+    if (H->em.E > 0) k_emit<<<H->em.E, 256, 0, st>>>(c, H->em, float(dt));
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+void drop_graphs(Handle* H) {
+    for (auto& kv : H->graphs) (void)hipGraphExecDestroy(kv.second);
+    H->graphs.clear();
+}
+
 void bind_state(Handle* H, const float* q, const float* qd, const float* f) {
     H->c.q = const_cast<float*>(q);
     H->c.qd = const_cast<float*>(qd);
@@ -2320,6 +2661,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
     c.sparse_corr = 1;
     c.S = S;
     c.NW = L->wall_particle_count;
+    c.one_way = P->one_way != 0;
     c.cap = P->neighbor_capacity;
     c.kind = P->kernel;
     c.friction = P->wall_friction;
@@ -2404,13 +2746,26 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         }
         base[2 * W] = int(tot);
         c.n_cells = int(tot);
-        H->wall_first = base[W];
+        c.wall_first = base[W];  // first cell of the wall grids
         TRY(dupload(H, (float**)&c.g_org, org.data(), 3 * W));
         TRY(dupload(H, (int**)&c.g_dims, dims.data(), 3 * W));
         TRY(dupload(H, (int**)&c.cell_base, base.data(), 2 * W + 1));
         TRY(dalloc(H, &c.cell_count, int64_t(c.n_cells) + 1));
         TRY(dalloc(H, &c.cell_start, int64_t(c.n_cells) + 1));
         TRY(dalloc(H, &c.cell_fill, int64_t(c.n_cells) + 1));
+        if (c.NW > 0) TRY(dalloc(H, &c.wall_near, int64_t(c.n_cells) - c.wall_first));
+        {  // compact wall index (ascending ids), the exchange window and the walls' work
+            std::vector<float> wv(std::max(n, 1), 0.0f);
+            if (n > 0) HIPCHK(hipMemcpy(wv.data(), M->wall_volume, sizeof(float) * n, hipMemcpyDeviceToHost));
+            std::vector<int> cidx(std::max(n, 1), -1);
+            int nw = 0;
+            for (int i = 0; i < n; ++i)
+                if (wv[i] > 0.0f) cidx[i] = nw++;
+            H->n_wall = nw;
+            TRY(dupload(H, (int**)&c.wcidx, cidx.data(), std::max(n, 1)));
+            TRY(dalloc(H, &c.pw_exch, 3 * int64_t(std::max(nw, 1))));
+            TRY(dalloc(H, &c.pw_work, W));
+        }
         H->n_scan_blocks = blocks_for(c.n_cells, 1024);
         TRY(dalloc(H, &c.scan_part, H->n_scan_blocks));
         TRY(dalloc(H, &c.items, n));
@@ -2458,6 +2813,16 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         for (double** p : {&c.n_active, &c.err_sum, &c.rz, &c.stepb}) TRY(dalloc(H, p, W));
         for (double** p : {&c.sums, &c.flow_z, &c.flow_xy, &c.head}) TRY(dalloc(H, p, 2 * W));
         TRY(dalloc(H, &c.meter, 3 * int64_t(std::max(S, 1))));
+        {  // ledgers (runtime.py:93-107) and an empty sink table
+            EmitCtx& e = H->em;
+            for (long long** p : {&e.mass0, &e.mass_in, &e.mass_out, &e.mass_act, &e.flux_e, &e.flux_g})
+                TRY(dalloc(H, p, W));
+            TRY(dalloc(H, &e.escaped, W));
+            TRY(dalloc(H, &e.emit_overflow, W));
+            int* sink_start = nullptr;
+            TRY(dalloc(H, &sink_start, W + 1));
+            e.sink_start = sink_start;
+        }
         TRY(dalloc(H, &c.part, 3 * int64_t(std::max(H->n_chunks, 1))));
         TRY(dalloc(H, &c.n_unconv, 1));
         TRY(dalloc(H, &H->qd_saved, 3 * int64_t(n)));
@@ -2483,6 +2848,14 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
                 rc = fail(DFSPH_EDEVICE, "hipEventCreate failed");
                 goto fail;
             }
+        // runtime.py:132-134 init_mass: the build-time active mass is the ledger's initial mass
+        if (n > 0) {
+            k_active_mass<<<H->n_chunks, 256, 0, nullptr>>>(c, H->em.mass0);
+            if (hipDeviceSynchronize() != hipSuccess) {
+                rc = fail(DFSPH_EDEVICE, "initial mass failed");
+                goto fail;
+            }
+        }
     }
     *out = H;
     return DFSPH_OK;
@@ -2502,6 +2875,7 @@ int32_t dfsph_destroy(void* h) {
     for (auto& kv : H->graphs) (void)hipGraphExecDestroy(kv.second);
     if (H->cap_stream) (void)hipStreamDestroy(H->cap_stream);
     for (void* p : H->owned) (void)hipFree(p);
+    for (void* p : H->emit_owned) (void)hipFree(p);
     if (H->h_unconv) (void)hipHostFree(H->h_unconv);
     for (hipEvent_t e : H->ev_unconv)
         if (e) (void)hipEventDestroy(e);
@@ -2602,7 +2976,9 @@ int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void
     SolveSetup S1 = solve_setup(H, 1, dt, -1, SolveOverride{});
     rc = run_solve(H, S1, st, SEG_TAIL_DEN, [&](hipStream_t s) {
         if (H->log_targets) k_targets<<<blocks_for(H->c.n), 256, 0, s>>>(H->c, dtf, H->log_targets);
-        return phase_advect(H, dt, s);
+        int r = phase_walls(H, s);
+        if (r == 0) r = phase_advect(H, dt, s);
+        return r ? r : phase_emitters(H, dt, s);
     });
     return rc;
 }
@@ -2856,6 +3232,27 @@ int32_t dfsph_get_buffer(void* h, int32_t which, void** ptr, int64_t* nb) {
         case DFSPH_BUF_DIV_ERR: B(c.st_div_err, 4 * W)
         case DFSPH_BUF_SHAPE_METER: B(c.meter, 24 * int64_t(std::max(c.S, 1)))
         case DFSPH_BUF_FEATURES: B(c.feat, 64 * n)
+        case DFSPH_BUF_MASS_INITIAL: B(H->em.mass0, 8 * W)
+        case DFSPH_BUF_MASS_IN: B(H->em.mass_in, 8 * W)
+        case DFSPH_BUF_MASS_OUT: B(H->em.mass_out, 8 * W)
+        case DFSPH_BUF_MASS_ACTIVE: B(H->em.mass_act, 8 * W)
+        case DFSPH_BUF_ESCAPED: B(H->em.escaped, 4 * W)
+        case DFSPH_BUF_EMIT_OVERFLOW: B(H->em.emit_overflow, 4 * W)
+        case DFSPH_BUF_FLUX_E: B(H->em.flux_e, 8 * W)
+        case DFSPH_BUF_FLUX_G: B(H->em.flux_g, 8 * W)
+        case DFSPH_BUF_EMIT_T:
+            if (H->em.E == 0) return fail(DFSPH_EINVAL, "no emitters");
+            B(H->em.t_acc, 4 * int64_t(H->em.E))
+        case DFSPH_BUF_EMIT_CURSOR:
+            if (H->em.E == 0) return fail(DFSPH_EINVAL, "no emitters");
+            B(H->em.cursor, 4 * int64_t(H->em.E))
+        case DFSPH_BUF_EMIT_ENABLED:
+            if (H->em.E == 0) return fail(DFSPH_EINVAL, "no emitters");
+            B(H->em.enabled, 4 * int64_t(H->em.E))
+        case DFSPH_BUF_FLAGS: B(c.flags, 4 * int64_t(c.n))
+        case DFSPH_BUF_PHASE: B(c.phase, int64_t(c.n))
+        case DFSPH_BUF_PW_EXCH: B(c.pw_exch, 24 * int64_t(std::max(H->n_wall, 1)))
+        case DFSPH_BUF_PW_WORK: B(c.pw_work, 8 * W)
         default: return fail(DFSPH_EINVAL, "unknown buffer id");
     }
 #undef B
@@ -2914,6 +3311,161 @@ int32_t dfsph_report(void* h, DfsphReport* out, void* stream) {
     for (auto& it : items)
         if (it.dst) HIPCHK(hipMemcpyAsync(it.dst, it.src, it.bytes, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return DFSPH_OK;
+}
+
+// emitters.py:32-255 EmitterTable / EmitterDevice and runtime.py:197-304 EmitterManager (add_emitter, add_sink,
+// _upload, carry_from): the whole table, validated and copied to the device; the clock, cursor and enabled flag of
+// the first min(old, new) emitters carry over.
+int32_t dfsph_set_emitters(void* h, const DfsphEmitters* t) {
+    if (!h || !t) return fail(DFSPH_EINVAL, "null argument");
+    Handle* H = as_handle(h);
+    Ctx& c = H->c;
+    const int W = c.W, E = t->emitter_count, K = t->sink_count;
+    if (E < 0 || K < 0) return fail(DFSPH_EINVAL, "dfsph_set_emitters: negative count");
+    std::vector<int> ws(W + 1);
+    HIPCHK(hipMemcpy(ws.data(), c.ws, sizeof(int) * (W + 1), hipMemcpyDeviceToHost));
+    if (E > 0) {
+        if (!t->world || !t->xform || !t->velocity || !t->interval || !t->offset_start || !t->pool_start)
+            return fail(DFSPH_EINVAL, "dfsph_set_emitters: null emitter array");
+        if (t->offset_start[0] != 0 || t->pool_start[0] != 0)
+            return fail(DFSPH_EINVAL, "dfsph_set_emitters: offset_start[0] and pool_start[0] must be 0");
+        for (int e = 0; e < E; ++e) {
+            int w = t->world[e];
+            if (w < 0 || w >= W) return fail(DFSPH_EINVAL, "add_emitter: world " + std::to_string(w) + " out of range");
+            if (!(t->interval[e] > 0.0f)) return fail(DFSPH_EINVAL, "add_emitter: interval must be > 0");
+            if (t->offset_start[e + 1] < t->offset_start[e] || t->pool_start[e + 1] < t->pool_start[e])
+                return fail(DFSPH_EINVAL, "dfsph_set_emitters: offset_start and pool_start must be ascending");
+            for (int k = t->pool_start[e]; k < t->pool_start[e + 1]; ++k) {
+                int i = t->pool_ids[k];
+                if (i < ws[w] || i >= ws[w + 1])
+                    return fail(DFSPH_EINVAL, "dfsph_set_emitters: pool id " + std::to_string(i) +
+                                                  " is not a particle of the emitter's world");
+            }
+        }
+    }
+    for (int k = 0; k < K; ++k)
+        if (t->sink_world[k] < 0 || t->sink_world[k] >= W)
+            return fail(DFSPH_EINVAL, "add_sink: world " + std::to_string(t->sink_world[k]) + " out of range");
+    HIPCHK(hipDeviceSynchronize());
+    EmitCtx& em = H->em;
+    const int E_old = em.E;
+    std::vector<void*> owned;
+    auto up = [&](auto** dst, const auto* src, int64_t count) -> int {
+        using Tv = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+        void* p = nullptr;
+        size_t bytes = size_t(std::max<int64_t>(count, 1)) * sizeof(Tv);
+        if (hipMalloc(&p, bytes) != hipSuccess) return fail(DFSPH_EDEVICE, "hipMalloc failed");
+        owned.push_back(p);
+        if (hipMemset(p, 0, bytes) != hipSuccess) return fail(DFSPH_EDEVICE, "hipMemset failed");
+        if (src && count > 0 && hipMemcpy(p, src, size_t(count) * sizeof(Tv), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(DFSPH_EDEVICE, "hipMemcpy failed");
+        *dst = static_cast<std::remove_pointer_t<decltype(dst)>>(p);
+        return DFSPH_OK;
+    };
+    EmitCtx ne = em;  // the ledgers stay
+    int rc = DFSPH_OK;
+    // sinks grouped by world (the retire test is "in any sink of the world", so their order does not matter)
+    std::vector<int> sstart(W + 1, 0), sorder(K);
+    for (int k = 0; k < K; ++k) sstart[t->sink_world[k] + 1] += 1;
+    for (int w = 0; w < W; ++w) sstart[w + 1] += sstart[w];
+    {
+        std::vector<int> fillp(sstart.begin(), sstart.end() - 1);
+        for (int k = 0; k < K; ++k) sorder[fillp[t->sink_world[k]]++] = k;
+    }
+    std::vector<float> sx(7 * size_t(K)), sh(3 * size_t(K));
+    for (int k = 0; k < K; ++k) {
+        std::copy(t->sink_xform + 7 * sorder[k], t->sink_xform + 7 * sorder[k] + 7, sx.begin() + 7 * k);
+        std::copy(t->sink_half + 3 * sorder[k], t->sink_half + 3 * sorder[k] + 3, sh.begin() + 3 * k);
+    }
+    const int n_off = E > 0 ? t->offset_start[E] : 0, n_pool = E > 0 ? t->pool_start[E] : 0;
+    int *d_world = nullptr, *d_off_start = nullptr, *d_pool_start = nullptr, *d_pool = nullptr, *d_sstart = nullptr;
+    float *d_xf = nullptr, *d_vel = nullptr, *d_int = nullptr, *d_off = nullptr, *d_sx = nullptr, *d_sh = nullptr;
+    if ((rc = up(&d_world, t->world, E)) || (rc = up(&d_xf, t->xform, 7 * int64_t(E))) ||
+        (rc = up(&d_vel, t->velocity, 3 * int64_t(E))) || (rc = up(&d_int, t->interval, E)) ||
+        (rc = up(&d_off_start, t->offset_start, E > 0 ? E + 1 : 0)) || (rc = up(&d_off, t->offsets, 3 * int64_t(n_off))) ||
+        (rc = up(&d_pool_start, t->pool_start, E > 0 ? E + 1 : 0)) || (rc = up(&d_pool, t->pool_ids, n_pool)) ||
+        (rc = up(&ne.t_acc, (const float*)nullptr, E)) || (rc = up(&ne.cursor, (const int*)nullptr, E)) ||
+        (rc = up(&ne.enabled, (const int*)nullptr, E)) || (rc = up(&d_sstart, sstart.data(), W + 1)) ||
+        (rc = up(&d_sx, sx.data(), 7 * int64_t(K))) || (rc = up(&d_sh, sh.data(), 3 * int64_t(K)))) {
+        for (void* p : owned) (void)hipFree(p);
+        return rc;
+    }
+    std::vector<int> on(std::max(E, 1), 1);
+    HIPCHK(hipMemcpy(ne.enabled, on.data(), sizeof(int) * std::max(E, 1), hipMemcpyHostToDevice));
+    const int carry = std::min(E_old, E);  // EmitterDevice.carry_from
+    if (carry > 0) {
+        HIPCHK(hipMemcpy(ne.t_acc, em.t_acc, sizeof(float) * carry, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(ne.cursor, em.cursor, sizeof(int) * carry, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(ne.enabled, em.enabled, sizeof(int) * carry, hipMemcpyDeviceToDevice));
+    }
+    ne.E = E;
+    ne.has_bounds = t->has_bounds != 0;
+    ne.lo = f3{t->bounds_lo[0], t->bounds_lo[1], t->bounds_lo[2]};
+    ne.hi = f3{t->bounds_hi[0], t->bounds_hi[1], t->bounds_hi[2]};
+    ne.e_world = d_world;
+    ne.e_xform = d_xf;
+    ne.e_vel = d_vel;
+    ne.e_interval = d_int;
+    ne.off_start = d_off_start;
+    ne.off_local = d_off;
+    ne.pool_start = d_pool_start;
+    ne.pool_ids = d_pool;
+    ne.sink_start = d_sstart;
+    ne.s_xform = d_sx;
+    ne.s_half = d_sh;
+    for (void* p : H->emit_owned) (void)hipFree(p);
+    H->emit_owned = owned;
+    H->em = ne;
+    H->sink_count = K;
+    H->emit_active = E > 0 || K > 0 || ne.has_bounds;
+    c.speed_later = H->emit_active;
+    drop_graphs(H);  // captured launches hold the previous tables
+    return DFSPH_OK;
+}
+
+// SolverDFSPH.set_emitter_enabled (solver.py:438-447): a closed emitter emits nothing and its clock stops
+int32_t dfsph_set_emitter_enabled(void* h, int32_t e, int32_t on) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    Handle* H = as_handle(h);
+    if (e < 0 || e >= H->em.E)
+        return fail(DFSPH_EINVAL, "set_emitter_enabled: emitter " + std::to_string(e) + " does not exist (" +
+                                      std::to_string(H->em.E) + " emitters)");
+    int v = on != 0;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(H->em.enabled + e, &v, sizeof(int), hipMemcpyHostToDevice));
+    return DFSPH_OK;
+}
+
+// the step's last phase on its own: sinks, escape box, emitters and the speed statistics
+int32_t dfsph_emitters_step(void* h, float* q, float* qd, double dt, void* stream) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    if (!(dt > 0.0)) return fail(DFSPH_EINVAL, "dt must be > 0");
+    Handle* H = as_handle(h);
+    bind_state(H, q, qd, nullptr);
+    return phase_emitters(H, dt, static_cast<hipStream_t>(stream));
+}
+
+// SolverDFSPH.retire (solver.py:428-431, ports.py:187-189): mask [n] int32 on the device, non-zero = retire
+int32_t dfsph_retire(void* h, const int32_t* mask, float* q, float* qd, void* stream) {
+    if (!h || !mask) return fail(DFSPH_EINVAL, "null argument");
+    Handle* H = as_handle(h);
+    bind_state(H, q, qd, nullptr);
+    if (H->c.n == 0) return DFSPH_OK;
+    k_retire_mask<<<blocks_for(H->c.n), 256, 0, static_cast<hipStream_t>(stream)>>>(H->c, H->em, mask);
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
+// runtime.py:126-130 _active_mass into DFSPH_BUF_MASS_ACTIVE (the flags and phase of the model as they are now)
+int32_t dfsph_active_mass(void* h, void* stream) {
+    if (!h) return fail(DFSPH_EINVAL, "null handle");
+    Handle* H = as_handle(h);
+    Ctx& c = H->c;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    k_fill_i32<<<blocks_for(2 * int64_t(c.W)), 256, 0, st>>>(reinterpret_cast<int*>(H->em.mass_act), 0, 2 * int64_t(c.W));
+    if (c.n > 0) k_active_mass<<<H->n_chunks, 256, 0, st>>>(c, H->em.mass_act);
+    LAUNCH_CHECK();
     return DFSPH_OK;
 }
 

@@ -20,7 +20,7 @@ from . import kernels as kn
 from .config import DFSPHConfig, full_neighbor_count
 from .model import GeoType, Model, ParticleFlags
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 DFSPH_OK, DFSPH_EINVAL, DFSPH_ECAPACITY, DFSPH_EDEVICE = 0, -1, -2, -3
 
@@ -28,7 +28,14 @@ DFSPH_OK, DFSPH_EINVAL, DFSPH_ECAPACITY, DFSPH_EDEVICE = 0, -1, -2, -3
  BUF_SLOT_PHI, BUF_SLOT_XBL, BUF_SLOT_NBL, BUF_SLOT_IMPULSE, BUF_RATIO, BUF_ALPHA, BUF_NEFF, BUF_ACTIVE, BUF_REG,
  BUF_SRC, BUF_KAPPA, BUF_ACC, BUF_ERR, BUF_LOWER, BUF_KV, BUF_KD, BUF_VTMP, BUF_CONVERGED, BUF_ITERS, BUF_ERR_MEAN,
  BUF_ERR_MAX, BUF_N_ACTIVE, BUF_TOL_SCALE, BUF_OVERFLOW, BUF_MAX_SPEED, BUF_ITERS_DEN, BUF_ITERS_DIV, BUF_DEN_ERR,
- BUF_DIV_ERR, BUF_SHAPE_METER, BUF_FEATURES) = range(40)
+ BUF_DIV_ERR, BUF_SHAPE_METER, BUF_FEATURES, BUF_MASS_INITIAL, BUF_MASS_IN, BUF_MASS_OUT, BUF_MASS_ACTIVE, BUF_ESCAPED,
+ BUF_EMIT_OVERFLOW, BUF_FLUX_E, BUF_FLUX_G, BUF_EMIT_T, BUF_EMIT_CURSOR, BUF_EMIT_ENABLED, BUF_FLAGS,
+ BUF_PHASE, BUF_PW_EXCH, BUF_PW_WORK) = range(55)
+IMP_SCALE = float(2**50)    # impulse fixed point, bodies.py:25
+
+FIX_SCALE = float(2**40)    # mass fixed point, kernels.py:32-33
+WORK_SCALE = float(2**50)   # energy fixed point, bodies.py:24
+FORCE_SCALE = float(2**40)  # bodies.py:23
 
 # kernel ids of dfsph_profile_read
 # tuning options (dfsph_set_option); none changes a result bit
@@ -68,7 +75,7 @@ class DfsphParams(ctypes.Structure):
         ("tot_min", c_double), ("lattice_grad", c_double), ("compression_relief_speed", c_double),
         ("contact_separation_speed", c_double), ("wall_offset", c_double), ("wall_amplitude", c_double),
         ("taper0", c_double), ("taper1", c_double), ("reach", c_double), ("curvature", c_int),
-        ("project_eps", c_double), ("sweep_len", c_double),
+        ("project_eps", c_double), ("sweep_len", c_double), ("one_way", c_int),
     ]
 
 
@@ -93,6 +100,14 @@ class DfsphReport(ctypes.Structure):
                 ("iters_divergence", P(c_int)), ("neighbor_overflow", P(c_int)), ("max_speed", P(c_float)),
                 ("density_err_max", P(c_float)), ("density_tol", P(c_float)), ("projected", P(c_int)),
                 ("swept", P(c_int)), ("nonfinite", P(c_int)), ("n_active", P(c_double))]
+
+
+class DfsphEmitters(ctypes.Structure):
+    _fields_ = [("emitter_count", c_int), ("world", P(c_int)), ("xform", P(c_float)), ("velocity", P(c_float)),
+                ("interval", P(c_float)), ("offset_start", P(c_int)), ("offsets", P(c_float)),
+                ("pool_start", P(c_int)), ("pool_ids", P(c_int)), ("sink_count", c_int), ("sink_world", P(c_int)),
+                ("sink_xform", P(c_float)), ("sink_half", P(c_float)), ("has_bounds", c_int),
+                ("bounds_lo", c_float * 3), ("bounds_hi", c_float * 3)]
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -147,6 +162,7 @@ def params_from_config(cfg: DFSPHConfig, thinnest: float = math.inf) -> DfsphPar
     p.curvature = int(cfg.boundary_curvature_order)
     p.project_eps = PROJECT_EPS * cfg.particle_radius
     p.sweep_len = sweep_length(h, thinnest)
+    p.one_way = int(cfg.coupling == "one_way")
     return p
 
 
@@ -169,13 +185,14 @@ class LayoutTables:
     wall_volume: np.ndarray
     wall_count: int
     thinnest: float
+    shape_count: int = 0  # the model's shapes (the arrays above hold one placeholder entry when there are none)
 
     def struct(self, n: int) -> DfsphLayout:
         L = DfsphLayout()
         L.n, L.world_count = n, len(self.world_start) - 1
         L.world_start = _ptr(self.world_start, c_int)
         L.grid_origin, L.grid_dims = _ptr(self.grid_origin, c_float), _ptr(self.grid_dims, c_int)
-        L.shape_count = len(self.shape_geo)
+        L.shape_count = int(self.shape_count)
         L.shape_start, L.shape_ids = _ptr(self.shape_start, c_int), _ptr(self.shape_ids, c_int)
         L.shape_geo, L.shape_scale = _ptr(self.shape_geo, c_int), _ptr(self.shape_scale, c_float)
         L.shape_xform, L.shape_body = _ptr(self.shape_xform, c_float), _ptr(self.shape_body, c_int)
@@ -202,9 +219,10 @@ def world_shape_table(shape_world: np.ndarray, shape_flags: np.ndarray, shape_ty
 
 def grid_boxes(q: np.ndarray, world_start: np.ndarray, model: Model, ids_per_world: list[np.ndarray], h: float,
                bounds=None) -> tuple[np.ndarray, np.ndarray]:
-    """Per-world cell-grid origin and dims covering the world's particles, its boxes/spheres and the escape box,
-    padded by 2 h. The grid only accelerates the search: positions outside it are clamped to its edge cells, which
-    keeps every pair within h in adjacent cells."""
+    """Per-world cell-grid origin and dims covering the world's particles (spares too) and its boxes/spheres, padded
+    by 2 h, and ``bounds`` when given. The grid only accelerates the search: positions outside it are clamped to its
+    edge cells, which keeps every pair within h in adjacent cells; the escape box is left out by the caller (an
+    automatic one is padded by at least 0.25 m and would multiply the cells)."""
     W = len(world_start) - 1
     cell = h * (1.0 + GRID_CELL_SLACK)
     org = np.zeros((W, 3), np.float32)
@@ -250,7 +268,7 @@ def layout_from_model(model: Model, cfg: DFSPHConfig) -> LayoutTables:
     vol[wall] = (8.0 * radius[wall] ** 3).astype(np.float32)
     q = model.particle_q.cpu().numpy().astype(np.float64)
     per_world = [ids[start[w]:start[w + 1]] for w in range(W)]
-    org, dims = grid_boxes(q, ws, model, per_world, cfg.h, cfg.world_bounds)
+    org, dims = grid_boxes(q, ws, model, per_world, cfg.h)
     S = model.shape_count
     f32 = lambda a, shape: np.ascontiguousarray(np.asarray(a, np.float32).reshape(shape))  # noqa: E731
     i32 = lambda a: np.ascontiguousarray(np.asarray(a, np.int32).reshape(-1))  # noqa: E731
@@ -263,7 +281,7 @@ def layout_from_model(model: Model, cfg: DFSPHConfig) -> LayoutTables:
         shape_radius=f32(model.shape_collision_radius if S else np.zeros(1), -1),
         shape_mu=f32(model.shape_material_mu if S else np.zeros(1), -1),
         wall_volume=vol, wall_count=int(np.count_nonzero(wall)),
-        thinnest=thinnest_wall(model, ids) if S else math.inf,
+        thinnest=thinnest_wall(model, ids) if S else math.inf, shape_count=S,
     )
 
 
@@ -320,6 +338,11 @@ def load_library() -> ctypes.CDLL:
         "dfsph_write_buffer": ([vp, i32, vp, i64], i32),
         "dfsph_zero_stats": ([vp, vp], i32),
         "dfsph_report": ([vp, P(DfsphReport), vp], i32),
+        "dfsph_set_emitters": ([vp, P(DfsphEmitters)], i32),
+        "dfsph_set_emitter_enabled": ([vp, i32, i32], i32),
+        "dfsph_emitters_step": ([vp, vp, vp, dbl, vp], i32),
+        "dfsph_active_mass": ([vp, vp], i32),
+        "dfsph_retire": ([vp, vp, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -2,10 +2,11 @@
 
 Every tuning option (DFSPH_OPT_*: the host convergence-check interval, i.e. the iteration schedule of the pressure
 solves, the neighbour-list block size, the pair-gradient cache, the sparse correction pass, hipGraph replay) gives
-bit-identical results, so the tuner can time them on the live simulation: it snapshots the state, the warm starts and
-the shape meters, times short step windows for each candidate (one option at a time, each from the snapshot,
-median of ``rounds``), keeps the fastest value, and restores the snapshot. The caller's simulation continues exactly
-as if no tuning had happened. The reference picks launch parameters inside Warp; this has no counterpart there.
+bit-identical results, so the tuner can time them on the live simulation: it snapshots the state, the warm starts,
+the ledgers, the emitters and the shape meters, times short step windows for each candidate (one option at a time,
+each from the snapshot, median of ``rounds``), keeps the fastest value, and restores the snapshot. The caller's
+simulation continues exactly as if no tuning had happened. The reference picks launch parameters inside Warp; this
+has no counterpart there.
 """
 
 from __future__ import annotations
@@ -32,23 +33,24 @@ NAMES = {_abi.OPT_CHECK_INTERVAL: "check_interval", _abi.OPT_BUILD_BLOCK: "build
 
 
 class _Snapshot:
+    """Everything a step changes: the state, the solver's state arrays (warm starts, flags, phase, emitted and retired
+    mass, the emitters' clocks and cursors), the shape meters and the rest of the per-world ledger."""
+
     def __init__(self, solver, state):
         self.q = state.particle_q.clone()
         self.qd = state.particle_qd.clone()
         self.arrays = solver.alloy_state_arrays(None)
-        S = solver.model.shape_count
-        self.meters = solver.buffer(_abi.BUF_SHAPE_METER, np.int64, (S, 3)) if S else None
+        S, W = solver.model.shape_count, solver.model.world_count
+        self.buffers = {_abi.BUF_SHAPE_METER: solver.buffer(_abi.BUF_SHAPE_METER, np.int64, (S, 3))} if S else {}
+        for which, dt in ((_abi.BUF_EMIT_OVERFLOW, np.int32), (_abi.BUF_FLUX_E, np.int64), (_abi.BUF_FLUX_G, np.int64)):
+            self.buffers[which] = solver.buffer(which, dt, (W,))
 
     def restore(self, solver, state):
         state.particle_q.copy_(self.q)
         state.particle_qd.copy_(self.qd)
         solver.alloy_restore_state_arrays(self.arrays)
-        if self.meters is not None:
-            import ctypes
-
-            m = np.ascontiguousarray(self.meters)
-            solver._call(solver.lib.dfsph_write_buffer, _abi.BUF_SHAPE_METER, m.ctypes.data_as(ctypes.c_void_p),
-                         m.nbytes)
+        for which, a in self.buffers.items():
+            solver._write(which, a)
 
 
 def _window_ms(solver, state, dt: float, warm: int, steps: int) -> float:

@@ -8,6 +8,7 @@ density error; the state adapter exposes ``alloy_state_arrays`` with its item ->
 from __future__ import annotations
 
 import dataclasses
+import logging
 from collections.abc import Mapping
 from typing import Any
 
@@ -17,12 +18,17 @@ from ..core.registry import (HealthSample, SolverCapability, register_health_ada
                              register_state_adapter)
 from .config import DFSPHConfig, register_custom_attributes
 
+log = logging.getLogger("alloy.sph")
+
 _CONFIG_FIELDS = frozenset(f.name for f in dataclasses.fields(DFSPHConfig))
 
 
 def _factory(view: Any, kwargs: Mapping[str, Any]):
     """Entry factory (plugin.py:36-65): ``config`` or Config fields, plus capacity_per_world; unknown kwargs raise
-    TypeError. The reference's automatic escape box and cloth absorption (plugin.py:48-64) are outside this path."""
+    TypeError. Without ``config`` and without a ``world_bounds`` kwarg the escape box comes from the scene's content
+    (``escape_box.auto_box``); an explicit ``world_bounds`` (None too) or a full ``config`` is kept as given. Cloth
+    absorption (plugin.py:46, 60-62) belongs to the cloth coupling, outside this path."""
+    from . import escape_box
     from .solver import SolverDFSPH
 
     kw = dict(kwargs)
@@ -31,11 +37,20 @@ def _factory(view: Any, kwargs: Mapping[str, Any]):
     fields = {k: kw.pop(k) for k in list(kw) if k in _CONFIG_FIELDS}
     if kw:
         raise TypeError(f"sph factory: unknown kwargs {sorted(kw)}")
+    auto = None
     if cfg is None:
+        if "world_bounds" not in fields:
+            auto = escape_box.auto_box(view)
+            fields["world_bounds"] = None if auto is None else auto[0]
+            log.info("sph factory: escape box %s from the scene's content (retired particles: report().escaped)",
+                     fields["world_bounds"])
         cfg = DFSPHConfig(**fields)
     elif fields:
         cfg = dataclasses.replace(cfg, **fields)
-    return SolverDFSPH(view, cfg, capacity_per_world=cap)
+    solver = SolverDFSPH(view, cfg, capacity_per_world=cap)
+    if auto is not None:
+        solver.escape_box_pad = auto[1]
+    return solver
 
 
 class _SPHEmitter:

@@ -2,12 +2,13 @@
 
 Same construction, methods, errors and step order as alloy/solvers/sph/solver.py:57-484: ``step`` runs neighbours
 -> boundary slots -> density and factor -> divergence solve -> gravity, external forces and XSPH -> density solve ->
-shape meters -> advection and wall projection -> speed statistics, all on the current torch stream. Every work
-buffer is allocated once at construction; ``step`` allocates nothing.
+shape meters -> advection and wall projection -> sinks, escape box and box emitters -> speed statistics, all on the
+current torch stream. Every work buffer is allocated once at construction; ``step`` allocates nothing.
 
-Subsystems of the reference that this build does not implement (implicit viscosity, surface tension, ambient
-pressure, soft walls, Verlet skin, bodies attached to shapes, emitters/sinks) are refused at construction with a
-ValueError naming the config field, rather than silently ignored.
+The mass ledger is int64 fixed point on the device (2^40 per kg, runtime.py:93-153), so active + out == initial + in
+holds exactly. Subsystems of the reference that this build does not implement (implicit viscosity, surface tension,
+ambient pressure, soft walls, Verlet skin, bodies attached to shapes and the body-attached patch emitters) are
+refused with an error naming them, rather than silently ignored.
 """
 
 from __future__ import annotations
@@ -21,6 +22,7 @@ import numpy as np
 import torch
 
 from . import _abi
+from .emitters import EmitterManager
 from .config import (PHASE_LIQUID, DFSPHConfig, register_custom_attributes, resolve_viscosity, snap_down,
                      stable_dt)
 from .model import Model, ModelFlags, ParticleFlags, State
@@ -61,6 +63,43 @@ def hydrostatic_guess(model: Model, state: State, spacing: float) -> np.ndarray:
         np.maximum.at(top, inv, z)
         out[sel] = (gn * np.maximum(top[inv] - z + 0.5 * spacing, 0.0)).astype(np.float32)
     return out
+
+
+class ParticleWallReactions:
+    """The liquid's reaction on the particle walls (particle_walls.py:236-282 world_work / particle_impulse /
+    new_window / harvest): each particle-wall slot's impulse is split over the wall particles in int64 at 2^50 every
+    step, summed over an exchange window; ``count`` wall particles in ascending id order."""
+
+    def __init__(self, solver):
+        self._s = solver
+        self.vid = np.nonzero(solver.tables.wall_volume > 0.0)[0]
+        self.count = int(len(self.vid))
+
+    def world_work(self) -> np.ndarray:
+        """The last step's work [J] of the walls on the liquid, per world."""
+        if not self.count:
+            return np.zeros(0)
+        W = self._s.model.world_count
+        return self._s.buffer(_abi.BUF_PW_WORK, np.int64, (W,)).astype(np.float64) / _abi.WORK_SCALE
+
+    def particle_impulse(self) -> np.ndarray:
+        """The exchange window's impulse [N s] per wall particle, (count, 3)."""
+        if not self.count:
+            return np.zeros((0, 3))
+        return self._s.buffer(_abi.BUF_PW_EXCH, np.int64, (self.count, 3)).astype(np.float64) / _abi.IMP_SCALE
+
+    def new_window(self) -> None:
+        if self.count:
+            self._s._write(_abi.BUF_PW_EXCH, np.zeros((self.count, 3), np.int64))
+
+    def harvest(self, local_to_global, out: torch.Tensor, dt: float) -> None:
+        """out[local_to_global[vid]] = the window's impulse / dt (a force [N]) for the wall particles mapped >= 0."""
+        if not self.count:
+            return
+        l2g = np.asarray(local_to_global)[self.vid]
+        keep = l2g >= 0
+        f = torch.as_tensor(self.particle_impulse()[keep] / float(dt), dtype=out.dtype, device=out.device)
+        out[torch.as_tensor(l2g[keep], device=out.device)] = f
 
 
 def _stream_ptr(device: torch.device):
@@ -114,14 +153,25 @@ class SolverDFSPH:
         self._flags0 = model.particle_flags.clone()
         self._phase0 = model.sph.phase.clone()
         W = model.world_count
-        self._mass_in = np.zeros(W)
-        self._mass_out = np.zeros(W)
-        self._escaped = np.zeros(W, np.int32)
-        self._ws_dev = torch.as_tensor(model.world_start_np, dtype=torch.int64, device=self.device)
-        self._mass_initial = self._active_mass()  # runtime.py:132-134 init_mass
+        self.emitters = EmitterManager(model, cfg, self._upload_emitters, self._emit_cursors)
+        self.pw = ParticleWallReactions(self)
+        if self.emitters.active:  # the escape box
+            self.emitters.upload()
         log.info("SolverDFSPH: %d particles in %d worlds, r=%g m, h=%g m, %d shapes, %d wall particles, solver=%s",
                  model.particle_count, W, cfg.particle_radius, cfg.h, model.shape_count, self.tables.wall_count,
                  self.pressure_solver)
+
+    def _upload_emitters(self, st) -> None:
+        self._call(self.lib.dfsph_set_emitters, ctypes.byref(st.struct), what="SolverDFSPH emitters")
+
+    def _emit_cursors(self) -> np.ndarray:
+        E = len(self.emitters.table.world) if hasattr(self, "emitters") else 0
+        if E == 0 or self.emitters.struct is None or self.emitters.struct.struct.emitter_count == 0:
+            return np.zeros(E, np.int32)
+        n_dev = int(self.emitters.struct.struct.emitter_count)
+        out = np.zeros(E, np.int32)
+        out[:n_dev] = self.buffer(_abi.BUF_EMIT_CURSOR, np.int32, (n_dev,))
+        return out
 
     # ------------------------------------------------------------------------------------------------- checks
     def _create_native(self) -> None:
@@ -264,11 +314,13 @@ class SolverDFSPH:
         """Per-world report of the last step (runtime.py:136-147); the step's sync point."""
         r = self._raw_report()
         mi = self.config.max_iterations
-        mass = self._active_mass()
+        led = self.mass_ledger()
+        kg = lambda a: a.astype(np.float64) / _abi.FIX_SCALE  # noqa: E731
+        W = self.model.world_count
         return SPHReport(
             density_err=r["density_err"].astype(np.float64), divergence_err=r["divergence_err"].astype(np.float64),
-            iters_density=r["iters_density"], iters_divergence=r["iters_divergence"], mass_active=mass,
-            mass_in=self._mass_in.copy(), mass_out=self._mass_out.copy(), escaped=self._escaped.copy(),
+            iters_density=r["iters_density"], iters_divergence=r["iters_divergence"], mass_active=kg(led["active"]),
+            mass_in=kg(led["in"]), mass_out=kg(led["out"]), escaped=self.buffer(_abi.BUF_ESCAPED, np.int32, (W,)),
             neighbor_overflow=r["neighbor_overflow"], max_speed=r["max_speed"].astype(np.float64),
             unconverged=(r["iters_density"] >= mi) | (r["iters_divergence"] >= mi),
             density_tol=r["density_tol"].astype(np.float64))
@@ -276,24 +328,20 @@ class SolverDFSPH:
     def diagnostics(self) -> dict[str, np.ndarray]:
         """Extra per-world counters (runtime.py:155-163)."""
         r = self._raw_report()
+        W = self.model.world_count
         return {"density_err_max": r["density_err_max"].astype(np.float64), "projected": r["projected"],
-                "nonfinite": r["nonfinite"], "swept": r["swept"], "n_active": r["n_active"]}
-
-    def _active_mass(self) -> np.ndarray:
-        """Active liquid mass per world [kg]: a float64 prefix sum over the contiguous world ranges (no atomics)."""
-        m = self.model
-        if m.particle_count == 0:
-            return np.zeros(m.world_count)
-        act = ((m.particle_flags & int(ParticleFlags.ACTIVE)) != 0) & (m.sph.phase >= PHASE_LIQUID)
-        w = torch.where(act, m.particle_mass.double(), torch.zeros((), dtype=torch.float64, device=self.device))
-        cum = torch.cat([torch.zeros(1, dtype=torch.float64, device=self.device), torch.cumsum(w, 0)])
-        at = cum.index_select(0, self._ws_dev).cpu().numpy()  # W + 1 values cross to the host, not n
-        return at[1:] - at[:-1]
+                "nonfinite": r["nonfinite"], "swept": r["swept"], "n_active": r["n_active"],
+                "emitter_overflow": self.buffer(_abi.BUF_EMIT_OVERFLOW, np.int32, (W,))}
 
     def mass_ledger(self) -> dict[str, np.ndarray]:
-        """Per-world mass ledger [kg] (runtime.py:129-134): active + out == initial + in."""
-        return {"initial": self._mass_initial.copy(), "in": self._mass_in.copy(), "out": self._mass_out.copy(),
-                "active": self._active_mass()}
+        """Per-world mass ledger, int64 at ``scale`` per kg (runtime.py:136-140): active + out == initial + in,
+        exactly."""
+        W = self.model.world_count
+        self._call(self.lib.dfsph_active_mass, _stream_ptr(self.device))
+        return {"initial": self.buffer(_abi.BUF_MASS_INITIAL, np.int64, (W,)),
+                "in": self.buffer(_abi.BUF_MASS_IN, np.int64, (W,)),
+                "out": self.buffer(_abi.BUF_MASS_OUT, np.int64, (W,)),
+                "active": self.buffer(_abi.BUF_MASS_ACTIVE, np.int64, (W,)), "scale": np.float64(_abi.FIX_SCALE)}
 
     def shape_impulse(self) -> np.ndarray:
         """Cumulative fluid impulse on each shape since the last reset [N s], (S, 3) float64 (solver.py:399-401)."""
@@ -342,11 +390,24 @@ class SolverDFSPH:
         for which in (_abi.BUF_KV, _abi.BUF_KD):
             k = self.buffer(which, np.float32, (max(n, 1),))
             k[:n][sel.cpu().numpy()] = 0.0
-            self._call(self.lib.dfsph_write_buffer, which, k.ctypes.data_as(ctypes.c_void_p), k.nbytes)
-        self._mass_in[mask] = 0.0
-        self._mass_out[mask] = 0.0
-        self._escaped[mask] = 0
-        self._mass_initial[mask] = self._active_mass()[mask]
+            self._write(which, k)
+        # the ledger restarts with the restored particles; the initial mass stays (runtime.py:190-194)
+        for which, dt in ((_abi.BUF_ESCAPED, np.int32), (_abi.BUF_EMIT_OVERFLOW, np.int32), (_abi.BUF_MASS_IN, np.int64),
+                          (_abi.BUF_MASS_OUT, np.int64), (_abi.BUF_FLUX_E, np.int64), (_abi.BUF_FLUX_G, np.int64)):
+            a = self.buffer(which, dt, (W,))
+            a[mask] = 0
+            self._write(which, a)
+        picked = self.emitters.rewound_emitters(mask)  # emitters.rewind: clock and cursor back to 0
+        E = len(self.emitters.table.world)
+        if E and len(picked):
+            for which, dt in ((_abi.BUF_EMIT_T, np.float32), (_abi.BUF_EMIT_CURSOR, np.int32)):
+                a = self.buffer(which, dt, (E,))
+                a[picked] = 0
+                self._write(which, a)
+
+    def _write(self, which: int, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        self._call(self.lib.dfsph_write_buffer, which, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
 
     def choose_dt(self, control_dt: float) -> float:
         """CFL step from the last step's max speed, snapped onto the D4 grid (solver.py:424-426)."""
@@ -380,27 +441,62 @@ class SolverDFSPH:
                   .to(out.dtype))
 
     def alloy_capacity_report(self) -> Mapping[str, int]:
-        return {"neighbor_overflow": int(self._raw_report()["neighbor_overflow"].sum()), "emitter_overflow": 0}
+        W = self.model.world_count
+        return {"neighbor_overflow": int(self._raw_report()["neighbor_overflow"].sum()),
+                "emitter_overflow": int(self.buffer(_abi.BUF_EMIT_OVERFLOW, np.int32, (W,)).sum())}
+
+    def _ledger_keys(self):
+        out = [("sph.mass_in", _abi.BUF_MASS_IN, np.int64), ("sph.mass_out", _abi.BUF_MASS_OUT, np.int64),
+               ("sph.escaped", _abi.BUF_ESCAPED, np.int32)]
+        if self.emitters.table.world:
+            out += [("sph.emit_t", _abi.BUF_EMIT_T, np.float32), ("sph.emit_cursor", _abi.BUF_EMIT_CURSOR, np.int32)]
+        return out
 
     def alloy_state_arrays(self, state: State | None = None) -> dict[str, np.ndarray]:
-        """Arrays a snapshot must restore bitwise for this solver to replay (coupling_hooks.py:115-126)."""
+        """Arrays a snapshot must restore bitwise for this solver to replay (coupling_hooks.py:115-126,
+        runtime.py:178-179 and 389-393: the ledger and the emitters' clocks and cursors)."""
         n = max(self.model.particle_count, 1)
-        return {"sph.kappa_v": self.buffer(_abi.BUF_KV, np.float32, (n,)),
-                "sph.kappa_d": self.buffer(_abi.BUF_KD, np.float32, (n,)),
-                "sph.phase": self.model.sph.phase.cpu().numpy().copy(),
-                "sph.flags": self.model.particle_flags.cpu().numpy().copy()}
+        out = {"sph.kappa_v": self.buffer(_abi.BUF_KV, np.float32, (n,)),
+               "sph.kappa_d": self.buffer(_abi.BUF_KD, np.float32, (n,)),
+               "sph.phase": self.model.sph.phase.cpu().numpy().copy(),
+               "sph.flags": self.model.particle_flags.cpu().numpy().copy()}
+        W, E = self.model.world_count, len(self.emitters.table.world)
+        for key, which, dt in self._ledger_keys():
+            out[key] = self.buffer(which, dt, (E if key.startswith("sph.emit") else W,))
+        return out
 
     def alloy_restore_state_arrays(self, arrays: Mapping[str, np.ndarray]) -> None:
         for key, which in (("sph.kappa_v", _abi.BUF_KV), ("sph.kappa_d", _abi.BUF_KD)):
-            a = np.ascontiguousarray(arrays[key], dtype=np.float32)
-            self._call(self.lib.dfsph_write_buffer, which, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+            self._write(which, np.ascontiguousarray(arrays[key], dtype=np.float32))
         self.model.sph.phase.copy_(torch.tensor(arrays["sph.phase"], device=self.device))
         self.model.particle_flags.copy_(torch.tensor(arrays["sph.flags"], device=self.device))
+        for key, which, dt in self._ledger_keys():
+            if key in arrays:
+                self._write(which, np.ascontiguousarray(arrays[key], dtype=dt))
 
     def alloy_state_worlds(self) -> dict[str, np.ndarray]:
         """Item -> world maps of the state arrays (coupling_hooks.py:128-138)."""
         w = self.model.particle_world.cpu().numpy()
-        return {"sph.kappa_v": w, "sph.kappa_d": w, "sph.phase": w, "sph.flags": w}
+        out = {"sph.kappa_v": w, "sph.kappa_d": w, "sph.phase": w, "sph.flags": w}
+        wl = np.arange(self.model.world_count, dtype=np.int32)
+        ew = np.asarray(self.emitters.table.world, np.int32)
+        for key, _, _ in self._ledger_keys():
+            out[key] = ew if key.startswith("sph.emit") else wl
+        return out
+
+    def export_flux_energy(self, health) -> None:
+        """Energy emitted minus energy retired since the last call, into ``health``'s W_ext (solver.py:413-418,
+        runtime.py:29-37 and 360-369): W_ext[w] = flux_e + flux_g z_ref[w], then the flux ledger is zeroed."""
+        W = self.model.world_count
+        fe = self.buffer(_abi.BUF_FLUX_E, np.int64, (W,))
+        fg = self.buffer(_abi.BUF_FLUX_G, np.int64, (W,))
+        z = health.state_arrays()["health.energy_z_ref"]
+        z = np.asarray(z.detach().cpu().numpy() if isinstance(z, torch.Tensor) else
+                       (z.numpy() if hasattr(z, "numpy") else z), np.float64)[:W]
+        w_ext = fe.astype(np.float64) / _abi.WORK_SCALE + fg.astype(np.float64) / _abi.FORCE_SCALE * z
+        for which in (_abi.BUF_FLUX_E, _abi.BUF_FLUX_G):
+            self._write(which, np.zeros(W, np.int64))
+        health.add_external_work(w_ext)
 
     def notify_model_changed(self, flags: int) -> None:
         """solver.py:343-351. Shape properties (transform, scale, radius, friction): the native handle is rebuilt from
@@ -410,12 +506,24 @@ class SolverDFSPH:
             self._gravity.copy_(self.model.gravity.reshape(-1))
         if flags & int(ModelFlags.SHAPE_PROPERTIES):
             saved = self.alloy_state_arrays(None)
+            W = self.model.world_count
+            ledger = {which: self.buffer(which, dt, (W,)) for which, dt in (
+                (_abi.BUF_MASS_INITIAL, np.int64), (_abi.BUF_EMIT_OVERFLOW, np.int32), (_abi.BUF_FLUX_E, np.int64),
+                (_abi.BUF_FLUX_G, np.int64))}
+            E = len(self.emitters.table.world)
+            enabled = self.buffer(_abi.BUF_EMIT_ENABLED, np.int32, (E,)) if E else None
             S = self.model.shape_count
             meters = self.buffer(_abi.BUF_SHAPE_METER, np.int64, (max(S, 1), 3)) if S else None
             opts = {o: self.get_option(o) for o in (_abi.OPT_CHECK_INTERVAL, _abi.OPT_GRAPHS, _abi.OPT_PAIR_CACHE)}
             self.lib.dfsph_destroy(self._h)
             self._h = None
             self._create_native()
+            if self.emitters.active:
+                self.emitters.upload()
+                if enabled is not None:
+                    self._write(_abi.BUF_EMIT_ENABLED, enabled)
+            for which, a in ledger.items():
+                self._write(which, a)
             self.alloy_restore_state_arrays(saved)
             if meters is not None and self.model.shape_count == S:
                 self._call(self.lib.dfsph_write_buffer, _abi.BUF_SHAPE_METER, meters.ctypes.data_as(ctypes.c_void_p),
@@ -431,36 +539,44 @@ class SolverDFSPH:
         out.zero_()
 
     def retire(self, state: State, mask: torch.Tensor) -> None:
-        """Retire the masked liquid particles for good (solver.py:428-431): they leave the active set (flags cleared)
-        and their mass goes to the worlds' mass_out."""
-        del state
-        m = self.model
-        sel = (mask.to(torch.bool) & ((m.particle_flags & int(ParticleFlags.ACTIVE)) != 0)
-               & (m.sph.phase >= PHASE_LIQUID))
-        w = m.particle_world.long().clamp(min=0)
-        out = torch.zeros(m.world_count, dtype=torch.float64, device=self.device).index_add_(
-            0, w, torch.where(sel, m.particle_mass.double(), torch.zeros((), dtype=torch.float64, device=self.device)))
-        m.particle_flags[sel] = m.particle_flags[sel] & ~int(ParticleFlags.ACTIVE)
-        self._mass_out += out.cpu().numpy()
+        """Retire the masked liquid particles for good (solver.py:428-431, ports.py:66-93): ACTIVE cleared, phase
+        retired, velocity zeroed; their mass and energy leave the ledgers as at a sink."""
+        n = self.model.particle_count
+        if n == 0:
+            return
+        m32 = mask.to(device=self.device, dtype=torch.int32).contiguous()
+        if tuple(m32.shape) != (n,):
+            raise ValueError(f"SolverDFSPH.retire: mask must have {n} entries")
+        self._checked_state(state, n)
+        self._call(self.lib.dfsph_retire, _dptr(m32), _dptr(state.particle_q), _dptr(state.particle_qd),
+                   _stream_ptr(self.device), what="SolverDFSPH.retire")
 
-    def _no_emitters(self, what: str):
-        raise NotImplementedError(f"SolverDFSPH.{what}: emitters, sinks and patch emitters (emitters.py) are not "
-                                  "built in this solver")
-
-    def add_emitter(self, world, region, half_extents, rate, velocity, pool=None) -> int:
-        self._no_emitters("add_emitter")
+    def add_emitter(self, world: int, region, half_extents, rate: float, velocity, pool=None) -> int:
+        """Emit liquid from a box region at ``rate`` [m^3/s] with ``velocity`` [m/s] by activating spares
+        (solver.py:433-436, runtime.py:243-270); ``region`` is p(3) + q(4, xyzw). Returns the emitter's index."""
+        return self.emitters.add_emitter(world, region, half_extents, rate, velocity, pool=pool)
 
     def set_emitter_enabled(self, e: int, on: bool) -> None:
-        self._no_emitters("set_emitter_enabled")
+        """Open or close emitter ``e`` between steps: a closed emitter emits nothing and its clock pauses
+        (solver.py:438-447)."""
+        n = len(self.emitters.table.world)
+        if not 0 <= int(e) < n:
+            raise ValueError(f"set_emitter_enabled: emitter {e} does not exist ({n} emitters)")
+        self._call(self.lib.dfsph_set_emitter_enabled, int(e), int(bool(on)))
+
+    def add_sink(self, world: int, region, half_extents) -> int:
+        """Retire particles that enter a box region; their mass goes to ``mass_out`` (solver.py:460-462)."""
+        return self.emitters.add_sink(world, region, half_extents)
+
+    def _no_patch(self, what: str):
+        raise NotImplementedError(f"SolverDFSPH.{what}: patch emitters move with bodies (patch.py); bodies attached "
+                                  "to shapes are not built in this solver")
 
     def add_patch_emitter(self, world_mask, body, local_points, velocity=(0.0, 0.0, 0.0)) -> int:
-        self._no_emitters("add_patch_emitter")
+        self._no_patch("add_patch_emitter")
 
     def trigger(self, pid: int, world_flags) -> None:
-        self._no_emitters("trigger")
-
-    def add_sink(self, world, region, half_extents) -> int:
-        self._no_emitters("add_sink")
+        self._no_patch("trigger")
 
     @staticmethod
     def emit(builder, obj, decision) -> None:

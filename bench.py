@@ -19,7 +19,10 @@ bracketed by device synchronisations and timed with HIP events on the solver's s
   (profiles/), or null without one;
 - "cpu_baseline": the CPU oracle (the same algorithm, float32, one core) on a bounded sample of the workload;
 - "fp64_density_err": mean max(rho/rho0 - 1, 0) of the final state's liquid, evaluated by the FP64 oracle.
-`--workload dam_break_125k` (the 125,000-particle dam break) and the others measure a single world.
+`--workload dam_break_125k` (the 125,000-particle dam break) and the others measure a single world;
+`--workload pour_worlds_2048` the batched pour pack (2,048 worlds, each a spout filling a swaying particle-wall bowl
+from a pool of U{1,000..4,000} spares, 1,000 steps): there `value` counts the liquid active at each step (particle-
+steps/s = sum over steps of the active liquid / seconds), the caller's per-step wall drive inside the timed region.
 """
 
 from __future__ import annotations
@@ -43,6 +46,7 @@ WORKLOADS = {  # steps: the timed steps each workload is specified with
     "dam_break_125k": dict(kind="dam_break", n_side=50, box=1.0, seed=1, steps=500),
     "dam_break_8k": dict(kind="dam_break", n_side=20, box=0.4, seed=0, steps=200),
     "dam_break_1m": dict(kind="dam_break", n_side=100, box=2.0, seed=3, steps=200),
+    "pour_worlds_2048": dict(kind="pour_worlds", worlds=2048, seed=4, steps=1000),
 }
 CPU_SAMPLE_PARTICLE_STEPS = 3.0e6  # about 20 s of the serial oracle
 OPTION_IDS = {"check_interval": 1, "graphs": 2, "pair_cache": 3, "sparse_correction": 5, "build_block": 6}
@@ -53,11 +57,37 @@ def make_scene(wl: dict, device: str, subset=None):
 
     if wl["kind"] == "dam_break":
         return scenes.dam_break(wl["n_side"], wl["box"], wl["seed"], device=device)
+    if wl["kind"] == "pour_worlds":
+        return scenes.pour_worlds(wl["worlds"], seed=wl["seed"], device=device, subset=subset)
     return scenes.bowl_worlds(wl["worlds"], seed=wl["seed"], device=device, subset=subset)
+
+
+class Run:
+    """A scene's solver and state plus the caller's per-step work (the swaying bowls of pour worlds)."""
+
+    def __init__(self, sc, solver, state):
+        self.sc, self.solver, self.state, self.t = sc, solver, state, 0.0
+
+    def step(self, dt: float) -> None:
+        self.sc.drive(self.state, self.t)
+        self.solver.step(self.state, self.state, None, None, dt)
+        self.t += dt
+
+    def active(self):
+        """Active liquid particles now, as a device scalar (no sync)."""
+        m = self.solver.model
+        return (((m.particle_flags & 1) != 0) & (m.sph.phase >= 1)).sum()
 
 
 def describe(wl: dict, sc) -> str:
     n = sum(sc.fluid_counts)
+    if wl["kind"] == "pour_worlds":
+        return (f"{wl['worlds']} batched pour worlds (0.32 m box tanks, a two-layer particle-wall bowl of radius 0.08 m "
+                "swaying along x with amplitude 0.05 m and period 1 s at a phase U(0, 2 pi) per world, a spout at "
+                "z = 0.25 m with half extents 0.01 m emitting a 2 x 2 layer at 2 m/s every 5 ms, U{1000..4000} spares "
+                f"per world, escape box the tank + 0.1 m, seed {wl['seed']}): {n} spares, "
+                f"{sc.model.particle_count - n} wall particles, no liquid at step 0, dt 1 ms, 0.1 % tolerances, "
+                "projected CG")
     if wl["kind"] == "dam_break":
         return (f"dam break, {n} particles ({wl['n_side']}^3 lattice, spacing 0.01 m, jitter 1e-4 m, seed {wl['seed']}) "
                 f"in a {wl['box']} m SDF box tank, dt 1 ms, 0.1 % density and divergence tolerance, projected CG, "
@@ -87,24 +117,29 @@ def load_learned(mode: str, device: str):
     return WarmStartMLP.load(DEFAULT_WEIGHTS, device), int(solves)
 
 
-def timed_run(solver, state, steps: int, dt: float, prof_mask: int = 0, sample: int = 1) -> dict:
+def timed_run(run: Run, steps: int, dt: float, prof_mask: int = 0, sample: int = 1, count_active: bool = False) -> dict:
     """`steps` steps timed with HIP events on the current stream, a report (the step's only host sync point) every
     10 steps; over the same region, per-launch timing of a pseudo-random 1 / `sample` of the launches of the kernels
-    in `prof_mask` (a timed launch costs the host more than a plain one; sampling keeps that out of the step time)."""
+    in `prof_mask` (a timed launch costs the host more than a plain one; sampling keeps that out of the step time).
+    `count_active`: sum the active liquid after every step on the device (particle-steps of a growing pour)."""
     import torch
 
     from alloy_amd.sph import _abi
 
+    solver = run.solver
     lib, h = solver.lib, solver._h
     stream = torch.cuda.current_stream()
     if prof_mask:
         _abi.check(lib.dfsph_set_profiling(h, prof_mask, sample), lib)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     den, it_den, it_div, unconv = [], [], [], 0
+    acc = torch.zeros((), dtype=torch.int64, device=solver.device) if count_active else None
     torch.cuda.synchronize()
     e0.record(stream)
     for k in range(steps):
-        solver.step(state, state, None, None, dt)
+        run.step(dt)
+        if acc is not None:
+            acc += run.active()
         if k % 10 == 9:
             r = solver.report()
             den.append(float(np.mean(r.density_err)))
@@ -117,6 +152,8 @@ def timed_run(solver, state, steps: int, dt: float, prof_mask: int = 0, sample: 
            "density_err_max_sampled": float(np.max(den)) if den else None,
            "iters_density_mean": float(np.mean(it_den)) if it_den else None,
            "iters_divergence_mean": float(np.mean(it_div)) if it_div else None, "unconverged_sampled_steps": unconv}
+    if acc is not None:
+        out["particle_steps"] = int(acc.item())
     if prof_mask:
         ks = {}
         for kid, name in enumerate(_abi.KERNEL_NAMES):
@@ -138,20 +175,22 @@ def plain_sample(wl: dict, worlds: list[int], steps: int, warmup: int, dt: float
     from alloy_amd.sph import _abi
 
     n_tot, ms_tot, den, its = 0, 0.0, [], []
+    pour = wl["kind"] == "pour_worlds"
     for w in worlds:
         sc = make_scene(wl, "cuda", subset=[w])
         solver = sc.make_solver()
         if pair_cache is not None:
             solver.set_option(_abi.OPT_PAIR_CACHE, pair_cache)
         state = sc.model.state()
+        run = Run(sc, solver, state)
         for _ in range(warmup):
-            solver.step(state, state, None, None, dt)
-        r = timed_run(solver, state, steps, dt)
-        n_tot += sc.fluid_counts[0] * steps
+            run.step(dt)
+        r = timed_run(run, steps, dt, count_active=pour)
+        n_tot += r["particle_steps"] if pour else sc.fluid_counts[0] * steps
         ms_tot += r["total_ms"]
         den.append(r["density_err_mean"])
         its.append((r["iters_density_mean"], r["iters_divergence_mean"]))
-        del solver, state, sc
+        del solver, state, sc, run
         torch.cuda.empty_cache()
     return {"value": n_tot / (ms_tot * 1e-3), "unit": "particle-steps/s", "worlds_sampled": len(worlds),
             "world_ids": worlds, "steps": steps, "warmup": warmup, "ms_per_world_step": ms_tot / (steps * len(worlds)),
@@ -164,23 +203,50 @@ def plain_sample(wl: dict, worlds: list[int], steps: int, warmup: int, dt: float
 def cpu_baseline(wl: dict, steps: int, warmup: int, sample_w: int) -> dict:
     """The CPU oracle (float32, serial C++) on a bounded sample: the whole scene for single-world workloads, an
     even sample of `sample_w` worlds (each exactly as in the pack) for the batched one."""
-    from alloy_amd.sph import scenes
+    from alloy_amd.sph import _abi, scenes
     from oracle.oracle import OracleDFSPH
 
-    subset = sample_worlds(wl["worlds"], sample_w) if wl["kind"] == "bowl_worlds" else None
+    pour = wl["kind"] == "pour_worlds"
+    subset = sample_worlds(wl["worlds"], sample_w) if wl["kind"] in ("bowl_worlds", "pour_worlds") else None
     sc = make_scene(wl, "cpu", subset=subset)
     if steps is None:
-        steps = int(max(2, min(40, CPU_SAMPLE_PARTICLE_STEPS // max(sum(sc.fluid_counts), 1))))
+        steps = wl["steps"] if pour else int(max(2, min(40, CPU_SAMPLE_PARTICLE_STEPS // max(sum(sc.fluid_counts), 1))))
     o = OracleDFSPH(sc.model, sc.config, 32)
-    for _ in range(warmup):
+    if pour:  # the scene's emitters through the same host bookkeeping, and the swaying walls
+        from alloy_amd.sph.emitters import EmitterManager
+
+        def cursors():
+            c = np.asarray(o.buffer(_abi.BUF_EMIT_CURSOR)).copy()
+            return np.concatenate([c, np.zeros(len(mgr.table.world) - len(c), np.int32)])
+
+        mgr = EmitterManager(sc.model, sc.config, o.set_emitters, cursors)
+        for args in sc.emitters:
+            mgr.add_emitter(*args)
+        drive_state = sc.model.state()
+
+        wall_ids = sc.sway["ids"].numpy()
+
+    def step(k):
+        if pour:
+            sc.drive(drive_state, k * scenes.DT)
+            o.qd[wall_ids] = drive_state.particle_qd.numpy()[wall_ids]
         o.step(scenes.DT)
+
+    for k in range(warmup):
+        step(k)
+    act = lambda: int(np.count_nonzero(((o.buffer(_abi.BUF_FLAGS) & 1) != 0) & (o.buffer(_abi.BUF_PHASE) >= 1)))  # noqa: E731
+    n_ps = 0
     t = time.perf_counter()
-    for _ in range(steps):
-        o.step(scenes.DT)
+    for k in range(warmup, warmup + steps):
+        step(k)
+        n_ps += act() if pour else 0
     dt = time.perf_counter() - t
     n = sum(sc.fluid_counts)
-    what = f"{len(subset)} of the {wl['worlds']} worlds ({n} liquid particles)" if subset else f"{n} particles"
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+    if not pour:
+        n_ps = n * steps
+    what = (f"{len(subset)} of the {wl['worlds']} worlds ({n} {'spares' if pour else 'liquid particles'})" if subset
+            else f"{n} particles")
+    return {"value": n_ps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
             "sample": f"CPU oracle (float32, serial C++ -O3, one core) on {what} of the same workload from its "
                       f"initial state: {warmup} warm-up + {steps} timed steps ({dt:.1f} s)"}
 
@@ -263,6 +329,7 @@ def main() -> int:
     dt = scenes.DT
     _log(f"[bench] workload {args.workload} {wl} steps={args.steps} warmup={args.warmup}")
     sc = make_scene(wl, "cuda")
+    flags0, phase0 = sc.model.particle_flags.clone(), sc.model.sph.phase.clone()
     solver = sc.make_solver()
     solver.set_check_interval(args.check_every)
     solver.set_graphs(args.graphs)
@@ -273,10 +340,12 @@ def main() -> int:
         net.attach(solver)
         solver.set_option(_abi.OPT_LEARNED_SOLVES, solves)
     state = sc.model.state()
+    pour = wl["kind"] == "pour_worlds"
     n_fluid = sum(sc.fluid_counts)
     W = sc.model.world_count
+    main_run = Run(sc, solver, state)
     for _ in range(args.warmup):
-        solver.step(state, state, None, None, dt)
+        main_run.step(dt)
     torch.cuda.synchronize()
     _log("[bench] warm-up done")
     tuned = None
@@ -285,14 +354,15 @@ def main() -> int:
 
         tuned = autotune(solver, state, dt, log=_log)
     roof_mask = (1 << _abi.KID_ACCEL) | (1 << _abi.KID_APPLY)
-    run = timed_run(solver, state, args.steps, dt, prof_mask=roof_mask, sample=args.roofline_sample)
+    run = timed_run(main_run, args.steps, dt, prof_mask=roof_mask, sample=args.roofline_sample, count_active=pour)
     total_ms = run["total_ms"]
     _log(f"[bench] timed {args.steps} steps in {total_ms:.1f} ms")
     ks = run["kernels"]
     dom = max(ks, key=lambda k: ks[k][1])
     nl, tl, bl = ks[dom]
     achieved = bl / (tl * 1e-3) / 1e9
-    value = n_fluid * args.steps / (total_ms * 1e-3)
+    particle_steps = run["particle_steps"] if pour else n_fluid * args.steps
+    value = particle_steps / (total_ms * 1e-3)
     solver_desc = ("batched, one launch per phase over all worlds" if W > 1 else "single world") + (
         f", learned warm start (trained network; solves {'divergence+density' if solves == 3 else 'density' if solves == 2 else 'divergence'})"
         if net is not None else ", plain warm start")
@@ -310,6 +380,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": describe(wl, sc), "particles": n_fluid, "worlds": W, "solver": solver_desc},
+        "particle_steps": particle_steps,
         "density_err_mean": run["density_err_mean"],
         "density_err_max_sampled": run["density_err_max_sampled"],
         "iters_density_mean": run["iters_density_mean"],
@@ -325,7 +396,7 @@ def main() -> int:
                    "seconds": round(tuned["seconds"], 2) if tuned else 0.0},
     }
     if args.breakdown_steps > 0:
-        bd = timed_run(solver, state, args.breakdown_steps, dt, prof_mask=(1 << len(_abi.KERNEL_NAMES)) - 1)
+        bd = timed_run(main_run, args.breakdown_steps, dt, prof_mask=(1 << len(_abi.KERNEL_NAMES)) - 1)
         out["kernel_ms_per_step"] = {k: round(v[1] / args.breakdown_steps, 4) for k, v in bd["kernels"].items()}
         out["kernel_launches_per_step"] = {k: round(v[0] / args.breakdown_steps, 2) for k, v in bd["kernels"].items()}
         out["breakdown_steps"] = args.breakdown_steps
@@ -335,10 +406,13 @@ def main() -> int:
         out["fp64_density_err"] = fp64_density_error(sc, state.particle_q.cpu().numpy(),
                                                      state.particle_qd.cpu().numpy())
         _log(f"[bench] fp64 density error {out['fp64_density_err']:.3e}")
-    del solver, state
+    del solver, state, main_run
     torch.cuda.empty_cache()
     if net is not None and not args.no_rule_compare:
         _log("[bench] the same run with the rule warm start")
+        if pour:  # the first run emitted from the model's spares: back to the build-time flags and phase
+            sc.model.particle_flags.copy_(flags0)
+            sc.model.sph.phase.copy_(phase0)
         s2 = sc.make_solver()
         s2.set_check_interval(args.check_every)
         s2.set_graphs(args.graphs)
@@ -347,17 +421,19 @@ def main() -> int:
         for name, v in (tuned["options"].items() if tuned else ()):
             s2.set_option(OPTION_IDS[name], v)
         st2 = sc.model.state()
+        run2 = Run(sc, s2, st2)
         for _ in range(args.warmup):
-            s2.step(st2, st2, None, None, dt)
-        r2 = timed_run(s2, st2, args.steps, dt)
+            run2.step(dt)
+        r2 = timed_run(run2, args.steps, dt, count_active=pour)
+        ps2 = r2["particle_steps"] if pour else n_fluid * args.steps
         out["batched_rule_start"] = {
-            "value": n_fluid * args.steps / (r2["total_ms"] * 1e-3), "ms_per_step": r2["total_ms"] / args.steps,
+            "value": ps2 / (r2["total_ms"] * 1e-3), "ms_per_step": r2["total_ms"] / args.steps,
             "iters_density_mean": r2["iters_density_mean"], "iters_divergence_mean": r2["iters_divergence_mean"],
             "density_err_mean": r2["density_err_mean"]}
         out["learned_vs_rule_start"] = value / out["batched_rule_start"]["value"]
-        del s2, st2
+        del s2, st2, run2
         torch.cuda.empty_cache()
-    if wl["kind"] == "bowl_worlds" and not args.no_plain:
+    if wl["kind"] in ("bowl_worlds", "pour_worlds") and not args.no_plain:
         _log(f"[bench] plain single-world DFSPH on {args.plain_worlds} sampled worlds")
         plain = plain_sample(wl, sample_worlds(W, args.plain_worlds), args.steps, args.warmup, dt, args.pair_cache)
         out["plain"] = plain

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define DFSPH_ABI_VERSION 2
+#define DFSPH_ABI_VERSION 3
 
 #define DFSPH_OK 0
 #define DFSPH_EINVAL (-1)
@@ -72,6 +72,8 @@ typedef struct DfsphParams {
     int32_t curvature;               /* boundary_curvature_order (0 or 1) */
     double project_eps;              /* PROJECT_EPS * r (solver.py:483) */
     double sweep_len;                /* integrate.sweep_length(h, thinnest) */
+    int32_t one_way;                 /* coupling == "one_way": no shape meters, no particle-wall reactions
+                                        (bodies.py:58-59, particle_walls.py:156-157) */
 } DfsphParams;
 
 /* Static description of the particle pack and its walls (layout.py:47-72, boundary.py:79-95). */
@@ -164,8 +166,48 @@ enum DfsphBuffer {
     DFSPH_BUF_DIV_ERR,        /* f32    [W] */
     DFSPH_BUF_SHAPE_METER,    /* int64  [S, 3] impulse per shape, 2^50 fixed point (bodies.py:21-25) */
     DFSPH_BUF_FEATURES,       /* f32    [n, 16] learned warm-start features (solve inputs) */
+    /* mass ledger and emitter state (runtime.py:93-107, emitters.py:200-216); masses are int64 at 2^40 per kg
+     * (kernels.py:142-144), so active + out == initial + in holds exactly */
+    DFSPH_BUF_MASS_INITIAL,   /* int64  [W] active liquid mass at construction (init_mass) */
+    DFSPH_BUF_MASS_IN,        /* int64  [W] emitted */
+    DFSPH_BUF_MASS_OUT,       /* int64  [W] retired by sinks, the escape box or dfsph_retire */
+    DFSPH_BUF_MASS_ACTIVE,    /* int64  [W] as of the last dfsph_active_mass */
+    DFSPH_BUF_ESCAPED,        /* int32  [W] particles retired by the escape box */
+    DFSPH_BUF_EMIT_OVERFLOW,  /* int32  [W] layer sites an exhausted pool could not fill */
+    DFSPH_BUF_FLUX_E,         /* int64  [W] energy brought in minus taken out, m (v^2/2 - g.x), 2^50 per J */
+    DFSPH_BUF_FLUX_G,         /* int64  [W] its slope m g_z for the monitor's z_ref, 2^40 per N */
+    DFSPH_BUF_EMIT_T,         /* f32    [E] layer clock of each emitter */
+    DFSPH_BUF_EMIT_CURSOR,    /* int32  [E] spares each emitter has used */
+    DFSPH_BUF_EMIT_ENABLED,   /* int32  [E] */
+    DFSPH_BUF_FLAGS,          /* int32  [n] the model's particle_flags (the emitter pass writes them) */
+    DFSPH_BUF_PHASE,          /* int8   [n] the model's sph:phase */
+    DFSPH_BUF_PW_EXCH,        /* int64  [NW, 3] particle-wall impulse of the exchange window, 2^50 per N s, rows in
+                                 ascending id order of the wall particles (particle_walls.py:138-177) */
+    DFSPH_BUF_PW_WORK,        /* int64  [W] the particle walls' work on the liquid in the last step, 2^50 per J */
     DFSPH_BUF_COUNT_
 };
+
+/* Box emitters, sinks and the escape box (emitters.py:32-255, runtime.py:197-304). Host arrays, copied by
+ * dfsph_set_emitters; E emitters, K sinks. An emitter emits a layer (its offsets, through its region transform,
+ * with its velocity) every `interval` seconds of simulated time, activating the next spares of its pool in order. */
+typedef struct DfsphEmitters {
+    int32_t emitter_count;           /* E */
+    const int32_t* world;            /* [E] */
+    const float* xform;              /* [E*7] region p(3) q(4, xyzw) */
+    const float* velocity;           /* [E*3] */
+    const float* interval;           /* [E] n_layer V / rate [s] (runtime.py:266) */
+    const int32_t* offset_start;     /* [E+1] CSR into offsets */
+    const float* offsets;            /* [offset_start[E]*3] layer lattice in the region frame (emitters.py:46-64) */
+    const int32_t* pool_start;       /* [E+1] CSR into pool_ids */
+    const int32_t* pool_ids;         /* spare particle ids of each emitter, in emission order */
+    int32_t sink_count;              /* K */
+    const int32_t* sink_world;       /* [K] */
+    const float* sink_xform;         /* [K*7] */
+    const float* sink_half;          /* [K*3] half extents */
+    int32_t has_bounds;              /* escape box (DFSPHConfig.world_bounds) */
+    float bounds_lo[3];
+    float bounds_hi[3];
+} DfsphEmitters;
 
 /* Library */
 int32_t dfsph_abi_version(void);
@@ -274,6 +316,21 @@ int32_t dfsph_zero_stats(void* handle, void* stream);   /* WorldStats.reset coun
 
 /* Per-world report (runtime.py:136-147): the sync point of a step. */
 int32_t dfsph_report(void* handle, DfsphReport* out, void* stream);
+
+/* Emitters, sinks and the escape box (runtime.py:231-304 add_emitter / add_sink / _upload, EmitterDevice.carry_from):
+ * replaces the tables; the first min(old, new) emitters keep their clock, cursor and enabled flag. With any emitter,
+ * sink or escape box, dfsph_step ends with the emitter pass (runtime.py:318-347): sinks and the escape box retire
+ * liquid, then each due emitter emits a layer; the step's speed statistics are taken after it. */
+int32_t dfsph_set_emitters(void* handle, const DfsphEmitters* table);
+/* SolverDFSPH.set_emitter_enabled (solver.py:438-447); an unknown emitter is DFSPH_EINVAL */
+int32_t dfsph_set_emitter_enabled(void* handle, int32_t emitter, int32_t on);
+/* the emitter pass alone (after dfsph_advect_project, whose speed statistics it then supplies) */
+int32_t dfsph_emitters_step(void* handle, float* q, float* qd, double dt, void* stream);
+/* SolverDFSPH.retire (solver.py:428-431, ports.py:66-93): the liquid particles with mask[i] != 0 (device int32 [n])
+ * leave for good (ACTIVE cleared, phase -2, velocity 0); their mass and energy leave the ledgers as at a sink */
+int32_t dfsph_retire(void* handle, const int32_t* mask, float* q, float* qd, void* stream);
+/* active liquid mass per world into DFSPH_BUF_MASS_ACTIVE (runtime.py:126-130) */
+int32_t dfsph_active_mass(void* handle, void* stream);
 
 #ifdef __cplusplus
 }

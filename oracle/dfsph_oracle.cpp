@@ -224,7 +224,7 @@ struct Oracle {
     T h, spacing, r, n_full, tot_min, min_nb, omega, warm_den, warm_div, xsph, v_sep;
     T offset, amplitude, taper0, taper1, reach, eps, sweep_len, lattice_grad;
     double relief_speed, rest_speed, flow_ratio, tol_den, tol_div, tol_min, err_factor, lattice_grad_d;
-    int kind, solver, min_it, max_it, cap, div_solve, two_sided, friction, curvature;
+    int kind, solver, min_it, max_it, cap, div_solve, two_sided, friction, curvature, one_way = 0;
 
     int n = 0, W = 0, S = 0, NW = 0;  // NW: wall particles
     std::vector<int> ws;              // world start [W+1]
@@ -262,6 +262,17 @@ struct Oracle {
     std::vector<int> st_overflow, st_projected, st_swept, st_nonfinite, it_den, it_div;
     std::vector<T> st_max_speed, st_den_err, st_div_err, st_den_err_max, st_den_tol;
     std::vector<int64_t> meter;
+    std::vector<int> wcidx;            // compact index of each wall particle (ascending ids), -1 otherwise
+    std::vector<int64_t> pw_exch, pw_work;  // [NW*3] exchange window, [W] walls' work (2^50 fixed point)
+    // emitters, sinks, escape box (emitters.py:32-255) and the int64 ledgers (runtime.py:93-107)
+    int E = 0, has_bounds = 0, emit_active = 0;
+    V3<T> lo{}, hi{};
+    std::vector<int> e_world, off_start, pool_start, pool_ids, cursor, enabled, sink_start;
+    std::vector<Xform<T>> e_xform, s_xform;
+    std::vector<V3<T>> e_vel, off_local, s_half;
+    std::vector<T> e_interval, t_acc;
+    std::vector<int64_t> mass0, mass_in, mass_out, mass_act, flux_e, flux_g;
+    std::vector<int> escaped, emit_overflow;
     // chunks
     std::vector<int> ch_begin, ch_end, w_chunk;
     // state (bound per call)
@@ -305,6 +316,7 @@ struct Oracle {
         reach = T(P->reach);
         eps = T(P->project_eps);
         sweep_len = T(P->sweep_len);
+        one_way = P->one_way != 0;
         lattice_grad = T(P->lattice_grad);
         lattice_grad_d = P->lattice_grad;
         relief_speed = P->compression_relief_speed;
@@ -407,6 +419,16 @@ struct Oracle {
         for (auto* v : {&n_active, &err_sum, &rz, &stepb}) v->assign(W, 0.0);
         for (auto* v : {&sums, &flow_z, &flow_xy, &head}) v->assign(2 * W, 0.0);
         meter.assign(3 * size_t(std::max(S, 1)), 0);
+        for (auto* v : {&mass0, &mass_in, &mass_out, &mass_act, &flux_e, &flux_g}) v->assign(W, 0);
+        wcidx.assign(std::max(n, 1), -1);
+        int nw = 0;
+        for (int i = 0; i < n; ++i)
+            if (wvol[i] > T(0)) wcidx[i] = nw++;
+        pw_exch.assign(3 * size_t(std::max(nw, 1)), 0);
+        pw_work.assign(W, 0);
+        escaped.assign(W, 0);
+        emit_overflow.assign(W, 0);
+        sink_start.assign(W + 1, 0);
         // chunk table (reduce.py:167-182)
         w_chunk.push_back(0);
         for (int w = 0; w < W; ++w) {
@@ -416,6 +438,8 @@ struct Oracle {
             }
             w_chunk.push_back(int(ch_begin.size()));
         }
+        active_mass();  // runtime.py:132-134 init_mass
+        mass0 = mass_act;
         return DFSPH_OK;
     }
 
@@ -1324,6 +1348,7 @@ struct Oracle {
 
     // ---------------------------------------------------------------------------- shape meters (bodies.py:37-)
     void meters() {
+        if (one_way) return;  // bodies.py:58-59
         for (int i = 0; i < n; ++i)
             for (int k = 0; k < 4; ++k) {
                 int s = s_shape[4 * i + k];
@@ -1334,6 +1359,60 @@ struct Oracle {
                 meter[3 * s] += int64_t(std::round(double(J.x) * sc));
                 meter[3 * s + 1] += int64_t(std::round(double(J.y) * sc));
                 meter[3 * s + 2] += int64_t(std::round(double(J.z) * sc));
+            }
+    }
+
+    // particle_walls.py:138-177 _accumulate_kernel (impulse split over the wall particles, the walls' work)
+    void pw_accumulate() {
+        std::fill(pw_work.begin(), pw_work.end(), 0);
+        if (NW == 0) return;
+        const double IMP = 1125899906842624.0;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 4; ++k) {
+                if (s_shape[4 * i + k] != S) continue;
+                V3<T> J = get3(s_imp, 4 * size_t(i) + k);
+                if (J.x == T(0) && J.y == T(0) && J.z == T(0)) continue;
+                int w = world[i];
+                pw_work[w] += std::llround(double(-dot(J, get3(s_vb, 4 * size_t(i) + k))) * IMP);
+                if (one_way) continue;
+                V3<T> x = qv(i), gc, mv;
+                T col;
+                pw_sample(x, w, col, gc, mv);
+                if (col <= T(0)) continue;
+                int64_t left[3] = {std::llround(double(J.x) * IMP), std::llround(double(J.y) * IMP),
+                                   std::llround(double(J.z) * IMP)};
+                int best = -1;
+                T sbest = T(0);
+                int g = W + w, cx, cy, cz;
+                cell_of(g, x, cx, cy, cz);
+                const int* d = &g_dims[3 * w];
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            int X = cx + dx, Y = cy + dy, Z = cz + dz;
+                            if (X < 0 || Y < 0 || Z < 0 || X >= d[0] || Y >= d[1] || Z >= d[2]) continue;
+                            int cc = cell_base[g] + (X * d[1] + Y) * d[2] + Z;
+                            for (int e = cell_start[cc]; e < cell_start[cc + 1]; ++e) {
+                                int j = cell_items[e];
+                                T sj = wvol[j] * kernel_w(length(x - qv(j)), h, kind) / col;
+                                if (sj > T(0)) {
+                                    int cj = wcidx[j];
+                                    int64_t part[3] = {std::llround(double(J.x * sj) * IMP),
+                                                       std::llround(double(J.y * sj) * IMP),
+                                                       std::llround(double(J.z * sj) * IMP)};
+                                    for (int q = 0; q < 3; ++q) {
+                                        pw_exch[3 * size_t(cj) + q] += part[q];
+                                        left[q] -= part[q];
+                                    }
+                                    if (sj > sbest) {
+                                        sbest = sj;
+                                        best = cj;
+                                    }
+                                }
+                            }
+                        }
+                if (best >= 0)
+                    for (int q = 0; q < 3; ++q) pw_exch[3 * size_t(best) + q] += left[q];
             }
     }
 
@@ -1503,8 +1582,12 @@ struct Oracle {
                 st_projected[world[i]] += 1;
             }
         }
-        // speed_kernel (integrate.py:312-326)
         for (int w = 0; w < W; ++w) st_max_speed[w] = T(0);
+        if (!emit_active) speed();  // with emitters, sinks or an escape box, after their pass
+    }
+
+    // speed_kernel (integrate.py:312-326)
+    void speed() {
         for (int i = 0; i < n; ++i) {
             if (!is_fluid(i)) continue;
             V3<T> v = vv(i), x = qv(i);
@@ -1516,6 +1599,158 @@ struct Oracle {
                 st_max_speed[w] = tmax(st_max_speed[w], T(1.0e30));
             }
         }
+    }
+
+    // ---------------------------------------------------------------------------- emitters
+    static int64_t fixed(double x, double scale) { return std::llround(x * scale); }  // bodies.py:31-32 _fx
+    static int64_t to_fixed(T m) { return fixed(double(m), 1099511627776.0); }      // kernels.py:142-144
+
+    // emitters.py:82-89 flux_add
+    void flux_add(int w, T m, V3<T> v, V3<T> x, T sign) {
+        V3<T> g = get3(gravity, w);
+        T en = sign * m * (T(0.5) * dot(v, v) - dot(g, x));
+        flux_e[w] += fixed(double(en), 1125899906842624.0);
+        flux_g[w] += fixed(double(sign * m * g.z), 1099511627776.0);
+    }
+
+    // emitters.py:127-130 _in_box
+    static bool in_box(const Xform<T>& X, V3<T> half, V3<T> x) {
+        V3<T> xl = xf_inv_point(X, x);
+        return std::fabs(xl.x) <= half.x && std::fabs(xl.y) <= half.y && std::fabs(xl.z) <= half.z;
+    }
+
+    // emitters.py:133-165 retire_kernel
+    void retire() {
+        for (int i = 0; i < n; ++i) {
+            if (!is_fluid(i)) continue;
+            int w = world[i];
+            V3<T> x = qv(i);
+            int gone = 0;
+            for (int k = sink_start[w]; k < sink_start[w + 1]; ++k)
+                if (in_box(s_xform[k], s_half[k], x)) gone = 1;
+            if (has_bounds && gone == 0) {
+                bool out = x.x < lo.x || x.y < lo.y || x.z < lo.z || x.x > hi.x || x.y > hi.y || x.z > hi.z;
+                if (out || !std::isfinite(x.x + x.y + x.z)) {
+                    gone = 2;
+                    escaped[w] += 1;
+                }
+            }
+            if (gone != 0) {
+                V3<T> v = vv(i);
+                if (std::isfinite(x.x + x.y + x.z + v.x + v.y + v.z)) flux_add(w, mass[i], v, x, T(-1));
+                flags[i] = flags[i] & ~ACTIVE;
+                phase[i] = int8_t(-2);
+                setv(i, V3<T>{T(0), T(0), T(0)});
+                mass_out[w] += to_fixed(mass[i]);
+            }
+        }
+    }
+
+    // emitters.py:67-79 emit_clock_kernel, 92-117 emit_kernel, 120-124 advance_cursor_kernel
+    void emit(T dt) {
+        std::vector<int> now(E, 0);
+        for (int e = 0; e < E; ++e) {
+            if (enabled[e] == 0) continue;
+            T t = t_acc[e] + dt;
+            if (t >= e_interval[e]) {
+                t -= e_interval[e];
+                now[e] = 1;
+            }
+            t_acc[e] = t;
+        }
+        for (int e = 0; e < E; ++e) {
+            if (!now[e]) continue;
+            int w = e_world[e], psize = pool_start[e + 1] - pool_start[e];
+            for (int r = 0; r < off_start[e + 1] - off_start[e]; ++r) {
+                int slot = cursor[e] + r;
+                if (slot >= psize) {
+                    emit_overflow[w] += 1;
+                    continue;
+                }
+                int i = pool_ids[pool_start[e] + slot];
+                setq(i, xf_point(e_xform[e], off_local[off_start[e] + r]));
+                setv(i, e_vel[e]);
+                flags[i] = flags[i] | ACTIVE;
+                phase[i] = int8_t(1);
+                mass_in[w] += to_fixed(mass[i]);
+                flux_add(w, mass[i], vv(i), qv(i), T(1));
+            }
+        }
+        for (int e = 0; e < E; ++e)
+            if (now[e]) cursor[e] = std::min(cursor[e] + (off_start[e + 1] - off_start[e]), pool_start[e + 1] - pool_start[e]);
+    }
+
+    // runtime.py:318-326 EmitterManager.step, then the step's speed statistics (solver.py:266-267)
+    void emitters_step(T dt) {
+        if (!emit_active) return;
+        retire();
+        if (E > 0) emit(dt);
+        speed();
+    }
+
+    // runtime.py:126-130 _active_mass
+    void active_mass() {
+        std::fill(mass_act.begin(), mass_act.end(), 0);
+        for (int i = 0; i < n; ++i)
+            if (is_fluid(i)) mass_act[world[i]] += to_fixed(mass[i]);
+    }
+
+    int set_emitters(const DfsphEmitters* t) {
+        int En = t->emitter_count, K = t->sink_count;
+        for (int e = 0; e < En; ++e) {
+            int w = t->world[e];
+            if (w < 0 || w >= W) return fail(DFSPH_EINVAL, "add_emitter: world out of range");
+            for (int k = t->pool_start[e]; k < t->pool_start[e + 1]; ++k)
+                if (t->pool_ids[k] < ws[w] || t->pool_ids[k] >= ws[w + 1])
+                    return fail(DFSPH_EINVAL, "pool id is not a particle of the emitter's world");
+        }
+        for (int k = 0; k < K; ++k)
+            if (t->sink_world[k] < 0 || t->sink_world[k] >= W) return fail(DFSPH_EINVAL, "add_sink: world out of range");
+        std::vector<T> t_old = t_acc;
+        std::vector<int> c_old = cursor, en_old = enabled;
+        E = En;
+        e_world.assign(t->world, t->world + En);
+        e_xform.resize(En);
+        e_vel.resize(En);
+        e_interval.resize(En);
+        for (int e = 0; e < En; ++e) {
+            const float* x = t->xform + 7 * e;
+            e_xform[e] = Xform<T>{V3<T>{T(x[0]), T(x[1]), T(x[2])}, T(x[3]), T(x[4]), T(x[5]), T(x[6])};
+            e_vel[e] = V3<T>{T(t->velocity[3 * e]), T(t->velocity[3 * e + 1]), T(t->velocity[3 * e + 2])};
+            e_interval[e] = T(t->interval[e]);
+        }
+        off_start.assign(t->offset_start, t->offset_start + (En > 0 ? En + 1 : 0));
+        pool_start.assign(t->pool_start, t->pool_start + (En > 0 ? En + 1 : 0));
+        int n_off = En > 0 ? off_start[En] : 0, n_pool = En > 0 ? pool_start[En] : 0;
+        off_local.resize(n_off);
+        for (int k = 0; k < n_off; ++k)
+            off_local[k] = V3<T>{T(t->offsets[3 * k]), T(t->offsets[3 * k + 1]), T(t->offsets[3 * k + 2])};
+        pool_ids.assign(t->pool_ids, t->pool_ids + n_pool);
+        t_acc.assign(En, T(0));
+        cursor.assign(En, 0);
+        enabled.assign(En, 1);
+        for (int e = 0; e < std::min<int>(En, int(t_old.size())); ++e) {  // EmitterDevice.carry_from
+            t_acc[e] = t_old[e];
+            cursor[e] = c_old[e];
+            enabled[e] = en_old[e];
+        }
+        sink_start.assign(W + 1, 0);
+        for (int k = 0; k < K; ++k) sink_start[t->sink_world[k] + 1] += 1;
+        for (int w = 0; w < W; ++w) sink_start[w + 1] += sink_start[w];
+        s_xform.resize(K);
+        s_half.resize(K);
+        std::vector<int> fillp(sink_start.begin(), sink_start.end() - 1);
+        for (int k = 0; k < K; ++k) {
+            int d = fillp[t->sink_world[k]]++;
+            const float* x = t->sink_xform + 7 * k;
+            s_xform[d] = Xform<T>{V3<T>{T(x[0]), T(x[1]), T(x[2])}, T(x[3]), T(x[4]), T(x[5]), T(x[6])};
+            s_half[d] = V3<T>{T(t->sink_half[3 * k]), T(t->sink_half[3 * k + 1]), T(t->sink_half[3 * k + 2])};
+        }
+        has_bounds = t->has_bounds != 0;
+        lo = V3<T>{T(t->bounds_lo[0]), T(t->bounds_lo[1]), T(t->bounds_lo[2])};
+        hi = V3<T>{T(t->bounds_hi[0]), T(t->bounds_hi[1]), T(t->bounds_hi[2])};
+        emit_active = E > 0 || K > 0 || has_bounds;
+        return DFSPH_OK;
     }
 
     // ---------------------------------------------------------------------------- prime (prime.py:53-115)
@@ -1562,7 +1797,9 @@ struct Oracle {
         nonpressure(dt);
         solve(1, dt_host, -1, warm_den_k);
         meters();
+        pw_accumulate();
         advect_project(dt);
+        emitters_step(dt);
     }
 };
 
@@ -1617,6 +1854,21 @@ int get_buffer_t(Oracle<T>* o, int which, void** ptr, int64_t* nb) {
         case DFSPH_BUF_DEN_ERR: OB(st_den_err)
         case DFSPH_BUF_DIV_ERR: OB(st_div_err)
         case DFSPH_BUF_SHAPE_METER: OB(meter)
+        case DFSPH_BUF_MASS_INITIAL: OB(mass0)
+        case DFSPH_BUF_MASS_IN: OB(mass_in)
+        case DFSPH_BUF_MASS_OUT: OB(mass_out)
+        case DFSPH_BUF_MASS_ACTIVE: OB(mass_act)
+        case DFSPH_BUF_ESCAPED: OB(escaped)
+        case DFSPH_BUF_EMIT_OVERFLOW: OB(emit_overflow)
+        case DFSPH_BUF_FLUX_E: OB(flux_e)
+        case DFSPH_BUF_FLUX_G: OB(flux_g)
+        case DFSPH_BUF_EMIT_T: OB(t_acc)
+        case DFSPH_BUF_EMIT_CURSOR: OB(cursor)
+        case DFSPH_BUF_EMIT_ENABLED: OB(enabled)
+        case DFSPH_BUF_FLAGS: OB(flags)
+        case DFSPH_BUF_PHASE: OB(phase)
+        case DFSPH_BUF_PW_EXCH: OB(pw_exch)
+        case DFSPH_BUF_PW_WORK: OB(pw_work)
         default: return fail(DFSPH_EINVAL, "unknown buffer id");
     }
 #undef OB
@@ -1747,6 +1999,34 @@ int32_t orc_advect_project(void* h, void* q, void* qd, double dt) {
 
 int32_t orc_meters(void* h) {
     DISPATCH(h, { o->meters(); });
+    return DFSPH_OK;
+}
+
+int32_t orc_set_emitters(void* h, const DfsphEmitters* t) {
+    if (!t) return fail(DFSPH_EINVAL, "null argument");
+    DISPATCH(h, { return o->set_emitters(t); });
+    return DFSPH_OK;
+}
+
+int32_t orc_set_emitter_enabled(void* h, int32_t e, int32_t on) {
+    DISPATCH(h, {
+        if (e < 0 || e >= o->E) return fail(DFSPH_EINVAL, "set_emitter_enabled: emitter does not exist");
+        o->enabled[e] = on != 0;
+    });
+    return DFSPH_OK;
+}
+
+int32_t orc_emitters_step(void* h, void* q, void* qd, double dt) {
+    DISPATCH(h, {
+        using TT = std::remove_pointer_t<decltype(o->q)>;
+        bind(o, q, qd, nullptr);
+        o->emitters_step(TT(dt));
+    });
+    return DFSPH_OK;
+}
+
+int32_t orc_active_mass(void* h) {
+    DISPATCH(h, { o->active_mass(); });
     return DFSPH_OK;
 }
 

@@ -54,6 +54,10 @@ def load() -> ctypes.CDLL:
         "orc_advect_project": ([vp, vp, vp, dbl], i32),
         "orc_meters": ([vp], i32),
         "orc_zero_impulse": ([vp], i32),
+        "orc_set_emitters": ([vp, P(_abi.DfsphEmitters)], i32),
+        "orc_set_emitter_enabled": ([vp, i32, i32], i32),
+        "orc_emitters_step": ([vp, vp, vp, dbl], i32),
+        "orc_active_mass": ([vp], i32),
         "orc_get_buffer": ([vp, i32, P(vp), P(i64)], i32),
         "orc_reduce": ([vp, vp, i32, P(dbl), vp], i32),
         "orc_kernel_w": ([i32, dbl, dbl, i32], dbl),
@@ -79,7 +83,11 @@ _BUF_DTYPES = {
     _abi.BUF_NBR: np.uint16, _abi.BUF_NBR_HI: np.uint8, _abi.BUF_COUNT: np.int32, _abi.BUF_GROUP: np.int32,
     _abi.BUF_SLOT_SHAPE: np.int32, _abi.BUF_CONVERGED: np.int32, _abi.BUF_ITERS: np.int32,
     _abi.BUF_N_ACTIVE: np.float64, _abi.BUF_OVERFLOW: np.int32, _abi.BUF_ITERS_DEN: np.int32,
-    _abi.BUF_ITERS_DIV: np.int32, _abi.BUF_SHAPE_METER: np.int64,
+    _abi.BUF_ITERS_DIV: np.int32, _abi.BUF_SHAPE_METER: np.int64, _abi.BUF_MASS_INITIAL: np.int64,
+    _abi.BUF_MASS_IN: np.int64, _abi.BUF_MASS_OUT: np.int64, _abi.BUF_MASS_ACTIVE: np.int64, _abi.BUF_ESCAPED: np.int32,
+    _abi.BUF_EMIT_OVERFLOW: np.int32, _abi.BUF_FLUX_E: np.int64, _abi.BUF_FLUX_G: np.int64,
+    _abi.BUF_EMIT_CURSOR: np.int32, _abi.BUF_EMIT_ENABLED: np.int32, _abi.BUF_FLAGS: np.int32, _abi.BUF_PHASE: np.int8,
+    _abi.BUF_PW_EXCH: np.int64, _abi.BUF_PW_WORK: np.int64,
 }
 
 
@@ -168,15 +176,34 @@ class OracleDFSPH:
     def advect_project(self, dt: float):
         _check(self.lib.orc_advect_project(self.h, self._p(self.q), self._p(self.qd), float(dt)), "orc_advect_project")
 
+    def set_emitters(self, table) -> None:
+        """The emitter, sink and escape-box table (an ``alloy_amd.sph.emitters.EmitterStruct``, e.g. a solver's
+        ``emitters.struct``)."""
+        self._emit_keep = table
+        _check(self.lib.orc_set_emitters(self.h, ctypes.byref(table.struct)), "orc_set_emitters")
+
+    def set_emitter_enabled(self, e: int, on: bool) -> None:
+        _check(self.lib.orc_set_emitter_enabled(self.h, int(e), int(bool(on))), "orc_set_emitter_enabled")
+
+    def emitters_step(self, dt: float) -> None:
+        _check(self.lib.orc_emitters_step(self.h, self._p(self.q), self._p(self.qd), float(dt)), "orc_emitters_step")
+
+    def mass_ledger(self) -> dict[str, np.ndarray]:
+        _check(self.lib.orc_active_mass(self.h), "orc_active_mass")
+        return {"initial": self.buffer(_abi.BUF_MASS_INITIAL).copy(), "in": self.buffer(_abi.BUF_MASS_IN).copy(),
+                "out": self.buffer(_abi.BUF_MASS_OUT).copy(), "active": self.buffer(_abi.BUF_MASS_ACTIVE).copy()}
+
     def buffer(self, which: int) -> np.ndarray:
         """A numpy view of an oracle buffer (float buffers in the oracle's precision)."""
         ptr, nb = ctypes.c_void_p(), ctypes.c_int64()
         _check(self.lib.orc_get_buffer(self.h, which, ctypes.byref(ptr), ctypes.byref(nb)), "orc_get_buffer")
         dt = np.dtype(_BUF_DTYPES.get(which, self.real))
         if which in (_abi.BUF_ERR_MEAN, _abi.BUF_ERR_MAX, _abi.BUF_TOL_SCALE, _abi.BUF_MAX_SPEED, _abi.BUF_DEN_ERR,
-                     _abi.BUF_DIV_ERR):
+                     _abi.BUF_DIV_ERR, _abi.BUF_EMIT_T):
             dt = np.dtype(self.real)
         count = nb.value // dt.itemsize
+        if count == 0:
+            return np.zeros(0, dt)
         arr = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dt))), (count,))
         return arr
 

@@ -40,7 +40,7 @@ def test_python_binding_covers_the_header():
         assert fn.restype is not None or n == "dfsph_last_error", n
 
 
-@pytest.mark.parametrize("struct", ["DfsphParams", "DfsphLayout", "DfsphModel", "DfsphReport"])
+@pytest.mark.parametrize("struct", ["DfsphParams", "DfsphLayout", "DfsphModel", "DfsphReport", "DfsphEmitters"])
 def test_struct_layout_matches_c(struct):
     cls = getattr(_abi, struct)
     fields = [f[0] for f in cls._fields_]

@@ -79,3 +79,27 @@ def test_unbuilt_subsystems_are_refused(field, value):
     cfg = dataclasses.replace(sc.config, **{field: value})
     with pytest.raises(ValueError, match="not built"):
         SolverDFSPH(sc.model, cfg)
+
+
+def test_auto_escape_box_covers_the_content_with_its_pad():
+    """test_escape_box.py:46-53 of the reference: the plate's bounding sphere and the block, padded by 0.25 m; the
+    ground plane is left out; an empty model has no box."""
+    import numpy as np
+
+    from alloy_amd.sph import escape_box
+    from alloy_amd.sph.build import add_fluid_block, wall_box
+    from alloy_amd.sph.model import GeoType, ModelBuilder
+
+    cfg = DFSPHConfig(particle_radius=2.0e-3)
+    b = ModelBuilder(gravity=(0.0, 0.0, -9.81))
+    b.add_shape(GeoType.PLANE, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    b.begin_world()
+    wall_box(b, (0.0, 0.0, 0.095), (0.05, 0.05, 0.005))
+    add_fluid_block(b, cfg, (-0.008, -0.008, 0.1), (4, 4, 4))
+    b.end_world()
+    m = b.finalize("cpu")
+    lo, hi = escape_box.auto_world_bounds(m)
+    rp = float(np.linalg.norm([0.05, 0.05, 0.005]))
+    assert np.allclose(lo, np.array([-rp, -rp, 0.095 - rp]) - 0.25, atol=1e-6)
+    assert np.allclose(hi, np.array([rp, rp, 0.095 + rp]) + 0.25, atol=1e-6)
+    assert escape_box.auto_world_bounds(ModelBuilder().finalize("cpu")) is None
