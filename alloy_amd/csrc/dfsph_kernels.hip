@@ -90,6 +90,7 @@ struct Ctx {
     const int* wcidx;    // [n] compact index of each particle-wall particle (ascending ids), -1 otherwise
     long long* pw_exch;  // [NW*3] impulse of the exchange window per wall particle, 2^50 N s
     long long* pw_work;  // [W] the walls' work on the liquid in the last step, 2^50 J
+    float* pw_col;       // [n] the particle-wall colour of each liquid particle, from the step's slot pass
     int* cell_fill;
     int* items;
     int* key;
@@ -529,11 +530,14 @@ __global__ void __launch_bounds__(256) k_scatter(Ctx c) {
 }
 
 // ascending ids within each cell: the fixed visit order of the colour sums (wall cells) and a nearly sorted
-// arrival order for the list build (fluid cells)
-__global__ void __launch_bounds__(256) k_sort_cells(Ctx c, int first, int last) {
-    int cc = first + blockIdx.x * blockDim.x + threadIdx.x;
-    if (cc >= last) return;
+// arrival order for the list build (fluid cells). One thread per occupant of the items array; the one at its cell's
+// first slot sorts that cell (empty cells, most of a grid, cost nothing).
+__global__ void __launch_bounds__(256) k_sort_cells(Ctx c) {
+    int a0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a0 >= c.cell_start[c.n_cells]) return;
+    int cc = c.key[c.items[a0]];
     int b = c.cell_start[cc], e = c.cell_start[cc + 1];
+    if (a0 != b) return;
     for (int a = b + 1; a < e; ++a) {
         int v = c.items[a];
         float4 pv = c.q4s[a];
@@ -713,9 +717,7 @@ __global__ void __launch_bounds__(256) k_pw_accumulate(Ctx c) {
                   static_cast<unsigned long long>(llround(double(-dot(J, ld3(c.s_vb, b4 + k))) * IMP)));
         if (c.one_way) continue;
         f3 x = xyz(c.q4[i]);
-        float col;
-        f3 gc, mv;
-        pw_sample(c, x, w, col, gc, mv);
+        float col = c.pw_col[i];  // pw_sample at the same positions in this step's k_slots
         if (col <= 0.0f) continue;
         long long left[3] = {llround(double(J.x) * IMP), llround(double(J.y) * IMP), llround(double(J.z) * IMP)};
         int best = -1;
@@ -881,15 +883,21 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
         R.nrm[k] = f3{0.0f, 0.0f, 0.0f};
     }
     int64_t b4 = 4 * int64_t(i);
-    for (int k = 0; k < 4; ++k) st3(c.s_imp, b4 + k, f3{0.0f, 0.0f, 0.0f});  // _begin_step (solver.py:289-291)
     if (!is_fluid(c, i)) {
+        // A row whose four slots are empty is already clear: its psi, grad psi and impulses are zero (the fill
+        // writes zeros for empty slots, impulses go only to slots with a wall) and nothing writes the row of a
+        // particle that is not liquid. Only a particle that left the liquid with a slot in use is cleared.
+        const int4 sh = *reinterpret_cast<const int4*>(c.s_shape + b4);
+        if (sh.x == -1 && sh.y == -1 && sh.z == -1 && sh.w == -1) return;
         for (int k = 0; k < 4; ++k) {
+            st3(c.s_imp, b4 + k, f3{0.0f, 0.0f, 0.0f});
             c.s_shape[b4 + k] = -1;
             c.s_psi[b4 + k] = 0.0f;
             st3(c.s_gpsi, b4 + k, f3{0.0f, 0.0f, 0.0f});
         }
         return;
     }
+    for (int k = 0; k < 4; ++k) st3(c.s_imp, b4 + k, f3{0.0f, 0.0f, 0.0f});  // _begin_step (solver.py:289-291)
     int w = c.world[i];
     f3 x = xyz(c.q4[i]);
     int nn = 0;
@@ -913,6 +921,7 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
     bool have_pw = false;
     if (c.NW > 0) {
         pw_sample(c, x, w, pcol, pgc, pmv);
+        c.pw_col[i] = pcol;
         have_pw = true;
         float phi = 1.0e30f;
         if (!(pcol < 1.0e-4f || length(pgc) <= 0.0f)) phi = colour_distance(c, fmn(pcol, 1.0f));
@@ -2335,7 +2344,7 @@ int phase_neighbors(Handle* H, hipStream_t st) {
     k_scan_top<<<1, 256, 0, st>>>(c.scan_part, H->n_scan_blocks, c.cell_start, c.n_cells);
     k_scan_apply<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part, c.cell_start, c.cell_fill);
     k_scatter<<<blocks_for(c.n), 256, 0, st>>>(c);
-    k_sort_cells<<<blocks_for(c.n_cells), 256, 0, st>>>(c, 0, c.n_cells);  // ascending ids in every cell
+    k_sort_cells<<<blocks_for(c.n), 256, 0, st>>>(c);  // ascending ids in every cell
     if (c.NW > 0) {
         int64_t nw_cells = int64_t(c.n_cells) - c.wall_first;
         k_fill_u8<<<std::min(blocks_for(nw_cells), 4096), 256, 0, st>>>(c.wall_near, nw_cells);
@@ -2765,6 +2774,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
             TRY(dupload(H, (int**)&c.wcidx, cidx.data(), std::max(n, 1)));
             TRY(dalloc(H, &c.pw_exch, 3 * int64_t(std::max(nw, 1))));
             TRY(dalloc(H, &c.pw_work, W));
+            TRY(dalloc(H, &c.pw_col, nw > 0 ? n : 1));
         }
         H->n_scan_blocks = blocks_for(c.n_cells, 1024);
         TRY(dalloc(H, &c.scan_part, H->n_scan_blocks));
@@ -2950,7 +2960,7 @@ int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void
         int r;
         if ((r = phase_neighbors(H, s))) return r;
         if ((r = phase_slots(H, s))) return r;
-        const bool learned = H->mlp_params || H->log_feat;
+        const bool learned = (H->mlp_params && H->learned_solves != 0) || H->log_feat;
         if ((r = phase_density(H, s, learned ? dtf : 0.0f))) return r;
         if ((r = phase_flow(H, s))) return r;
         if (learned) {  // learned warm start: features (written by the density pass) -> MLP -> initial kappa

@@ -29,7 +29,8 @@ CANDIDATES = (
     (_abi.OPT_GRAPHS, (0, 1)),
 )
 NAMES = {_abi.OPT_CHECK_INTERVAL: "check_interval", _abi.OPT_BUILD_BLOCK: "build_block",
-         _abi.OPT_SPARSE_CORRECTION: "sparse_correction", _abi.OPT_PAIR_CACHE: "pair_cache", _abi.OPT_GRAPHS: "graphs"}
+         _abi.OPT_SPARSE_CORRECTION: "sparse_correction", _abi.OPT_PAIR_CACHE: "pair_cache", _abi.OPT_GRAPHS: "graphs",
+         _abi.OPT_LEARNED_SOLVES: "learned_solves"}
 
 
 class _Snapshot:
@@ -67,10 +68,18 @@ def _window_ms(solver, state, dt: float, warm: int, steps: int) -> float:
 
 
 def autotune(solver, state, dt: float, *, steps: int = 8, warm: int = 2, rounds: int = 2, candidates=CANDIDATES,
-             log: Callable[[str], None] | None = None) -> dict:
+             learned: bool = False, log: Callable[[str], None] | None = None) -> dict:
     """Tune ``solver``'s options on ``state`` in place (the state and the solver come back unchanged) and return
-    {"options": {name: value}, "ms_per_step": {name: {value: median ms}}, "seconds": tuning time}."""
+    {"options": {name: value}, "ms_per_step": {name: {value: median ms}}, "seconds": tuning time}.
+
+    ``learned``: with a learned warm start attached, also time it against the reference's rule (DFSPH_OPT_LEARNED_SOLVES
+    0). That choice is not bit-identical: both starts converge to the same tolerance, but the iterates differ, so the
+    simulation after tuning follows the chosen start (it is restored to the snapshot either way)."""
     t0 = time.perf_counter()
+    if learned and getattr(solver, "_warm_model", None) is not None:
+        cur = solver.get_option(_abi.OPT_LEARNED_SOLVES)
+        if cur:
+            candidates = tuple(candidates) + ((_abi.OPT_LEARNED_SOLVES, (cur, 0)),)
     snap = _Snapshot(solver, state)
     current = {opt: solver.get_option(opt) for opt, _ in candidates}
     table: dict[str, dict] = {}

@@ -131,12 +131,16 @@ class EmitterStruct:
 
 class EmitterManager:
     """Host bookkeeping of one solver's emitters and sinks (runtime.py:197-304); ``upload(struct)`` hands the table
-    to the device (the library, or the oracle in tests)."""
+    to the device (the library, or the oracle in tests). With ``eager`` False a change marks the table dirty and
+    ``flush`` uploads it (the solver flushes before a step or a read of emitter state), so adding thousands of
+    emitters uploads once."""
 
-    def __init__(self, model, config, upload, cursors):
+    def __init__(self, model, config, upload, cursors, eager: bool = True):
         self.model, self.config = model, config
         self._upload_fn = upload  # callable(EmitterStruct)
         self._cursors = cursors   # callable() -> np.ndarray of the device cursors (or zeros)
+        self.eager = eager
+        self.dirty = False
         self.table = EmitterTable()
         b = config.world_bounds
         self.bounds = None if b is None else (tuple(float(v) for v in b[0]), tuple(float(v) for v in b[1]))
@@ -146,16 +150,17 @@ class EmitterManager:
     def active(self) -> bool:
         return bool(self.table.world) or bool(self.table.sink_world) or self.bounds is not None
 
-    def _host(self):
-        m = self.model
-        return (m.sph.phase.cpu().numpy(), m.particle_world.cpu().numpy(), m.particle_flags.cpu().numpy())
-
     def _spares(self, w: int) -> np.ndarray:
-        return spare_ids(*self._host(), w)
+        """Inactive spares of world w, ascending ids: only the world's own slice crosses to the host."""
+        m = self.model
+        a, b = int(m.world_start_np[w]), int(m.world_start_np[w + 1])
+        ph = m.sph.phase[a:b].cpu().numpy()
+        fl = m.particle_flags[a:b].cpu().numpy()
+        return (a + spare_ids(ph, np.full(b - a, w), fl, w)).astype(np.int32)
 
     def _first_sph(self, world: int) -> int:
         ws = self.model.world_start_np
-        ph = self.model.sph.phase.cpu().numpy()[ws[world]:ws[world + 1]]
+        ph = self.model.sph.phase[ws[world]:ws[world + 1]].cpu().numpy()
         idx = np.nonzero((ph >= PHASE_SPARE) | (ph == PHASE_HELD))[0]
         if len(idx) == 0:
             raise ValueError(f"world {world} has no SPH particles or spares")
@@ -219,8 +224,17 @@ class EmitterManager:
         return len(self.table.sink_world) - 1
 
     def upload(self) -> None:
+        if not self.eager:
+            self.dirty = True
+            return
+        self.flush(force=True)
+
+    def flush(self, force: bool = False) -> None:
+        if not (force or self.dirty):
+            return
         self.struct = EmitterStruct(self.table, self.bounds)
         self._upload_fn(self.struct)
+        self.dirty = False
 
     def rewound_emitters(self, world_mask: np.ndarray | None) -> np.ndarray:
         """Indices of the emitters in the masked worlds (all when None): ``emitters.rewind`` sets their clock and

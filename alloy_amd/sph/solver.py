@@ -153,10 +153,10 @@ class SolverDFSPH:
         self._flags0 = model.particle_flags.clone()
         self._phase0 = model.sph.phase.clone()
         W = model.world_count
-        self.emitters = EmitterManager(model, cfg, self._upload_emitters, self._emit_cursors)
+        self.emitters = EmitterManager(model, cfg, self._upload_emitters, self._emit_cursors, eager=False)
         self.pw = ParticleWallReactions(self)
         if self.emitters.active:  # the escape box
-            self.emitters.upload()
+            self.emitters.flush(force=True)
         log.info("SolverDFSPH: %d particles in %d worlds, r=%g m, h=%g m, %d shapes, %d wall particles, solver=%s",
                  model.particle_count, W, cfg.particle_radius, cfg.h, model.shape_count, self.tables.wall_count,
                  self.pressure_solver)
@@ -168,9 +168,11 @@ class SolverDFSPH:
         E = len(self.emitters.table.world) if hasattr(self, "emitters") else 0
         if E == 0 or self.emitters.struct is None or self.emitters.struct.struct.emitter_count == 0:
             return np.zeros(E, np.int32)
-        n_dev = int(self.emitters.struct.struct.emitter_count)
+        n_dev = int(self.emitters.struct.struct.emitter_count)  # the table on the device (a newer one may wait)
         out = np.zeros(E, np.int32)
-        out[:n_dev] = self.buffer(_abi.BUF_EMIT_CURSOR, np.int32, (n_dev,))
+        dev = np.empty(n_dev, np.int32)
+        self._call(self.lib.dfsph_read_buffer, _abi.BUF_EMIT_CURSOR, dev.ctypes.data_as(ctypes.c_void_p), dev.nbytes)
+        out[:n_dev] = dev
         return out
 
     # ------------------------------------------------------------------------------------------------- checks
@@ -262,6 +264,8 @@ class SolverDFSPH:
         if state_out is not state_in:
             state_out.particle_q.copy_(state_in.particle_q)
             state_out.particle_qd.copy_(state_in.particle_qd)
+        if self.emitters.dirty:
+            self.emitters.flush()
         f = state_in.particle_f
         self._call(self.lib.dfsph_step, _dptr(state_out.particle_q), _dptr(state_out.particle_qd), _dptr(f),
                    float(dt), _stream_ptr(self.device), what="SolverDFSPH.step")
@@ -292,6 +296,8 @@ class SolverDFSPH:
     # ------------------------------------------------------------------------------------------------- reports
     def buffer(self, which: int, dtype, shape) -> np.ndarray:
         """Host copy of a solver buffer (tests, diagnostics; syncs)."""
+        if self.emitters.dirty:
+            self.emitters.flush()
         out = np.empty(shape, dtype=dtype)
         self._call(self.lib.dfsph_read_buffer, which, out.ctypes.data_as(ctypes.c_void_p), out.nbytes)
         return out
@@ -406,6 +412,8 @@ class SolverDFSPH:
                 self._write(which, a)
 
     def _write(self, which: int, a: np.ndarray) -> None:
+        if self.emitters.dirty:
+            self.emitters.flush()
         a = np.ascontiguousarray(a)
         self._call(self.lib.dfsph_write_buffer, which, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
 
@@ -562,6 +570,7 @@ class SolverDFSPH:
         n = len(self.emitters.table.world)
         if not 0 <= int(e) < n:
             raise ValueError(f"set_emitter_enabled: emitter {e} does not exist ({n} emitters)")
+        self.emitters.flush()
         self._call(self.lib.dfsph_set_emitter_enabled, int(e), int(bool(on)))
 
     def add_sink(self, world: int, region, half_extents) -> int:

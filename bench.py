@@ -352,7 +352,8 @@ def main() -> int:
     if not args.no_autotune:
         from alloy_amd.sph.autotune import autotune
 
-        tuned = autotune(solver, state, dt, log=_log)
+        tuned = autotune(solver, state, dt, learned=True, log=_log)
+    learned_used = net is not None and solver.get_option(_abi.OPT_LEARNED_SOLVES) != 0
     roof_mask = (1 << _abi.KID_ACCEL) | (1 << _abi.KID_APPLY)
     run = timed_run(main_run, args.steps, dt, prof_mask=roof_mask, sample=args.roofline_sample, count_active=pour)
     total_ms = run["total_ms"]
@@ -365,7 +366,8 @@ def main() -> int:
     value = particle_steps / (total_ms * 1e-3)
     solver_desc = ("batched, one launch per phase over all worlds" if W > 1 else "single world") + (
         f", learned warm start (trained network; solves {'divergence+density' if solves == 3 else 'density' if solves == 2 else 'divergence'})"
-        if net is not None else ", plain warm start")
+        if learned_used else ", the reference's warm-start rule" + (
+            " (the autotuner timed the learned start slower on this pack)" if net is not None else ""))
     out = {
         "metric": METRIC,
         "value": value,
@@ -393,6 +395,7 @@ def main() -> int:
         "kernel_ms_per_step_timed": {k: round(v[1] / args.steps, 4) for k, v in ks.items()},
         "tuning": {"autotuned": tuned is not None,
                    "options": tuned["options"] if tuned else None,
+                   "ms_per_step": tuned["ms_per_step"] if tuned else None,
                    "seconds": round(tuned["seconds"], 2) if tuned else 0.0},
     }
     if args.breakdown_steps > 0:
@@ -408,7 +411,7 @@ def main() -> int:
         _log(f"[bench] fp64 density error {out['fp64_density_err']:.3e}")
     del solver, state, main_run
     torch.cuda.empty_cache()
-    if net is not None and not args.no_rule_compare:
+    if learned_used and not args.no_rule_compare:
         _log("[bench] the same run with the rule warm start")
         if pour:  # the first run emitted from the model's spares: back to the build-time flags and phase
             sc.model.particle_flags.copy_(flags0)
@@ -419,7 +422,8 @@ def main() -> int:
         if args.pair_cache is not None:
             s2.set_option(_abi.OPT_PAIR_CACHE, args.pair_cache)
         for name, v in (tuned["options"].items() if tuned else ()):
-            s2.set_option(OPTION_IDS[name], v)
+            if name in OPTION_IDS:
+                s2.set_option(OPTION_IDS[name], v)
         st2 = sc.model.state()
         run2 = Run(sc, s2, st2)
         for _ in range(args.warmup):

@@ -27,10 +27,10 @@ def _exact(led):
 
 
 def _oracle_for(solver, sc):
-    from alloy_amd.sph import _abi
     from oracle.oracle import OracleDFSPH
 
     o = OracleDFSPH(sc.model, sc.config, 32)
+    solver.emitters.flush()  # the solver uploads its table lazily (at the next step); the oracle needs it now
     if solver.emitters.struct is not None:
         o.set_emitters(solver.emitters.struct)
     return o
