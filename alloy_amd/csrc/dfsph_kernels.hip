@@ -17,6 +17,7 @@
 #include <type_traits>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -91,6 +92,23 @@ struct Ctx {
     long long* pw_exch;  // [NW*3] impulse of the exchange window per wall particle, 2^50 N s
     long long* pw_work;  // [W] the walls' work on the liquid in the last step, 2^50 J
     float* pw_col;       // [n] the particle-wall colour of each liquid particle, from the step's slot pass
+    // Verlet-skin reuse (neighbors_skin.py): superset rows within h + skin at each world's last build
+    float skin;          // 0: off
+    int cap_s;           // superset row capacity (<= 64)
+    uint16_t* sup;       // [cap_s][n]
+    uint8_t* sup_hi;     // [cap_s][n] when wide, else [1]
+    int* sup_count;      // [n]; -1: this step's row was built plain (the superset overflowed)
+    float4* q_ref;       // [n] position at the world's last build
+    int* g_ref;          // [n] group at the world's last build (-2: never built)
+    int* skin_flag;      // [W] the world rebuilds this step
+    int* skin_force;     // [W] the world rebuilds next step
+    // Weiler 2018 implicit viscosity (viscosity.py): per-world preconditioned CG on the velocities
+    const float* mu;     // [n] sph:viscosity [Pa s]
+    float visc_d;        // 2 (dim + 2) / the rest-lattice factor of the operator
+    float *vb, *vx, *vr, *vz, *vp, *vap;  // [3n] right-hand side v*, solution, residual, preconditioned residual, ...
+    float* vminv;        // [9n] inverse of each particle's 3x3 diagonal block
+    int *v_conv, *v_iters;                // [W]
+    double *v_rz, *v_bb, *v_step;         // [W]
     int* cell_fill;
     int* items;
     int* key;
@@ -305,7 +323,8 @@ __device__ __forceinline__ void chunk_reduce(const Ctx& c, double x, double y, d
     }
 }
 
-enum WorldMode { W_NACTIVE = 0, W_CG_INIT, W_CG_PAP, W_CG_BETA, W_JACOBI, W_FLOW_Z, W_FLOW_XY, W_HEAD };
+enum WorldMode { W_NACTIVE = 0, W_CG_INIT, W_CG_PAP, W_CG_BETA, W_JACOBI, W_FLOW_Z, W_FLOW_XY, W_HEAD, W_VISC_BEGIN,
+                 W_VISC_ALPHA, W_VISC_CHECK };
 
 struct StopSpec {
     float tol, tol_max;
@@ -323,6 +342,8 @@ __global__ void __launch_bounds__(256) k_world(Ctx c, int mode, StopSpec sp) {
     int w = blockIdx.x;
     if (mode == W_CG_PAP && w == 0 && threadIdx.x == 0) *c.n_unconv = 0;  // recounted by W_CG_BETA
     if ((mode == W_CG_PAP || mode == W_CG_BETA || mode == W_JACOBI) && c.converged[w] != 0) return;
+    if (mode == W_VISC_ALPHA && w == 0 && threadIdx.x == 0) *c.n_unconv = 0;  // recounted by W_VISC_CHECK
+    if ((mode == W_VISC_ALPHA || mode == W_VISC_CHECK) && c.v_conv[w] != 0) return;
     __shared__ double lds[CHUNK];
     double a = 0.0, b = 0.0, m = -3.4e38;
     for (int k = c.w_chunk[w] + int(threadIdx.x); k < c.w_chunk[w + 1]; k += CHUNK) {
@@ -385,6 +406,35 @@ __global__ void __launch_bounds__(256) k_world(Ctx c, int mode, StopSpec sp) {
             c.head[2 * w] = sa;
             c.head[2 * w + 1] = sb;
             break;
+        case W_VISC_BEGIN:  // viscosity.py:120-132
+            c.v_rz[w] = sa;
+            c.v_bb[w] = sb;
+            c.v_iters[w] = 0;
+            if (c.n_active[w] > 0.0 && sa > 0.0) {
+                c.v_conv[w] = 0;
+                atomicAdd(c.n_unconv, 1);
+            } else {
+                c.v_conv[w] = 1;
+            }
+            break;
+        case W_VISC_ALPHA: {  // viscosity.py:135-145
+            double a = 0.0;
+            if (sa > 1.0e-30) a = c.v_rz[w] / sa;
+            c.v_step[w] = a;
+            break;
+        }
+        case W_VISC_CHECK: {  // viscosity.py:166-185: stop at |r|^2 <= tol^2 |b|^2; beta into v_step
+            int it = c.v_iters[w] + 1;
+            c.v_iters[w] = it;
+            double beta = 0.0;
+            if (c.v_rz[w] > 1.0e-30) beta = sa / c.v_rz[w];
+            c.v_rz[w] = sa;
+            c.v_step[w] = beta;
+            float tol = sp.tol;
+            if (sb <= double(tol * tol) * c.v_bb[w] || it >= sp.max_it) c.v_conv[w] = 1;
+            else atomicAdd(c.n_unconv, 1);
+            break;
+        }
     }
 }
 
@@ -583,23 +633,15 @@ __global__ void k_fill_u8(uint8_t* p, int64_t n) {
 #define DFSPH_BUILD_U 8  // candidate records in flight per lane in the list build (build-time tuning)
 #endif
 constexpr int BUILD_U = DFSPH_BUILD_U;
-// BB: the block size (DFSPH_OPT_BUILD_BLOCK); the row of thread t is row[k * BB + t]
+// BB: the block size (DFSPH_OPT_BUILD_BLOCK); the row of thread t is row[k * BB + t]. One row: the lowest `cap`
+// world-local ids within sqrt(r2) of particle i (group g >= 0), ascending, in LDS; returns (count, candidates).
 template <bool WIDE, int BB>
-__global__ void __launch_bounds__(BB) k_build_lists(Ctx c) {
+__device__ __forceinline__ void build_row(const Ctx& c, int i, int g, f3 xi, float r2, int cap,
+                                          typename std::conditional<WIDE, int, uint16_t>::type* L, int& nn_out,
+                                          int& total_out) {
     using RowT = typename std::conditional<WIDE, int, uint16_t>::type;
-    __shared__ RowT row[64 * BB];
-    int i = xcd_particle();
-    if (i >= c.n) return;
-    int g = c.group[i];
-    if (g < 0) {
-        c.count[i] = 0;
-        return;
-    }
-    RowT* L = row + threadIdx.x;
-    const int cap = c.cap;
+    const float h2 = r2;
     int base = c.ws[g];
-    f3 xi = xyz(c.q4[i]);
-    float h2 = c.h * c.h;
     int cx, cy, cz;
     cell_of(c, g, xi, cx, cy, cz);
     const int* d = c.g_dims + 3 * g;
@@ -646,13 +688,127 @@ __global__ void __launch_bounds__(BB) k_build_lists(Ctx c) {
             }
         }
     }
+    nn_out = nn;
+    total_out = total;
+}
+
+template <bool WIDE, int BB>
+__device__ __forceinline__ void write_row(const Ctx& c, int i, int g, int nn, int total,
+                                          const typename std::conditional<WIDE, int, uint16_t>::type* L) {
     for (int k = 0; k < nn; ++k) {
         int v = L[k * BB];
         c.nbr[int64_t(k) * c.n + i] = uint16_t(v & 65535);
         if (WIDE) c.nbr_hi[int64_t(k) * c.n + i] = uint8_t(v >> 16);
     }
-    if (total > cap) atomicAdd(c.st_overflow + g, 1);
+    if (total > c.cap) atomicAdd(c.st_overflow + g, 1);
     c.count[i] = nn;
+}
+
+// SUP false: the step's rows (neighbors.py:160-195). SUP true (neighbors_skin.py:57-106): the superset rows within
+// h + skin of the worlds flagged this step, each particle's reference position and group stored; a superset that
+// overflows its capacity is replaced by this step's plain row and forces the world's next rebuild.
+template <bool WIDE, int BB, bool SUP>
+__global__ void __launch_bounds__(BB) k_build_lists(Ctx c) {
+    using RowT = typename std::conditional<WIDE, int, uint16_t>::type;
+    __shared__ RowT row[64 * BB];
+    int i = xcd_particle();
+    if (i >= c.n) return;
+    int g = c.group[i];
+    if (SUP) {
+        int w = c.world[i];
+        if (w < 0 || c.skin_flag[w] == 0) return;
+        c.q_ref[i] = c.q4[i];
+        c.g_ref[i] = g;
+        if (g < 0) {
+            c.sup_count[i] = 0;
+            return;
+        }
+    } else if (g < 0) {
+        c.count[i] = 0;
+        return;
+    }
+    RowT* L = row + threadIdx.x;
+    f3 xi = xyz(c.q4[i]);
+    int nn, total;
+    if (!SUP) {
+        build_row<WIDE, BB>(c, i, g, xi, c.h * c.h, c.cap, L, nn, total);
+        write_row<WIDE, BB>(c, i, g, nn, total, L);
+        return;
+    }
+    float hs = c.h + c.skin;
+    build_row<WIDE, BB>(c, i, g, xi, hs * hs, c.cap_s, L, nn, total);
+    if (total <= c.cap_s) {
+        for (int k = 0; k < nn; ++k) {
+            int v = L[k * BB];
+            c.sup[int64_t(k) * c.n + i] = uint16_t(v & 65535);
+            if (WIDE) c.sup_hi[int64_t(k) * c.n + i] = uint8_t(v >> 16);
+        }
+        c.sup_count[i] = nn;
+        return;
+    }
+    build_row<WIDE, BB>(c, i, g, xi, c.h * c.h, c.cap, L, nn, total);
+    write_row<WIDE, BB>(c, i, g, nn, total, L);
+    c.sup_count[i] = -1;
+    c.skin_force[g] = 1;
+}
+
+// neighbors_skin.py:34-42 seed: this step's flags start from the forced worlds
+__global__ void k_skin_seed(Ctx c) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= c.W) return;
+    c.skin_flag[w] = c.skin_force[w];
+    c.skin_force[w] = 0;
+}
+
+// neighbors_skin.py:45-60 check: a world rebuilds when one of its particles changed group or moved more than
+// 0.49 skin since its last build (a NaN move counts)
+__global__ void __launch_bounds__(256) k_skin_check(Ctx c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    int w = c.world[i];
+    if (w < 0) return;
+    int g = c.group[i];
+    bool moved = g != c.g_ref[i];
+    if (!moved && g >= 0) {
+        f3 d = xyz(c.q4[i]) - xyz(c.q_ref[i]);
+        float half = 0.49f * c.skin;
+        moved = !(dot(d, d) <= half * half);
+    }
+    if (moved) c.skin_flag[w] = 1;
+}
+
+// neighbors_skin.py:115-145 filter: the superset entries now within h, in the row's ascending order, the first
+// `cap` of them, overflow counted: the rows, counts and overflow of a plain build at these positions
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_skin_filter(Ctx c) {
+    int i = xcd_particle();
+    if (i >= c.n) return;
+    int g = c.group[i];
+    if (g < 0) {
+        c.count[i] = 0;
+        return;
+    }
+    int ns = c.sup_count[i];
+    if (ns < 0) return;  // a plain row written by this step's build
+    int base = c.ws[g];
+    f3 xi = xyz(c.q4[i]);
+    float h2 = c.h * c.h;
+    int n = 0, total = 0;
+    for (int k = 0; k < ns; ++k) {
+        int jl = int(c.sup[int64_t(k) * c.n + i]);
+        if (WIDE) jl += int(c.sup_hi[int64_t(k) * c.n + i]) << 16;
+        f3 dd = xyz(c.q4[base + jl]) - xi;
+        if (dot(dd, dd) < h2) {
+            total += 1;
+            if (n < c.cap) {
+                c.nbr[int64_t(n) * c.n + i] = uint16_t(jl & 65535);
+                if (WIDE) c.nbr_hi[int64_t(n) * c.n + i] = uint8_t(jl >> 16);
+                n += 1;
+            }
+        }
+    }
+    if (total > c.cap) atomicAdd(c.st_overflow + g, 1);
+    c.count[i] = n;
 }
 
 // ----------------------------------------------------------------------------------------------- particle walls
@@ -1957,6 +2113,155 @@ __global__ void __launch_bounds__(256) k_advect_project(Ctx c, float dt) {
     }
 }
 
+// ----------------------------------------------------------------------------------------------- implicit viscosity
+// Weiler et al. 2018 as in viscosity.py:1-220: A v = v*, (A v)_i = v_i - dt / rho_i sum_j coeff_ij (x_ij . (v_i - v_j))
+// grad W_ij, a per-world CG with a 3x3 block-Jacobi preconditioner and fixed-order float64 dot products.
+struct M33 {
+    float m[9];  // row major
+};
+__device__ __forceinline__ f3 mul33(const M33& a, f3 v) {
+    return f3{a.m[0] * v.x + a.m[1] * v.y + a.m[2] * v.z, a.m[3] * v.x + a.m[4] * v.y + a.m[5] * v.z,
+              a.m[6] * v.x + a.m[7] * v.y + a.m[8] * v.z};
+}
+// the inverse by cofactors over the determinant (expanded along the first row); a singular block gives zeros
+__device__ __forceinline__ M33 inverse33(const M33& a) {
+    const float* m = a.m;
+    float c00 = m[4] * m[8] - m[5] * m[7], c01 = m[5] * m[6] - m[3] * m[8], c02 = m[3] * m[7] - m[4] * m[6];
+    float det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+    M33 r;
+    if (det == 0.0f) {
+        for (int k = 0; k < 9; ++k) r.m[k] = 0.0f;
+        return r;
+    }
+    float inv = 1.0f / det;
+    r.m[0] = c00 * inv;
+    r.m[1] = (m[2] * m[7] - m[1] * m[8]) * inv;
+    r.m[2] = (m[1] * m[5] - m[2] * m[4]) * inv;
+    r.m[3] = c01 * inv;
+    r.m[4] = (m[0] * m[8] - m[2] * m[6]) * inv;
+    r.m[5] = (m[2] * m[3] - m[0] * m[5]) * inv;
+    r.m[6] = c02 * inv;
+    r.m[7] = (m[1] * m[6] - m[0] * m[7]) * inv;
+    r.m[8] = (m[0] * m[4] - m[1] * m[3]) * inv;
+    return r;
+}
+// viscosity.py:39-47 _visc_coeff
+__device__ __forceinline__ float visc_coeff(const Ctx& c, int i, int j, f3 xij) {
+    float rho_j = c.ratio[j] * c.rho0[j];
+    if (rho_j <= 0.0f) return 0.0f;
+    float mu_ij = 0.5f * (c.mu[i] + c.mu[j]);
+    return c.visc_d * mu_ij * (c.mass[j] / rho_j) / (dot(xij, xij) + 0.01f * c.h * c.h);
+}
+
+// viscosity.py:50-71 visc_apply_kernel: out = A vec and the chunk partial of vec . out
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_visc_apply(Ctx c, float dt, const float* vec, float* out) {
+    CHUNK_PROLOGUE
+    int w = c.ch_world[b_];
+    if (c.v_conv[w] != 0) return;
+    double px = 0.0;
+    if (valid) {
+        if (!is_fluid(c, i)) {
+            st3(out, i, f3{0.0f, 0.0f, 0.0f});
+        } else {
+            const KC K = make_kc(c.h, c.kind);
+            f3 xi = xyz(c.q4[i]), vi = ld3(vec, i), a{0.0f, 0.0f, 0.0f};
+            int cnt = c.count[i], base = c.ws[c.world[i]];
+            for (int k = 0; k < cnt; ++k) {
+                int j = nbr_at<WIDE>(c, base, i, k);
+                f3 xij = xi - xyz(c.q4[j]);
+                a = a + (visc_coeff(c, i, j, xij) * dot(xij, vi - ld3(vec, j))) * kernel_grad(xij, K);
+            }
+            float rho_i = fmx(c.ratio[i] * c.rho0[i], 1.0e-6f);
+            f3 r = vi - (dt / rho_i) * a;
+            st3(out, i, r);
+            px = double(dot(vi, r));
+        }
+    }
+    chunk_reduce(c, px, 0.0, valid ? 0.0 : -3.4e38);
+}
+
+// viscosity.py:74-111 visc_init_kernel: the block preconditioner, r = b - A x, z = M^-1 r, p = z; partial (r.z, b.b)
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_visc_init(Ctx c, float dt) {
+    CHUNK_PROLOGUE
+    double px = 0.0, py = 0.0;
+    if (valid) {
+        if (!is_fluid(c, i)) {
+            st3(c.vr, i, f3{0.0f, 0.0f, 0.0f});
+            st3(c.vz, i, f3{0.0f, 0.0f, 0.0f});
+            st3(c.vp, i, f3{0.0f, 0.0f, 0.0f});
+        } else {
+            const KC K = make_kc(c.h, c.kind);
+            f3 xi = xyz(c.q4[i]);
+            float rho_i = fmx(c.ratio[i] * c.rho0[i], 1.0e-6f);
+            M33 blk{{1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f}};
+            int cnt = c.count[i], base = c.ws[c.world[i]];
+            for (int k = 0; k < cnt; ++k) {
+                int j = nbr_at<WIDE>(c, base, i, k);
+                f3 xij = xi - xyz(c.q4[j]);
+                float s = (dt / rho_i) * visc_coeff(c, i, j, xij);
+                f3 g = kernel_grad(xij, K);
+                const float gv[3] = {g.x, g.y, g.z}, xv[3] = {xij.x, xij.y, xij.z};
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) blk.m[3 * r + q] = blk.m[3 * r + q] - s * (gv[r] * xv[q]);
+            }
+            M33 mi = inverse33(blk);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) c.vminv[9 * int64_t(i) + k] = mi.m[k];
+            f3 bi = ld3(c.vb, i);
+            f3 ri = bi - ld3(c.vap, i);
+            f3 zi = mul33(mi, ri);
+            st3(c.vr, i, ri);
+            st3(c.vz, i, zi);
+            st3(c.vp, i, zi);
+            px = double(dot(ri, zi));
+            py = double(dot(bi, bi));
+        }
+    }
+    chunk_reduce(c, px, py, valid ? 0.0 : -3.4e38);
+}
+
+// viscosity.py:148-163 visc_update_kernel: x += a p, r -= a Ap, z = M^-1 r; partial (r.z, r.r)
+__global__ void __launch_bounds__(256) k_visc_update(Ctx c) {
+    CHUNK_PROLOGUE
+    int w = c.ch_world[b_];
+    if (c.v_conv[w] != 0) return;
+    double px = 0.0, py = 0.0;
+    if (valid && is_fluid(c, i)) {
+        float a = float(c.v_step[w]);
+        st3(c.vx, i, ld3(c.vx, i) + a * ld3(c.vp, i));
+        f3 ri = ld3(c.vr, i) - a * ld3(c.vap, i);
+        M33 mi;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) mi.m[k] = c.vminv[9 * int64_t(i) + k];
+        f3 zi = mul33(mi, ri);
+        st3(c.vr, i, ri);
+        st3(c.vz, i, zi);
+        px = double(dot(ri, zi));
+        py = double(dot(ri, ri));
+    }
+    chunk_reduce(c, px, py, valid ? 0.0 : -3.4e38);
+}
+
+// viscosity.py:188-197 visc_direction_kernel; block 0 stores the unconverged-world count for the host
+__global__ void __launch_bounds__(256) k_visc_direction(Ctx c, int* flag_out) {
+    CHUNK_PROLOGUE
+    if (flag_out && blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(flag_out, *c.n_unconv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    int w = c.ch_world[b_];
+    if (c.v_conv[w] != 0 || !valid || !is_fluid(c, i)) return;
+    st3(c.vp, i, ld3(c.vz, i) + float(c.v_step[w]) * ld3(c.vp, i));
+}
+
+// viscosity.py:200-204 visc_store_kernel
+__global__ void __launch_bounds__(256) k_visc_store(Ctx c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < c.n && is_fluid(c, i)) st3(c.qd, i, ld3(c.vx, i));
+}
+
 // ----------------------------------------------------------------------------------------------- emitters
 // Box emitters, sinks, the escape box and the int64 ledgers (emitters.py:32-255, runtime.py:197-409). The tables are
 // device copies of the host's emitter table; per-world ledgers are int64 fixed point, so their atomic sums do not
@@ -2313,9 +2618,9 @@ Handle* as_handle(void* h) { return static_cast<Handle*>(h); }
 
 // ----------------------------------------------------------------------------------------------- phases (host)
 template <bool WIDE, int BB>
-void launch_build_lists_bb(Handle* H, hipStream_t st) {
+void launch_build_lists_bb(Handle* H, hipStream_t st, bool sup) {
     const Ctx& c = H->c;
-    auto kf = k_build_lists<WIDE, BB>;
+    auto kf = sup ? k_build_lists<WIDE, BB, true> : k_build_lists<WIDE, BB, false>;
     hipEvent_t ea, eb;
     unsigned long long* pb = nullptr;
     int grid = blocks_for(c.n, BB);
@@ -2323,14 +2628,14 @@ void launch_build_lists_bb(Handle* H, hipStream_t st) {
     else hipLaunchKernelGGL(kf, dim3(grid), dim3(BB), 0, st, c);
 }
 // the list build with the tuned block size (64, 128 or 256; 256 only with uint16 rows: 64 KB of LDS otherwise)
-void launch_build_lists(Handle* H, hipStream_t st) {
+void launch_build_lists(Handle* H, hipStream_t st, bool sup = false) {
     if (H->c.wide) {
-        if (H->build_block == 64) launch_build_lists_bb<true, 64>(H, st);
-        else launch_build_lists_bb<true, 128>(H, st);
+        if (H->build_block == 64) launch_build_lists_bb<true, 64>(H, st, sup);
+        else launch_build_lists_bb<true, 128>(H, st, sup);
     } else {
-        if (H->build_block == 64) launch_build_lists_bb<false, 64>(H, st);
-        else if (H->build_block == 256) launch_build_lists_bb<false, 256>(H, st);
-        else launch_build_lists_bb<false, 128>(H, st);
+        if (H->build_block == 64) launch_build_lists_bb<false, 64>(H, st, sup);
+        else if (H->build_block == 256) launch_build_lists_bb<false, 256>(H, st, sup);
+        else launch_build_lists_bb<false, 128>(H, st, sup);
     }
 }
 
@@ -2350,7 +2655,15 @@ int phase_neighbors(Handle* H, hipStream_t st) {
         k_fill_u8<<<std::min(blocks_for(nw_cells), 4096), 256, 0, st>>>(c.wall_near, nw_cells);
         k_mark_wall_cells<<<blocks_for(c.n), 256, 0, st>>>(c);
     }
-    launch_build_lists(H, st);
+    if (c.skin > 0.0f) {  // neighbors_skin.py:179-203: flags, the flagged worlds' superset rows, then the filter
+        k_skin_seed<<<blocks_for(c.W), 256, 0, st>>>(c);
+        k_skin_check<<<blocks_for(c.n), 256, 0, st>>>(c);
+        launch_build_lists(H, st, true);
+        if (c.wide) k_skin_filter<true><<<blocks_for(c.n), 256, 0, st>>>(c);
+        else k_skin_filter<false><<<blocks_for(c.n), 256, 0, st>>>(c);
+    } else {
+        launch_build_lists(H, st);
+    }
     LAUNCH_CHECK();
     return DFSPH_OK;
 }
@@ -2585,6 +2898,41 @@ int phase_nonpressure(Handle* H, double dt, hipStream_t st) {
     return DFSPH_OK;
 }
 
+// viscosity.py:257-283 ImplicitViscosity.solve: v* = the velocities after the non-pressure update; CG per world with
+// the same lookahead loop as the pressure solves; the solution replaces the liquid's velocities
+int run_visc(Handle* H, double dt_h, hipStream_t st) {
+    Ctx& c = H->c;
+    if (c.n == 0 || !H->P.viscosity_implicit) return DFSPH_OK;
+    const float dt = float(dt_h);
+    StopSpec sp{float(H->P.visc_tol), 0.0f, 0, H->P.visc_max_iterations, 0};
+    const int64_t n3 = 3 * int64_t(c.n);
+    k_copy_f32<<<blocks_for(n3), 256, 0, st>>>(c.vb, c.qd, n3);
+    k_copy_f32<<<blocks_for(n3), 256, 0, st>>>(c.vx, c.qd, n3);
+    k_fill_i32<<<blocks_for(c.W), 256, 0, st>>>(c.v_conv, 0, c.W);
+    LAUNCH_W(k_visc_apply, H->n_chunks, 256, c, dt, c.vx, c.vap);
+    LAUNCH_W(k_visc_init, H->n_chunks, 256, c, dt);
+    k_fill_i32<<<1, 256, 0, st>>>(c.n_unconv, 0, 1);
+    k_world<<<c.W, 256, 0, st>>>(c, W_VISC_BEGIN, sp);
+    int rc, nu = 0;
+    for (int batch = 0;; ++batch) {
+        int slot = batch & 1;
+        LAUNCH_W(k_visc_apply, H->n_chunks, 256, c, dt, c.vp, c.vap);
+        k_world<<<c.W, 256, 0, st>>>(c, W_VISC_ALPHA, sp);
+        k_visc_update<<<H->n_chunks, 256, 0, st>>>(c);
+        k_world<<<c.W, 256, 0, st>>>(c, W_VISC_CHECK, sp);
+        k_visc_direction<<<H->n_chunks, 256, 0, st>>>(c, H->d_unconv + slot);
+        LAUNCH_CHECK();
+        if ((rc = post_unconv(H, st, slot, true))) return rc;
+        if (batch == 0) continue;
+        if ((rc = wait_unconv(H, slot ^ 1, &nu))) return rc;
+        if (nu == 0) break;
+        if (batch > 100000) return fail(DFSPH_EDEVICE, "viscosity solve did not terminate");
+    }
+    k_visc_store<<<blocks_for(c.n), 256, 0, st>>>(c);
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
 int phase_advect(Handle* H, double dt, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
@@ -2647,6 +2995,9 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
     if (P->neighbor_capacity < 8 || P->neighbor_capacity > 64)
         return fail(DFSPH_EINVAL, "neighbor_capacity must be in [8, 64]");
     if (!(P->h > 0.0)) return fail(DFSPH_EINVAL, "h must be > 0");
+    if (P->skin > 0.0 && L->wall_particle_count > 0)  // neighbors.py:249-254
+        return fail(DFSPH_EINVAL, "SolverDFSPH: particle walls need the grid rebuilt every step (reneighbor_skin off)");
+    if (!(P->skin >= 0.0) || P->skin > P->h) return fail(DFSPH_EINVAL, "reneighbor_skin must be in [0, 1] (x h)");
     const int W = L->world_count, n = L->n, S = L->shape_count;
     if (L->world_start[0] != 0 || L->world_start[W] != n)
         return fail(DFSPH_EINVAL, "world_start must cover every particle (world_start[0] = 0, world_start[W] = n)");
@@ -2671,13 +3022,16 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
     c.S = S;
     c.NW = L->wall_particle_count;
     c.one_way = P->one_way != 0;
+    c.mu = M->viscosity;
+    c.visc_d = float(P->visc_d);
     c.cap = P->neighbor_capacity;
     c.kind = P->kernel;
     c.friction = P->wall_friction;
     c.curvature = P->curvature;
     c.two_sided = P->divergence_two_sided;
     c.h = float(P->h);
-    c.cell = float(P->h * (1.0 + 1.0e-5));
+    c.cell = float((P->h + std::max(P->skin, 0.0)) * (1.0 + 1.0e-5));  // a skin widens the cells to h + skin
+    c.skin = float(std::max(P->skin, 0.0));
     c.r = float(P->particle_radius);
     c.n_full = float(P->n_full);
     c.tot_min = float(P->tot_min);
@@ -2795,6 +3149,27 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         TRY(dalloc(H, &c.nbr, int64_t(c.cap) * std::max(n, 1)));
         TRY(dalloc(H, &c.nbr_hi, c.wide ? int64_t(c.cap) * std::max(n, 1) : 1));
         TRY(dalloc(H, &c.pgrad, int64_t(c.cap) * std::max(n, 1)));
+        if (P->viscosity_implicit) {  // viscosity.py:213-234
+            for (float** p : {&c.vb, &c.vx, &c.vr, &c.vz, &c.vp, &c.vap}) TRY(dalloc(H, p, 3 * int64_t(n)));
+            TRY(dalloc(H, &c.vminv, 9 * int64_t(n)));
+            for (int** p : {&c.v_conv, &c.v_iters}) TRY(dalloc(H, p, W));
+            for (double** p : {&c.v_rz, &c.v_bb, &c.v_step}) TRY(dalloc(H, p, W));
+            if (!M->viscosity) {
+                rc = fail(DFSPH_EINVAL, "the implicit viscosity needs the model's sph:viscosity (DfsphModel.viscosity)");
+                goto fail;
+            }
+        }
+        if (c.skin > 0.0f) {  // Verlet-skin state (neighbors_skin.py:162-177)
+            c.cap_s = P->skin_rows > 0 ? std::min(P->skin_rows, 64) : 64;
+            TRY(dalloc(H, &c.sup, int64_t(c.cap_s) * std::max(n, 1)));
+            TRY(dalloc(H, &c.sup_hi, c.wide ? int64_t(c.cap_s) * std::max(n, 1) : 1));
+            TRY(dalloc(H, &c.sup_count, n));
+            TRY(dalloc(H, &c.q_ref, n));
+            std::vector<int> never(std::max(n, 1), INT_MIN);  // never built: every world builds first
+            TRY(dupload(H, &c.g_ref, never.data(), std::max(n, 1)));
+            TRY(dalloc(H, &c.skin_flag, W));
+            TRY(dalloc(H, &c.skin_force, W));
+        }
         TRY(dalloc(H, &c.count, n));
         TRY(dalloc(H, &c.group, n));
         // slots
@@ -2928,7 +3303,9 @@ int32_t dfsph_nonpressure(void* h, const float* q, float* qd, const float* f, do
     if (!(dt > 0.0)) return fail(DFSPH_EINVAL, "dt must be > 0");
     Handle* H = as_handle(h);
     bind_state(H, q, qd, f);
-    return phase_nonpressure(H, dt, static_cast<hipStream_t>(stream));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int rc = phase_nonpressure(H, dt, st);
+    return rc ? rc : run_visc(H, dt, st);  // solver.py:295-308
 }
 
 int32_t dfsph_advect_project(void* h, float* q, float* qd, double dt, void* stream) {
@@ -2983,6 +3360,7 @@ int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void
         rc = run_segment(H, st, SEG_NONPRESSURE, 0, dtf, [&](hipStream_t s) { return phase_nonpressure(H, dt, s); });
     }
     if (rc) return rc;
+    if ((rc = run_visc(H, dt, st))) return rc;  // solver.py:306-308: part of the non-pressure update
     SolveSetup S1 = solve_setup(H, 1, dt, -1, SolveOverride{});
     rc = run_solve(H, S1, st, SEG_TAIL_DEN, [&](hipStream_t s) {
         if (H->log_targets) k_targets<<<blocks_for(H->c.n), 256, 0, s>>>(H->c, dtf, H->log_targets);
@@ -3020,6 +3398,7 @@ int32_t dfsph_prime(void* h, float* q, float* qd, const float* guess, double dt,
     if ((rc = phase_density(H, st))) return rc;
     k_scale_f32_rev<<<blocks_for(c.n), 256, 0, st>>>(c.kv, guess, float(dt), c.n);
     if ((rc = phase_nonpressure(H, dt, st))) return rc;
+    if ((rc = run_visc(H, dt, st))) return rc;  // prime.py:78: _nonpressure(predict=False)
     SolveOverride ov;
     ov.tol = tol;
     ov.max_it = max_iterations;
@@ -3263,6 +3642,12 @@ int32_t dfsph_get_buffer(void* h, int32_t which, void** ptr, int64_t* nb) {
         case DFSPH_BUF_PHASE: B(c.phase, int64_t(c.n))
         case DFSPH_BUF_PW_EXCH: B(c.pw_exch, 24 * int64_t(std::max(H->n_wall, 1)))
         case DFSPH_BUF_PW_WORK: B(c.pw_work, 8 * W)
+        case DFSPH_BUF_VISC_ITERS:
+            if (!H->P.viscosity_implicit) return fail(DFSPH_EINVAL, "no implicit viscosity");
+            B(c.v_iters, 4 * W)
+        case DFSPH_BUF_SKIN_FLAG:
+            if (c.skin <= 0.0f) return fail(DFSPH_EINVAL, "no Verlet skin");
+            B(c.skin_flag, 4 * W)
         default: return fail(DFSPH_EINVAL, "unknown buffer id");
     }
 #undef B

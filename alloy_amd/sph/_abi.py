@@ -30,7 +30,7 @@ DFSPH_OK, DFSPH_EINVAL, DFSPH_ECAPACITY, DFSPH_EDEVICE = 0, -1, -2, -3
  BUF_ERR_MAX, BUF_N_ACTIVE, BUF_TOL_SCALE, BUF_OVERFLOW, BUF_MAX_SPEED, BUF_ITERS_DEN, BUF_ITERS_DIV, BUF_DEN_ERR,
  BUF_DIV_ERR, BUF_SHAPE_METER, BUF_FEATURES, BUF_MASS_INITIAL, BUF_MASS_IN, BUF_MASS_OUT, BUF_MASS_ACTIVE, BUF_ESCAPED,
  BUF_EMIT_OVERFLOW, BUF_FLUX_E, BUF_FLUX_G, BUF_EMIT_T, BUF_EMIT_CURSOR, BUF_EMIT_ENABLED, BUF_FLAGS,
- BUF_PHASE, BUF_PW_EXCH, BUF_PW_WORK) = range(55)
+ BUF_PHASE, BUF_PW_EXCH, BUF_PW_WORK, BUF_SKIN_FLAG, BUF_VISC_ITERS) = range(57)
 IMP_SCALE = float(2**50)    # impulse fixed point, bodies.py:25
 
 FIX_SCALE = float(2**40)    # mass fixed point, kernels.py:32-33
@@ -75,7 +75,9 @@ class DfsphParams(ctypes.Structure):
         ("tot_min", c_double), ("lattice_grad", c_double), ("compression_relief_speed", c_double),
         ("contact_separation_speed", c_double), ("wall_offset", c_double), ("wall_amplitude", c_double),
         ("taper0", c_double), ("taper1", c_double), ("reach", c_double), ("curvature", c_int),
-        ("project_eps", c_double), ("sweep_len", c_double), ("one_way", c_int),
+        ("project_eps", c_double), ("sweep_len", c_double), ("one_way", c_int), ("skin", c_double),
+        ("skin_rows", c_int), ("viscosity_implicit", c_int), ("visc_d", c_double), ("visc_tol", c_double),
+        ("visc_max_iterations", c_int),
     ]
 
 
@@ -92,7 +94,7 @@ class DfsphLayout(ctypes.Structure):
 class DfsphModel(ctypes.Structure):
     _fields_ = [("mass", ctypes.c_void_p), ("rest_density", ctypes.c_void_p), ("flags", ctypes.c_void_p),
                 ("phase", ctypes.c_void_p), ("world", ctypes.c_void_p), ("gravity", ctypes.c_void_p),
-                ("wall_volume", ctypes.c_void_p)]
+                ("wall_volume", ctypes.c_void_p), ("viscosity", ctypes.c_void_p)]
 
 
 class DfsphReport(ctypes.Structure):
@@ -163,7 +165,27 @@ def params_from_config(cfg: DFSPHConfig, thinnest: float = math.inf) -> DfsphPar
     p.project_eps = PROJECT_EPS * cfg.particle_radius
     p.sweep_len = sweep_length(h, thinnest)
     p.one_way = int(cfg.coupling == "one_way")
+    p.skin = (cfg.reneighbor_skin or 0.0) * h
+    p.skin_rows = skin_capacity(h, sp, p.skin) if p.skin > 0.0 else 0
+    p.viscosity_implicit = 0  # set by the solver when the viscosity resolves to weiler2018
+    p.visc_d = kn.LAPLACE_D / kn.laplacian_lattice_factor(sp, h, kind)
+    p.visc_tol, p.visc_max_iterations = cfg.viscosity_tolerance, cfg.viscosity_max_iterations
     return p
+
+
+SKIN_FILL_MARGIN = 1.75  # superset capacity / interior lattice neighbours within h + skin (neighbors_skin.py:146)
+
+
+def skin_capacity(h: float, spacing: float, skin: float) -> int:
+    """Superset row capacity (neighbors_skin.py:149-158): 1.75 x the neighbours of an interior lattice particle
+    within h + skin, at least 16; this library holds at most 64 (a fuller row is built plain that step and its
+    world rebuilds next step: reuse is lost, never exactness)."""
+    rad = (h + skin) / spacing
+    m = int(math.ceil(rad)) + 1
+    ax = np.arange(-m, m + 1, dtype=np.float64)
+    d2 = ax[:, None, None] ** 2 + ax[None, :, None] ** 2 + ax[None, None, :] ** 2
+    fill = int(np.count_nonzero(d2 < rad * rad)) - 1
+    return min(64, max(16, int(math.ceil(SKIN_FILL_MARGIN * fill))))
 
 
 # ------------------------------------------------------------------------------------------------ layout
@@ -268,7 +290,7 @@ def layout_from_model(model: Model, cfg: DFSPHConfig) -> LayoutTables:
     vol[wall] = (8.0 * radius[wall] ** 3).astype(np.float32)
     q = model.particle_q.cpu().numpy().astype(np.float64)
     per_world = [ids[start[w]:start[w + 1]] for w in range(W)]
-    org, dims = grid_boxes(q, ws, model, per_world, cfg.h)
+    org, dims = grid_boxes(q, ws, model, per_world, cfg.h + (cfg.reneighbor_skin or 0.0) * cfg.h)
     S = model.shape_count
     f32 = lambda a, shape: np.ascontiguousarray(np.asarray(a, np.float32).reshape(shape))  # noqa: E731
     i32 = lambda a: np.ascontiguousarray(np.asarray(a, np.int32).reshape(-1))  # noqa: E731

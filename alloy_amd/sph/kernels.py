@@ -126,6 +126,20 @@ def _lattice_dwdr(spacing: float, h: float, kind: str) -> tuple[np.ndarray, np.n
     return g[sel], r[sel], dw
 
 
+LAPLACE_D = 10.0  # 2 (dim + 2), the Weiler/Monaghan viscous Laplacian's coefficient (viscosity.py:34)
+
+
+def laplacian_lattice_factor(spacing: float, h: float, kind: str = "cubic") -> float:
+    """Discrete / exact ratio of the viscous Laplacian 2 (d + 2) sum_j V (v_ij . x_ij) / (|x_ij|^2 + 0.01 h^2) grad W_ij
+    on the rest lattice (V = 1 / S) for the shear v = (z^2 / 2, 0, 0), whose exact Laplacian is 1 (kernels.py:252-262):
+    the implicit viscosity divides its coefficient by it, so the configured viscosity is the effective one."""
+    g, r, dw = _lattice_dwdr(spacing, h, kind)
+    grad_x = dw * g[:, 0] / r
+    v_ij_x = -0.5 * g[:, 2] ** 2
+    vol = 1.0 / lattice_sum(spacing, h, kind)
+    return float(LAPLACE_D * vol * np.sum(v_ij_x * g[:, 0] / (r**2 + 0.01 * h * h) * grad_x))
+
+
 def lattice_diagonal(spacing: float, h: float, kind: str = "cubic") -> float:
     """Interior rest-lattice DFSPH diagonal sum_j (V |dW_ij|)^2 with V = 1/S [1/m^2] (kernels.py:265-270)."""
     _g, _r, dw = _lattice_dwdr(spacing, h, kind)

@@ -6,9 +6,10 @@ shape meters -> advection and wall projection -> sinks, escape box and box emitt
 current torch stream. Every work buffer is allocated once at construction; ``step`` allocates nothing.
 
 The mass ledger is int64 fixed point on the device (2^40 per kg, runtime.py:93-153), so active + out == initial + in
-holds exactly. Subsystems of the reference that this build does not implement (implicit viscosity, surface tension,
-ambient pressure, soft walls, Verlet skin, bodies attached to shapes and the body-attached patch emitters) are
-refused with an error naming them, rather than silently ignored.
+holds exactly. ``reneighbor_skin`` reuses superset neighbour rows between rebuilds (neighbors_skin.py; the rows stay
+equal to a plain build, bit for bit). Subsystems of the reference that this build does not implement (implicit
+viscosity, surface tension, ambient pressure, soft walls, bodies attached to shapes and the body-attached patch
+emitters) are refused with an error naming them, rather than silently ignored.
 """
 
 from __future__ import annotations
@@ -134,9 +135,6 @@ class SolverDFSPH:
         self._check_layout(model)
         self._check_capacity(capacity_per_world)
         self.viscosity = resolve_viscosity(cfg, float(model.sph.viscosity.max().item()) if model.particle_count else 0.0)
-        if self.viscosity == "weiler2018":
-            raise ValueError("SolverDFSPH: viscosity 'weiler2018' (implicit viscosity) is not built here; "
-                             "use viscosity='xsph' or 'none'")
         phase = model.sph.phase.cpu().numpy()
         mass = model.particle_mass.cpu().numpy()[phase >= 0]
         rho0 = model.sph.rest_density.cpu().numpy()[phase >= 0]
@@ -182,14 +180,17 @@ class SolverDFSPH:
         self.tables = _abi.layout_from_model(model, cfg)
         self._params = _abi.params_from_config(cfg, self.tables.thinnest)
         self._params.pressure_solver = 0 if self.pressure_solver == "cg" else 1
-        if self.viscosity == "none":
+        if self.viscosity != "xsph":  # solver.py:299: XSPH only with viscosity "xsph"
             self._params.xsph_c = 0.0
+        self._params.viscosity_implicit = int(self.viscosity == "weiler2018")
+        self._mu = model.sph.viscosity.to(torch.float32).contiguous()
         self._wall_volume = torch.tensor(self.tables.wall_volume, device=self.device)
         self._gravity = model.gravity.reshape(-1).contiguous().clone()
         m = _abi.DfsphModel()
         m.mass, m.rest_density = _dptr(model.particle_mass), _dptr(model.sph.rest_density)
         m.flags, m.phase, m.world = _dptr(model.particle_flags), _dptr(model.sph.phase), _dptr(model.particle_world)
         m.gravity, m.wall_volume = _dptr(self._gravity), _dptr(self._wall_volume)
+        m.viscosity = _dptr(self._mu)
         self._model_struct = m
         layout = self.tables.struct(model.particle_count)
         h = ctypes.c_void_p()
@@ -202,9 +203,7 @@ class SolverDFSPH:
         unbuilt = [
             (cfg.surface_tension != "none", "surface_tension"),
             (cfg.ambient_pressure is not None, "ambient_pressure"),
-            (cfg.reneighbor_skin is not None, "reneighbor_skin"),
             (cfg.boundary_viscosity is not None and cfg.boundary_viscosity > 0.0, "boundary_viscosity"),
-            (cfg.viscosity == "weiler2018", "viscosity='weiler2018'"),
         ]
         for bad, name in unbuilt:
             if bad:

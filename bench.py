@@ -49,7 +49,8 @@ WORKLOADS = {  # steps: the timed steps each workload is specified with
     "pour_worlds_2048": dict(kind="pour_worlds", worlds=2048, seed=4, steps=1000),
 }
 CPU_SAMPLE_PARTICLE_STEPS = 3.0e6  # about 20 s of the serial oracle
-OPTION_IDS = {"check_interval": 1, "graphs": 2, "pair_cache": 3, "sparse_correction": 5, "build_block": 6}
+OPTION_IDS = {"check_interval": 1, "graphs": 2, "pair_cache": 3, "learned_solves": 4, "sparse_correction": 5,
+              "build_block": 6}
 
 
 def make_scene(wl: dict, device: str, subset=None):
@@ -291,6 +292,9 @@ def main() -> int:
                          "result bit)")
     ap.add_argument("--graphs", action="store_true", help="replay the step's segments as hipGraphs")
     ap.add_argument("--pair-cache", type=int, default=None, help="DFSPH_PAIR_CACHE_* mask (default: the library's)")
+    ap.add_argument("--options", default="",
+                    help="name=value,... of library options set before the warm-up (e.g. the autotuner's choice, to "
+                         "profile exactly the timed configuration with --no-autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=None,
                     help="timed oracle steps (default: about 3e6 particle-steps of the sample)")
@@ -339,6 +343,9 @@ def main() -> int:
     if net is not None:
         net.attach(solver)
         solver.set_option(_abi.OPT_LEARNED_SOLVES, solves)
+    for item in filter(None, args.options.split(",")):
+        name, v = item.split("=")
+        solver.set_option(OPTION_IDS[name.strip()], int(v))
     state = sc.model.state()
     pour = wl["kind"] == "pour_worlds"
     n_fluid = sum(sc.fluid_counts)

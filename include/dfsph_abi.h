@@ -74,6 +74,13 @@ typedef struct DfsphParams {
     double sweep_len;                /* integrate.sweep_length(h, thinnest) */
     int32_t one_way;                 /* coupling == "one_way": no shape meters, no particle-wall reactions
                                         (bodies.py:58-59, particle_walls.py:156-157) */
+    double skin;                     /* Verlet skin [m] = reneighbor_skin x h, 0 off (neighbors_skin.py); the cell
+                                        grid's side is then h + skin */
+    int32_t skin_rows;               /* superset row capacity (skin_capacity, neighbors_skin.py:149-158), at most 64 */
+    int32_t viscosity_implicit;      /* 1: Weiler 2018 implicit viscosity after the non-pressure update (viscosity.py) */
+    double visc_d;                   /* LAPLACE_D / laplacian_lattice_factor (solver.py:157-160) */
+    double visc_tol;                 /* viscosity_tolerance: stop at |r| <= tol |v*| per world */
+    int32_t visc_max_iterations;
 } DfsphParams;
 
 /* Static description of the particle pack and its walls (layout.py:47-72, boundary.py:79-95). */
@@ -106,6 +113,7 @@ typedef struct DfsphModel {
     const int32_t* world;            /* particle_world */
     const float* gravity;            /* [W*3] */
     const float* wall_volume;        /* V_j = 8 r^3 of particle-wall particles, 0 otherwise (particle_walls.py:227) */
+    const float* viscosity;          /* sph:viscosity [Pa s] (read by the implicit viscosity; may be NULL without it) */
 } DfsphModel;
 
 /* Host copy of the per-world report (types.py:11-28 SPHReport); arrays of length W owned by the caller. */
@@ -184,6 +192,8 @@ enum DfsphBuffer {
     DFSPH_BUF_PW_EXCH,        /* int64  [NW, 3] particle-wall impulse of the exchange window, 2^50 per N s, rows in
                                  ascending id order of the wall particles (particle_walls.py:138-177) */
     DFSPH_BUF_PW_WORK,        /* int64  [W] the particle walls' work on the liquid in the last step, 2^50 per J */
+    DFSPH_BUF_SKIN_FLAG,      /* int32  [W] worlds whose Verlet-skin rows were rebuilt in the last list build */
+    DFSPH_BUF_VISC_ITERS,     /* int32  [W] iterations of the last implicit viscosity solve */
     DFSPH_BUF_COUNT_
 };
 

@@ -225,6 +225,13 @@ struct Oracle {
     T offset, amplitude, taper0, taper1, reach, eps, sweep_len, lattice_grad;
     double relief_speed, rest_speed, flow_ratio, tol_den, tol_div, tol_min, err_factor, lattice_grad_d;
     int kind, solver, min_it, max_it, cap, div_solve, two_sided, friction, curvature, one_way = 0;
+    // Weiler 2018 implicit viscosity (viscosity.py)
+    int visc_on = 0, visc_max_it = 0;
+    T visc_d = T(0);
+    float visc_tol = 0.0f;
+    std::vector<T> mu, vb, vx, vr, vz, vp, vap, vminv;
+    std::vector<int> v_conv, v_iters;
+    std::vector<double> v_rz, v_bb, v_step;
 
     int n = 0, W = 0, S = 0, NW = 0;  // NW: wall particles
     std::vector<int> ws;              // world start [W+1]
@@ -317,6 +324,10 @@ struct Oracle {
         eps = T(P->project_eps);
         sweep_len = T(P->sweep_len);
         one_way = P->one_way != 0;
+        visc_on = P->viscosity_implicit != 0;
+        visc_d = T(P->visc_d);
+        visc_tol = float(P->visc_tol);
+        visc_max_it = P->visc_max_iterations;
         lattice_grad = T(P->lattice_grad);
         lattice_grad_d = P->lattice_grad;
         relief_speed = P->compression_relief_speed;
@@ -374,7 +385,7 @@ struct Oracle {
             sh_mu[s] = T(L->shape_mu[s]);
         }
         // grid: fluid cells of every world, then wall cells of every world
-        cell = T(P->h * (1.0 + 1.0e-5));
+        cell = T((P->h + std::max(P->skin, 0.0)) * (1.0 + 1.0e-5));  // the product's grid (wider with a skin)
         g_org.resize(3 * W);
         g_dims.resize(3 * W);
         cell_base.resize(2 * W + 1);
@@ -420,6 +431,16 @@ struct Oracle {
         for (auto* v : {&sums, &flow_z, &flow_xy, &head}) v->assign(2 * W, 0.0);
         meter.assign(3 * size_t(std::max(S, 1)), 0);
         for (auto* v : {&mass0, &mass_in, &mass_out, &mass_act, &flux_e, &flux_g}) v->assign(W, 0);
+        mu.assign(std::max(n, 1), T(0));
+        if (M->viscosity)
+            for (int i = 0; i < n; ++i) mu[i] = T(M->viscosity[i]);
+        if (visc_on) {
+            for (auto* v : {&vb, &vx, &vr, &vz, &vp, &vap}) v->assign(3 * size_t(std::max(n, 1)), T(0));
+            vminv.assign(9 * size_t(std::max(n, 1)), T(0));
+            v_conv.assign(W, 0);
+            v_iters.assign(W, 0);
+            for (auto* v : {&v_rz, &v_bb, &v_step}) v->assign(W, 0.0);
+        }
         wcidx.assign(std::max(n, 1), -1);
         int nw = 0;
         for (int i = 0; i < n; ++i)
@@ -1362,6 +1383,144 @@ struct Oracle {
             }
     }
 
+    // ---------------------------------------------------------------------------- implicit viscosity (viscosity.py)
+    T visc_coeff(int i, int j, V3<T> xij) const {  // viscosity.py:39-47
+        T rho_j = ratio[j] * rho0[j];
+        if (rho_j <= T(0)) return T(0);
+        T mu_ij = T(0.5) * (mu[i] + mu[j]);
+        return visc_d * mu_ij * (mass[j] / rho_j) / (dot(xij, xij) + T(0.01) * h * h);
+    }
+    static V3<T> mul33(const T* m, V3<T> v) {
+        return {m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+                m[6] * v.x + m[7] * v.y + m[8] * v.z};
+    }
+    // the inverse by cofactors over the determinant (expanded along the first row); a singular block gives zeros
+    static void inverse33(const T* m, T* r) {
+        T c00 = m[4] * m[8] - m[5] * m[7], c01 = m[5] * m[6] - m[3] * m[8], c02 = m[3] * m[7] - m[4] * m[6];
+        T det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+        if (det == T(0)) {
+            for (int k = 0; k < 9; ++k) r[k] = T(0);
+            return;
+        }
+        T inv = T(1) / det;
+        r[0] = c00 * inv;
+        r[1] = (m[2] * m[7] - m[1] * m[8]) * inv;
+        r[2] = (m[1] * m[5] - m[2] * m[4]) * inv;
+        r[3] = c01 * inv;
+        r[4] = (m[0] * m[8] - m[2] * m[6]) * inv;
+        r[5] = (m[2] * m[3] - m[0] * m[5]) * inv;
+        r[6] = c02 * inv;
+        r[7] = (m[1] * m[6] - m[0] * m[7]) * inv;
+        r[8] = (m[0] * m[4] - m[1] * m[3]) * inv;
+    }
+    T rho_i_of(int i) const { return tmax(ratio[i] * rho0[i], T(1.0e-6)); }
+    // viscosity.py:50-71: out = A vec, vals[2i] = vec_i . out_i (converged worlds untouched)
+    void visc_apply(T dt, const std::vector<T>& vec, std::vector<T>& out, std::vector<T>& vals) {
+        for (int i = 0; i < n; ++i) {
+            vals[2 * i] = vals[2 * i + 1] = T(0);
+            if (v_conv[world[i] < 0 ? 0 : world[i]] != 0 && world[i] >= 0) continue;
+            if (!is_fluid(i)) {
+                put3(out, i, V3<T>{T(0), T(0), T(0)});
+                continue;
+            }
+            V3<T> xi = qv(i), vi = get3(vec, i), a{T(0), T(0), T(0)};
+            for (int k = 0; k < count[i]; ++k) {
+                int j = nbr_j(i, k);
+                V3<T> xij = xi - qv(j);
+                a = a + (visc_coeff(i, j, xij) * dot(xij, vi - get3(vec, j))) * kernel_grad(xij, h, kind);
+            }
+            V3<T> r = vi - (dt / rho_i_of(i)) * a;
+            put3(out, i, r);
+            vals[2 * i] = dot(vi, r);
+        }
+    }
+    void visc_solve(T dt) {  // viscosity.py:243-283
+        std::vector<T> vals(2 * size_t(std::max(n, 1)), T(0));
+        std::vector<double> sums(2 * size_t(W));
+        std::vector<T> mx(W);
+        for (int i = 0; i < n; ++i) {
+            put3(vb, i, vv(i));
+            put3(vx, i, vv(i));
+        }
+        std::fill(v_conv.begin(), v_conv.end(), 0);
+        visc_apply(dt, vx, vap, vals);
+        for (int i = 0; i < n; ++i) {  // visc_init_kernel
+            vals[2 * i] = vals[2 * i + 1] = T(0);
+            if (!is_fluid(i)) {
+                for (auto* v : {&vr, &vz, &vp}) put3(*v, i, V3<T>{T(0), T(0), T(0)});
+                continue;
+            }
+            V3<T> xi = qv(i);
+            T rho_i = rho_i_of(i);
+            T blk[9] = {T(1), T(0), T(0), T(0), T(1), T(0), T(0), T(0), T(1)};
+            for (int k = 0; k < count[i]; ++k) {
+                int j = nbr_j(i, k);
+                V3<T> xij = xi - qv(j);
+                T s = (dt / rho_i) * visc_coeff(i, j, xij);
+                V3<T> g = kernel_grad(xij, h, kind);
+                const T gv[3] = {g.x, g.y, g.z}, xv[3] = {xij.x, xij.y, xij.z};
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q) blk[3 * r + q] = blk[3 * r + q] - s * (gv[r] * xv[q]);
+            }
+            inverse33(blk, &vminv[9 * size_t(i)]);
+            V3<T> bi = get3(vb, i), ri = bi - get3(vap, i);
+            V3<T> zi = mul33(&vminv[9 * size_t(i)], ri);
+            put3(vr, i, ri);
+            put3(vz, i, zi);
+            put3(vp, i, zi);
+            vals[2 * i] = dot(ri, zi);
+            vals[2 * i + 1] = dot(bi, bi);
+        }
+        reduce(vals, 2, sums, mx.data());
+        int unconv = 0;
+        for (int w = 0; w < W; ++w) {  // visc_begin_kernel
+            v_rz[w] = sums[2 * w];
+            v_bb[w] = sums[2 * w + 1];
+            v_iters[w] = 0;
+            v_conv[w] = (n_active[w] > 0.0 && sums[2 * w] > 0.0) ? 0 : 1;
+            unconv += v_conv[w] == 0;
+        }
+        while (unconv > 0) {
+            visc_apply(dt, vp, vap, vals);
+            reduce(vals, 2, sums, mx.data());
+            for (int w = 0; w < W; ++w) {  // visc_alpha_kernel
+                if (v_conv[w]) continue;
+                double a = 0.0;
+                if (sums[2 * w] > 1.0e-30) a = v_rz[w] / sums[2 * w];
+                v_step[w] = a;
+            }
+            for (int i = 0; i < n; ++i) {  // visc_update_kernel
+                vals[2 * i] = vals[2 * i + 1] = T(0);
+                if (!is_fluid(i) || v_conv[world[i]]) continue;
+                T a = T(v_step[world[i]]);
+                put3(vx, i, get3(vx, i) + a * get3(vp, i));
+                V3<T> ri = get3(vr, i) - a * get3(vap, i);
+                V3<T> zi = mul33(&vminv[9 * size_t(i)], ri);
+                put3(vr, i, ri);
+                put3(vz, i, zi);
+                vals[2 * i] = dot(ri, zi);
+                vals[2 * i + 1] = dot(ri, ri);
+            }
+            reduce(vals, 2, sums, mx.data());
+            unconv = 0;
+            for (int w = 0; w < W; ++w) {  // visc_check_kernel
+                if (v_conv[w]) continue;
+                int it = v_iters[w] + 1;
+                v_iters[w] = it;
+                double beta = 0.0;
+                if (v_rz[w] > 1.0e-30) beta = sums[2 * w] / v_rz[w];
+                v_rz[w] = sums[2 * w];
+                v_step[w] = beta;
+                if (sums[2 * w + 1] <= double(visc_tol * visc_tol) * v_bb[w] || it >= visc_max_it) v_conv[w] = 1;
+                else unconv += 1;
+            }
+            for (int i = 0; i < n; ++i)  // visc_direction_kernel
+                if (is_fluid(i) && !v_conv[world[i]]) put3(vp, i, get3(vz, i) + T(v_step[world[i]]) * get3(vp, i));
+        }
+        for (int i = 0; i < n; ++i)  // visc_store_kernel
+            if (is_fluid(i)) setv(i, get3(vx, i));
+    }
+
     // particle_walls.py:138-177 _accumulate_kernel (impulse split over the wall particles, the walls' work)
     void pw_accumulate() {
         std::fill(pw_work.begin(), pw_work.end(), 0);
@@ -1770,6 +1929,7 @@ struct Oracle {
         min_nb = T(0);
         err_factor = 1.0e9;
         nonpressure(dt);
+        if (visc_on) visc_solve(dt);
         solve(0, dt_host, -1);
         tol_div = s_tol;
         err_factor = s_fac;
@@ -1795,6 +1955,7 @@ struct Oracle {
         flow_speed();
         if (div_solve) solve(0, dt_host, -1, warm_div_k);
         nonpressure(dt);
+        if (visc_on) visc_solve(dt);  // solver.py:306-308
         solve(1, dt_host, -1, warm_den_k);
         meters();
         pw_accumulate();
@@ -1869,6 +2030,7 @@ int get_buffer_t(Oracle<T>* o, int which, void** ptr, int64_t* nb) {
         case DFSPH_BUF_PHASE: OB(phase)
         case DFSPH_BUF_PW_EXCH: OB(pw_exch)
         case DFSPH_BUF_PW_WORK: OB(pw_work)
+        case DFSPH_BUF_VISC_ITERS: OB(v_iters)
         default: return fail(DFSPH_EINVAL, "unknown buffer id");
     }
 #undef OB
@@ -1984,6 +2146,7 @@ int32_t orc_nonpressure(void* h, void* q, void* qd, const void* f, double dt) {
         using TT = std::remove_pointer_t<decltype(o->q)>;
         bind(o, q, qd, f);
         o->nonpressure(TT(dt));
+        if (o->visc_on) o->visc_solve(TT(dt));
     });
     return DFSPH_OK;
 }

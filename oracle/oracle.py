@@ -94,7 +94,8 @@ _BUF_DTYPES = {
 class OracleDFSPH:
     """The CPU oracle for one model: ``precision`` 32 (bitwise parity with the HIP path) or 64 (FP64 reference)."""
 
-    def __init__(self, model: Model, cfg: DFSPHConfig, precision: int = 32, pressure_solver: str | None = None):
+    def __init__(self, model: Model, cfg: DFSPHConfig, precision: int = 32, pressure_solver: str | None = None,
+                 viscosity: str | None = None):
         self.lib = load()
         self.model, self.cfg, self.precision = model, cfg, precision
         self.real = np.float64 if precision == 64 else np.float32
@@ -103,18 +104,23 @@ class OracleDFSPH:
         self.params = _abi.params_from_config(cfg, self.tables.thinnest)
         if pressure_solver is not None:
             self.params.pressure_solver = 0 if pressure_solver == "cg" else 1
-        if cfg.viscosity == "none":
+        from alloy_amd.sph.config import resolve_viscosity
+
+        visc = viscosity or resolve_viscosity(cfg, float(model.sph.viscosity.max().item()) if self.n else 0.0)
+        if visc != "xsph":
             self.params.xsph_c = 0.0
+        self.params.viscosity_implicit = int(visc == "weiler2018")
         host = lambda t, dt: np.ascontiguousarray(t.detach().cpu().numpy().astype(dt))  # noqa: E731
         self._keep = dict(
             mass=host(model.particle_mass, np.float32), rho0=host(model.sph.rest_density, np.float32),
             flags=host(model.particle_flags, np.int32), phase=host(model.sph.phase, np.int8),
             world=host(model.particle_world, np.int32), gravity=host(model.gravity.reshape(-1), np.float32),
-            wvol=np.ascontiguousarray(self.tables.wall_volume, np.float32))
+            wvol=np.ascontiguousarray(self.tables.wall_volume, np.float32),
+            mu=host(model.sph.viscosity, np.float32))
         k = self._keep
         m = _abi.DfsphModel()
         for field_, key in (("mass", "mass"), ("rest_density", "rho0"), ("flags", "flags"), ("phase", "phase"),
-                            ("world", "world"), ("gravity", "gravity"), ("wall_volume", "wvol")):
+                            ("world", "world"), ("gravity", "gravity"), ("wall_volume", "wvol"), ("viscosity", "mu")):
             setattr(m, field_, k[key].ctypes.data_as(ctypes.c_void_p))
         layout = self.tables.struct(self.n)
         h = ctypes.c_void_p()

@@ -71,7 +71,7 @@ def test_solver_has_no_cpu_path():
 
 
 @pytest.mark.parametrize("field,value", [("surface_tension", "csf"), ("ambient_pressure", 100.0),
-                                         ("reneighbor_skin", 0.1), ("viscosity", "weiler2018")])
+                                         ("boundary_viscosity", 0.1)])
 def test_unbuilt_subsystems_are_refused(field, value):
     from alloy_amd.sph.solver import SolverDFSPH
 
@@ -103,3 +103,25 @@ def test_auto_escape_box_covers_the_content_with_its_pad():
     assert np.allclose(lo, np.array([-rp, -rp, 0.095 - rp]) - 0.25, atol=1e-6)
     assert np.allclose(hi, np.array([rp, rp, 0.095 + rp]) + 0.25, atol=1e-6)
     assert escape_box.auto_world_bounds(ModelBuilder().finalize("cpu")) is None
+
+
+@pytest.mark.parametrize("skin, ok", [(None, True), (0.0, True), (0.1, True), (-0.1, False), (1.5, False)])
+def test_skin_config_range(skin, ok):
+    """test_neighbors_skin.py:97-105 of the reference."""
+    cfg = DFSPHConfig(reneighbor_skin=skin)
+    if ok:
+        cfg.validate()
+    else:
+        with pytest.raises(ValueError, match="reneighbor_skin"):
+            cfg.validate()
+
+
+def test_skin_capacity_is_the_padded_rest_fill():
+    """neighbors_skin.py:146-158: 1.75 x the interior lattice neighbours within h + skin (h = 2 spacings: the 26
+    interior neighbours of config.py:230-236; 32 within 2.2 spacings), at least 16 and, here, at most 64."""
+    from alloy_amd.sph._abi import skin_capacity
+
+    assert skin_capacity(2.0, 1.0, 0.0) == 46        # ceil(1.75 x 26)
+    assert skin_capacity(2.0, 1.0, 0.2) == 56        # ceil(1.75 x 32)
+    assert skin_capacity(2.0, 1.0, 2.0) == 64        # 4 spacings: ceil(1.75 x 250) held to 64
+    assert skin_capacity(0.5, 1.0, 0.0) == 16        # no lattice neighbour within half a spacing
