@@ -87,7 +87,7 @@ _BUF_DTYPES = {
     _abi.BUF_MASS_IN: np.int64, _abi.BUF_MASS_OUT: np.int64, _abi.BUF_MASS_ACTIVE: np.int64, _abi.BUF_ESCAPED: np.int32,
     _abi.BUF_EMIT_OVERFLOW: np.int32, _abi.BUF_FLUX_E: np.int64, _abi.BUF_FLUX_G: np.int64,
     _abi.BUF_EMIT_CURSOR: np.int32, _abi.BUF_EMIT_ENABLED: np.int32, _abi.BUF_FLAGS: np.int32, _abi.BUF_PHASE: np.int8,
-    _abi.BUF_PW_EXCH: np.int64, _abi.BUF_PW_WORK: np.int64,
+    _abi.BUF_PW_EXCH: np.int64, _abi.BUF_PW_WORK: np.int64, _abi.BUF_VISC_ITERS: np.int32,
 }
 
 

@@ -70,13 +70,13 @@ def test_solver_has_no_cpu_path():
         SolverDFSPH(sc.model, sc.config)
 
 
-@pytest.mark.parametrize("field,value", [("surface_tension", "csf"), ("ambient_pressure", 100.0),
-                                         ("boundary_viscosity", 0.1)])
-def test_unbuilt_subsystems_are_refused(field, value):
+@pytest.mark.parametrize("overrides", [{"surface_tension": "csf"}, {"ambient_pressure": 100.0},
+                                       {"viscosity": "weiler2018", "boundary_viscosity": 0.1}])
+def test_unbuilt_subsystems_are_refused(overrides):
     from alloy_amd.sph.solver import SolverDFSPH
 
     sc = dam_break(4, 0.1)
-    cfg = dataclasses.replace(sc.config, **{field: value})
+    cfg = dataclasses.replace(sc.config, **overrides)
     with pytest.raises(ValueError, match="not built"):
         SolverDFSPH(sc.model, cfg)
 
