@@ -109,6 +109,11 @@ struct Ctx {
     float* vminv;        // [9n] inverse of each particle's 3x3 diagonal block
     int *v_conv, *v_iters;                // [W]
     double *v_rz, *v_bb, *v_step;         // [W]
+    // its wall term (wall_viscosity.py), present when wall_coeff > 0
+    float wall_coeff, mu_b;               // mu_b x wall_coefficient_scale; mu_b [Pa s]
+    float* wall_m;                        // [9n] sum over the slots of the wall block M_ik
+    float* wall_w;                        // [3n] sum over the slots of w_ik = M_ik v_b
+    float *vshift, *vraw, *vgk;           // [3n], [3n], [n] the pressure guess's wall-velocity shift and its inputs
     int* cell_fill;
     int* items;
     int* key;
@@ -2153,6 +2158,168 @@ __device__ __forceinline__ float visc_coeff(const Ctx& c, int i, int j, f3 xij) 
     return c.visc_d * mu_ij * (c.mass[j] / rho_j) / (dot(xij, xij) + 0.01f * c.h * c.h);
 }
 
+// wall_viscosity.py:96-105 _tangents: t1, t2 orthonormal to the unit vector n
+__device__ __forceinline__ void wall_tangents(f3 n, f3& t1, f3& t2) {
+    f3 a{1.0f, 0.0f, 0.0f};
+    if (fabsf(n.x) > 0.6f) a = f3{0.0f, 1.0f, 0.0f};
+    t1 = normalize(cross(n, a));
+    t2 = cross(n, t1);
+}
+
+// wall_viscosity.py:108-150 slot_pair: the wall block M and w = M v_b of one slot (false for an empty slot or one
+// whose term vanishes). The normal block comes from four samples around x_k = x_i - d n; the tangential block is the
+// calibrated no-slip stress, faded out from 2r to 3r; v_b is the slot's wall velocity plus the particle's shift.
+__device__ __forceinline__ bool wall_slot_pair(const Ctx& c, int64_t slot, f3 xi, f3 shift, M33& m, f3& w) {
+    if (c.s_shape[slot] < 0) return false;
+    const KC K = make_kc(c.h, c.kind);
+    const float r = c.r, phi = c.s_phi[slot];
+    const float d = fmx(phi + 0.5f * r, 2.0f * r);  // TimeStep.cpp:293 of SPlisHSPlasH
+    const float wd = kernel_w(d, K);
+    f3 n = xi - ld3(c.s_xb, slot);
+    const float ln = length(n);
+    float psi = c.s_psi[slot];
+    if (phi > c.taper0) {  // the untapered half-space volume
+        float v0, v1;
+        halfspace_volume(phi + c.offset, c.h, 0.0f, c.kind, v0, v1);
+        psi = v0 * c.amplitude;
+    }
+    if (wd <= 0.0f || psi <= 0.0f || ln < 1.0e-9f) return false;
+    n = n / ln;
+    const f3 xk = xi - d * n;
+    f3 t1, t2;
+    wall_tangents(n, t1, t2);
+    const float vol = 0.25f * psi / wd;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m.m[k] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float sg = 1.0f - 2.0f * float(q % 2);
+        const f3 xs = xk + (sg * 0.5f * c.h) * (q < 2 ? t1 : t2);
+        const f3 xis = xi - xs;
+        const f3 g = kernel_grad(xis, K);
+        const float cc = c.wall_coeff * vol / (dot(xis, xis) + 0.01f * c.h * c.h);
+        const float gv[3] = {g.x, g.y, g.z}, xv[3] = {xis.x, xis.y, xis.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) m.m[3 * a + b] = m.m[3 * a + b] + cc * (gv[a] * xv[b]);
+    }
+    const float u = fcl((phi - 2.0f * r) / r, 0.0f, 1.0f);
+    const float m_tt = -c.mu_b / (fmx(phi, r) * 2.0f * r) * (1.0f - u * u * (3.0f - 2.0f * u));
+    const float mn = dot(n, mul33(m, n));
+    const float nv[3] = {n.x, n.y, n.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float nn = nv[a] * nv[b];
+            m.m[3 * a + b] = mn * nn + m_tt * ((a == b ? 1.0f : 0.0f) - nn);
+        }
+    w = mul33(m, ld3(c.s_vb, slot) + shift);
+    return true;
+}
+
+// viscosity.py:235-243 pressure_guess_kernel (warm 1): the density solve's stored pressure as this step's guess
+__global__ void __launch_bounds__(256) k_visc_guess(Ctx c, float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    c.vgk[i] = is_fluid(c, i) ? fmn(c.kd[i], 2.5e-4f) / (dt * dt) : 0.0f;
+}
+
+// viscosity.py:246-258 guess_shift_kernel: raw = -dt a, a the guess's pressure acceleration (pressure.py:264-277)
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_visc_raw(Ctx c, float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    if (!is_fluid(c, i)) {
+        st3(c.vraw, i, f3{0.0f, 0.0f, 0.0f});
+        return;
+    }
+    const KC K = make_kc(c.h, c.kind);
+    const f3 xi = xyz(c.q4[i]);
+    const float ki = c.vgk[i] / c.rho0[i];
+    f3 a{0.0f, 0.0f, 0.0f};
+    int cnt = c.count[i], base = c.ws[c.world[i]];
+    for (int k = 0; k < cnt; ++k) {
+        int j = nbr_at<WIDE>(c, base, i, k);
+        f3 g = kernel_grad(xi - xyz(c.q4[j]), K);
+        a = a - (c.mass[j] * (ki + c.vgk[j] / c.rho0[j])) * g;
+    }
+    const int64_t b4 = 4 * int64_t(i);
+    for (int b = 0; b < 4; ++b)
+        if (c.s_shape[b4 + b] >= 0) a = a - c.vgk[i] * ld3(c.s_gpsi, b4 + b);
+    st3(c.vraw, i, (-dt) * a);
+}
+
+// viscosity.py:295-323 wall_shift_kernel: the Shepard average of raw over the liquid around a particle with a slot
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_visc_shift(Ctx c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const int64_t b4 = 4 * int64_t(i);
+    bool wall = false;
+    for (int b = 0; b < 4; ++b) wall = wall || c.s_shape[b4 + b] >= 0;
+    if (!is_fluid(c, i) || !wall) {
+        st3(c.vshift, i, f3{0.0f, 0.0f, 0.0f});
+        return;
+    }
+    const KC K = make_kc(c.h, c.kind);
+    const f3 xi = xyz(c.q4[i]);
+    float tot = c.mass[i] / c.rho0[i] * kernel_w(0.0f, K);
+    f3 acc = tot * ld3(c.vraw, i);
+    int cnt = c.count[i], base = c.ws[c.world[i]];
+    for (int k = 0; k < cnt; ++k) {
+        int j = nbr_at<WIDE>(c, base, i, k);
+        if (!is_fluid(c, j)) continue;
+        float wj = c.mass[j] / c.rho0[j] * kernel_w(length(xi - xyz(c.q4[j])), K);
+        acc = acc + wj * ld3(c.vraw, j);
+        tot = tot + wj;
+    }
+    st3(c.vshift, i, acc / tot);
+}
+
+// wall_viscosity.py:153-167 wall_terms_kernel: M_i = sum_k M_ik, w_i = sum_k w_ik
+__global__ void __launch_bounds__(256) k_visc_wall_terms(Ctx c, int has_shift) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    M33 m;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m.m[k] = 0.0f;
+    f3 w{0.0f, 0.0f, 0.0f};
+    if (is_fluid(c, i)) {
+        const f3 xi = xyz(c.q4[i]);
+        const f3 sh = has_shift ? ld3(c.vshift, i) : f3{0.0f, 0.0f, 0.0f};
+        for (int b = 0; b < 4; ++b) {
+            M33 mk;
+            f3 wk;
+            if (!wall_slot_pair(c, 4 * int64_t(i) + b, xi, sh, mk, wk)) continue;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) m.m[k] = m.m[k] + mk.m[k];
+            w = w + wk;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c.wall_m[9 * int64_t(i) + k] = m.m[k];
+    st3(c.wall_w, i, w);
+}
+
+// wall_viscosity.py:170-184 wall_reaction_kernel: each slot's wall takes -(m_i dt / rho_i)(M_ik v_i - w_ik)
+__global__ void __launch_bounds__(256) k_visc_wall_reaction(Ctx c, float dt, int has_shift) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n || !is_fluid(c, i)) return;
+    const f3 xi = xyz(c.q4[i]), vi = ld3(c.qd, i);
+    const float rho_i = fmx(c.ratio[i] * c.rho0[i], 1.0e-6f);
+    const float f = c.mass[i] * dt / rho_i;
+    const f3 sh = has_shift ? ld3(c.vshift, i) : f3{0.0f, 0.0f, 0.0f};
+    for (int b = 0; b < 4; ++b) {
+        const int64_t slot = 4 * int64_t(i) + b;
+        M33 mk;
+        f3 wk;
+        if (!wall_slot_pair(c, slot, xi, sh, mk, wk)) continue;  // an empty slot, or no term: no impulse
+        st3(c.s_imp, slot, ld3(c.s_imp, slot) - f * (mul33(mk, vi) - wk));
+    }
+}
+
 // viscosity.py:50-71 visc_apply_kernel: out = A vec and the chunk partial of vec . out
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_visc_apply(Ctx c, float dt, const float* vec, float* out) {
@@ -2174,6 +2341,12 @@ __global__ void __launch_bounds__(256) k_visc_apply(Ctx c, float dt, const float
             }
             float rho_i = fmx(c.ratio[i] * c.rho0[i], 1.0e-6f);
             f3 r = vi - (dt / rho_i) * a;
+            if (c.wall_coeff > 0.0f) {  // viscosity.py:69-70
+                M33 wm;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) wm.m[k] = c.wall_m[9 * int64_t(i) + k];
+                r = r - (dt / rho_i) * mul33(wm, vi);
+            }
             st3(out, i, r);
             px = double(dot(vi, r));
         }
@@ -2208,10 +2381,17 @@ __global__ void __launch_bounds__(256) k_visc_init(Ctx c, float dt) {
 #pragma unroll
                     for (int q = 0; q < 3; ++q) blk.m[3 * r + q] = blk.m[3 * r + q] - s * (gv[r] * xv[q]);
             }
+            f3 bi = ld3(c.vb, i);
+            if (c.wall_coeff > 0.0f) {  // viscosity.py:97-99: the wall block and the moving walls' drag
+                const float s = dt / rho_i;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) blk.m[k] = blk.m[k] - s * c.wall_m[9 * int64_t(i) + k];
+                bi = bi - s * ld3(c.wall_w, i);
+                st3(c.vb, i, bi);
+            }
             M33 mi = inverse33(blk);
 #pragma unroll
             for (int k = 0; k < 9; ++k) c.vminv[9 * int64_t(i) + k] = mi.m[k];
-            f3 bi = ld3(c.vb, i);
             f3 ri = bi - ld3(c.vap, i);
             f3 zi = mul33(mi, ri);
             st3(c.vr, i, ri);
@@ -2900,12 +3080,21 @@ int phase_nonpressure(Handle* H, double dt, hipStream_t st) {
 
 // viscosity.py:257-283 ImplicitViscosity.solve: v* = the velocities after the non-pressure update; CG per world with
 // the same lookahead loop as the pressure solves; the solution replaces the liquid's velocities
-int run_visc(Handle* H, double dt_h, hipStream_t st) {
+int run_visc(Handle* H, double dt_h, hipStream_t st, bool predict) {
     Ctx& c = H->c;
     if (c.n == 0 || !H->P.viscosity_implicit) return DFSPH_OK;
     const float dt = float(dt_h);
     StopSpec sp{float(H->P.visc_tol), 0.0f, 0, H->P.visc_max_iterations, 0};
     const int64_t n3 = 3 * int64_t(c.n);
+    const bool wall = c.wall_coeff > 0.0f;
+    if (wall) {  // solver.py:310-324 _wall_terms: with predict, the walls' velocities carry the pressure guess
+        if (predict) {
+            k_visc_guess<<<blocks_for(c.n), 256, 0, st>>>(c, dt);
+            LAUNCH_W(k_visc_raw, blocks_for(c.n), 256, c, dt);
+            LAUNCH_W(k_visc_shift, blocks_for(c.n), 256, c);
+        }
+        k_visc_wall_terms<<<blocks_for(c.n), 256, 0, st>>>(c, predict ? 1 : 0);
+    }
     k_copy_f32<<<blocks_for(n3), 256, 0, st>>>(c.vb, c.qd, n3);
     k_copy_f32<<<blocks_for(n3), 256, 0, st>>>(c.vx, c.qd, n3);
     k_fill_i32<<<blocks_for(c.W), 256, 0, st>>>(c.v_conv, 0, c.W);
@@ -2929,6 +3118,7 @@ int run_visc(Handle* H, double dt_h, hipStream_t st) {
         if (batch > 100000) return fail(DFSPH_EDEVICE, "viscosity solve did not terminate");
     }
     k_visc_store<<<blocks_for(c.n), 256, 0, st>>>(c);
+    if (wall) k_visc_wall_reaction<<<blocks_for(c.n), 256, 0, st>>>(c, dt, predict ? 1 : 0);
     LAUNCH_CHECK();
     return DFSPH_OK;
 }
@@ -2998,6 +3188,10 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
     if (P->skin > 0.0 && L->wall_particle_count > 0)  // neighbors.py:249-254
         return fail(DFSPH_EINVAL, "SolverDFSPH: particle walls need the grid rebuilt every step (reneighbor_skin off)");
     if (!(P->skin >= 0.0) || P->skin > P->h) return fail(DFSPH_EINVAL, "reneighbor_skin must be in [0, 1] (x h)");
+    if (!(P->boundary_viscosity >= 0.0) || !(P->visc_wall_coeff >= 0.0))
+        return fail(DFSPH_EINVAL, "boundary_viscosity must be >= 0");
+    if (P->boundary_viscosity > 0.0 && !P->viscosity_implicit)  // solver.py:164-165
+        return fail(DFSPH_EINVAL, "boundary_viscosity needs the implicit (weiler2018) viscosity");
     const int W = L->world_count, n = L->n, S = L->shape_count;
     if (L->world_start[0] != 0 || L->world_start[W] != n)
         return fail(DFSPH_EINVAL, "world_start must cover every particle (world_start[0] = 0, world_start[W] = n)");
@@ -3024,6 +3218,8 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
     c.one_way = P->one_way != 0;
     c.mu = M->viscosity;
     c.visc_d = float(P->visc_d);
+    c.wall_coeff = P->viscosity_implicit ? float(P->visc_wall_coeff) : 0.0f;
+    c.mu_b = float(P->boundary_viscosity);
     c.cap = P->neighbor_capacity;
     c.kind = P->kernel;
     c.friction = P->wall_friction;
@@ -3157,6 +3353,11 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
             if (!M->viscosity) {
                 rc = fail(DFSPH_EINVAL, "the implicit viscosity needs the model's sph:viscosity (DfsphModel.viscosity)");
                 goto fail;
+            }
+            if (c.wall_coeff > 0.0f) {  // wall_viscosity.py: per-particle wall block, drag and guess shift
+                TRY(dalloc(H, &c.wall_m, 9 * int64_t(n)));
+                for (float** p : {&c.wall_w, &c.vshift, &c.vraw}) TRY(dalloc(H, p, 3 * int64_t(n)));
+                TRY(dalloc(H, &c.vgk, int64_t(n)));
             }
         }
         if (c.skin > 0.0f) {  // Verlet-skin state (neighbors_skin.py:162-177)
@@ -3305,7 +3506,7 @@ int32_t dfsph_nonpressure(void* h, const float* q, float* qd, const float* f, do
     bind_state(H, q, qd, f);
     hipStream_t st = static_cast<hipStream_t>(stream);
     int rc = phase_nonpressure(H, dt, st);
-    return rc ? rc : run_visc(H, dt, st);  // solver.py:295-308
+    return rc ? rc : run_visc(H, dt, st, true);  // solver.py:295-308
 }
 
 int32_t dfsph_advect_project(void* h, float* q, float* qd, double dt, void* stream) {
@@ -3360,7 +3561,7 @@ int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void
         rc = run_segment(H, st, SEG_NONPRESSURE, 0, dtf, [&](hipStream_t s) { return phase_nonpressure(H, dt, s); });
     }
     if (rc) return rc;
-    if ((rc = run_visc(H, dt, st))) return rc;  // solver.py:306-308: part of the non-pressure update
+    if ((rc = run_visc(H, dt, st, true))) return rc;  // solver.py:306-308: part of the non-pressure update
     SolveSetup S1 = solve_setup(H, 1, dt, -1, SolveOverride{});
     rc = run_solve(H, S1, st, SEG_TAIL_DEN, [&](hipStream_t s) {
         if (H->log_targets) k_targets<<<blocks_for(H->c.n), 256, 0, s>>>(H->c, dtf, H->log_targets);
@@ -3398,7 +3599,7 @@ int32_t dfsph_prime(void* h, float* q, float* qd, const float* guess, double dt,
     if ((rc = phase_density(H, st))) return rc;
     k_scale_f32_rev<<<blocks_for(c.n), 256, 0, st>>>(c.kv, guess, float(dt), c.n);
     if ((rc = phase_nonpressure(H, dt, st))) return rc;
-    if ((rc = run_visc(H, dt, st))) return rc;  // prime.py:78: _nonpressure(predict=False)
+    if ((rc = run_visc(H, dt, st, false))) return rc;  // prime.py:78: _nonpressure(predict=False)
     SolveOverride ov;
     ov.tol = tol;
     ov.max_it = max_iterations;

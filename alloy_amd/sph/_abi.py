@@ -17,10 +17,11 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import kernels as kn
+from . import wall_viscosity as wv
 from .config import DFSPHConfig, full_neighbor_count
 from .model import GeoType, Model, ParticleFlags
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 DFSPH_OK, DFSPH_EINVAL, DFSPH_ECAPACITY, DFSPH_EDEVICE = 0, -1, -2, -3
 
@@ -77,7 +78,7 @@ class DfsphParams(ctypes.Structure):
         ("taper0", c_double), ("taper1", c_double), ("reach", c_double), ("curvature", c_int),
         ("project_eps", c_double), ("sweep_len", c_double), ("one_way", c_int), ("skin", c_double),
         ("skin_rows", c_int), ("viscosity_implicit", c_int), ("visc_d", c_double), ("visc_tol", c_double),
-        ("visc_max_iterations", c_int),
+        ("visc_max_iterations", c_int), ("boundary_viscosity", c_double), ("visc_wall_coeff", c_double),
     ]
 
 
@@ -170,6 +171,9 @@ def params_from_config(cfg: DFSPHConfig, thinnest: float = math.inf) -> DfsphPar
     p.viscosity_implicit = 0  # set by the solver when the viscosity resolves to weiler2018
     p.visc_d = kn.LAPLACE_D / kn.laplacian_lattice_factor(sp, h, kind)
     p.visc_tol, p.visc_max_iterations = cfg.viscosity_tolerance, cfg.viscosity_max_iterations
+    p.boundary_viscosity = float(cfg.boundary_viscosity or 0.0)
+    p.visc_wall_coeff = (p.boundary_viscosity * wv.wall_coefficient_scale(sp, h, kind)
+                         if p.boundary_viscosity > 0.0 else 0.0)
     return p
 
 

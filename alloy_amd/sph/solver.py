@@ -135,6 +135,9 @@ class SolverDFSPH:
         self._check_layout(model)
         self._check_capacity(capacity_per_world)
         self.viscosity = resolve_viscosity(cfg, float(model.sph.viscosity.max().item()) if model.particle_count else 0.0)
+        if cfg.boundary_viscosity and self.viscosity != "weiler2018":  # solver.py:164-165
+            raise ValueError("SolverDFSPH: boundary_viscosity needs the weiler2018 viscosity "
+                             f"(resolved {self.viscosity!r})")
         phase = model.sph.phase.cpu().numpy()
         mass = model.particle_mass.cpu().numpy()[phase >= 0]
         rho0 = model.sph.rest_density.cpu().numpy()[phase >= 0]
@@ -203,7 +206,6 @@ class SolverDFSPH:
         unbuilt = [
             (cfg.surface_tension != "none", "surface_tension"),
             (cfg.ambient_pressure is not None, "ambient_pressure"),
-            (cfg.boundary_viscosity is not None and cfg.boundary_viscosity > 0.0, "boundary_viscosity"),
         ]
         for bad, name in unbuilt:
             if bad:

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define DFSPH_ABI_VERSION 3
+#define DFSPH_ABI_VERSION 4
 
 #define DFSPH_OK 0
 #define DFSPH_EINVAL (-1)
@@ -81,6 +81,9 @@ typedef struct DfsphParams {
     double visc_d;                   /* LAPLACE_D / laplacian_lattice_factor (solver.py:157-160) */
     double visc_tol;                 /* viscosity_tolerance: stop at |r| <= tol |v*| per world */
     int32_t visc_max_iterations;
+    double boundary_viscosity;       /* mu_b [Pa s] of the wall term, 0 = free-slip walls (config.py:98,
+                                        wall_viscosity.py); needs viscosity_implicit */
+    double visc_wall_coeff;          /* mu_b x wall_coefficient_scale (solver.py:160-162) */
 } DfsphParams;
 
 /* Static description of the particle pack and its walls (layout.py:47-72, boundary.py:79-95). */

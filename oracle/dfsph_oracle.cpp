@@ -232,6 +232,9 @@ struct Oracle {
     std::vector<T> mu, vb, vx, vr, vz, vp, vap, vminv;
     std::vector<int> v_conv, v_iters;
     std::vector<double> v_rz, v_bb, v_step;
+    // its wall term (wall_viscosity.py)
+    T wall_coeff = T(0), mu_b = T(0);
+    std::vector<T> wall_m, wall_w, vshift, vraw, vgk;
 
     int n = 0, W = 0, S = 0, NW = 0;  // NW: wall particles
     std::vector<int> ws;              // world start [W+1]
@@ -328,6 +331,8 @@ struct Oracle {
         visc_d = T(P->visc_d);
         visc_tol = float(P->visc_tol);
         visc_max_it = P->visc_max_iterations;
+        wall_coeff = visc_on ? T(P->visc_wall_coeff) : T(0);
+        mu_b = T(P->boundary_viscosity);
         lattice_grad = T(P->lattice_grad);
         lattice_grad_d = P->lattice_grad;
         relief_speed = P->compression_relief_speed;
@@ -440,6 +445,11 @@ struct Oracle {
             v_conv.assign(W, 0);
             v_iters.assign(W, 0);
             for (auto* v : {&v_rz, &v_bb, &v_step}) v->assign(W, 0.0);
+            if (wall_coeff > T(0)) {
+                wall_m.assign(9 * size_t(std::max(n, 1)), T(0));
+                for (auto* v : {&wall_w, &vshift, &vraw}) v->assign(3 * size_t(std::max(n, 1)), T(0));
+                vgk.assign(std::max(n, 1), T(0));
+            }
         }
         wcidx.assign(std::max(n, 1), -1);
         int nw = 0;
@@ -1414,6 +1424,128 @@ struct Oracle {
         r[8] = (m[0] * m[4] - m[1] * m[3]) * inv;
     }
     T rho_i_of(int i) const { return tmax(ratio[i] * rho0[i], T(1.0e-6)); }
+    // wall_viscosity.py:96-105 _tangents
+    static void wall_tangents(V3<T> nv, V3<T>& t1, V3<T>& t2) {
+        V3<T> a{T(1), T(0), T(0)};
+        if (std::abs(nv.x) > T(0.6)) a = V3<T>{T(0), T(1), T(0)};
+        t1 = normalize(cross(nv, a));
+        t2 = cross(nv, t1);
+    }
+    // wall_viscosity.py:108-150 slot_pair: M (row major) and w = M v_b of slot k of particle i; false when empty
+    bool wall_slot_pair(int i, int k, V3<T> xi, V3<T> shift, T* m, V3<T>& w) const {
+        size_t sl = 4 * size_t(i) + k;
+        if (s_shape[sl] < 0) return false;
+        T phi = s_phi[sl];
+        T d = tmax(phi + T(0.5) * r, T(2) * r);
+        T wd = kernel_w(d, h, kind);
+        V3<T> nv = xi - get3(s_xb, sl);
+        T ln = length(nv);
+        T psi = s_psi[sl];
+        if (phi > taper0) {
+            T v0, v1;
+            halfspace_volume(phi + offset, h, T(0), kind, v0, v1);
+            psi = v0 * amplitude;
+        }
+        if (wd <= T(0) || psi <= T(0) || ln < T(1.0e-9)) return false;
+        nv = nv / ln;
+        V3<T> xk = xi - d * nv, t1, t2;
+        wall_tangents(nv, t1, t2);
+        T vol = T(0.25) * psi / wd;
+        for (int q = 0; q < 9; ++q) m[q] = T(0);
+        for (int q = 0; q < 4; ++q) {
+            T sg = T(1) - T(2) * T(q % 2);
+            V3<T> xis = xi - (xk + (sg * T(0.5) * h) * (q < 2 ? t1 : t2));
+            V3<T> g = kernel_grad(xis, h, kind);
+            T cc = wall_coeff * vol / (dot(xis, xis) + T(0.01) * h * h);
+            const T gv[3] = {g.x, g.y, g.z}, xv[3] = {xis.x, xis.y, xis.z};
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) m[3 * a + b] = m[3 * a + b] + cc * (gv[a] * xv[b]);
+        }
+        T u = tclamp((phi - T(2) * r) / r, T(0), T(1));
+        T m_tt = -mu_b / (tmax(phi, r) * T(2) * r) * (T(1) - u * u * (T(3) - T(2) * u));
+        T mn = dot(nv, mul33(m, nv));
+        const T nn3[3] = {nv.x, nv.y, nv.z};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                T nn = nn3[a] * nn3[b];
+                m[3 * a + b] = mn * nn + m_tt * ((a == b ? T(1) : T(0)) - nn);
+            }
+        w = mul33(m, get3(s_vb, sl) + shift);
+        return true;
+    }
+    // solver.py:310-324 _wall_terms with viscosity.py:235-323 (the pressure guess's shift) and
+    // wall_viscosity.py:153-167 (wall_terms_kernel)
+    void visc_wall_terms(T dt, bool predict) {
+        if (predict) {
+            for (int i = 0; i < n; ++i) vgk[i] = is_fluid(i) ? tmin(kd[i], T(2.5e-4)) / (dt * dt) : T(0);
+            for (int i = 0; i < n; ++i) {
+                if (!is_fluid(i)) {
+                    put3(vraw, i, V3<T>{T(0), T(0), T(0)});
+                    continue;
+                }
+                V3<T> xi = qv(i), a{T(0), T(0), T(0)};
+                T ki = vgk[i] / rho0[i];
+                for (int k = 0; k < count[i]; ++k) {
+                    int j = nbr_j(i, k);
+                    V3<T> g = kernel_grad(xi - qv(j), h, kind);
+                    a = a - (mass[j] * (ki + vgk[j] / rho0[j])) * g;
+                }
+                for (int b = 0; b < 4; ++b)
+                    if (s_shape[4 * size_t(i) + b] >= 0) a = a - vgk[i] * get3(s_gpsi, 4 * size_t(i) + b);
+                put3(vraw, i, (-dt) * a);
+            }
+            for (int i = 0; i < n; ++i) {
+                bool wall = false;
+                for (int b = 0; b < 4; ++b) wall = wall || s_shape[4 * size_t(i) + b] >= 0;
+                if (!is_fluid(i) || !wall) {
+                    put3(vshift, i, V3<T>{T(0), T(0), T(0)});
+                    continue;
+                }
+                V3<T> xi = qv(i);
+                T tot = mass[i] / rho0[i] * kernel_w(T(0), h, kind);
+                V3<T> acc_ = tot * get3(vraw, i);
+                for (int k = 0; k < count[i]; ++k) {
+                    int j = nbr_j(i, k);
+                    if (!is_fluid(j)) continue;
+                    T wj = mass[j] / rho0[j] * kernel_w(length(xi - qv(j)), h, kind);
+                    acc_ = acc_ + wj * get3(vraw, j);
+                    tot = tot + wj;
+                }
+                put3(vshift, i, acc_ / tot);
+            }
+        }
+        for (int i = 0; i < n; ++i) {
+            T m[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+            V3<T> w{T(0), T(0), T(0)};
+            if (is_fluid(i)) {
+                V3<T> xi = qv(i), sh = predict ? get3(vshift, i) : V3<T>{T(0), T(0), T(0)};
+                for (int b = 0; b < 4; ++b) {
+                    T mk[9];
+                    V3<T> wk;
+                    if (!wall_slot_pair(i, b, xi, sh, mk, wk)) continue;
+                    for (int q = 0; q < 9; ++q) m[q] = m[q] + mk[q];
+                    w = w + wk;
+                }
+            }
+            for (int q = 0; q < 9; ++q) wall_m[9 * size_t(i) + q] = m[q];
+            put3(wall_w, i, w);
+        }
+    }
+    // wall_viscosity.py:170-184 wall_reaction_kernel
+    void visc_wall_reaction(T dt, bool predict) {
+        for (int i = 0; i < n; ++i) {
+            if (!is_fluid(i)) continue;
+            V3<T> xi = qv(i), vi = vv(i), sh = predict ? get3(vshift, i) : V3<T>{T(0), T(0), T(0)};
+            T f = mass[i] * dt / rho_i_of(i);
+            for (int b = 0; b < 4; ++b) {
+                T mk[9];
+                V3<T> wk;
+                if (!wall_slot_pair(i, b, xi, sh, mk, wk)) continue;
+                size_t sl = 4 * size_t(i) + b;
+                put3(s_imp, sl, get3(s_imp, sl) - f * (mul33(mk, vi) - wk));
+            }
+        }
+    }
     // viscosity.py:50-71: out = A vec, vals[2i] = vec_i . out_i (converged worlds untouched)
     void visc_apply(T dt, const std::vector<T>& vec, std::vector<T>& out, std::vector<T>& vals) {
         for (int i = 0; i < n; ++i) {
@@ -1430,14 +1562,16 @@ struct Oracle {
                 a = a + (visc_coeff(i, j, xij) * dot(xij, vi - get3(vec, j))) * kernel_grad(xij, h, kind);
             }
             V3<T> r = vi - (dt / rho_i_of(i)) * a;
+            if (wall_coeff > T(0)) r = r - (dt / rho_i_of(i)) * mul33(&wall_m[9 * size_t(i)], vi);
             put3(out, i, r);
             vals[2 * i] = dot(vi, r);
         }
     }
-    void visc_solve(T dt) {  // viscosity.py:243-283
+    void visc_solve(T dt, bool predict) {  // viscosity.py:243-283
         std::vector<T> vals(2 * size_t(std::max(n, 1)), T(0));
         std::vector<double> sums(2 * size_t(W));
         std::vector<T> mx(W);
+        if (wall_coeff > T(0)) visc_wall_terms(dt, predict);
         for (int i = 0; i < n; ++i) {
             put3(vb, i, vv(i));
             put3(vx, i, vv(i));
@@ -1462,8 +1596,15 @@ struct Oracle {
                 for (int r = 0; r < 3; ++r)
                     for (int q = 0; q < 3; ++q) blk[3 * r + q] = blk[3 * r + q] - s * (gv[r] * xv[q]);
             }
+            V3<T> bi = get3(vb, i);
+            if (wall_coeff > T(0)) {
+                T s = dt / rho_i;
+                for (int q = 0; q < 9; ++q) blk[q] = blk[q] - s * wall_m[9 * size_t(i) + q];
+                bi = bi - s * get3(wall_w, i);
+                put3(vb, i, bi);
+            }
             inverse33(blk, &vminv[9 * size_t(i)]);
-            V3<T> bi = get3(vb, i), ri = bi - get3(vap, i);
+            V3<T> ri = bi - get3(vap, i);
             V3<T> zi = mul33(&vminv[9 * size_t(i)], ri);
             put3(vr, i, ri);
             put3(vz, i, zi);
@@ -1519,6 +1660,7 @@ struct Oracle {
         }
         for (int i = 0; i < n; ++i)  // visc_store_kernel
             if (is_fluid(i)) setv(i, get3(vx, i));
+        if (wall_coeff > T(0)) visc_wall_reaction(dt, predict);
     }
 
     // particle_walls.py:138-177 _accumulate_kernel (impulse split over the wall particles, the walls' work)
@@ -1929,7 +2071,7 @@ struct Oracle {
         min_nb = T(0);
         err_factor = 1.0e9;
         nonpressure(dt);
-        if (visc_on) visc_solve(dt);
+        if (visc_on) visc_solve(dt, false);
         solve(0, dt_host, -1);
         tol_div = s_tol;
         err_factor = s_fac;
@@ -1955,7 +2097,7 @@ struct Oracle {
         flow_speed();
         if (div_solve) solve(0, dt_host, -1, warm_div_k);
         nonpressure(dt);
-        if (visc_on) visc_solve(dt);  // solver.py:306-308
+        if (visc_on) visc_solve(dt, true);  // solver.py:306-308
         solve(1, dt_host, -1, warm_den_k);
         meters();
         pw_accumulate();
@@ -2146,7 +2288,7 @@ int32_t orc_nonpressure(void* h, void* q, void* qd, const void* f, double dt) {
         using TT = std::remove_pointer_t<decltype(o->q)>;
         bind(o, q, qd, f);
         o->nonpressure(TT(dt));
-        if (o->visc_on) o->visc_solve(TT(dt));
+        if (o->visc_on) o->visc_solve(TT(dt), true);
     });
     return DFSPH_OK;
 }

@@ -193,3 +193,141 @@ def test_columns_per_world_viscosity_match_the_oracle_bitwise(options):
         iters.append(o.buffer(_abi.BUF_VISC_ITERS)[:2].tolist())
     _log(f"[viscosity] per-world viscous iterations {iters[-3:]}")
     assert any(a != b for a, b in iters)
+
+
+# ---------------------------------------------------------------------------------------------- the wall term
+def _film_gpu(e, mu_b, T, options=None):
+    import torch
+
+    from alloy_amd.sph import _abi
+    from alloy_amd.sph.solver import SolverDFSPH
+    from tests.test_oracle_wall_viscosity import film_measure, film_model
+
+    m, cfg = film_model(e, mu_b, device="cuda")
+    s = SolverDFSPH(m, cfg)
+    for name, val in (options or {}).items():
+        s.set_option(getattr(_abi, "OPT_" + name.upper()), val)
+    st = m.state()
+    dt = 2.0 * cfg.particle_radius
+    for _ in range(int(round(T / dt))):
+        s.step(st, st, None, None, dt)
+    torch.cuda.synchronize()
+    return film_measure(st.particle_q.cpu().numpy().astype(np.float64),
+                        st.particle_qd.cpu().numpy().astype(np.float64), cfg)
+
+
+@pytest.mark.parametrize("e", [4.0e-3, 2.0e-3, 1.0e-3])
+def test_film_drains_at_the_nusselt_speed(e):
+    """test_wall_viscosity.py:164-171: honey films 1, 2 and 4 mm thick on a static wall drain within 15 % of the
+    Nusselt speed rho g e^2 / (3 mu); nothing enters the wall."""
+    from tests.test_oracle_wall_viscosity import G, HONEY_MU, HONEY_RHO
+
+    f = _film_gpu(e, HONEY_MU, 0.05)
+    nusselt = HONEY_RHO * G * f["e"] ** 2 / (3.0 * HONEY_MU)
+    _log(f"[wall viscosity] e = {e * 1e3:.0f} mm: u / Nusselt = {f['u'] / nusselt:.4f}")
+    assert abs(f["u"] - nusselt) <= 0.15 * nusselt, (f, nusselt)
+    assert f["x_min"] > 0.0, f
+
+
+def test_ten_times_wall_viscosity_stays_stable():
+    """test_wall_viscosity.py:185-194: 10x mu_b on a 2 mm film for 0.6 s stays out of the wall with bounded speeds,
+    never faster than no-slip drainage."""
+    from tests.test_oracle_wall_viscosity import G, HONEY_MU, HONEY_RHO
+
+    f = _film_gpu(2.0e-3, 10.0 * HONEY_MU, 0.6)
+    nusselt = HONEY_RHO * G * f["e"] ** 2 / (3.0 * HONEY_MU)
+    _log(f"[wall viscosity] 10x mu_b: u / Nusselt = {f['u'] / nusselt:.4f}, v_max / Nusselt = {f['v_max'] / nusselt:.3f}")
+    assert f["x_min"] > 0.0, f
+    assert f["v_max"] < 3.0 * nusselt, (f, nusselt)
+    assert 0.0 < f["u"] <= 1.15 * nusselt, (f, nusselt)
+
+
+def test_boundary_viscosity_needs_the_implicit_viscosity():
+    from alloy_amd.sph.solver import SolverDFSPH
+    from tests.test_oracle_wall_viscosity import film_model
+
+    m, cfg = film_model(4.0e-3, 6.0, ny=4, nz=4, device="cuda")
+    import dataclasses
+
+    m.sph.viscosity.fill_(1.0e-3)  # 'auto' resolves to xsph below 1e-2 Pa s
+    with pytest.raises(ValueError, match="boundary_viscosity needs the weiler2018"):
+        SolverDFSPH(m, dataclasses.replace(cfg, viscosity="auto"))
+
+
+@pytest.mark.parametrize("options", [{}, {"graphs": 1, "check_interval": 2, "pair_cache": 7, "sparse_correction": 1}])
+def test_film_matches_the_oracle_bitwise(options):
+    """A 2 mm honey film with the wall term, 30 steps (the pressure guess shifts the wall velocity from the second
+    step on): positions, velocities, iteration counts, the slot impulses and the shape meters equal the oracle's."""
+    import torch
+
+    from alloy_amd.sph import _abi
+    from alloy_amd.sph.solver import SolverDFSPH
+    from oracle.oracle import OracleDFSPH
+    from tests.test_oracle_wall_viscosity import film_model
+
+    m, cfg = film_model(2.0e-3, 6.0, ny=8, nz=24, device="cuda")
+    s, o = SolverDFSPH(m, cfg), OracleDFSPH(m, cfg, 32)
+    for name, val in options.items():
+        s.set_option(getattr(_abi, "OPT_" + name.upper()), val)
+    st = m.state()
+    n = m.particle_count
+    dt = 2.0 * cfg.particle_radius
+    for k in range(30):
+        s.step(st, st, None, None, dt)
+        o.step(dt)
+        _compare(s, o, st, f"film step {k}")
+        assert np.array_equal(s.buffer(_abi.BUF_SLOT_IMPULSE, np.float32, (n * 12,)),
+                              o.buffer(_abi.BUF_SLOT_IMPULSE)[:n * 12]), f"film step {k}: slot impulses"
+    torch.cuda.synchronize()
+    S = m.shape_count
+    assert np.array_equal(s.buffer(_abi.BUF_SHAPE_METER, np.int64, (S * 3,)), o.buffer(_abi.BUF_SHAPE_METER)[:S * 3])
+
+
+def test_primed_columns_with_wall_viscosity_match_the_oracle_bitwise():
+    """Two worlds with the wall term (1 and 0.05 Pa s, mu_b = 1) in SDF tanks, primed (no pressure guess), 20 steps."""
+    import dataclasses
+
+    from alloy_amd.sph import _abi
+    from alloy_amd.sph.solver import hydrostatic_guess
+    from oracle.oracle import OracleDFSPH
+
+    sc = _columns([1.0, 0.05], [8, 6])
+    sc.config = dataclasses.replace(sc.config, boundary_viscosity=1.0)
+    s = sc.make_solver()
+    o = OracleDFSPH(sc.model, sc.config, 32)
+    st = sc.model.state()
+    s.prime(st, DT)
+    o.prime(hydrostatic_guess(sc.model, sc.model.state(), sc.config.spacing), DT)
+    _compare(s, o, st, "prime")
+    for k in range(20):
+        s.step(st, st, None, None, DT)
+        o.step(DT)
+        _compare(s, o, st, f"step {k}")
+    assert s.report().max_speed.max() < 0.05
+
+
+def test_pour_worlds_with_wall_viscosity_match_the_oracle_bitwise():
+    """Four pour worlds (spout, swaying particle-wall bowl) at 1 Pa s with the wall term, 80 steps: the moving bowl's
+    slot velocities enter the wall term; every 20 steps the GPU state equals the oracle's."""
+    import dataclasses
+
+    import torch
+
+    from alloy_amd.sph import scenes
+    from tests.test_emitters_gpu import _oracle_for
+
+    base = scenes.pour_worlds(2048, seed=4, device="cuda", subset=range(0, 2048, 512))
+    cfg = dataclasses.replace(base.config, viscosity="weiler2018", boundary_viscosity=1.0)
+    sc = dataclasses.replace(base, config=cfg)
+    sc.model.sph.viscosity.fill_(1.0)
+    s = sc.make_solver()
+    st = sc.model.state()
+    o = _oracle_for(s, sc)
+    for k in range(80):
+        sc.drive(st, k * DT)
+        o.qd[...] = st.particle_qd.cpu().numpy()
+        s.step(st, st, None, None, DT)
+        o.step(DT)
+        if k % 20 == 19:
+            _compare(s, o, st, f"pour step {k + 1}")
+    torch.cuda.synchronize()
