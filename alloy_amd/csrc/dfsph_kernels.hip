@@ -114,6 +114,14 @@ struct Ctx {
     float* wall_m;                        // [9n] sum over the slots of the wall block M_ik
     float* wall_w;                        // [3n] sum over the slots of w_ik = M_ik v_b
     float *vshift, *vraw, *vgk;           // [3n], [3n], [n] the pressure guess's wall-velocity shift and its inputs
+    // continuum surface force with a contact angle (csf.py, csf_sparse.py), present when csf != 0
+    int csf;
+    float csf_cs, csf_inv_i, csf_lo, csf_hi, csf_kcoh;  // c_s, 1 / I, CSF_BAND, 32 / (pi h^9)
+    const float *sigma, *gamma;           // [n] surface tension [N/m], the pair force's gamma
+    float *sh_cos, *sh_sin;               // [S] contact angle (sin -1: free)
+    float *t_mcount, *t_kraw, *t_ktmp, *t_kappa;           // [n]
+    float *t_grad, *t_g1, *t_g2, *t_nhat, *t_nwall, *t_anorm;  // [3n]
+    int *t_surf, *t_wslot;                // [n]
     int* cell_fill;
     int* items;
     int* key;
@@ -2442,6 +2450,268 @@ __global__ void __launch_bounds__(256) k_visc_store(Ctx c) {
     if (i < c.n && is_fluid(c, i)) st3(c.qd, i, ld3(c.vx, i));
 }
 
+// ----------------------------------------------------------------------------------------------- surface tension
+// Continuum surface force with the Breinlinger contact angle (csf.py:63-249) and Akinci's pair force for small pieces
+// (csf_sparse.py:51-88). Every pass reads positions, its inputs and the neighbour rows; the two velocity updates read
+// no velocity of another particle, so they update in place.
+constexpr float CSF_SURFACE_GRAD = 0.3f;  // |g| h above this marks a surface particle (csf.py:53)
+
+__device__ __forceinline__ float csf_weight(float count, float lo, float hi) {  // csf_sparse.py:44-48
+    float x = fcl((count - lo) / (hi - lo), 0.0f, 1.0f);
+    return x * x * (3.0f - 2.0f * x);
+}
+
+// csf_sparse.py:51-62 max_count_kernel and csf.py:63-80 grad_kernel: the largest neighbour count around i, and the
+// colour gradient with the wall slots counted as liquid
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_csf_grad(Ctx c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const KC K = make_kc(c.h, c.kind);
+    const int cnt = c.count[i];
+    const int base = c.world[i] >= 0 ? c.ws[c.world[i]] : 0;
+    int s = cnt;
+    for (int k = 0; k < cnt; ++k) s = max(s, c.count[nbr_at<WIDE>(c, base, i, k)]);
+    c.t_mcount[i] = float(s);
+    f3 g{0.0f, 0.0f, 0.0f};
+    if (is_fluid(c, i)) {
+        const f3 xi = xyz(c.q4[i]);
+        const float inv = 1.0f / c.rho0[i];
+        for (int k = 0; k < cnt; ++k) {
+            int j = nbr_at<WIDE>(c, base, i, k);
+            g = g + (c.mass[j] * inv) * kernel_grad(xi - xyz(c.q4[j]), K);
+        }
+        const int64_t b4 = 4 * int64_t(i);
+        for (int b = 0; b < 4; ++b)
+            if (c.s_shape[b4 + b] >= 0) g = g + 0.5f * ld3(c.s_gpsi, b4 + b);
+    }
+    st3(c.t_grad, i, g);
+}
+
+// csf.py:83-101 shepard_vec_kernel: the Shepard mean over the neighbours, self included
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_csf_shepard_vec(Ctx c, const float* vin, float* vout) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const f3 vi = ld3(vin, i);
+    if (!is_fluid(c, i)) {
+        st3(vout, i, vi);
+        return;
+    }
+    const KC K = make_kc(c.h, c.kind);
+    const f3 xi = xyz(c.q4[i]);
+    const float w0 = kernel_w(0.0f, K);
+    f3 s = w0 * vi;
+    float den = w0;
+    const int cnt = c.count[i], base = c.ws[c.world[i]];
+    for (int k = 0; k < cnt; ++k) {
+        int j = nbr_at<WIDE>(c, base, i, k);
+        float w = kernel_w(length(xi - xyz(c.q4[j])), K);
+        s = s + w * ld3(vin, j);
+        den = den + w;
+    }
+    st3(vout, i, s / den);
+}
+
+// csf.py:104-141 normal_kernel: the surface flag, the normal and the Breinlinger turn at a wall with a contact angle
+__global__ void __launch_bounds__(256) k_csf_normal(Ctx c, const float* gdir) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const f3 z{0.0f, 0.0f, 0.0f};
+    st3(c.t_nhat, i, z);
+    st3(c.t_nwall, i, z);
+    c.t_wslot[i] = -1;
+    c.t_surf[i] = 0;
+    const f3 gd = ld3(gdir, i);
+    const float lg = length(gd);
+    if (!is_fluid(c, i) || length(ld3(c.t_grad, i)) * c.h <= CSF_SURFACE_GRAD || lg <= 0.0f) return;
+    f3 n = (-gd) / lg;
+    const int64_t b4 = 4 * int64_t(i);
+    int best = -1;
+    float dmin = 1.0e30f;
+    for (int b = 0; b < 4; ++b)
+        if (c.s_shape[b4 + b] >= 0 && c.s_phi[b4 + b] < dmin) {
+            dmin = c.s_phi[b4 + b];
+            best = b;
+        }
+    if (best >= 0) {
+        const int s = c.s_shape[b4 + best];
+        if (s < c.S && c.sh_sin[s] >= 0.0f) {
+            f3 nw = xyz(c.q4[i]) - ld3(c.s_xb, b4 + best);
+            const float lw = length(nw);
+            if (lw > 1.0e-9f) {
+                nw = nw / lw;
+                if (dmin < 0.0f) nw = -nw;
+                const f3 t = n - dot(n, nw) * nw;
+                const float lt = length(t);
+                if (lt > 1.0e-6f) {
+                    n = c.sh_cos[s] * nw + (c.sh_sin[s] / lt) * t;
+                    st3(c.t_nwall, i, nw);
+                    c.t_wslot[i] = best;
+                }
+            }
+        }
+    }
+    st3(c.t_nhat, i, n);
+    c.t_surf[i] = 1;
+}
+
+// csf.py:144-172 curvature_kernel: kappa = trace(B A^-1), the least-squares tangential gradient of n over the
+// surface neighbours; A = sum w t t^T (+ 1e-2 trace(A) n n^T), B = sum w (P dn) t^T, t = P (x_j - x_i)
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_csf_curvature(Ctx c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    c.t_kraw[i] = 0.0f;
+    if (c.t_surf[i] == 0) return;
+    const KC K = make_kc(c.h, c.kind);
+    const f3 xi = xyz(c.q4[i]), ni = ld3(c.t_nhat, i);
+    const float nv[3] = {ni.x, ni.y, ni.z};
+    M33 P, A, B;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            P.m[3 * a + b] = (a == b ? 1.0f : 0.0f) - nv[a] * nv[b];
+            A.m[3 * a + b] = 0.0f;
+            B.m[3 * a + b] = 0.0f;
+        }
+    int cnt = 0;
+    const int nc = c.count[i], base = c.ws[c.world[i]];
+    for (int k = 0; k < nc; ++k) {
+        int j = nbr_at<WIDE>(c, base, i, k);
+        if (c.t_surf[j] == 0) continue;
+        const f3 xj = xyz(c.q4[j]);
+        const float w = kernel_w(length(xi - xj), K);
+        const f3 t = mul33(P, xj - xi);
+        const f3 d = mul33(P, ld3(c.t_nhat, j) - ni);
+        const float tv[3] = {t.x, t.y, t.z}, dv[3] = {d.x, d.y, d.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                A.m[3 * a + b] = A.m[3 * a + b] + w * (tv[a] * tv[b]);
+                B.m[3 * a + b] = B.m[3 * a + b] + w * (dv[a] * tv[b]);
+            }
+        cnt += 1;
+    }
+    const float scale = A.m[0] + A.m[4] + A.m[8];
+    if (cnt < 4 || scale <= 0.0f) return;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) A.m[3 * a + b] = A.m[3 * a + b] + (1.0e-2f * scale) * (nv[a] * nv[b]);
+    const M33 Ai = inverse33(A);
+    float tr = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) tr = tr + (B.m[3 * a] * Ai.m[a] + B.m[3 * a + 1] * Ai.m[3 + a] + B.m[3 * a + 2] * Ai.m[6 + a]);
+    c.t_kraw[i] = tr;
+}
+
+// csf.py:175-194 shepard_surface_kernel: the Shepard mean of the curvature over the surface neighbours
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_csf_shepard_surf(Ctx c, const float* kin, float* kout) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    kout[i] = kin[i];
+    if (c.t_surf[i] == 0) return;
+    const KC K = make_kc(c.h, c.kind);
+    const f3 xi = xyz(c.q4[i]);
+    const float w0 = kernel_w(0.0f, K);
+    float num = w0 * kin[i], den = w0;
+    const int cnt = c.count[i], base = c.ws[c.world[i]];
+    for (int k = 0; k < cnt; ++k) {
+        int j = nbr_at<WIDE>(c, base, i, k);
+        if (c.t_surf[j] == 0) continue;
+        float w = kernel_w(length(xi - xyz(c.q4[j])), K);
+        num = num + w * kin[j];
+        den = den + w;
+    }
+    kout[i] = num / den;
+}
+
+// csf.py:197-226 csf_kernel: v_i += dt w_i a_i; at a contact line the part pointing away from the wall goes to the
+// wall's slot impulse
+__global__ void __launch_bounds__(256) k_csf_apply(Ctx c, const float* kin, float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    c.t_kappa[i] = 0.0f;
+    const float wi = csf_weight(c.t_mcount[i], c.csf_lo, c.csf_hi);
+    if (c.t_surf[i] == 0 || c.sigma[i] == 0.0f || wi <= 0.0f) return;
+    float kap = kin[i];
+    const float lg = length(ld3(c.t_grad, i));
+    const float corr = 1.0f + 0.5f * kap * (c.ratio[i] - c.csf_cs) / lg;  // centre -> interface
+    if (corr > 0.5f) kap = kap / corr;
+    c.t_kappa[i] = kap;
+    const float rho_i = fmx(c.ratio[i] * c.rho0[i], 1.0e-6f);
+    f3 dv = (-(wi * dt * c.sigma[i] * kap * lg * c.csf_inv_i / rho_i)) * ld3(c.t_nhat, i);
+    const int b = c.t_wslot[i];
+    if (b >= 0) {
+        const f3 nw = ld3(c.t_nwall, i);
+        const float dn = dot(dv, nw);
+        if (dn > 0.0f) {
+            dv = dv - dn * nw;
+            const int64_t slot = 4 * int64_t(i) + b;
+            st3(c.s_imp, slot, ld3(c.s_imp, slot) + (c.mass[i] * dn) * nw);
+        }
+    }
+    st3(c.qd, i, ld3(c.qd, i) + dv);
+}
+
+// tension.py:101-115 normals_kernel: Akinci's normal h sum_j (m_j / rho_j) grad W_ij
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_csf_akinci_normals(Ctx c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    f3 n{0.0f, 0.0f, 0.0f};
+    if (is_fluid(c, i)) {
+        const KC K = make_kc(c.h, c.kind);
+        const f3 xi = xyz(c.q4[i]);
+        const int cnt = c.count[i], base = c.ws[c.world[i]];
+        for (int k = 0; k < cnt; ++k) {
+            int j = nbr_at<WIDE>(c, base, i, k);
+            const float rho_j = c.ratio[j] * c.rho0[j];
+            if (rho_j > 0.0f) n = n + (c.mass[j] / rho_j) * kernel_grad(xi - xyz(c.q4[j]), K);
+        }
+        n = c.h * n;
+    }
+    st3(c.t_anorm, i, n);
+}
+
+// tension.py:81-98 cohesion_c and curvature_weight
+__device__ __forceinline__ float cohesion_c(float r, float h, float k) {
+    if (r > h || r <= 0.0f) return 0.0f;
+    const float s = (h - r) * (h - r) * (h - r) * r * r * r;
+    if (r > 0.5f * h) return k * s;
+    const float h3 = h * h * h;
+    return k * (2.0f * s - h3 * h3 / 64.0f);
+}
+
+// csf_sparse.py:65-88 sparse_pair_kernel: v_i += dt sum_j c_ij K_ij gamma_ij [curvature + cohesion] on small pieces
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_csf_pairs(Ctx c, float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n || !is_fluid(c, i) || c.gamma[i] == 0.0f) return;
+    const f3 xi = xyz(c.q4[i]), ni = ld3(c.t_anorm, i);
+    const float rho_i = c.ratio[i] * c.rho0[i];
+    const float wi = csf_weight(c.t_mcount[i], c.csf_lo, c.csf_hi);
+    f3 a{0.0f, 0.0f, 0.0f};
+    const int cnt = c.count[i], base = c.ws[c.world[i]];
+    for (int k = 0; k < cnt; ++k) {
+        int j = nbr_at<WIDE>(c, base, i, k);
+        const float cc = 1.0f - 0.5f * (wi + csf_weight(c.t_mcount[j], c.csf_lo, c.csf_hi));
+        const float g = 0.5f * (c.gamma[i] + c.gamma[j]);
+        const float rho_j = c.ratio[j] * c.rho0[j];
+        if (cc <= 0.0f || g == 0.0f || rho_i + rho_j <= 0.0f) continue;
+        const f3 rv = xi - xyz(c.q4[j]);
+        const float r = length(rv);
+        f3 acc = (4.0f * fmx(1.0f - r / c.h, 0.0f)) * (ni - ld3(c.t_anorm, j));
+        if (r > 1.0e-6f * c.h) acc = acc + (c.mass[j] * cohesion_c(r, c.h, c.csf_kcoh) / r) * rv;
+        a = a - (cc * 2.0f * c.rho0[i] / (rho_i + rho_j) * g) * acc;
+    }
+    st3(c.qd, i, ld3(c.qd, i) + dt * a);
+}
+
 // ----------------------------------------------------------------------------------------------- emitters
 // Box emitters, sinks, the escape box and the int64 ledgers (emitters.py:32-255, runtime.py:197-409). The tables are
 // device copies of the host's emitter table; per-world ledgers are int64 fixed point, so their atomic sums do not
@@ -3069,13 +3339,36 @@ int phase_solve(Handle* H, int mode, double dt_h, int fixed_iters, hipStream_t s
     return rc;
 }
 
+// csf.py:311-332 CSFTension.apply: one stage of the non-pressure update
+int run_csf(Handle* H, float dt, hipStream_t st) {
+    Ctx& c = H->c;
+    if (!c.csf || c.n == 0) return DFSPH_OK;
+    const int nb = blocks_for(c.n);
+    LAUNCH_W(k_csf_grad, nb, 256, c);
+    // NORMAL_PASSES + 1 = 3 Shepard passes: grad -> g1 -> g2 -> g1
+    LAUNCH_W(k_csf_shepard_vec, nb, 256, c, c.t_grad, c.t_g1);
+    LAUNCH_W(k_csf_shepard_vec, nb, 256, c, c.t_g1, c.t_g2);
+    LAUNCH_W(k_csf_shepard_vec, nb, 256, c, c.t_g2, c.t_g1);
+    k_csf_normal<<<nb, 256, 0, st>>>(c, c.t_g1);
+    LAUNCH_W(k_csf_curvature, nb, 256, c);
+    // CURVATURE_PASSES = 3: kraw -> ktmp -> kraw -> ktmp
+    LAUNCH_W(k_csf_shepard_surf, nb, 256, c, c.t_kraw, c.t_ktmp);
+    LAUNCH_W(k_csf_shepard_surf, nb, 256, c, c.t_ktmp, c.t_kraw);
+    LAUNCH_W(k_csf_shepard_surf, nb, 256, c, c.t_kraw, c.t_ktmp);
+    k_csf_apply<<<nb, 256, 0, st>>>(c, c.t_ktmp, dt);
+    LAUNCH_W(k_csf_akinci_normals, nb, 256, c);
+    LAUNCH_W(k_csf_pairs, nb, 256, c, dt);
+    LAUNCH_CHECK();
+    return DFSPH_OK;
+}
+
 int phase_nonpressure(Handle* H, double dt, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
     LAUNCH_WT(KID_NONPRESSURE, k_nonpressure, blocks_for(c.n), 256, c, float(dt));
     k_copy_f32<<<blocks_for(3 * int64_t(c.n)), 256, 0, st>>>(c.qd, c.vtmp, 3 * int64_t(c.n));
     LAUNCH_CHECK();
-    return DFSPH_OK;
+    return run_csf(H, float(dt), st);  // solver.py:304-305: surface tension after the non-pressure update
 }
 
 // viscosity.py:257-283 ImplicitViscosity.solve: v* = the velocities after the non-pressure update; CG per world with
@@ -3220,6 +3513,12 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
     c.visc_d = float(P->visc_d);
     c.wall_coeff = P->viscosity_implicit ? float(P->visc_wall_coeff) : 0.0f;
     c.mu_b = float(P->boundary_viscosity);
+    c.csf = P->csf != 0;
+    c.csf_cs = float(P->csf_c_surface);
+    c.csf_inv_i = float(P->csf_inv_i);
+    c.csf_lo = float(P->csf_band_lo);
+    c.csf_hi = float(P->csf_band_hi);
+    c.csf_kcoh = float(P->csf_k_cohesion);
     c.cap = P->neighbor_capacity;
     c.kind = P->kernel;
     c.friction = P->wall_friction;
@@ -3339,6 +3638,16 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         TRY(dupload(H, (float**)&c.sh_x, L->shape_xform, 7 * S));
         TRY(dupload(H, (float**)&c.sh_radius, L->shape_radius, S));
         TRY(dupload(H, (float**)&c.sh_mu, L->shape_mu, S));
+        {  // contact angles (csf.py:281-296); none given: every contact line free
+            std::vector<float> cs(std::max(S, 1), 0.0f), sn(std::max(S, 1), -1.0f);
+            if (L->shape_cos && L->shape_sin)
+                for (int s = 0; s < S; ++s) {
+                    cs[s] = L->shape_cos[s];
+                    sn[s] = L->shape_sin[s];
+                }
+            TRY(dupload(H, &c.sh_cos, cs.data(), std::max(S, 1)));
+            TRY(dupload(H, &c.sh_sin, sn.data(), std::max(S, 1)));
+        }
         // lists
         H->nbr_bytes = int64_t(c.cap) * std::max(n, 1) * 2;
         H->nbr_hi_bytes = c.wide ? int64_t(c.cap) * std::max(n, 1) : 1;
@@ -3359,6 +3668,18 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
                 for (float** p : {&c.wall_w, &c.vshift, &c.vraw}) TRY(dalloc(H, p, 3 * int64_t(n)));
                 TRY(dalloc(H, &c.vgk, int64_t(n)));
             }
+        }
+        if (c.csf) {  // csf.py:298-309: the per-step work arrays
+            if (!M->surface_tension || !M->csf_gamma) {
+                rc = fail(DFSPH_EINVAL, "surface_tension 'csf' needs DfsphModel.surface_tension and csf_gamma");
+                goto fail;
+            }
+            c.sigma = M->surface_tension;
+            c.gamma = M->csf_gamma;
+            for (float** p : {&c.t_mcount, &c.t_kraw, &c.t_ktmp, &c.t_kappa}) TRY(dalloc(H, p, n));
+            for (float** p : {&c.t_grad, &c.t_g1, &c.t_g2, &c.t_nhat, &c.t_nwall, &c.t_anorm})
+                TRY(dalloc(H, p, 3 * int64_t(n)));
+            for (int** p : {&c.t_surf, &c.t_wslot}) TRY(dalloc(H, p, n));
         }
         if (c.skin > 0.0f) {  // Verlet-skin state (neighbors_skin.py:162-177)
             c.cap_s = P->skin_rows > 0 ? std::min(P->skin_rows, 64) : 64;
@@ -3846,6 +4167,12 @@ int32_t dfsph_get_buffer(void* h, int32_t which, void** ptr, int64_t* nb) {
         case DFSPH_BUF_VISC_ITERS:
             if (!H->P.viscosity_implicit) return fail(DFSPH_EINVAL, "no implicit viscosity");
             B(c.v_iters, 4 * W)
+        case DFSPH_BUF_CSF_SURF:
+            if (!c.csf) return fail(DFSPH_EINVAL, "no csf surface tension");
+            B(c.t_surf, 4 * n)
+        case DFSPH_BUF_CSF_KAPPA:
+            if (!c.csf) return fail(DFSPH_EINVAL, "no csf surface tension");
+            B(c.t_kappa, 4 * n)
         case DFSPH_BUF_SKIN_FLAG:
             if (c.skin <= 0.0f) return fail(DFSPH_EINVAL, "no Verlet skin");
             B(c.skin_flag, 4 * W)

@@ -17,11 +17,12 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import kernels as kn
+from . import csf
 from . import wall_viscosity as wv
 from .config import DFSPHConfig, full_neighbor_count
 from .model import GeoType, Model, ParticleFlags
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 DFSPH_OK, DFSPH_EINVAL, DFSPH_ECAPACITY, DFSPH_EDEVICE = 0, -1, -2, -3
 
@@ -31,7 +32,7 @@ DFSPH_OK, DFSPH_EINVAL, DFSPH_ECAPACITY, DFSPH_EDEVICE = 0, -1, -2, -3
  BUF_ERR_MAX, BUF_N_ACTIVE, BUF_TOL_SCALE, BUF_OVERFLOW, BUF_MAX_SPEED, BUF_ITERS_DEN, BUF_ITERS_DIV, BUF_DEN_ERR,
  BUF_DIV_ERR, BUF_SHAPE_METER, BUF_FEATURES, BUF_MASS_INITIAL, BUF_MASS_IN, BUF_MASS_OUT, BUF_MASS_ACTIVE, BUF_ESCAPED,
  BUF_EMIT_OVERFLOW, BUF_FLUX_E, BUF_FLUX_G, BUF_EMIT_T, BUF_EMIT_CURSOR, BUF_EMIT_ENABLED, BUF_FLAGS,
- BUF_PHASE, BUF_PW_EXCH, BUF_PW_WORK, BUF_SKIN_FLAG, BUF_VISC_ITERS) = range(57)
+ BUF_PHASE, BUF_PW_EXCH, BUF_PW_WORK, BUF_SKIN_FLAG, BUF_VISC_ITERS, BUF_CSF_SURF, BUF_CSF_KAPPA) = range(59)
 IMP_SCALE = float(2**50)    # impulse fixed point, bodies.py:25
 
 FIX_SCALE = float(2**40)    # mass fixed point, kernels.py:32-33
@@ -79,6 +80,8 @@ class DfsphParams(ctypes.Structure):
         ("project_eps", c_double), ("sweep_len", c_double), ("one_way", c_int), ("skin", c_double),
         ("skin_rows", c_int), ("viscosity_implicit", c_int), ("visc_d", c_double), ("visc_tol", c_double),
         ("visc_max_iterations", c_int), ("boundary_viscosity", c_double), ("visc_wall_coeff", c_double),
+        ("csf", c_int), ("csf_c_surface", c_double), ("csf_inv_i", c_double), ("csf_band_lo", c_double),
+        ("csf_band_hi", c_double), ("csf_k_cohesion", c_double),
     ]
 
 
@@ -88,14 +91,15 @@ class DfsphLayout(ctypes.Structure):
         ("grid_dims", P(c_int)), ("shape_count", c_int), ("shape_start", P(c_int)), ("shape_ids", P(c_int)),
         ("shape_geo", P(c_int)), ("shape_scale", P(c_float)), ("shape_xform", P(c_float)),
         ("shape_body", P(c_int)), ("shape_radius", P(c_float)), ("shape_mu", P(c_float)),
-        ("wall_particle_count", c_int),
+        ("wall_particle_count", c_int), ("shape_cos", P(c_float)), ("shape_sin", P(c_float)),
     ]
 
 
 class DfsphModel(ctypes.Structure):
     _fields_ = [("mass", ctypes.c_void_p), ("rest_density", ctypes.c_void_p), ("flags", ctypes.c_void_p),
                 ("phase", ctypes.c_void_p), ("world", ctypes.c_void_p), ("gravity", ctypes.c_void_p),
-                ("wall_volume", ctypes.c_void_p), ("viscosity", ctypes.c_void_p)]
+                ("wall_volume", ctypes.c_void_p), ("viscosity", ctypes.c_void_p),
+                ("surface_tension", ctypes.c_void_p), ("csf_gamma", ctypes.c_void_p)]
 
 
 class DfsphReport(ctypes.Structure):
@@ -174,6 +178,12 @@ def params_from_config(cfg: DFSPHConfig, thinnest: float = math.inf) -> DfsphPar
     p.boundary_viscosity = float(cfg.boundary_viscosity or 0.0)
     p.visc_wall_coeff = (p.boundary_viscosity * wv.wall_coefficient_scale(sp, h, kind)
                          if p.boundary_viscosity > 0.0 else 0.0)
+    p.csf = int(cfg.surface_tension == "csf")
+    if p.csf:
+        big_i, c_s = csf.flat_lattice_constants(sp, h, kind)
+        p.csf_c_surface, p.csf_inv_i = c_s, 1.0 / big_i
+        p.csf_band_lo, p.csf_band_hi = csf.CSF_BAND
+        p.csf_k_cohesion = 32.0 / (math.pi * h**9)
     return p
 
 
@@ -212,6 +222,8 @@ class LayoutTables:
     wall_count: int
     thinnest: float
     shape_count: int = 0  # the model's shapes (the arrays above hold one placeholder entry when there are none)
+    shape_cos: np.ndarray | None = None  # contact angles (csf), one placeholder entry when there are no shapes
+    shape_sin: np.ndarray | None = None
 
     def struct(self, n: int) -> DfsphLayout:
         L = DfsphLayout()
@@ -224,6 +236,8 @@ class LayoutTables:
         L.shape_xform, L.shape_body = _ptr(self.shape_xform, c_float), _ptr(self.shape_body, c_int)
         L.shape_radius, L.shape_mu = _ptr(self.shape_radius, c_float), _ptr(self.shape_mu, c_float)
         L.wall_particle_count = int(self.wall_count)
+        if self.shape_cos is not None:
+            L.shape_cos, L.shape_sin = _ptr(self.shape_cos, c_float), _ptr(self.shape_sin, c_float)
         return L
 
 
@@ -296,6 +310,7 @@ def layout_from_model(model: Model, cfg: DFSPHConfig) -> LayoutTables:
     per_world = [ids[start[w]:start[w + 1]] for w in range(W)]
     org, dims = grid_boxes(q, ws, model, per_world, cfg.h + (cfg.reneighbor_skin or 0.0) * cfg.h)
     S = model.shape_count
+    cos, sin = csf.shape_angles(model.sph.shape_contact_angle if S else np.zeros(0))
     f32 = lambda a, shape: np.ascontiguousarray(np.asarray(a, np.float32).reshape(shape))  # noqa: E731
     i32 = lambda a: np.ascontiguousarray(np.asarray(a, np.int32).reshape(-1))  # noqa: E731
     return LayoutTables(
@@ -308,6 +323,7 @@ def layout_from_model(model: Model, cfg: DFSPHConfig) -> LayoutTables:
         shape_mu=f32(model.shape_material_mu if S else np.zeros(1), -1),
         wall_volume=vol, wall_count=int(np.count_nonzero(wall)),
         thinnest=thinnest_wall(model, ids) if S else math.inf, shape_count=S,
+        shape_cos=f32(cos, -1), shape_sin=f32(sin, -1),
     )
 
 

@@ -23,14 +23,17 @@ def lattice_points(lo: Vec3, counts: tuple[int, int, int], spacing: float) -> np
 
 
 def add_fluid_points(builder: ModelBuilder, cfg: DFSPHConfig, points: np.ndarray, rest_density: float = 1000.0,
-                     viscosity: float = 1.0e-3, velocity: Vec3 = (0.0, 0.0, 0.0)) -> range:
-    """Active liquid particles at ``points`` in the builder's current world (build.py:41-57)."""
+                     viscosity: float = 1.0e-3, velocity: Vec3 = (0.0, 0.0, 0.0), surface_tension: float = 0.0,
+                     adhesion: float = 0.0) -> range:
+    """Active liquid particles at ``points`` in the builder's current world (build.py:41-57); ``surface_tension``
+    [N/m] acts with ``surface_tension="csf"``."""
     register_custom_attributes(builder)
     pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
     if len(pts) == 0:
         return range(builder.particle_count, builder.particle_count)
     return builder.add_particles(pts, velocity, particle_mass(cfg, rest_density), cfg.particle_radius,
-                                 int(ParticleFlags.ACTIVE), rest_density, viscosity, PHASE_LIQUID)
+                                 int(ParticleFlags.ACTIVE), rest_density, viscosity, PHASE_LIQUID, surface_tension,
+                                 adhesion)
 
 
 def add_fluid_block(builder: ModelBuilder, cfg: DFSPHConfig, lo: Vec3, counts: tuple[int, int, int],
@@ -55,10 +58,12 @@ def add_spares(builder: ModelBuilder, cfg: DFSPHConfig, count: int, rest_density
                                  rest_density, viscosity, PHASE_SPARE)
 
 
-def wall_box(builder: ModelBuilder, center: Vec3, half: Vec3, mu: float = 0.5) -> int:
-    """One static SDF wall box (build.py:104-110)."""
+def wall_box(builder: ModelBuilder, center: Vec3, half: Vec3, mu: float = 0.5,
+             contact_angle: float | None = None) -> int:
+    """One static SDF wall box (build.py:91-110); ``contact_angle`` [rad] for csf (None: a free contact line)."""
     register_custom_attributes(builder)
-    return builder.add_shape(GeoType.BOX, center, half, mu=mu)
+    return builder.add_shape(GeoType.BOX, center, half, mu=mu,
+                             contact_angle=-1.0 if contact_angle is None else float(contact_angle))
 
 
 def add_box_tank(builder: ModelBuilder, inner_lo: Vec3, inner_hi: Vec3, wall: float, *, open_top: bool = True,

@@ -48,11 +48,25 @@ class GeoType(IntEnum):
 
 @dataclass
 class SPHAttributes:
-    """model.sph: per-particle rest density [kg/m^3], viscosity [Pa s] and phase (config.py:244-261)."""
+    """model.sph: per-particle rest density [kg/m^3], viscosity [Pa s], phase, surface tension sigma [N/m] and
+    Akinci adhesion, and per shape the contact angle [rad] (-1: a free contact line) (config.py:244-261)."""
 
     rest_density: torch.Tensor
     viscosity: torch.Tensor
     phase: torch.Tensor
+    surface_tension: torch.Tensor | None = None  # [n] float32, 0: no tension
+    adhesion: torch.Tensor | None = None         # [n] float32 (read by akinci2013 only)
+    shape_contact_angle: np.ndarray | None = None  # [S] float32 host array, -1: free
+
+    def __post_init__(self):
+        n = self.rest_density.shape[0]
+        dev = self.rest_density.device
+        if self.surface_tension is None:
+            self.surface_tension = torch.zeros(n, dtype=torch.float32, device=dev)
+        if self.adhesion is None:
+            self.adhesion = torch.zeros(n, dtype=torch.float32, device=dev)
+        if self.shape_contact_angle is None:
+            self.shape_contact_angle = np.zeros(0, np.float32)
 
 
 @dataclass
@@ -109,6 +123,8 @@ class ModelBuilder:
         self._rho0: list = []
         self._mu: list = []
         self._phase: list = []
+        self._sigma: list = []
+        self._adh: list = []
         self._shapes: list[dict] = []
         self._world_gravity: list = []
         self.current_world = -1
@@ -139,7 +155,7 @@ class ModelBuilder:
 
     # ---------------------------------------------------------------------------------------- content
     def add_particles(self, pos, vel, mass, radius, flags, rest_density=1000.0, viscosity=1.0e-3,
-                      phase=-1) -> range:
+                      phase=-1, surface_tension=0.0, adhesion=0.0) -> range:
         if self.current_world < 0:
             raise ValueError("ModelBuilder: particles must be added inside a world")
         pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
@@ -155,13 +171,15 @@ class ModelBuilder:
         self._rho0.append(bc(rest_density, np.float64))
         self._mu.append(bc(viscosity, np.float64))
         self._phase.append(bc(phase, np.int8))
+        self._sigma.append(bc(surface_tension, np.float64))
+        self._adh.append(bc(adhesion, np.float64))
         self._count += n
         return range(first, first + n)
 
     def add_shape(self, geo: int, center, half, quat=(0.0, 0.0, 0.0, 1.0), body: int = -1, mu: float = 0.5,
-                  radius: float | None = None) -> int:
+                  radius: float | None = None, contact_angle: float = -1.0) -> int:
         """A shape in the current world (or global with no world open). ``half``: box half extents, sphere
-        radius in x, plane footprint (<= 0: infinite) in x/y."""
+        radius in x, plane footprint (<= 0: infinite) in x/y; ``contact_angle`` [rad] for csf (-1: free)."""
         c = np.asarray(center, dtype=np.float64)
         hx = np.asarray(half, dtype=np.float64).reshape(3)
         if radius is None:
@@ -169,6 +187,7 @@ class ModelBuilder:
                 else 1.0e30
         self._shapes.append(dict(geo=int(geo), center=c, half=hx, quat=np.asarray(quat, np.float64), body=int(body),
                                  world=self.current_world, mu=float(mu), radius=float(radius),
+                                 contact_angle=float(contact_angle),
                                  flags=int(ShapeFlags.COLLIDE_PARTICLES | ShapeFlags.COLLIDE_SHAPES)))
         return len(self._shapes) - 1
 
@@ -209,7 +228,9 @@ class ModelBuilder:
             particle_world_start=torch.tensor(ws.astype(np.int32), device=dev),
             gravity=torch.tensor(grav, device=dev),
             sph=SPHAttributes(rest_density=f32(cat(self._rho0, np.float64)), viscosity=f32(cat(self._mu, np.float64)),
-                              phase=torch.tensor(cat(self._phase, np.int8), device=dev)),
+                              phase=torch.tensor(cat(self._phase, np.int8), device=dev),
+                              surface_tension=f32(cat(self._sigma, np.float64)), adhesion=f32(cat(self._adh, np.float64)),
+                              shape_contact_angle=np.asarray([d["contact_angle"] for d in sh], np.float32)),
             shape_count=S,
             shape_type=np.asarray([d["geo"] for d in sh], np.int32),
             shape_scale=np.asarray([d["half"] for d in sh], np.float32).reshape(S, 3),

@@ -22,7 +22,7 @@ from typing import ClassVar
 import numpy as np
 import torch
 
-from . import _abi
+from . import _abi, csf
 from .emitters import EmitterManager
 from .config import (PHASE_LIQUID, DFSPHConfig, register_custom_attributes, resolve_viscosity, snap_down,
                      stable_dt)
@@ -30,6 +30,8 @@ from .model import Model, ModelFlags, ParticleFlags, State
 from .types import SPHReport
 
 log = logging.getLogger("alloy.sph")
+
+PARTICLE_Q, PARTICLE_QD = 1 << 4, 1 << 5  # Newton's StateFlags bits of the coupling notifications
 
 CapacityError = _abi.CapacityError
 
@@ -138,6 +140,22 @@ class SolverDFSPH:
         if cfg.boundary_viscosity and self.viscosity != "weiler2018":  # solver.py:164-165
             raise ValueError("SolverDFSPH: boundary_viscosity needs the weiler2018 viscosity "
                              f"(resolved {self.viscosity!r})")
+        csf.reject_unread_inputs(model, cfg)  # layout.py:80-95
+        self.tension = cfg.surface_tension  # solver.py:83-87, csf.py:298-309
+        self.capillary_dt_violations = 0
+        self._dt_limit = csf.capillary_limit(model, cfg) if self.tension == "csf" else float("inf")
+        if self._dt_limit < cfg.dt_max:  # the capillary step bounds the stable step
+            import dataclasses
+
+            self.config = cfg = dataclasses.replace(cfg, dt_max=max(self._dt_limit, cfg.dt_min))
+        self.tension_rh_range = None
+        if self.tension == "csf":
+            viscous = self.viscosity == "weiler2018"
+            self.tension_rh_range = csf.RH_RANGE_VISCOUS if viscous else csf.RH_RANGE_XSPH
+            if not viscous and model.particle_count:
+                log.warning("SPH csf: low-viscosity liquid (viscosity 'xsph'): a free drop's l = 2 oscillation grows "
+                            "instead of decaying; quantitative for the shape, the Laplace pressure and the first "
+                            "oscillation periods at R/h %.1f-%.1f only", *csf.RH_RANGE_XSPH)
         phase = model.sph.phase.cpu().numpy()
         mass = model.particle_mass.cpu().numpy()[phase >= 0]
         rho0 = model.sph.rest_density.cpu().numpy()[phase >= 0]
@@ -156,6 +174,8 @@ class SolverDFSPH:
         W = model.world_count
         self.emitters = EmitterManager(model, cfg, self._upload_emitters, self._emit_cursors, eager=False)
         self.pw = ParticleWallReactions(self)
+        self._soft_back = 0.0     # after a staggered sync: move the particle walls back by v dt once (coupling)
+        self._pass_saved = None   # the exchange's solver state, restored by a restart pass
         if self.emitters.active:  # the escape box
             self.emitters.flush(force=True)
         log.info("SolverDFSPH: %d particles in %d worlds, r=%g m, h=%g m, %d shapes, %d wall particles, solver=%s",
@@ -183,8 +203,7 @@ class SolverDFSPH:
         self.tables = _abi.layout_from_model(model, cfg)
         self._params = _abi.params_from_config(cfg, self.tables.thinnest)
         self._params.pressure_solver = 0 if self.pressure_solver == "cg" else 1
-        if self.viscosity != "xsph":  # solver.py:299: XSPH only with viscosity "xsph"
-            self._params.xsph_c = 0.0
+        self._params.xsph_c = csf.xsph_coefficient(cfg, self.viscosity)  # solver.py:87, 299
         self._params.viscosity_implicit = int(self.viscosity == "weiler2018")
         self._mu = model.sph.viscosity.to(torch.float32).contiguous()
         self._wall_volume = torch.tensor(self.tables.wall_volume, device=self.device)
@@ -194,6 +213,10 @@ class SolverDFSPH:
         m.flags, m.phase, m.world = _dptr(model.particle_flags), _dptr(model.sph.phase), _dptr(model.particle_world)
         m.gravity, m.wall_volume = _dptr(self._gravity), _dptr(self._wall_volume)
         m.viscosity = _dptr(self._mu)
+        if self.tension == "csf":
+            self._sigma = model.sph.surface_tension.to(torch.float32).contiguous()
+            self._gamma = torch.tensor(csf.liquid_gamma(model, cfg), device=self.device)
+            m.surface_tension, m.csf_gamma = _dptr(self._sigma), _dptr(self._gamma)
         self._model_struct = m
         layout = self.tables.struct(model.particle_count)
         h = ctypes.c_void_p()
@@ -204,7 +227,7 @@ class SolverDFSPH:
     @staticmethod
     def _refuse_unbuilt(cfg: DFSPHConfig) -> None:
         unbuilt = [
-            (cfg.surface_tension != "none", "surface_tension"),
+            (cfg.surface_tension not in ("none", "csf"), f"surface_tension={cfg.surface_tension!r}"),
             (cfg.ambient_pressure is not None, "ambient_pressure"),
         ]
         for bad, name in unbuilt:
@@ -267,6 +290,16 @@ class SolverDFSPH:
             state_out.particle_qd.copy_(state_in.particle_qd)
         if self.emitters.dirty:
             self.emitters.flush()
+        if self.tension == "csf" and dt > self._dt_limit * (1.0 + 1.0e-6):  # tension.py:280-287: never silent
+            if self.capillary_dt_violations == 0:
+                log.warning("SPH csf: dt %.3g s is above the capillary step limit %.3g s; the tension laws do not "
+                            "cover it", dt, self._dt_limit)
+            self.capillary_dt_violations += 1
+        if self._soft_back > 0.0 and self.pw.count:  # coupling_hooks.py:170-180, before the grid
+            vid = torch.as_tensor(self.pw.vid, device=self.device)
+            q, qd = state_out.particle_q, state_out.particle_qd
+            q[vid] = q[vid] + (-self._soft_back) * qd[vid]
+        self._soft_back = 0.0
         f = state_in.particle_f
         self._call(self.lib.dfsph_step, _dptr(state_out.particle_q), _dptr(state_out.particle_qd), _dptr(f),
                    float(dt), _stream_ptr(self.device), what="SolverDFSPH.step")
@@ -330,7 +363,9 @@ class SolverDFSPH:
             mass_in=kg(led["in"]), mass_out=kg(led["out"]), escaped=self.buffer(_abi.BUF_ESCAPED, np.int32, (W,)),
             neighbor_overflow=r["neighbor_overflow"], max_speed=r["max_speed"].astype(np.float64),
             unconverged=(r["iters_density"] >= mi) | (r["iters_divergence"] >= mi),
-            density_tol=r["density_tol"].astype(np.float64))
+            density_tol=r["density_tol"].astype(np.float64), tension=self.tension,
+            tension_qualitative=False, tension_rh_range=self.tension_rh_range,
+            capillary_dt_violations=self.capillary_dt_violations)
 
     def diagnostics(self) -> dict[str, np.ndarray]:
         """Extra per-world counters (runtime.py:155-163)."""
@@ -469,16 +504,91 @@ class SolverDFSPH:
                "sph.kappa_d": self.buffer(_abi.BUF_KD, np.float32, (n,)),
                "sph.phase": self.model.sph.phase.cpu().numpy().copy(),
                "sph.flags": self.model.particle_flags.cpu().numpy().copy()}
+        if self.model.shape_count:  # a re-run period must not count its walls' impulse twice
+            out["sph.meter_j"] = self.buffer(_abi.BUF_SHAPE_METER, np.int64, (self.model.shape_count, 3))
         W, E = self.model.world_count, len(self.emitters.table.world)
         for key, which, dt in self._ledger_keys():
             out[key] = self.buffer(which, dt, (E if key.startswith("sph.emit") else W,))
         return out
 
+    # ------------------------------------------------------------------ coupling hooks (coupling_hooks.py:37-142)
+    def _pass_arrays(self) -> dict[str, np.ndarray]:
+        """What every pass of one coupling exchange starts from: the warm starts, the shape meters and, with any
+        emitter, sink or escape box, what the emitter pass mutates (runtime.py:364-386)."""
+        n = max(self.model.particle_count, 1)
+        out = {"sph.kappa_v": self.buffer(_abi.BUF_KV, np.float32, (n,)),
+               "sph.kappa_d": self.buffer(_abi.BUF_KD, np.float32, (n,))}
+        if self.model.shape_count:
+            out["sph.meter_j"] = self.buffer(_abi.BUF_SHAPE_METER, np.int64, (self.model.shape_count, 3))
+        if self.emitters.active:
+            out["sph.phase"] = self.model.sph.phase.cpu().numpy().copy()
+            out["sph.flags"] = self.model.particle_flags.cpu().numpy().copy()
+            W, E = self.model.world_count, len(self.emitters.table.world)
+            for key, which, dt in self._ledger_keys():
+                out[key] = self.buffer(which, dt, (E if key.startswith("sph.emit") else W,))
+        return out
+
+    def coupling_notify_input_state_update(self, state: State, flags: int, *, iteration_restart: bool = False,
+                                           dt: float = 0.0) -> None:
+        """A new coupling pass (coupling_hooks.py:43-70): a restart pass restores the exchange's warm starts, shape
+        meters and emitter state, so every pass starts from the same solver state and the meters count the pass
+        that sticks; any other notification saves them. Either way a new particle-wall impulse window begins."""
+        del state
+        if self.model.particle_count == 0:
+            return
+        if int(flags) & PARTICLE_Q and self.pw.count:
+            self._soft_back = float(dt)
+        if iteration_restart:
+            if self._pass_saved is not None:
+                self.alloy_restore_state_arrays(self._pass_saved)
+        else:
+            self._pass_saved = self._pass_arrays()
+        self.pw.new_window()
+
+    def coupling_harvest_proxy_wrenches(self, body_local_to_proxy_global, out_body_f, *, body_qd_before=None,
+                                        state=None, state_out=None, contacts=None, dt: float = 0.0) -> None:
+        """Body proxies (coupling_hooks.py:72-85): this library has no bodies, so there is nothing to harvest."""
+        if len(body_local_to_proxy_global) == 0:
+            return
+        raise NotImplementedError("SolverDFSPH: shapes attached to bodies are not built in this DFSPH implementation")
+
+    def coupling_rewind_proxy_particle(self, particle_local_to_proxy_global, state, coupling_forces,
+                                       particle_gravity_acceleration, dt: float) -> None:
+        """Lagged particle proxies are refused for particle walls (coupling_hooks.py:87-94)."""
+        if len(particle_local_to_proxy_global) and self.pw.count:
+            raise ValueError("SolverDFSPH particle walls need a staggered proxy pair (PairPolicy / Proxy mode "
+                             "'staggered' with Aitken relaxation); lagged proxies diverge for a liquid-wall pair")
+
+    def coupling_harvest_proxy_particle_forces(self, particle_local_to_proxy_global, out_particle_f, *,
+                                               dt: float = 0.0, **kwargs) -> None:
+        """Each mapped wall particle's exchange-averaged force from the liquid (coupling_hooks.py:96-110)."""
+        del kwargs
+        if len(particle_local_to_proxy_global) == 0:
+            return
+        if not dt > 0.0:
+            raise ValueError("SolverDFSPH harvest requires dt > 0")
+        if self.pw.count == 0:
+            raise ValueError("SolverDFSPH: proxy particles but no particle walls")
+        self.pw.harvest(particle_local_to_proxy_global, out_particle_f, dt)
+
+    def coupling_prepare_proxy_contacts(self, state, contacts, *, contacts_freshly_detected: bool = False):
+        return contacts  # no Contacts rows are read: boundaries come from shape SDFs and particle walls
+
+    def coupling_supports_full_surface_soft_contacts(self) -> bool:
+        return True
+
+    def coupling_supports_inertial_property_refresh(self) -> bool:
+        return False
+
     def alloy_restore_state_arrays(self, arrays: Mapping[str, np.ndarray]) -> None:
         for key, which in (("sph.kappa_v", _abi.BUF_KV), ("sph.kappa_d", _abi.BUF_KD)):
             self._write(which, np.ascontiguousarray(arrays[key], dtype=np.float32))
-        self.model.sph.phase.copy_(torch.tensor(arrays["sph.phase"], device=self.device))
-        self.model.particle_flags.copy_(torch.tensor(arrays["sph.flags"], device=self.device))
+        if "sph.meter_j" in arrays:
+            self._write(_abi.BUF_SHAPE_METER, np.ascontiguousarray(arrays["sph.meter_j"], dtype=np.int64))
+        if "sph.phase" in arrays:
+            self.model.sph.phase.copy_(torch.tensor(arrays["sph.phase"], device=self.device))
+        if "sph.flags" in arrays:
+            self.model.particle_flags.copy_(torch.tensor(arrays["sph.flags"], device=self.device))
         for key, which, dt in self._ledger_keys():
             if key in arrays:
                 self._write(which, np.ascontiguousarray(arrays[key], dtype=dt))
@@ -487,6 +597,8 @@ class SolverDFSPH:
         """Item -> world maps of the state arrays (coupling_hooks.py:128-138)."""
         w = self.model.particle_world.cpu().numpy()
         out = {"sph.kappa_v": w, "sph.kappa_d": w, "sph.phase": w, "sph.flags": w}
+        if self.model.shape_count:
+            out["sph.meter_j"] = np.asarray(self.model.shape_world, np.int32)
         wl = np.arange(self.model.world_count, dtype=np.int32)
         ew = np.asarray(self.emitters.table.world, np.int32)
         for key, _, _ in self._ledger_keys():

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define DFSPH_ABI_VERSION 4
+#define DFSPH_ABI_VERSION 5
 
 #define DFSPH_OK 0
 #define DFSPH_EINVAL (-1)
@@ -84,6 +84,11 @@ typedef struct DfsphParams {
     double boundary_viscosity;       /* mu_b [Pa s] of the wall term, 0 = free-slip walls (config.py:98,
                                         wall_viscosity.py); needs viscosity_implicit */
     double visc_wall_coeff;          /* mu_b x wall_coefficient_scale (solver.py:160-162) */
+    int32_t csf;                     /* 1: surface_tension "csf" (csf.py), a stage of the non-pressure update */
+    double csf_c_surface;            /* c_s of flat_lattice_constants (csf.py:252-278) */
+    double csf_inv_i;                /* 1 / I of flat_lattice_constants */
+    double csf_band_lo, csf_band_hi; /* CSF_BAND (csf_sparse.py:40) */
+    double csf_k_cohesion;           /* 32 / (pi h^9), Akinci's cohesion spline factor (tension.py:81-92) */
 } DfsphParams;
 
 /* Static description of the particle pack and its walls (layout.py:47-72, boundary.py:79-95). */
@@ -105,6 +110,8 @@ typedef struct DfsphLayout {
     const float* shape_radius;       /* host [S] origin radius (layout.py:151-163) */
     const float* shape_mu;           /* host [S] Coulomb friction of the wall projection */
     int32_t wall_particle_count;     /* particles flagged ACTIVE|PROXY acting as particle walls */
+    const float* shape_cos;          /* host [S] cos of sph:shape_contact_angle (csf.py:281-296); NULL: all free */
+    const float* shape_sin;          /* host [S] its sin, -1 for a free contact line */
 } DfsphLayout;
 
 /* Model arrays (device pointers, length n unless stated). */
@@ -117,6 +124,8 @@ typedef struct DfsphModel {
     const float* gravity;            /* [W*3] */
     const float* wall_volume;        /* V_j = 8 r^3 of particle-wall particles, 0 otherwise (particle_walls.py:227) */
     const float* viscosity;          /* sph:viscosity [Pa s] (read by the implicit viscosity; may be NULL without it) */
+    const float* surface_tension;    /* sph:surface_tension sigma [N/m] (read by csf; may be NULL without it) */
+    const float* csf_gamma;          /* the small-piece pair force's gamma per particle (csf_sparse.py:91-99) */
 } DfsphModel;
 
 /* Host copy of the per-world report (types.py:11-28 SPHReport); arrays of length W owned by the caller. */
@@ -197,6 +206,8 @@ enum DfsphBuffer {
     DFSPH_BUF_PW_WORK,        /* int64  [W] the particle walls' work on the liquid in the last step, 2^50 per J */
     DFSPH_BUF_SKIN_FLAG,      /* int32  [W] worlds whose Verlet-skin rows were rebuilt in the last list build */
     DFSPH_BUF_VISC_ITERS,     /* int32  [W] iterations of the last implicit viscosity solve */
+    DFSPH_BUF_CSF_SURF,       /* int32  [n] csf: 1 on the surface particles of the last step */
+    DFSPH_BUF_CSF_KAPPA,      /* float  [n] csf: the curvature applied at the interface, 0 off the surface */
     DFSPH_BUF_COUNT_
 };
 

@@ -235,6 +235,12 @@ struct Oracle {
     // its wall term (wall_viscosity.py)
     T wall_coeff = T(0), mu_b = T(0);
     std::vector<T> wall_m, wall_w, vshift, vraw, vgk;
+    // continuum surface force (csf.py, csf_sparse.py)
+    int csf_on = 0;
+    T csf_cs = T(0), csf_inv_i = T(0), csf_lo = T(0), csf_hi = T(0), csf_kcoh = T(0);
+    std::vector<T> sigma, gamma_, sh_cos, sh_sin;
+    std::vector<T> t_mcount, t_kraw, t_ktmp, t_kappa, t_grad, t_g1, t_g2, t_nhat, t_nwall, t_anorm;
+    std::vector<int> t_surf, t_wslot;
 
     int n = 0, W = 0, S = 0, NW = 0;  // NW: wall particles
     std::vector<int> ws;              // world start [W+1]
@@ -333,6 +339,12 @@ struct Oracle {
         visc_max_it = P->visc_max_iterations;
         wall_coeff = visc_on ? T(P->visc_wall_coeff) : T(0);
         mu_b = T(P->boundary_viscosity);
+        csf_on = P->csf != 0;
+        csf_cs = T(P->csf_c_surface);
+        csf_inv_i = T(P->csf_inv_i);
+        csf_lo = T(P->csf_band_lo);
+        csf_hi = T(P->csf_band_hi);
+        csf_kcoh = T(P->csf_k_cohesion);
         lattice_grad = T(P->lattice_grad);
         lattice_grad_d = P->lattice_grad;
         relief_speed = P->compression_relief_speed;
@@ -436,6 +448,26 @@ struct Oracle {
         for (auto* v : {&sums, &flow_z, &flow_xy, &head}) v->assign(2 * W, 0.0);
         meter.assign(3 * size_t(std::max(S, 1)), 0);
         for (auto* v : {&mass0, &mass_in, &mass_out, &mass_act, &flux_e, &flux_g}) v->assign(W, 0);
+        sh_cos.assign(std::max(S, 1), T(0));
+        sh_sin.assign(std::max(S, 1), T(-1));
+        if (L->shape_cos && L->shape_sin)
+            for (int s = 0; s < S; ++s) {
+                sh_cos[s] = T(L->shape_cos[s]);
+                sh_sin[s] = T(L->shape_sin[s]);
+            }
+        if (csf_on) {
+            if (!M->surface_tension || !M->csf_gamma) return fail(DFSPH_EINVAL, "csf needs surface_tension and csf_gamma");
+            sigma.assign(std::max(n, 1), T(0));
+            gamma_.assign(std::max(n, 1), T(0));
+            for (int i = 0; i < n; ++i) {
+                sigma[i] = T(M->surface_tension[i]);
+                gamma_[i] = T(M->csf_gamma[i]);
+            }
+            for (auto* v : {&t_mcount, &t_kraw, &t_ktmp, &t_kappa}) v->assign(std::max(n, 1), T(0));
+            for (auto* v : {&t_grad, &t_g1, &t_g2, &t_nhat, &t_nwall, &t_anorm}) v->assign(3 * size_t(std::max(n, 1)), T(0));
+            t_surf.assign(std::max(n, 1), 0);
+            t_wslot.assign(std::max(n, 1), -1);
+        }
         mu.assign(std::max(n, 1), T(0));
         if (M->viscosity)
             for (int i = 0; i < n; ++i) mu[i] = T(M->viscosity[i]);
@@ -1375,6 +1407,216 @@ struct Oracle {
             put3(vtmp, i, v);
         }
         for (int i = 0; i < n; ++i) setv(i, get3(vtmp, i));
+        if (csf_on) csf_stage(dt);  // solver.py:304-305
+    }
+
+    // ---------------------------------------------------------------------------- surface tension (csf.py)
+    static T csf_weight(T count, T lo, T hi) {  // csf_sparse.py:44-48
+        T x = tclamp((count - lo) / (hi - lo), T(0), T(1));
+        return x * x * (T(3) - T(2) * x);
+    }
+    T cohesion_c(T r) const {  // tension.py:81-92
+        if (r > h || r <= T(0)) return T(0);
+        T s = (h - r) * (h - r) * (h - r) * r * r * r;
+        if (r > T(0.5) * h) return csf_kcoh * s;
+        T h3 = h * h * h;
+        return csf_kcoh * (T(2) * s - h3 * h3 / T(64));
+    }
+    void shepard_vec(const std::vector<T>& vin, std::vector<T>& vout) {  // csf.py:83-101
+        for (int i = 0; i < n; ++i) {
+            V3<T> vi = get3(vin, i);
+            if (!is_fluid(i)) {
+                put3(vout, i, vi);
+                continue;
+            }
+            V3<T> xi = qv(i);
+            T w0 = kernel_w(T(0), h, kind), den = w0;
+            V3<T> s = w0 * vi;
+            for (int k = 0; k < count[i]; ++k) {
+                int j = nbr_j(i, k);
+                T w = kernel_w(length(xi - qv(j)), h, kind);
+                s = s + w * get3(vin, j);
+                den = den + w;
+            }
+            put3(vout, i, s / den);
+        }
+    }
+    void shepard_surf(const std::vector<T>& kin, std::vector<T>& kout) {  // csf.py:175-194
+        for (int i = 0; i < n; ++i) {
+            kout[i] = kin[i];
+            if (t_surf[i] == 0) continue;
+            V3<T> xi = qv(i);
+            T w0 = kernel_w(T(0), h, kind), num = w0 * kin[i], den = w0;
+            for (int k = 0; k < count[i]; ++k) {
+                int j = nbr_j(i, k);
+                if (t_surf[j] == 0) continue;
+                T w = kernel_w(length(xi - qv(j)), h, kind);
+                num = num + w * kin[j];
+                den = den + w;
+            }
+            kout[i] = num / den;
+        }
+    }
+    void csf_stage(T dt) {  // csf.py:311-332
+        for (int i = 0; i < n; ++i) {  // max_count_kernel, grad_kernel
+            int s = count[i];
+            for (int k = 0; k < count[i]; ++k) s = std::max(s, count[nbr_j(i, k)]);
+            t_mcount[i] = T(s);
+            V3<T> g{T(0), T(0), T(0)};
+            if (is_fluid(i)) {
+                V3<T> xi = qv(i);
+                T inv = T(1) / rho0[i];
+                for (int k = 0; k < count[i]; ++k) {
+                    int j = nbr_j(i, k);
+                    g = g + (mass[j] * inv) * kernel_grad(xi - qv(j), h, kind);
+                }
+                for (int b = 0; b < 4; ++b)
+                    if (s_shape[4 * size_t(i) + b] >= 0) g = g + T(0.5) * get3(s_gpsi, 4 * size_t(i) + b);
+            }
+            put3(t_grad, i, g);
+        }
+        shepard_vec(t_grad, t_g1);
+        shepard_vec(t_g1, t_g2);
+        shepard_vec(t_g2, t_g1);
+        for (int i = 0; i < n; ++i) {  // normal_kernel
+            const V3<T> z{T(0), T(0), T(0)};
+            put3(t_nhat, i, z);
+            put3(t_nwall, i, z);
+            t_wslot[i] = -1;
+            t_surf[i] = 0;
+            V3<T> gd = get3(t_g1, i);
+            T lg = length(gd);
+            if (!is_fluid(i) || length(get3(t_grad, i)) * h <= T(0.3) || lg <= T(0)) continue;
+            V3<T> nn = (-gd) / lg;
+            int best = -1;
+            T dmin = T(1.0e30);
+            for (int b = 0; b < 4; ++b)
+                if (s_shape[4 * size_t(i) + b] >= 0 && s_phi[4 * size_t(i) + b] < dmin) {
+                    dmin = s_phi[4 * size_t(i) + b];
+                    best = b;
+                }
+            if (best >= 0) {
+                int s = s_shape[4 * size_t(i) + best];
+                if (s < S && sh_sin[s] >= T(0)) {
+                    V3<T> nw = qv(i) - get3(s_xb, 4 * size_t(i) + best);
+                    T lw = length(nw);
+                    if (lw > T(1.0e-9)) {
+                        nw = nw / lw;
+                        if (dmin < T(0)) nw = -nw;
+                        V3<T> t = nn - dot(nn, nw) * nw;
+                        T lt = length(t);
+                        if (lt > T(1.0e-6)) {
+                            nn = sh_cos[s] * nw + (sh_sin[s] / lt) * t;
+                            put3(t_nwall, i, nw);
+                            t_wslot[i] = best;
+                        }
+                    }
+                }
+            }
+            put3(t_nhat, i, nn);
+            t_surf[i] = 1;
+        }
+        for (int i = 0; i < n; ++i) {  // curvature_kernel
+            t_kraw[i] = T(0);
+            if (t_surf[i] == 0) continue;
+            V3<T> xi = qv(i), ni = get3(t_nhat, i);
+            const T nv[3] = {ni.x, ni.y, ni.z};
+            T P[9], A[9], B[9];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    P[3 * a + b] = (a == b ? T(1) : T(0)) - nv[a] * nv[b];
+                    A[3 * a + b] = T(0);
+                    B[3 * a + b] = T(0);
+                }
+            int cnt = 0;
+            for (int k = 0; k < count[i]; ++k) {
+                int j = nbr_j(i, k);
+                if (t_surf[j] == 0) continue;
+                V3<T> xj = qv(j);
+                T w = kernel_w(length(xi - xj), h, kind);
+                V3<T> t = mul33(P, xj - xi), d = mul33(P, get3(t_nhat, j) - ni);
+                const T tv[3] = {t.x, t.y, t.z}, dv[3] = {d.x, d.y, d.z};
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) {
+                        A[3 * a + b] = A[3 * a + b] + w * (tv[a] * tv[b]);
+                        B[3 * a + b] = B[3 * a + b] + w * (dv[a] * tv[b]);
+                    }
+                cnt += 1;
+            }
+            T scale = A[0] + A[4] + A[8];
+            if (cnt < 4 || scale <= T(0)) continue;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) A[3 * a + b] = A[3 * a + b] + (T(1.0e-2) * scale) * (nv[a] * nv[b]);
+            T Ai[9];
+            inverse33(A, Ai);
+            T tr = T(0);
+            for (int a = 0; a < 3; ++a)
+                tr = tr + (B[3 * a] * Ai[a] + B[3 * a + 1] * Ai[3 + a] + B[3 * a + 2] * Ai[6 + a]);
+            t_kraw[i] = tr;
+        }
+        shepard_surf(t_kraw, t_ktmp);
+        shepard_surf(t_ktmp, t_kraw);
+        shepard_surf(t_kraw, t_ktmp);
+        for (int i = 0; i < n; ++i) {  // csf_kernel
+            t_kappa[i] = T(0);
+            T wi = csf_weight(t_mcount[i], csf_lo, csf_hi);
+            if (t_surf[i] == 0 || sigma[i] == T(0) || wi <= T(0)) continue;
+            T kap = t_ktmp[i];
+            T lg = length(get3(t_grad, i));
+            T corr = T(1) + T(0.5) * kap * (ratio[i] - csf_cs) / lg;
+            if (corr > T(0.5)) kap = kap / corr;
+            t_kappa[i] = kap;
+            T rho_i = tmax(ratio[i] * rho0[i], T(1.0e-6));
+            V3<T> dv = (-(wi * dt * sigma[i] * kap * lg * csf_inv_i / rho_i)) * get3(t_nhat, i);
+            int b = t_wslot[i];
+            if (b >= 0) {
+                V3<T> nw = get3(t_nwall, i);
+                T dn = dot(dv, nw);
+                if (dn > T(0)) {
+                    dv = dv - dn * nw;
+                    size_t sl = 4 * size_t(i) + b;
+                    put3(s_imp, sl, get3(s_imp, sl) + (mass[i] * dn) * nw);
+                }
+            }
+            setv(i, vv(i) + dv);
+        }
+        for (int i = 0; i < n; ++i) {  // normals_kernel (tension.py:101-115)
+            V3<T> nn{T(0), T(0), T(0)};
+            if (is_fluid(i)) {
+                V3<T> xi = qv(i);
+                for (int k = 0; k < count[i]; ++k) {
+                    int j = nbr_j(i, k);
+                    T rho_j = ratio[j] * rho0[j];
+                    if (rho_j > T(0)) nn = nn + (mass[j] / rho_j) * kernel_grad(xi - qv(j), h, kind);
+                }
+                nn = h * nn;
+            }
+            put3(t_anorm, i, nn);
+        }
+        std::vector<T> dvp(3 * size_t(std::max(n, 1)), T(0));
+        std::vector<char> upd(std::max(n, 1), 0);
+        for (int i = 0; i < n; ++i) {  // sparse_pair_kernel (csf_sparse.py:65-88)
+            if (!is_fluid(i) || gamma_[i] == T(0)) continue;
+            V3<T> xi = qv(i), ni = get3(t_anorm, i), a{T(0), T(0), T(0)};
+            T rho_i = ratio[i] * rho0[i];
+            T wi = csf_weight(t_mcount[i], csf_lo, csf_hi);
+            for (int k = 0; k < count[i]; ++k) {
+                int j = nbr_j(i, k);
+                T cc = T(1) - T(0.5) * (wi + csf_weight(t_mcount[j], csf_lo, csf_hi));
+                T g = T(0.5) * (gamma_[i] + gamma_[j]);
+                T rho_j = ratio[j] * rho0[j];
+                if (cc <= T(0) || g == T(0) || rho_i + rho_j <= T(0)) continue;
+                V3<T> rv = xi - qv(j);
+                T r = length(rv);
+                V3<T> acc = (T(4) * tmax(T(1) - r / h, T(0))) * (ni - get3(t_anorm, j));
+                if (r > T(1.0e-6) * h) acc = acc + (mass[j] * cohesion_c(r) / r) * rv;
+                a = a - (cc * T(2) * rho0[i] / (rho_i + rho_j) * g) * acc;
+            }
+            put3(dvp, i, dt * a);
+            upd[i] = 1;
+        }
+        for (int i = 0; i < n; ++i)
+            if (upd[i]) setv(i, vv(i) + get3(dvp, i));
     }
 
     // ---------------------------------------------------------------------------- shape meters (bodies.py:37-)
@@ -2173,6 +2415,8 @@ int get_buffer_t(Oracle<T>* o, int which, void** ptr, int64_t* nb) {
         case DFSPH_BUF_PW_EXCH: OB(pw_exch)
         case DFSPH_BUF_PW_WORK: OB(pw_work)
         case DFSPH_BUF_VISC_ITERS: OB(v_iters)
+        case DFSPH_BUF_CSF_SURF: OB(t_surf)
+        case DFSPH_BUF_CSF_KAPPA: OB(t_kappa)
         default: return fail(DFSPH_EINVAL, "unknown buffer id");
     }
 #undef OB

@@ -14,7 +14,7 @@ import subprocess
 
 import numpy as np
 
-from alloy_amd.sph import _abi
+from alloy_amd.sph import _abi, csf
 from alloy_amd.sph.config import DFSPHConfig
 from alloy_amd.sph.model import Model
 
@@ -87,7 +87,7 @@ _BUF_DTYPES = {
     _abi.BUF_MASS_IN: np.int64, _abi.BUF_MASS_OUT: np.int64, _abi.BUF_MASS_ACTIVE: np.int64, _abi.BUF_ESCAPED: np.int32,
     _abi.BUF_EMIT_OVERFLOW: np.int32, _abi.BUF_FLUX_E: np.int64, _abi.BUF_FLUX_G: np.int64,
     _abi.BUF_EMIT_CURSOR: np.int32, _abi.BUF_EMIT_ENABLED: np.int32, _abi.BUF_FLAGS: np.int32, _abi.BUF_PHASE: np.int8,
-    _abi.BUF_PW_EXCH: np.int64, _abi.BUF_PW_WORK: np.int64, _abi.BUF_VISC_ITERS: np.int32,
+    _abi.BUF_PW_EXCH: np.int64, _abi.BUF_PW_WORK: np.int64, _abi.BUF_VISC_ITERS: np.int32, _abi.BUF_CSF_SURF: np.int32,
 }
 
 
@@ -107,8 +107,7 @@ class OracleDFSPH:
         from alloy_amd.sph.config import resolve_viscosity
 
         visc = viscosity or resolve_viscosity(cfg, float(model.sph.viscosity.max().item()) if self.n else 0.0)
-        if visc != "xsph":
-            self.params.xsph_c = 0.0
+        self.params.xsph_c = csf.xsph_coefficient(cfg, visc)
         self.params.viscosity_implicit = int(visc == "weiler2018")
         host = lambda t, dt: np.ascontiguousarray(t.detach().cpu().numpy().astype(dt))  # noqa: E731
         self._keep = dict(
@@ -116,11 +115,13 @@ class OracleDFSPH:
             flags=host(model.particle_flags, np.int32), phase=host(model.sph.phase, np.int8),
             world=host(model.particle_world, np.int32), gravity=host(model.gravity.reshape(-1), np.float32),
             wvol=np.ascontiguousarray(self.tables.wall_volume, np.float32),
-            mu=host(model.sph.viscosity, np.float32))
+            mu=host(model.sph.viscosity, np.float32), sigma=host(model.sph.surface_tension, np.float32),
+            gamma=np.ascontiguousarray(csf.liquid_gamma(model, cfg), np.float32))
         k = self._keep
         m = _abi.DfsphModel()
         for field_, key in (("mass", "mass"), ("rest_density", "rho0"), ("flags", "flags"), ("phase", "phase"),
-                            ("world", "world"), ("gravity", "gravity"), ("wall_volume", "wvol"), ("viscosity", "mu")):
+                            ("world", "world"), ("gravity", "gravity"), ("wall_volume", "wvol"), ("viscosity", "mu"),
+                            ("surface_tension", "sigma"), ("csf_gamma", "gamma")):
             setattr(m, field_, k[key].ctypes.data_as(ctypes.c_void_p))
         layout = self.tables.struct(self.n)
         h = ctypes.c_void_p()

@@ -70,7 +70,7 @@ def test_solver_has_no_cpu_path():
         SolverDFSPH(sc.model, sc.config)
 
 
-@pytest.mark.parametrize("overrides", [{"surface_tension": "csf"}, {"ambient_pressure": 100.0}])
+@pytest.mark.parametrize("overrides", [{"surface_tension": "akinci2013"}, {"ambient_pressure": 100.0}])
 def test_unbuilt_subsystems_are_refused(overrides):
     from alloy_amd.sph.solver import SolverDFSPH
 
