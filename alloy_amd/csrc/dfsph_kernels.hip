@@ -122,6 +122,11 @@ struct Ctx {
     float *t_mcount, *t_kraw, *t_ktmp, *t_kappa;           // [n]
     float *t_grad, *t_g1, *t_g2, *t_nhat, *t_nwall, *t_anorm;  // [3n]
     int *t_surf, *t_wslot;                // [n]
+    // packed gathers of the pressure acceleration: [3][n] {x, y, z, vec / rho0} for the three vectors k_accel
+    // reads (kappa, the CG direction, the CG correction), and the per-world particle mass they stand in for
+    int packed;
+    float4* qk;
+    const float* w_mass;
     int* cell_fill;
     int* items;
     int* key;
@@ -200,6 +205,13 @@ constexpr int NB_U = DFSPH_NB_U;
 template <class T>
 __device__ __forceinline__ float4 load_pos(const T& d) { return d.q; }
 [[maybe_unused]] __device__ __forceinline__ float4 load_pos(float4 d) { return d; }
+
+// a vec / rho0 value of slot 0 (kappa), 1 (CG direction) or 2 (CG correction), and its packed copy
+enum VrSlot { VR_KAPPA = 0, VR_DIR = 1, VR_CORR = 2 };
+__device__ __forceinline__ void put_vr(const Ctx& c, float* vr, int slot, int i, float v) {
+    vr[i] = v;
+    if (c.packed) c.qk[int64_t(slot) * c.n + i].w = v;
+}
 template <bool WIDE>
 __device__ __forceinline__ void nbr_batch(const Ctx& c, int i, int cnt, int base, int k0, int (&j)[NB_U]) {
 #pragma unroll
@@ -506,6 +518,8 @@ __global__ void __launch_bounds__(256) k_groups_hist(Ctx c) {
     c.group[i] = g;
     f3 xi = Q(c, i);
     c.q4[i] = make_float4(xi.x, xi.y, xi.z, c.mass[i]);
+    if (c.packed)  // the w components are written by the vector's producer before any read this step
+        for (int s = 0; s < 3; ++s) c.qk[int64_t(s) * c.n + i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
     int k = -1;
     if (g != -1) {
         int gi = g >= 0 ? g : c.W + (-2 - g);
@@ -1344,7 +1358,7 @@ __global__ void __launch_bounds__(256) k_solve_init(Ctx c, int mode, float dt, f
     if (!is_fluid(c, i)) {
         c.src[i] = 0.0f;
         c.kappa[i] = 0.0f;
-        c.kr[i] = 0.0f;
+        put_vr(c, c.kr, VR_KAPPA, i, 0.0f);
         c.lower[i] = 0.0f;
         return;
     }
@@ -1371,7 +1385,7 @@ __global__ void __launch_bounds__(256) k_solve_init(Ctx c, int mode, float dt, f
         c.kappa[i] = warm * fmn(store[i], 2.5e-4f) / (dt * dt);
         if (learned) c.kappa[i] = fmx(learned[i], 0.0f);
     }
-    c.kr[i] = c.kappa[i] / c.rho0[i];
+    put_vr(c, c.kr, VR_KAPPA, i, c.kappa[i] / c.rho0[i]);
 }
 
 // pressure.py:355-367
@@ -1412,16 +1426,25 @@ struct QKR {
     float kr;
 };
 // kapr = kap / rho0 per particle, written by the kernel that produced kap (the same division, gathered once)
+// With c.packed (one mass per world) each neighbour is one 16 B gather {x, y, z, kap / rho0} of the vector's packed
+// copy, the mass the world's; otherwise a position-and-mass gather plus the quotient's. The terms are the same floats.
 template <bool WIDE>
-__device__ f3 pressure_accel(const Ctx& c, const float* kap, const float* kapr, int i) {
+__device__ f3 pressure_accel(const Ctx& c, const float* kap, const float* kapr, int slot, int i) {
     f3 xi = xyz(c.q4[i]);
     float ki = kap[i] / c.rho0[i];
     f3 a{0.0f, 0.0f, 0.0f};
     int cnt = c.count[i];
     int base = c.ws[c.world[i]];
-    for_nbrs_g<WIDE, QKR>(c, i, xi, cnt, base, (c.pair_cache & DFSPH_PAIR_CACHE_ACCEL) != 0,
-                          [&](int j) { return QKR{c.q4[j], kapr[j]}; },
-                          [&](int, QKR d, f3 g) { a = a - (d.q.w * (ki + d.kr)) * g; });
+    const bool cached = (c.pair_cache & DFSPH_PAIR_CACHE_ACCEL) != 0;
+    if (c.packed) {
+        const float mw = c.w_mass[c.world[i]];
+        const float4* qk = c.qk + int64_t(slot) * c.n;
+        for_nbrs_g<WIDE, float4>(c, i, xi, cnt, base, cached, [&](int j) { return qk[j]; },
+                                 [&](int, float4 d, f3 g) { a = a - (mw * (ki + d.w)) * g; });
+    } else {
+        for_nbrs_g<WIDE, QKR>(c, i, xi, cnt, base, cached, [&](int j) { return QKR{c.q4[j], kapr[j]}; },
+                              [&](int, QKR d, f3 g) { a = a - (d.q.w * (ki + d.kr)) * g; });
+    }
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
         if (c.s_shape[b4 + b] >= 0) a = a - kap[i] * ld3(c.s_gpsi, b4 + b);
@@ -1469,9 +1492,18 @@ __device__ __forceinline__ bool block_skip(const Ctx& c, int w, int gated) {
     return gated != 0 && c.restart[w] == 0;
 }
 
+// the packed copies {x, y, z, vec / rho0} of the three vectors from q4 and the quotient arrays
+__global__ void __launch_bounds__(256) k_pack_sync(Ctx c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 q = c.q4[i];
+    const float* vr[3] = {c.kr, c.pdr, c.corrr};
+    for (int s = 0; s < 3; ++s) c.qk[int64_t(s) * c.n + i] = make_float4(q.x, q.y, q.z, vr[s][i]);
+}
+
 // accel_kernel (pressure.py:280-291) / cg_accel_kernel (krylov.py:77-87)
 template <bool WIDE>
-__global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, const float* vecr, int gated) {
+__global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, const float* vecr, int slot, int gated) {
     CHUNK_PROLOGUE
     if (block_skip(c, c.ch_world[b_], gated)) return;
     unsigned nb = 0;  // algorithmic bytes of this particle (profiling)
@@ -1494,7 +1526,7 @@ __global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, const fl
                 }
             }
             if (full) {
-                f3 a = pressure_accel<WIDE>(c, vec, vecr, i);
+                f3 a = pressure_accel<WIDE>(c, vec, vecr, slot, i);
                 c.acc4[i] = make_float4(a.x, a.y, a.z, 0.0f);
                 nb = pass_bytes<WIDE>(c, i, 69u + (gated && c.sparse_corr ? 1u : 0u), 17u);
             }
@@ -1550,7 +1582,7 @@ __global__ void __launch_bounds__(256) k_cg_init(Ctx c, int mode, float dt) {
             c.act[i] = 0;
             c.z[i] = 0.0f;
             c.pd[i] = 0.0f;
-            c.pdr[i] = 0.0f;
+            put_vr(c, c.pdr, VR_DIR, i, 0.0f);
         } else {
             float sc = mode == 0 ? dt : dt * dt;
             int a = 0;
@@ -1563,7 +1595,7 @@ __global__ void __launch_bounds__(256) k_cg_init(Ctx c, int mode, float dt) {
             c.act[i] = a;
             c.z[i] = zi;
             c.pd[i] = zi;
-            c.pdr[i] = zi / c.rho0[i];
+            put_vr(c, c.pdr, VR_DIR, i, zi / c.rho0[i]);
             px = double(-wi * zi);
         }
     }
@@ -1577,7 +1609,7 @@ __global__ void __launch_bounds__(256) k_cg_step(Ctx c) {
     if (c.converged[w] != 0 || !valid) return;
     if (!is_fluid(c, i)) {
         c.corr[i] = 0.0f;
-        c.corrr[i] = 0.0f;
+        put_vr(c, c.corrr, VR_CORR, i, 0.0f);
         return;
     }
     double pap = c.sums[2 * w];
@@ -1594,10 +1626,10 @@ __global__ void __launch_bounds__(256) k_cg_step(Ctx c) {
             c.restart[w] = 1;  // a flag: only restart != 0 is ever tested (krylov.py:70-74, 231)
         }
         c.kappa[i] = k;
-        c.kr[i] = k / c.rho0[i];
+        put_vr(c, c.kr, VR_KAPPA, i, k / c.rho0[i]);
     }
     c.corr[i] = cc;
-    c.corrr[i] = cc / c.rho0[i];
+    put_vr(c, c.corrr, VR_CORR, i, cc / c.rho0[i]);
     c.wv[i] = c.wv[i] + a * c.ap[i];
 }
 
@@ -1646,7 +1678,7 @@ __global__ void __launch_bounds__(256) k_cg_direction(Ctx c, int* flag_out) {
     float p = 0.0f;
     if (c.act[i] != 0) p = c.z[i] + float(c.stepb[w]) * c.pd[i];
     c.pd[i] = p;
-    c.pdr[i] = p / c.rho0[i];
+    put_vr(c, c.pdr, VR_DIR, i, p / c.rho0[i]);
 }
 
 // jacobi_kernel (pressure.py:313-352) + the chunk partial of (error, -, max error); block (0,0) clears the
@@ -1677,7 +1709,7 @@ __global__ void __launch_bounds__(256) k_jacobi(Ctx c, int mode, float dt) {
             float lb = c.lower[i];
             float k_new = fmx(c.kappa[i] - c.omega * rr * c.alpha[i] * scale, lb);
             c.kappa[i] = k_new;
-            c.kr[i] = k_new / c.rho0[i];
+            put_vr(c, c.kr, VR_KAPPA, i, k_new / c.rho0[i]);
             float e = fmx(-rr, 0.0f);
             if (k_new > lb) e = fabsf(rr);
             if (mode == 0) {
@@ -1717,7 +1749,7 @@ __global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, f
         }
     }
     // apply_pressure_kernel
-    f3 a = pressure_accel<WIDE>(c, c.kappa, c.kr, i);
+    f3 a = pressure_accel<WIDE>(c, c.kappa, c.kr, VR_KAPPA, i);
     f3 v = V(c, i) + dt * a;
     float kap = c.kappa[i];
     float mk = c.mass[i] * kap * dt;
@@ -2959,6 +2991,7 @@ struct Handle {
     const float* mlp_params = nullptr;  // learned warm start on when set
     int learned_solves = 3;             // DFSPH_OPT_LEARNED_SOLVES: bit 0 divergence, bit 1 density
     int build_block = 128;              // DFSPH_OPT_BUILD_BLOCK
+    int packed_ok = 0;                  // every world's SPH particles share one mass: DFSPH_OPT_PACKED_ACCEL allowed
     float* log_feat = nullptr;          // when set, the step copies its features [n][16] here ...
     float* log_targets = nullptr;       // ... and its converged-kappa targets [n][2] here
     hipStream_t cap_stream = nullptr;
@@ -3224,7 +3257,7 @@ int enq_solve_head(Handle* H, const SolveSetup& S, hipStream_t st) {
               float(P.warm_start_density), S.relief, (const float*)S.store, S.learned);
     k_begin_solve<<<blocks_for(c.W), 256, 0, st>>>(c);
     if (P.pressure_solver == 0) {
-        LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.kappa, c.kr, 0);
+        LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.kappa, c.kr, VR_KAPPA, 0);
         LAUNCH_WT(KID_APPLY, k_apply, H->n_chunks, 256, c, S.mode, S.dt, c.kappa, c.wv, 0, 1, 0, 0);
         k_cg_init<<<H->n_chunks, 256, 0, st>>>(c, S.mode, S.dt);
         k_world<<<c.W, 256, 0, st>>>(c, W_CG_INIT, S.sp);
@@ -3240,17 +3273,17 @@ int enq_solve_batch(Handle* H, const SolveSetup& S, hipStream_t st, int* flag_ou
     float dt = S.dt;
     for (int rep = 0; rep < H->check_every; ++rep) {
         if (H->P.pressure_solver == 0) {
-            LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.pd, c.pdr, 0);
+            LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.pd, c.pdr, VR_DIR, 0);
             LAUNCH_WT(KID_APPLY, k_apply, H->n_chunks, 256, c, mode, dt, c.pd, c.ap, 0, 0, 0, 1);
             k_world<<<c.W, 256, 0, st>>>(c, W_CG_PAP, S.sp);
             k_cg_step<<<H->n_chunks, 256, 0, st>>>(c);
-            LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.corr, c.corrr, 1);
+            LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.corr, c.corrr, VR_CORR, 1);
             LAUNCH_WT(KID_APPLY, k_apply, H->n_chunks, 256, c, mode, dt, c.corr, c.wv, 1, 0, 1, 0);
             k_cg_finish<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
             k_world<<<c.W, 256, 0, st>>>(c, W_CG_BETA, S.sp);
             k_cg_direction<<<H->n_chunks, 256, 0, st>>>(c, rep == H->check_every - 1 ? flag_out : nullptr);
         } else {
-            LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.kappa, c.kr, 0);
+            LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.kappa, c.kr, VR_KAPPA, 0);
             LAUNCH_W(k_jacobi, H->n_chunks, 256, c, mode, dt);
             k_world<<<c.W, 256, 0, st>>>(c, W_JACOBI, S.sp);
         }
@@ -3275,7 +3308,8 @@ int run_segment(Handle* H, hipStream_t st, int kind, int mode, float dt, F enque
     if (!H->use_graphs || H->prof.on) return enqueue(st);
     uint32_t bits;
     std::memcpy(&bits, &dt, 4);
-    GraphKey key{kind, mode, H->check_every, H->c.pair_cache | (H->c.sparse_corr << 8) | (H->build_block << 16), bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
+    GraphKey key{kind, mode, H->check_every,
+                 H->c.pair_cache | (H->c.sparse_corr << 8) | (H->build_block << 16) | (H->c.packed << 28), bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
                  H->mlp_params, H->log_feat, H->log_targets};
     auto it = H->graphs.find(key);
     if (it == H->graphs.end()) {
@@ -3654,6 +3688,38 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         TRY(dalloc(H, &c.nbr, int64_t(c.cap) * std::max(n, 1)));
         TRY(dalloc(H, &c.nbr_hi, c.wide ? int64_t(c.cap) * std::max(n, 1) : 1));
         TRY(dalloc(H, &c.pgrad, int64_t(c.cap) * std::max(n, 1)));
+        {  // one mass per world (bitwise) over its SPH particles: the packed acceleration gathers may stand in for it
+            std::vector<float> hm(std::max(n, 1));
+            std::vector<int8_t> hp(std::max(n, 1));
+            if (n > 0) {
+                HIPCHK(hipMemcpy(hm.data(), M->mass, sizeof(float) * size_t(n), hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(hp.data(), M->phase, size_t(n), hipMemcpyDeviceToHost));
+            }
+            std::vector<float> wm(std::max(W, 1), 0.0f);
+            bool ok = true;
+            for (int w = 0; w < W && ok; ++w) {
+                bool seen = false;
+                for (int i = L->world_start[w]; i < L->world_start[w + 1]; ++i) {
+                    if (hp[i] < 0) continue;
+                    uint32_t a, b;
+                    std::memcpy(&a, &hm[i], 4);
+                    std::memcpy(&b, &wm[w], 4);
+                    if (!seen) {
+                        wm[w] = hm[i];
+                        seen = true;
+                    } else if (a != b) {
+                        ok = false;
+                        break;
+                    }
+                }
+            }
+            H->packed_ok = ok ? 1 : 0;
+            float* wm_dev = nullptr;
+            TRY(dupload(H, &wm_dev, wm.data(), std::max(W, 1)));
+            c.w_mass = wm_dev;
+            if (H->packed_ok) TRY(dalloc(H, &c.qk, 3 * int64_t(std::max(n, 1))));
+            c.packed = H->packed_ok;
+        }
         if (P->viscosity_implicit) {  // viscosity.py:213-234
             for (float** p : {&c.vb, &c.vx, &c.vr, &c.vz, &c.vp, &c.vap}) TRY(dalloc(H, p, 3 * int64_t(n)));
             TRY(dalloc(H, &c.vminv, 9 * int64_t(n)));
@@ -3999,6 +4065,19 @@ int32_t dfsph_set_option(void* h, int32_t option, int32_t value) {
             if (value < 0 || value > 1) return fail(DFSPH_EINVAL, "sparse correction: 0 or 1");
             H->c.sparse_corr = value;
             return DFSPH_OK;
+        case DFSPH_OPT_PACKED_ACCEL:
+            if (value < 0 || value > 1) return fail(DFSPH_EINVAL, "packed accel: 0 or 1");
+            if (value == 1 && !H->packed_ok)
+                return fail(DFSPH_EINVAL, "packed accel needs one particle mass per world");
+            if (H->c.packed != value) {
+                H->c.packed = value;
+                if (value) {  // bring the packed copies up to date, so a phase call before the next step reads them
+                    k_pack_sync<<<blocks_for(H->c.n), 256>>>(H->c);
+                    LAUNCH_CHECK();
+                    HIPCHK(hipDeviceSynchronize());
+                }
+            }
+            return DFSPH_OK;
         case DFSPH_OPT_LEARNED_SOLVES:
             if (value < 0 || value > 3) return fail(DFSPH_EINVAL, "learned solves: bit 0 divergence, bit 1 density");
             H->learned_solves = value;
@@ -4026,6 +4105,9 @@ int32_t dfsph_get_option(void* h, int32_t option, int32_t* value) {
             return DFSPH_OK;
         case DFSPH_OPT_SPARSE_CORRECTION:
             *value = H->c.sparse_corr;
+            return DFSPH_OK;
+        case DFSPH_OPT_PACKED_ACCEL:
+            *value = H->c.packed;
             return DFSPH_OK;
         case DFSPH_OPT_BUILD_BLOCK:
             *value = H->build_block;

@@ -41,8 +41,8 @@ FORCE_SCALE = float(2**40)  # bodies.py:23
 
 # kernel ids of dfsph_profile_read
 # tuning options (dfsph_set_option); none changes a result bit
-OPT_CHECK_INTERVAL, OPT_GRAPHS, OPT_PAIR_CACHE, OPT_LEARNED_SOLVES, OPT_SPARSE_CORRECTION, OPT_BUILD_BLOCK = (
-    1, 2, 3, 4, 5, 6)
+(OPT_CHECK_INTERVAL, OPT_GRAPHS, OPT_PAIR_CACHE, OPT_LEARNED_SOLVES, OPT_SPARSE_CORRECTION, OPT_BUILD_BLOCK,
+ OPT_PACKED_ACCEL) = (1, 2, 3, 4, 5, 6, 7)
 PAIR_CACHE_ACCEL, PAIR_CACHE_APPLY, PAIR_CACHE_OTHER, PAIR_CACHE_ALL = 1, 2, 4, 7
 
 (KID_ACCEL, KID_APPLY, KID_BUILD, KID_DENSITY, KID_SLOTS, KID_NONPRESSURE, KID_ADVECT, KID_SOLVE_INIT,

@@ -25,12 +25,13 @@ CANDIDATES = (
     (_abi.OPT_CHECK_INTERVAL, (1, 2, 3)),
     (_abi.OPT_BUILD_BLOCK, (128, 64, 256)),
     (_abi.OPT_SPARSE_CORRECTION, (1, 0)),
+    (_abi.OPT_PACKED_ACCEL, (1, 0)),
     (_abi.OPT_PAIR_CACHE, (0, _abi.PAIR_CACHE_APPLY, _abi.PAIR_CACHE_ALL)),
     (_abi.OPT_GRAPHS, (0, 1)),
 )
 NAMES = {_abi.OPT_CHECK_INTERVAL: "check_interval", _abi.OPT_BUILD_BLOCK: "build_block",
          _abi.OPT_SPARSE_CORRECTION: "sparse_correction", _abi.OPT_PAIR_CACHE: "pair_cache", _abi.OPT_GRAPHS: "graphs",
-         _abi.OPT_LEARNED_SOLVES: "learned_solves"}
+         _abi.OPT_LEARNED_SOLVES: "learned_solves", _abi.OPT_PACKED_ACCEL: "packed_accel"}
 
 
 class _Snapshot:

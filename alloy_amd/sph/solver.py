@@ -15,6 +15,7 @@ emitters) are refused with an error naming them, rather than silently ignored.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import logging
 from collections.abc import Mapping
 from typing import ClassVar
@@ -145,8 +146,6 @@ class SolverDFSPH:
         self.capillary_dt_violations = 0
         self._dt_limit = csf.capillary_limit(model, cfg) if self.tension == "csf" else float("inf")
         if self._dt_limit < cfg.dt_max:  # the capillary step bounds the stable step
-            import dataclasses
-
             self.config = cfg = dataclasses.replace(cfg, dt_max=max(self._dt_limit, cfg.dt_min))
         self.tension_rh_range = None
         if self.tension == "csf":

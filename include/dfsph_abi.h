@@ -286,8 +286,12 @@ enum DfsphOption {
     DFSPH_OPT_SPARSE_CORRECTION = 5, /* 1 (default): the CG iteration's correction product (krylov.py:238, A corr
                                         on clamped particles) evaluates only particles with a nonzero term and writes
                                         the others' exact zero results; 0 evaluates every particle. Bit-identical. */
-    DFSPH_OPT_BUILD_BLOCK = 6        /* threads per block of the neighbour-list build: 64, 128 (default) or 256
+    DFSPH_OPT_BUILD_BLOCK = 6,       /* threads per block of the neighbour-list build: 64, 128 (default) or 256
                                         (256 only without 24-bit rows). Bit-identical. */
+    DFSPH_OPT_PACKED_ACCEL = 7       /* 1 (default when every world's SPH particles share one mass): the pressure
+                                        acceleration gathers one 16 B record {x, y, z, kappa / rho0} per neighbour
+                                        from a packed copy, the mass the world's; 0: positions and quotients apart.
+                                        Bit-identical. */
 };
 enum DfsphPairCache {
     DFSPH_PAIR_CACHE_ACCEL = 1, /* pressure acceleration (k_accel, post-solve) */

@@ -50,7 +50,8 @@ def test_options_are_bit_identical(scene_name):
     for opts in ({_abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ALL}, {_abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_APPLY},
                  {_abi.OPT_CHECK_INTERVAL: 3}, {_abi.OPT_GRAPHS: 1, _abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ACCEL},
                  {_abi.OPT_SPARSE_CORRECTION: 0}, {_abi.OPT_SPARSE_CORRECTION: 0, _abi.OPT_GRAPHS: 1},
-                 {_abi.OPT_BUILD_BLOCK: 64}, {_abi.OPT_BUILD_BLOCK: 256}):
+                 {_abi.OPT_BUILD_BLOCK: 64}, {_abi.OPT_BUILD_BLOCK: 256}, {_abi.OPT_PACKED_ACCEL: 0},
+                 {_abi.OPT_PACKED_ACCEL: 0, _abi.OPT_GRAPHS: 1, _abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ALL}):
         q, v, it = _run(sc, 12, opts)
         _log(f"  {opts}: q equal {np.array_equal(q, q0)} qd equal {np.array_equal(v, v0)}")
         assert np.array_equal(q, q0) and np.array_equal(v, v0), opts
@@ -75,6 +76,36 @@ def test_option_abi_round_trip_and_errors():
     assert s.get_option(_abi.OPT_SPARSE_CORRECTION) == 1
     s.set_option(_abi.OPT_SPARSE_CORRECTION, 0)
     assert s.get_option(_abi.OPT_SPARSE_CORRECTION) == 0
+    assert s.get_option(_abi.OPT_PACKED_ACCEL) == 1  # one mass per world: on by default
+    s.set_option(_abi.OPT_PACKED_ACCEL, 0)
+    assert s.get_option(_abi.OPT_PACKED_ACCEL) == 0
+    s.set_option(_abi.OPT_PACKED_ACCEL, 1)
+    assert s.get_option(_abi.OPT_PACKED_ACCEL) == 1
+
+
+def test_packed_accel_needs_one_mass_per_world():
+    """Two rest densities (so two particle masses) in one world: the packed gathers cannot stand in for the masses,
+    so the option is off and refuses to turn on; the step still runs (positions and quotients apart)."""
+    import numpy as np
+
+    from alloy_amd.sph import _abi, scenes
+    from alloy_amd.sph.build import add_fluid_block
+    from alloy_amd.sph.model import ModelBuilder
+
+    cfg = scenes.bench_config(pressure_solver="jacobi")
+    b = ModelBuilder(gravity=(0.0, 0.0, -9.81))
+    b.begin_world()
+    add_fluid_block(b, cfg, (0.0, 0.0, 0.0), (4, 4, 3), rest_density=1000.0)
+    add_fluid_block(b, cfg, (0.0, 0.0, 3 * cfg.spacing), (4, 4, 3), rest_density=800.0)
+    b.end_world()
+    m = b.finalize("cuda")
+    s = scenes.Scene(m, cfg).make_solver()
+    assert s.get_option(_abi.OPT_PACKED_ACCEL) == 0
+    with pytest.raises(ValueError, match="one particle mass per world"):
+        s.set_option(_abi.OPT_PACKED_ACCEL, 1)
+    st = m.state()
+    s.step(st, st, None, None, DT)
+    assert np.isfinite(st.particle_q.cpu().numpy()).all()
 
 
 def test_profiled_bytes_match_the_host_formula():
@@ -130,7 +161,8 @@ def test_autotune_leaves_the_simulation_unchanged():
         b.step(sb, sb, None, None, DT)
     res = autotune(a, sa, DT, steps=2, warm=1, rounds=1, log=_log)
     _log(f"[autotune] {res['options']} in {res['seconds']:.1f} s")
-    assert set(res["options"]) == {"check_interval", "build_block", "sparse_correction", "pair_cache", "graphs"}
+    assert set(res["options"]) == {"check_interval", "build_block", "sparse_correction", "packed_accel", "pair_cache",
+                                   "graphs"}
     for _ in range(6):
         a.step(sa, sa, None, None, DT)
         b.step(sb, sb, None, None, DT)
