@@ -13,7 +13,11 @@
 // density/alpha by the rest-lattice identities (ratio 1 in the bulk and at flat walls, alpha = 1/lattice
 // diagonal); solves by residual <= tolerance, kappa >= lower bound, CG/Jacobi agreement and resting-column
 // physics; boundary slots by flat-wall, corner and facing-wall special cases; particle walls by the colour
-// inversion round trip; advect/project by x + v dt and the push-out invariant phi >= eps.
+// inversion round trip; advect/project by x + v dt and the push-out invariant phi >= eps; emitters and ledgers by
+// the reference's exact-ledger tests; the implicit viscosity by a shear mode's decay against nu k^2, rigid motion
+// left untouched and zero viscosity as the identity; its wall term by Nusselt film drainage and the walls'
+// impulse balancing the liquid's momentum; CSF by a lattice drop's curvature 2 / R, Laplace pressure and small
+// clusters gaining no energy (tests/test_oracle_viscosity.py, test_oracle_wall_viscosity.py, test_oracle_csf.py).
 //
 // Differences from the reference that are deliberate and shared with the HIP path:
 //  - the hash grid is a per-world dense cell grid (cell side h (1 + 1e-5), clamped at its edges). The neighbour
