@@ -71,6 +71,8 @@ struct Ctx {
     const int* ch_end;
     const int* ch_world;
     const int* w_chunk;
+    int* ch_live;  // [n_chunks] 1 when the chunk holds liquid this step (k_groups_hist); the solve passes skip the
+                   // other chunks, whose particles they would only write zeros for that no pass reads
     const float* g_org;
     const int* g_dims;
     const int* cell_base;
@@ -348,6 +350,16 @@ __device__ __forceinline__ void chunk_reduce(const Ctx& c, double x, double y, d
     }
 }
 
+// The chunk partial of a chunk without liquid: every term is +0 and every max term 0.0 (each chunk has a particle)
+__device__ __forceinline__ void chunk_zero(const Ctx& c) {
+    if (threadIdx.x == 0) {
+        double* p = c.part + 3 * int64_t(xcd_block(int(blockIdx.x), int(gridDim.x)));
+        p[0] = 0.0;
+        p[1] = 0.0;
+        p[2] = 0.0;
+    }
+}
+
 enum WorldMode { W_NACTIVE = 0, W_CG_INIT, W_CG_PAP, W_CG_BETA, W_JACOBI, W_FLOW_Z, W_FLOW_XY, W_HEAD, W_VISC_BEGIN,
                  W_VISC_ALPHA, W_VISC_CHECK };
 
@@ -516,6 +528,7 @@ __global__ void __launch_bounds__(256) k_groups_hist(Ctx c) {
     if ((fl & ACTIVE) != 0 && int(c.phase[i]) >= 1) g = c.world[i];
     else if (c.NW > 0 && c.wvol[i] > 0.0f && (fl & ACTIVE) != 0 && c.world[i] >= 0) g = -2 - c.world[i];
     c.group[i] = g;
+    if (g >= 0) c.ch_live[c.w_chunk[g] + (i - c.ws[g]) / CHUNK] = 1;
     f3 xi = Q(c, i);
     c.q4[i] = make_float4(xi.x, xi.y, xi.z, c.mass[i]);
     if (c.packed)  // the w components are written by the vector's producer before any read this step
@@ -1505,7 +1518,7 @@ __global__ void __launch_bounds__(256) k_pack_sync(Ctx c) {
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, const float* vecr, int slot, int gated) {
     CHUNK_PROLOGUE
-    if (block_skip(c, c.ch_world[b_], gated)) return;
+    if (block_skip(c, c.ch_world[b_], gated) || !c.ch_live[b_]) return;
     unsigned nb = 0;  // algorithmic bytes of this particle (profiling)
     if (valid) {
         if (!is_fluid(c, i)) {
@@ -1541,6 +1554,10 @@ __global__ void __launch_bounds__(256) k_apply(Ctx c, int mode, float dt, const 
                                                int bias, int accum, int reduce) {
     CHUNK_PROLOGUE
     if (block_skip(c, c.ch_world[b_], gated)) return;
+    if (!c.ch_live[b_]) {
+        if (reduce) chunk_zero(c);
+        return;
+    }
     unsigned nb = 0;  // algorithmic bytes of this particle (profiling)
     double px = 0.0;
     if (valid) {
@@ -1576,6 +1593,10 @@ __global__ void __launch_bounds__(256) k_apply(Ctx c, int mode, float dt, const 
 __global__ void __launch_bounds__(256) k_cg_init(Ctx c, int mode, float dt) {
     CHUNK_PROLOGUE
     if (c.converged[c.ch_world[b_]] != 0) return;
+    if (!c.ch_live[b_]) {
+        chunk_zero(c);
+        return;
+    }
     double px = 0.0;
     if (valid) {
         if (!is_fluid(c, i)) {
@@ -1606,7 +1627,7 @@ __global__ void __launch_bounds__(256) k_cg_init(Ctx c, int mode, float dt) {
 __global__ void __launch_bounds__(256) k_cg_step(Ctx c) {
     CHUNK_PROLOGUE
     int w = c.ch_world[b_];
-    if (c.converged[w] != 0 || !valid) return;
+    if (c.converged[w] != 0 || !valid || !c.ch_live[b_]) return;
     if (!is_fluid(c, i)) {
         c.corr[i] = 0.0f;
         put_vr(c, c.corrr, VR_CORR, i, 0.0f);
@@ -1638,6 +1659,11 @@ __global__ void __launch_bounds__(256) k_cg_finish(Ctx c, int mode, float dt) {
     CHUNK_PROLOGUE
     int w = c.ch_world[b_];
     if (c.converged[w] != 0) return;
+    if (!c.ch_live[b_]) {  // the error of a particle that is not liquid is 0
+        if (valid) c.err[i] = 0.0f;
+        chunk_zero(c);
+        return;
+    }
     double px = 0.0, py = 0.0;
     if (valid) {
         if (!is_fluid(c, i)) {
@@ -1674,7 +1700,7 @@ __global__ void __launch_bounds__(256) k_cg_direction(Ctx c, int* flag_out) {
     if (flag_out && blockIdx.x == 0 && threadIdx.x == 0)
         __hip_atomic_store(flag_out, *c.n_unconv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     int w = c.ch_world[b_];
-    if (c.converged[w] != 0 || !valid || !is_fluid(c, i)) return;
+    if (c.converged[w] != 0 || !valid || !c.ch_live[b_] || !is_fluid(c, i)) return;
     float p = 0.0f;
     if (c.act[i] != 0) p = c.z[i] + float(c.stepb[w]) * c.pd[i];
     c.pd[i] = p;
@@ -3127,6 +3153,7 @@ int phase_neighbors(Handle* H, hipStream_t st) {
     if (c.n == 0) return DFSPH_OK;
     k_fill_i32<<<std::min(blocks_for(int64_t(c.n_cells) + 1), 4096), 256, 0, st>>>(c.cell_count, 0,
                                                                                    int64_t(c.n_cells) + 1);
+    k_fill_i32<<<blocks_for(c.n_chunks), 256, 0, st>>>(c.ch_live, 0, c.n_chunks);
     k_groups_hist<<<blocks_for(c.n), 256, 0, st>>>(c);
     k_scan_reduce<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part);
     k_scan_top<<<1, 256, 0, st>>>(c.scan_part, H->n_scan_blocks, c.cell_start, c.n_cells);
@@ -3620,6 +3647,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         TRY(dupload(H, (int**)&c.ch_end, ce.data(), int64_t(ce.size())));
         TRY(dupload(H, (int**)&c.ch_world, cw.data(), int64_t(cw.size())));
         TRY(dupload(H, (int**)&c.w_chunk, wc.data(), int64_t(wc.size())));
+        TRY(dalloc(H, &c.ch_live, int64_t(cb.size())));
         std::vector<int> ws(L->world_start, L->world_start + W + 1);
         TRY(dupload(H, (int**)&c.ws, ws.data(), W + 1));
         // grid
