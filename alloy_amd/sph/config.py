@@ -1,10 +1,9 @@
 """DFSPHConfig: the solver's configuration, field for field as alloy/solvers/sph/config.py:46-199, with the same
 defaults and the same ``validate`` messages, plus the time-step helpers (config.py:209-236).
 
-Fields that select subsystems this build does not implement (implicit viscosity, surface tension, ambient
-pressure, Verlet skin, coupling modes other than the default) are kept so a caller's config round-trips; ``validate``
-accepts exactly what the reference accepts, and the solver raises ValueError when a field asks for a subsystem that
-is not built here (solver.py).
+Fields that select subsystems this library does not implement (ambient pressure, Akinci 2013 tension) are kept so
+a caller's config round-trips; ``validate`` accepts exactly what the reference accepts, and the solver raises
+ValueError when a field asks for a subsystem that is not implemented (solver.py).
 """
 
 from __future__ import annotations

@@ -1,15 +1,16 @@
 """SolverDFSPH: the DFSPH liquid plug-in, host side, driving the gfx950 kernels of libdfsph_hip.so.
 
 Same construction, methods, errors and step order as alloy/solvers/sph/solver.py:57-484: ``step`` runs neighbours
--> boundary slots -> density and factor -> divergence solve -> gravity, external forces and XSPH -> density solve ->
-shape meters -> advection and wall projection -> sinks, escape box and box emitters -> speed statistics, all on the
-current torch stream. Every work buffer is allocated once at construction; ``step`` allocates nothing.
+-> boundary slots -> density and factor -> divergence solve -> gravity, external forces and XSPH -> CSF surface
+tension -> implicit viscosity -> density solve -> shape meters -> advection and wall projection -> sinks, escape box
+and box emitters -> speed statistics, all on the current torch stream. Every work buffer is allocated once at
+construction; ``step`` allocates nothing.
 
 The mass ledger is int64 fixed point on the device (2^40 per kg, runtime.py:93-153), so active + out == initial + in
 holds exactly. ``reneighbor_skin`` reuses superset neighbour rows between rebuilds (neighbors_skin.py; the rows stay
-equal to a plain build, bit for bit). Subsystems of the reference that this build does not implement (implicit
-viscosity, surface tension, ambient pressure, soft walls, bodies attached to shapes and the body-attached patch
-emitters) are refused with an error naming them, rather than silently ignored.
+equal to a plain build, bit for bit). Subsystems of the reference that this library does not implement (ambient
+pressure, Akinci 2013 tension, soft walls, bodies attached to shapes and the body-attached patch emitters) are
+refused with an error naming them, rather than silently ignored.
 """
 
 from __future__ import annotations

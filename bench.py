@@ -309,8 +309,9 @@ def main() -> int:
                          "kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (worlds shard across "
-              "GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this benchmark measures one GPU; --gpus {args.gpus} is not supported (the batched packs "
+              "shard whole worlds across GPUs with no exchange, so a pack's rate scales with its GPU count)",
+              file=sys.stderr)
         return 2
 
     import torch

@@ -1,8 +1,8 @@
 /* C-ABI of the MI355X DFSPH liquid step (libdfsph_hip.so).
  *
  * The entry points replace the Warp launches of alloy/solvers/sph that SolverDFSPH.step issues
- * (alloy/solvers/sph/solver.py:218-268). Python binds them with ctypes (alloy_amd/sph/_abi.py); a
- * maintainer's binding recipe is in INTEGRATION.md.
+ * (alloy/solvers/sph/solver.py:218-268). Python binds them with ctypes (alloy_amd/sph/_abi.py), the
+ * binding a host that loads the library directly would copy.
  *
  * Conventions
  *  - Status 0 is success. A negative status comes with a thread-local message (dfsph_last_error):

@@ -1,5 +1,5 @@
 """The plug-in surface around the step (solver.py:343-479): shape-change notification, retiring particles, shape
-work and the body-attached patch emitters this build does not provide."""
+work and the body-attached patch emitters this library does not provide."""
 
 from __future__ import annotations
 
