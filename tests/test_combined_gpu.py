@@ -86,3 +86,42 @@ def test_all_stages_match_the_oracle_bitwise(options):
     assert np.all(led["in"] > 0)
     _log(f"[combined] {rebuilds} world rebuilds in 40 steps; emitted {led['in'].tolist()}, sunk {led['out'].tolist()}")
     assert rebuilds < 2 * 40
+
+
+def test_tension_and_wall_viscosity_in_particle_wall_bowls_match_the_oracle_bitwise():
+    """Two particle-wall bowls (r = 2.5 mm, small radii) of a viscous liquid with CSF tension and the wall term: the
+    colour-field slots feed the curvature's wall turn (free there: particle walls carry no contact angle), the wall
+    term and the reaction split; 30 steps bitwise, the walls' exchange impulses too."""
+    import dataclasses
+
+    import torch
+
+    from alloy_amd.sph import _abi, scenes
+    from alloy_amd.sph.config import DFSPHConfig
+    from oracle.oracle import OracleDFSPH
+
+    cfg = DFSPHConfig(particle_radius=2.5e-3, viscosity="weiler2018", boundary_viscosity=0.2, surface_tension="csf",
+                      rest_jitter_speed=None)
+    base = scenes.bowl_pool(2, bowl_radius=0.03, side=0.1, cfg=cfg, device="cuda", seed=3)
+    base.model.sph.viscosity.fill_(0.2)
+    liquid = base.model.sph.phase >= 0
+    base.model.sph.surface_tension.copy_(torch.where(liquid, 0.07, 0.0).to(torch.float32))
+    sc = dataclasses.replace(base, config=cfg)
+    s = sc.make_solver()
+    o = OracleDFSPH(sc.model, cfg, 32)
+    st = sc.model.state()
+    n = sc.model.particle_count
+    dt = min(1.0e-3, s.config.dt_max)
+    for k in range(30):
+        s.step(st, st, None, None, dt)
+        o.step(dt)
+        torch.cuda.synchronize()
+        what = f"bowls step {k}"
+        gq, gv = st.particle_q.cpu().numpy(), st.particle_qd.cpu().numpy()
+        assert np.array_equal(gq, o.q), f"{what}: positions differ at {np.nonzero((gq != o.q).any(1))[0][:8]}"
+        assert np.array_equal(gv, o.qd), f"{what}: velocities differ at {np.nonzero((gv != o.qd).any(1))[0][:8]}"
+        assert np.array_equal(s.buffer(_abi.BUF_CSF_SURF, np.int32, (n,)), o.buffer(_abi.BUF_CSF_SURF)[:n]), what
+    assert s.pw.count > 0
+    assert np.array_equal(s.buffer(_abi.BUF_PW_EXCH, np.int64, (s.pw.count * 3,)),
+                          o.buffer(_abi.BUF_PW_EXCH)[:s.pw.count * 3])
+    assert np.any(s.buffer(_abi.BUF_PW_EXCH, np.int64, (s.pw.count * 3,)) != 0)
