@@ -94,6 +94,11 @@ struct Ctx {
     long long* pw_exch;  // [NW*3] impulse of the exchange window per wall particle, 2^50 N s
     long long* pw_work;  // [W] the walls' work on the liquid in the last step, 2^50 J
     float* pw_col;       // [n] the particle-wall colour of each liquid particle, from the step's slot pass
+    // the wall particles the slot pass's colour sum visited, in its order: [PW_ROW][n] compact wall index and
+    // V_j W_ij, and the count (-1: more than PW_ROW; the reaction split then walks the cells itself)
+    int* wr_id;
+    float* wr_w;
+    int* wr_n;
     // Verlet-skin reuse (neighbors_skin.py): superset rows within h + skin at each world's last build
     float skin;          // 0: off
     int cap_s;           // superset row capacity (<= 64)
@@ -853,7 +858,10 @@ __global__ void __launch_bounds__(256) k_skin_filter(Ctx c) {
 
 // ----------------------------------------------------------------------------------------------- particle walls
 // particle_walls.py:65-84: colour, grad colour and mean velocity of world w's wall particles at x
-__device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv) {
+constexpr int PW_ROW = 32;  // wall particles recorded per liquid particle for the reaction split
+
+// rec >= 0: record the visited wall particles of liquid particle rec (wr_id / wr_w / wr_n)
+__device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv, int rec) {
     const KC K = make_kc(c.h, c.kind);
     col = 0.0f;
     gc = f3{0.0f, 0.0f, 0.0f};
@@ -861,7 +869,11 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv)
     if (c.NW == 0) return;
     int g = c.W + w, cx, cy, cz;
     int own = cell_of(c, g, x, cx, cy, cz);
-    if (!c.wall_near[own - c.wall_first]) return;  // no wall particle in the 27 cells: the sums below stay 0
+    if (!c.wall_near[own - c.wall_first]) {  // no wall particle in the 27 cells: the sums below stay 0
+        if (rec >= 0) c.wr_n[rec] = 0;
+        return;
+    }
+    int nrec = 0;
     const int* d = c.g_dims + 3 * w;
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
@@ -877,9 +889,15 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv)
                         col += wj;
                         gc = gc + c.wvol[j] * kernel_grad(dd, K);
                         mv = mv + wj * V(c, j);
+                        if (rec >= 0 && nrec < PW_ROW) {
+                            c.wr_id[int64_t(nrec) * c.n + rec] = c.wcidx[j];
+                            c.wr_w[int64_t(nrec) * c.n + rec] = wj;
+                        }
+                        nrec += 1;
                     }
                 }
             }
+    if (rec >= 0) c.wr_n[rec] = nrec <= PW_ROW ? nrec : -1;
     if (col > 0.0f) mv = mv / col;
 }
 // particle_walls.py:87-98
@@ -918,34 +936,42 @@ __global__ void __launch_bounds__(256) k_pw_accumulate(Ctx c) {
         long long left[3] = {llround(double(J.x) * IMP), llround(double(J.y) * IMP), llround(double(J.z) * IMP)};
         int best = -1;
         float sbest = 0.0f;
-        int g = c.W + w, cx, cy, cz;
-        cell_of(c, g, x, cx, cy, cz);
-        const int* d = c.g_dims + 3 * w;
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    int X = cx + dx, Y = cy + dy, Z = cz + dz;
-                    if (X < 0 || Y < 0 || Z < 0 || X >= d[0] || Y >= d[1] || Z >= d[2]) continue;
-                    int cc = c.cell_base[g] + (X * d[1] + Y) * d[2] + Z;
-                    for (int e = c.cell_start[cc]; e < c.cell_start[cc + 1]; ++e) {
-                        int j = c.items[e];
-                        float sj = c.wvol[j] * kernel_w(length(x - xyz(c.q4[j])), K) / col;
-                        if (sj > 0.0f) {
-                            int cj = c.wcidx[j];
-                            long long part[3] = {llround(double(J.x * sj) * IMP), llround(double(J.y * sj) * IMP),
-                                                 llround(double(J.z * sj) * IMP)};
-                            for (int q = 0; q < 3; ++q) {
-                                atomicAdd(reinterpret_cast<unsigned long long*>(c.pw_exch) + 3 * int64_t(cj) + q,
-                                          static_cast<unsigned long long>(part[q]));
-                                left[q] -= part[q];
-                            }
-                            if (sj > sbest) {
-                                sbest = sj;
-                                best = cj;
-                            }
+        auto share = [&](int cj, float wj) {  // wj = V_j W_ij as the slot pass summed it
+            float sj = wj / col;
+            if (sj > 0.0f) {
+                long long part[3] = {llround(double(J.x * sj) * IMP), llround(double(J.y * sj) * IMP),
+                                     llround(double(J.z * sj) * IMP)};
+                for (int q = 0; q < 3; ++q) {
+                    atomicAdd(reinterpret_cast<unsigned long long*>(c.pw_exch) + 3 * int64_t(cj) + q,
+                              static_cast<unsigned long long>(part[q]));
+                    left[q] -= part[q];
+                }
+                if (sj > sbest) {
+                    sbest = sj;
+                    best = cj;
+                }
+            }
+        };
+        const int nrow = c.wr_n[i];
+        if (nrow >= 0) {  // the slot pass's wall particles, in its visiting order
+            for (int k = 0; k < nrow; ++k) share(c.wr_id[int64_t(k) * c.n + i], c.wr_w[int64_t(k) * c.n + i]);
+        } else {  // more than the row holds: visit the cells again, in the same order
+            int g = c.W + w, cx, cy, cz;
+            cell_of(c, g, x, cx, cy, cz);
+            const int* d = c.g_dims + 3 * w;
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        int X = cx + dx, Y = cy + dy, Z = cz + dz;
+                        if (X < 0 || Y < 0 || Z < 0 || X >= d[0] || Y >= d[1] || Z >= d[2]) continue;
+                        int cc = c.cell_base[g] + (X * d[1] + Y) * d[2] + Z;
+                        for (int e = c.cell_start[cc]; e < c.cell_start[cc + 1]; ++e) {
+                            int j = c.items[e];
+                            float wj = c.wvol[j] * kernel_w(length(x - xyz(c.q4[j])), K);
+                            if (wj > 0.0f) share(c.wcidx[j], wj);
                         }
                     }
-                }
+        }
         if (best >= 0)
             for (int q = 0; q < 3; ++q)
                 atomicAdd(reinterpret_cast<unsigned long long*>(c.pw_exch) + 3 * int64_t(best) + q,
@@ -1116,7 +1142,7 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
     f3 pgc{0.0f, 0.0f, 0.0f}, pmv{0.0f, 0.0f, 0.0f};
     bool have_pw = false;
     if (c.NW > 0) {
-        pw_sample(c, x, w, pcol, pgc, pmv);
+        pw_sample(c, x, w, pcol, pgc, pmv, i);
         c.pw_col[i] = pcol;
         have_pw = true;
         float phi = 1.0e30f;
@@ -3686,6 +3712,9 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
             TRY(dalloc(H, &c.pw_exch, 3 * int64_t(std::max(nw, 1))));
             TRY(dalloc(H, &c.pw_work, W));
             TRY(dalloc(H, &c.pw_col, nw > 0 ? n : 1));
+            TRY(dalloc(H, &c.wr_n, nw > 0 ? n : 1));
+            TRY(dalloc(H, &c.wr_id, nw > 0 ? int64_t(PW_ROW) * n : 1));
+            TRY(dalloc(H, &c.wr_w, nw > 0 ? int64_t(PW_ROW) * n : 1));
         }
         H->n_scan_blocks = blocks_for(c.n_cells, 1024);
         TRY(dalloc(H, &c.scan_part, H->n_scan_blocks));
