@@ -881,13 +881,20 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv,
                 int X = cx + dx, Y = cy + dy, Z = cz + dz;
                 if (X < 0 || Y < 0 || Z < 0 || X >= d[0] || Y >= d[1] || Z >= d[2]) continue;
                 int cc = c.cell_base[g] + (X * d[1] + Y) * d[2] + Z;
+                // q4s: the grid's cell-ordered copy of q4 with the particle id in w (the same positions; one
+                // contiguous load per candidate). Outside the support W = 0 makes V_j W zero, so V_j is read for
+                // the candidates within h only.
                 for (int e = c.cell_start[cc]; e < c.cell_start[cc + 1]; ++e) {
-                    int j = c.items[e];
-                    f3 dd = x - xyz(c.q4[j]);
-                    float wj = c.wvol[j] * kernel_w(length(dd), K);
+                    const float4 p4 = c.q4s[e];
+                    f3 dd = x - f3{p4.x, p4.y, p4.z};
+                    float kw = kernel_w(length(dd), K);
+                    if (!(kw > 0.0f)) continue;
+                    int j = __float_as_int(p4.w);
+                    float vj = c.wvol[j];
+                    float wj = vj * kw;
                     if (wj > 0.0f) {
                         col += wj;
-                        gc = gc + c.wvol[j] * kernel_grad(dd, K);
+                        gc = gc + vj * kernel_grad(dd, K);
                         mv = mv + wj * V(c, j);
                         if (rec >= 0 && nrec < PW_ROW) {
                             c.wr_id[int64_t(nrec) * c.n + rec] = c.wcidx[j];
@@ -1141,12 +1148,13 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
     float pcol = 0.0f;
     f3 pgc{0.0f, 0.0f, 0.0f}, pmv{0.0f, 0.0f, 0.0f};
     bool have_pw = false;
+    float pw_phi = 1.0e30f;
     if (c.NW > 0) {
         pw_sample(c, x, w, pcol, pgc, pmv, i);
         c.pw_col[i] = pcol;
         have_pw = true;
-        float phi = 1.0e30f;
-        if (!(pcol < 1.0e-4f || length(pgc) <= 0.0f)) phi = colour_distance(c, fmn(pcol, 1.0f));
+        if (!(pcol < 1.0e-4f || length(pgc) <= 0.0f)) pw_phi = colour_distance(c, fmn(pcol, 1.0f));
+        const float phi = pw_phi;
         if (phi < c.reach && phi + c.offset < c.h)
             if (slot_insert(R, nn, c.S, phi) >= 0 && nn < 4) nn += 1;
     }
@@ -1165,7 +1173,9 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
             float lg = length(pgc);
             f3 nrm{0.0f, 0.0f, 1.0f};
             if (lg > 0.0f) nrm = (-pgc) / lg;
-            float phi = colour_distance(c, fcl(pcol, 0.0f, 1.0f));
+            // the insert's distance: the slot exists only if it was computed, and pcol >= 0 (a sum of V_j W > 0)
+            // makes fcl(pcol, 0, 1) = fmn(pcol, 1)
+            float phi = pw_phi;
             f3 xb = x - phi * nrm;
             float v0, v1;
             halfspace_volume(phi + c.offset, c.h, 0.0f, c.kind, v0, v1);
