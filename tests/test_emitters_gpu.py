@@ -242,3 +242,43 @@ def test_pour_worlds_with_swaying_bowls_match_the_oracle_bitwise():
     _log(f"[emitters] emitted {np.round(r.mass_in / float(sc.model.particle_mass[0].item())).astype(int).tolist()}")
     assert np.all(r.mass_in > 0) and _exact(s.mass_ledger())
     assert np.any(s.pw.world_work() != 0.0)  # a swaying bowl works on the liquid
+
+
+def test_dense_particle_walls_overflow_the_wall_rows_and_match_the_oracle_bitwise():
+    """A bowl lined at half the liquid spacing (four layers of radius r/2, the same thickness and volume): liquid at
+    the shell has far more than the 32 wall particles within h that the slot pass records per particle, so the
+    reaction split walks the cells again for those (particle_walls.py:138-177). 60 steps: the GPU state and the
+    walls' fixed-point impulses equal the oracle's bit for bit."""
+    from alloy_amd.sph import scenes
+    from alloy_amd.sph.build import PHASE_NONE, add_box_tank, add_fluid_points, bowl_points, lattice_points
+    from alloy_amd.sph.config import particle_mass
+    from alloy_amd.sph.model import ModelBuilder, ParticleFlags
+
+    cfg = scenes.bench_config()
+    sp, side, R = cfg.spacing, 0.2, 0.05
+    b = ModelBuilder(gravity=(0.0, 0.0, -9.81))
+    c = np.array([0.5 * side, 0.5 * side, 2.0 * sp + R])
+    b.begin_world()
+    add_box_tank(b, (0.0, 0.0, 0.0), (side, side, side), wall=cfg.h)
+    shell = bowl_points(tuple(c), R, 0.5 * sp, layers=4)
+    b.add_particles(shell, (0.0, 0.0, 0.0), particle_mass(cfg, 1000.0), 0.5 * cfg.particle_radius,
+                    int(ParticleFlags.ACTIVE | ParticleFlags.PROXY), 1000.0, 1.0e-3, PHASE_NONE)
+    m = int(np.ceil(R / sp)) + 1
+    pts = lattice_points(tuple(c - m * sp), (2 * m, 2 * m, m), sp)
+    pts = pts[(np.linalg.norm(pts - c, axis=1) < R - 0.5 * sp) & (pts[:, 2] < c[2])]
+    pts = pts + np.random.default_rng(3).uniform(-1e-4, 1e-4, pts.shape)
+    add_fluid_points(b, cfg, pts)
+    b.end_world()
+    sc = scenes.Scene(b.finalize("cuda"), cfg, [(0.0, 0.0, 0.0)], [(side, side, side)], [len(pts)])
+    d = np.linalg.norm(pts[:, None, :] - shell[None, :, :], axis=2)
+    assert int((d < cfg.h).sum(axis=1).max()) > 2 * 32  # rows overflow at the start, not only by chance
+    s = sc.make_solver()
+    st = sc.model.state()
+    o = _oracle_for(s, sc)
+    _log(f"[emitters] dense bowl: {len(shell)} wall particles at half spacing, {len(pts)} liquid")
+    for k in range(60):
+        s.step(st, st, None, None, DT)
+        o.step(DT)
+        if k % 30 == 29:
+            _compare(s, o, st, f"step {k + 1}")
+    assert np.any(s.pw.particle_impulse() != 0.0)
