@@ -4325,6 +4325,9 @@ int32_t dfsph_get_buffer(void* h, int32_t which, void** ptr, int64_t* nb) {
         case DFSPH_BUF_SKIN_FLAG:
             if (c.skin <= 0.0f) return fail(DFSPH_EINVAL, "no Verlet skin");
             B(c.skin_flag, 4 * W)
+        case DFSPH_BUF_PW_ROW_COUNT:
+            if (c.NW == 0) return fail(DFSPH_EINVAL, "no particle walls");
+            B(c.wr_n, 4 * n)
         default: return fail(DFSPH_EINVAL, "unknown buffer id");
     }
 #undef B

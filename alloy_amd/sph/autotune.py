@@ -36,7 +36,8 @@ NAMES = {_abi.OPT_CHECK_INTERVAL: "check_interval", _abi.OPT_BUILD_BLOCK: "build
 
 class _Snapshot:
     """Everything a step changes: the state, the solver's state arrays (warm starts, flags, phase, emitted and retired
-    mass, the emitters' clocks and cursors), the shape meters and the rest of the per-world ledger."""
+    mass, the emitters' clocks and cursors), the shape meters, the rest of the per-world ledger and the particle
+    walls' impulse totals of the current exchange window."""
 
     def __init__(self, solver, state):
         self.q = state.particle_q.clone()
@@ -46,6 +47,8 @@ class _Snapshot:
         self.buffers = {_abi.BUF_SHAPE_METER: solver.buffer(_abi.BUF_SHAPE_METER, np.int64, (S, 3))} if S else {}
         for which, dt in ((_abi.BUF_EMIT_OVERFLOW, np.int32), (_abi.BUF_FLUX_E, np.int64), (_abi.BUF_FLUX_G, np.int64)):
             self.buffers[which] = solver.buffer(which, dt, (W,))
+        if solver.pw.count:
+            self.buffers[_abi.BUF_PW_EXCH] = solver.buffer(_abi.BUF_PW_EXCH, np.int64, (solver.pw.count, 3))
 
     def restore(self, solver, state):
         state.particle_q.copy_(self.q)

@@ -22,7 +22,7 @@ import math
 
 import numpy as np
 
-from .kernels import kernel_w_np
+from .kernels import kernel_dwdr_np, kernel_w_np
 
 SURFACE_GRAD = 0.3  # |g| h above this marks a surface particle (csf.py:53)
 NORMAL_PASSES = 2  # extra Shepard passes of the colour gradient for the normal (csf.py:54)
@@ -60,76 +60,79 @@ def capillary_dt(spacing: float, rho0: float, sigma: float) -> float:
     return math.inf if sigma <= 0.0 else math.sqrt(rho0 * spacing**3 / (2.0 * math.pi * sigma))
 
 
-def _offsets(spacing: float, h: float) -> np.ndarray:
-    n = int(math.ceil(h / spacing)) + 1
-    ax = np.arange(-n, n + 1) * spacing
-    return np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+class _Planes:
+    """The rest lattice's neighbours within h of a site, grouped by lattice plane: plane dz (from -m to m, m =
+    ceil(h / spacing)) holds the offsets o = x_j - x_i with o_z = dz spacing. A flat free surface with the liquid on
+    layers l = 0, 1, 2, ... (layer l at z = -l spacing) gives a site of layer l exactly the planes dz <= l, so every
+    per-layer sum of the surface constants is a running sum of per-plane sums over dz.
+
+    Per plane: ``w`` = sum W(|o|) (the site itself included in plane 0), ``w_pair`` the same without the site,
+    ``gz`` = sum over o != 0 of the z component of grad_i W(x_i - x_j) = W'(r) (-o_z) / r, ``coh`` = sum C(r) / r
+    (|o_xy|^2 / 2 - o_z^2) (the cohesion spline's virial weight), ``taper`` = sum 4 max(1 - r / h, 0) (the
+    curvature term's weight)."""
+
+    def __init__(self, spacing: float, h: float, kind: str):
+        self.m = int(math.ceil(h / spacing))
+        self.dz = np.arange(-self.m, self.m + 1)
+        k = np.arange(-self.m - 1, self.m + 2, dtype=np.float64) * spacing
+        ox, oy = np.meshgrid(k, k, indexing="ij")
+        ox, oy = ox.ravel(), oy.ravel()
+        self.w, self.w_pair, self.gz, self.coh, self.taper = (np.zeros(len(self.dz)) for _ in range(5))
+        for p, d in enumerate(self.dz):
+            oz = d * spacing
+            r = np.sqrt(ox * ox + oy * oy + oz * oz)
+            inside = r < h
+            pair = inside & (r > 0.0)
+            self.w[p] = kernel_w_np(r[inside], h, kind).sum()
+            self.w_pair[p] = kernel_w_np(r[pair], h, kind).sum()
+            rp, xy2 = r[pair], ox[pair] ** 2 + oy[pair] ** 2
+            self.gz[p] = np.sum(kernel_dwdr_np(rp, h, kind) * (-oz) / rp)
+            self.coh[p] = np.sum(cohesion_c_np(rp, h) / rp * (0.5 * xy2 - oz * oz))
+            self.taper[p] = np.sum(4.0 * np.maximum(1.0 - rp / h, 0.0))
+
+    def upto(self, values: np.ndarray, layer: int) -> float:
+        """sum of the per-plane ``values`` over the planes a site of ``layer`` sees (dz <= layer)."""
+        return float(values[self.dz <= layer].sum())
 
 
 def flat_lattice_constants(spacing: float, h: float, kind: str = "cubic", layers: int = 6) -> tuple[float, float]:
-    """(I, c_s) of a flat rest-lattice free surface (csf.py:252-278): I = the sum over the surface layers
-    (|g| h > SURFACE_GRAD) of |g| spacing, the CSF delta normalisation; c_s = rho / rho0 extrapolated from the top
-    layer half a spacing out to the interface."""
-    off = _offsets(spacing, h)
-    rr = np.linalg.norm(off, axis=1)
-    keep = rr < h
-    off, rr = off[keep], rr[keep]
-    dl = np.round(off[:, 2] / spacing).astype(int)  # neighbour j sits on layer l - dl
-    w = kernel_w_np(rr, h, kind)
-    vol = 1.0 / float(w.sum())
-    e = 1.0e-7 * h
-    nz = rr > 0
-    dw = np.zeros_like(rr)
-    dw[nz] = (kernel_w_np(rr[nz] + e, h, kind) - kernel_w_np(rr[nz] - e, h, kind)) / (2.0 * e)
-    gz_unit = np.where(nz, dw * (-off[:, 2]) / np.where(nz, rr, 1.0), 0.0)
-    total, c0, g0 = 0.0, 0.0, 0.0
-    for layer in range(layers):
-        present = layer - dl >= 0
-        gz = abs(float(np.sum(vol * gz_unit[present])))
-        if gz * h > SURFACE_GRAD:
-            total += gz * spacing
-        if layer == 0:
-            c0, g0 = float(np.sum(vol * w[present])), gz
-    return total, c0 - 0.5 * spacing * g0
-
-
-def _slab_pairs(spacing: float, h: float, kind: str, rho0: float, mass: float, layers: int):
-    """Per top layer l < ``layers`` of the rest lattice's flat free surface: the offsets x_j - x_i to its neighbours
-    and the per-unit-gamma pair forces on i (cohesion vector, curvature z) (tension.py:204-230)."""
-    off = _offsets(spacing, h)
-    rr = np.linalg.norm(off, axis=1)
-    keep = (rr < h) & (rr > 0.0)
-    off, rr = off[keep], rr[keep]
-    dl = np.round(off[:, 2] / spacing).astype(int)
-    depth = layers + 2 * int(math.ceil(h / spacing)) + 2
-    w0 = float(kernel_w_np(np.zeros(1), h, kind)[0])
-    rho = np.array([mass * (w0 + kernel_w_np(rr[lay - dl >= 0], h, kind).sum()) for lay in range(depth)])
-    e = 1.0e-7 * h
-    dw = (kernel_w_np(rr + e, h, kind) - kernel_w_np(rr - e, h, kind)) / (2.0 * e)
-    grad_z = dw * (-off[:, 2]) / rr
-    nz = np.zeros(depth)  # Akinci's normal (z) per layer: h sum_j m / rho_j dW
-    for lay in range(depth - int(math.ceil(h / spacing)) - 1):
-        ok = lay - dl >= 0
-        nz[lay] = h * np.sum(mass / rho[lay - dl[ok]] * grad_z[ok])
-    out = []
-    for lay in range(layers):
-        ok = lay - dl >= 0
-        o, r, lj = off[ok], rr[ok], lay - dl[ok]
-        K = 2.0 * rho0 / (rho[lay] + rho[lj])
-        f_coh = (K * mass * mass * cohesion_c_np(r, h) / r)[:, None] * o
-        f_curv_z = -K * mass * (nz[lay] - nz[lj]) * 4.0 * np.maximum(1.0 - r / h, 0.0)
-        out.append((o, f_coh, f_curv_z))
-    return out
+    """(I, c_s) of a flat rest-lattice free surface (csf.py:225-250). With the volume V = 1 / (lattice kernel sum),
+    layer l has the colour c_l = V sum_{dz <= l} w and the colour gradient g_l = V sum_{dz <= l} gz (z only, by
+    symmetry). I, the CSF delta normalisation, is the gradient integrated across the surface layers: spacing times
+    the sum of |g_l| over the layers with |g_l| h > SURFACE_GRAD. c_s is rho / rho0 at the interface, c_0
+    extrapolated half a spacing outward along the gradient: c_0 - spacing |g_0| / 2."""
+    pl = _Planes(spacing, h, kind)
+    vol = 1.0 / float(pl.w.sum())
+    g = np.array([abs(vol * pl.upto(pl.gz, lay)) for lay in range(layers)])
+    big_i = spacing * float(g[g * h > SURFACE_GRAD].sum())
+    return big_i, vol * pl.upto(pl.w, 0) - 0.5 * spacing * float(g[0])
 
 
 def lattice_sigma_per_gamma(spacing: float, h: float, kind: str, rho0: float, mass: float, layers: int = 8) -> float:
-    """Kirkwood-Buff surface tension [N/m] per unit gamma of the rest lattice's flat free surface (tension.py:241-249):
-    sigma = (1/A) sum over the pairs across the surface of z_ij F_z - (x_ij F_x + y_ij F_y) / 2."""
+    """Kirkwood-Buff surface tension [N/m] per unit gamma of the rest lattice's flat free surface (tension.py:239-249).
+
+    The pair force on i from j (per unit gamma, csf_sparse.py's small-piece force) is K_ij (m^2 C(r) o / r - m
+    (n_i - n_j) 4 max(1 - r/h, 0)) with o = x_j - x_i, K_ij = 2 rho0 / (rho_i + rho_j), the layer densities rho_l =
+    m (W(0) + sum_{dz <= l} w_pair) and Akinci's normals n_l = h m sum_{dz <= l} gz / rho_{l - dz} (z only). The
+    virial sigma = 1 / (2 A) sum_i sum_j [z_ij F_z - (x_ij F_x + y_ij F_y) / 2], A = spacing^2 per lattice column,
+    x_ij = -o, then reduces per plane to the ``coh`` and ``taper`` sums. Pairs deeper than h below the surface cancel
+    by the lattice's cubic symmetry, so ``layers`` > h / spacing + 2 covers every term."""
+    pl = _Planes(spacing, h, kind)
+    depth = layers + 2 * pl.m + 2
+    w0 = float(kernel_w_np(np.zeros(1), h, kind)[0])
+    rho = np.array([mass * (w0 + pl.upto(pl.w_pair, lay)) for lay in range(depth)])
+    normal = np.zeros(depth)
+    for lay in range(depth - pl.m - 1):  # the layers whose every neighbour plane has a density
+        seen = pl.dz <= lay
+        normal[lay] = h * mass * float(np.sum(pl.gz[seen] / rho[lay - pl.dz[seen]]))
     sigma = 0.0
-    for o, f_coh, f_curv_z in _slab_pairs(spacing, h, kind, rho0, mass, layers):
-        vir = -o[:, 2] * (f_coh[:, 2] + f_curv_z) - 0.5 * (-o[:, 0] * f_coh[:, 0] - o[:, 1] * f_coh[:, 1])
-        sigma += 0.5 * float(vir.sum()) / spacing**2
-    return sigma
+    for lay in range(layers):
+        for p in np.nonzero(pl.dz <= lay)[0]:
+            d = int(pl.dz[p])
+            k_ij = 2.0 * rho0 / (rho[lay] + rho[lay - d])
+            curv = mass * (normal[lay] - normal[lay - d]) * d * spacing * pl.taper[p]
+            sigma += k_ij * (mass * mass * pl.coh[p] + curv)
+    return float(0.5 * sigma / spacing**2)
 
 
 def pair_gamma(sigma: np.ndarray, rho0: np.ndarray, mass: np.ndarray, spacing: float, h: float,
@@ -144,17 +147,20 @@ def pair_gamma(sigma: np.ndarray, rho0: np.ndarray, mass: np.ndarray, spacing: f
 
 
 def shape_angles(contact_angle: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
-    """(cos, sin) of each shape's contact angle; sin = -1 marks a free contact line (angle -1) (csf.py:281-296)."""
-    ang = np.asarray(contact_angle, np.float64)
-    if len(ang) == 0:
-        ang = np.full(1, -1.0)
-    bad = (ang > math.pi + 1.0e-6) | ((ang < 0.0) & (ang != -1.0))
-    if bad.any():
-        raise ValueError(f"sph:shape_contact_angle must be -1 (free) or in [0, pi] rad; shapes {np.nonzero(bad)[0]}")
-    free = ang < 0.0
-    cos = np.where(free, 0.0, np.cos(ang)).astype(np.float32)
-    sin = np.where(free, -1.0, np.sin(np.clip(ang, 0.0, math.pi))).astype(np.float32)
-    return cos, sin
+    """(cos, sin) of each shape's contact angle as float32, for the Breinlinger turn of the normal at a contact line
+    (csf.py:253-266). An angle of exactly -1 marks a free contact line, stored as sin = -1 (a sine no angle in
+    [0, pi] has); any other negative angle or one beyond pi is refused. No shapes: one free entry."""
+    theta = np.asarray(contact_angle, np.float64).reshape(-1)
+    if theta.size == 0:
+        theta = np.array([-1.0])
+    free = theta == -1.0
+    valid = free | ((theta >= 0.0) & (theta <= math.pi + 1.0e-6))
+    if not valid.all():
+        raise ValueError(f"sph:shape_contact_angle takes -1 (a free contact line) or an angle in [0, pi] rad; "
+                         f"shapes {np.flatnonzero(~valid).tolist()} are outside")
+    turned = np.clip(theta, 0.0, math.pi)
+    return (np.where(free, 0.0, np.cos(turned)).astype(np.float32),
+            np.where(free, -1.0, np.sin(turned)).astype(np.float32))
 
 
 def xsph_coefficient(cfg, viscosity: str) -> float:

@@ -41,6 +41,22 @@ def kernel_w_np(r: np.ndarray, h: float, kind: str = "cubic") -> np.ndarray:
     return w
 
 
+def kernel_dwdr_np(r: np.ndarray, h: float, kind: str = "cubic") -> np.ndarray:
+    """dW/dr [1/m^4] in float64, the derivative of ``kernel_w_np``'s polynomials: cubic 8/(pi h^4) (18 q^2 - 12 q) for
+    q <= 1/2 and -6 (1 - q)^2 8/(pi h^4) for q < 1; Wendland C2 -20 q (1 - q)^3 21/(2 pi h^4); 0 from q = 1."""
+    q = np.asarray(r, dtype=np.float64) / h
+    out = np.zeros_like(q)
+    if kind == "wendland_c2":
+        m = q < 1.0
+        out[m] = 21.0 / (2.0 * math.pi * h**4) * (-20.0 * q[m] * (1.0 - q[m]) ** 3)
+        return out
+    c = 8.0 / (math.pi * h**4)
+    near, far = q <= 0.5, (q > 0.5) & (q < 1.0)
+    out[near] = c * (18.0 * q[near] ** 2 - 12.0 * q[near])
+    out[far] = c * (-6.0 * (1.0 - q[far]) ** 2)
+    return out
+
+
 def _lattice_offsets(spacing: float, h: float) -> np.ndarray:
     """Offsets of a cubic lattice (spacing) covering the support ball, origin included, shape (M, 3)."""
     m = int(math.ceil(h / spacing)) + 1

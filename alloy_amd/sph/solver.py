@@ -117,6 +117,48 @@ def _dptr(t: torch.Tensor | None):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+def as_state_tensor(a, name: str, device: torch.device, n: int | None = None):
+    """A zero-copy torch view of a state array the caller owns: a torch tensor as it is, otherwise any device array
+    exporting DLPack (``__dlpack__``, as Warp arrays and the vendored Newton State's arrays do) or
+    ``__cuda_array_interface__``. vec3 arrays come as [n, 3] float32 (a flat [3 n] export is viewed as [n, 3]).
+    Host arrays are refused: the boundary takes device memory, and the step writes the caller's q and qd in place."""
+    if a is None or isinstance(a, torch.Tensor):
+        return a
+    if hasattr(a, "__dlpack__"):
+        t = torch.from_dlpack(a)
+    elif hasattr(a, "__cuda_array_interface__"):
+        t = torch.as_tensor(a, device=device)
+    else:
+        raise ValueError(f"SolverDFSPH: state.{name} is a {type(a).__name__}, neither a torch tensor nor a device array "
+                         "exporting __dlpack__ or __cuda_array_interface__")
+    want = torch.device(device)
+    if t.device.type != want.type or (want.index is not None and t.device.index != want.index):
+        raise ValueError(f"SolverDFSPH: state.{name} lives on {t.device}, the solver on {device}")
+    if t.dim() == 1 and (n is None or t.numel() == 3 * n) and t.numel() % 3 == 0:
+        t = t.view(-1, 3)
+    return t
+
+
+class _StateView:
+    """The caller's state seen through torch views of its arrays (as_state_tensor); writes reach the caller's
+    memory."""
+
+    __slots__ = ("particle_q", "particle_qd", "particle_f")
+
+    def __init__(self, state, device: torch.device, n: int):
+        self.particle_q = as_state_tensor(state.particle_q, "particle_q", device, n)
+        self.particle_qd = as_state_tensor(state.particle_qd, "particle_qd", device, n)
+        self.particle_f = as_state_tensor(getattr(state, "particle_f", None), "particle_f", device, n)
+
+
+def device_state(state, device: torch.device, n: int):
+    """``state`` itself when its arrays are torch tensors, else a _StateView of them."""
+    if isinstance(state.particle_q, torch.Tensor) and isinstance(state.particle_qd, torch.Tensor) and isinstance(
+            getattr(state, "particle_f", None), (torch.Tensor, type(None))):
+        return state
+    return _StateView(state, device, n)
+
+
 class SolverDFSPH:
     """Divergence-free SPH (Bender & Koschier 2017) with SDF and particle walls, on MI355X."""
 
@@ -283,6 +325,9 @@ class SolverDFSPH:
         n = self.model.particle_count
         if n == 0:
             return
+        same = state_out is state_in
+        state_in = device_state(state_in, self.device, n)
+        state_out = state_in if same else device_state(state_out, self.device, n)
         self._checked_state(state_in, n)
         self._checked_state(state_out, n)
         if state_out is not state_in:
@@ -300,7 +345,7 @@ class SolverDFSPH:
             q, qd = state_out.particle_q, state_out.particle_qd
             q[vid] = q[vid] + (-self._soft_back) * qd[vid]
         self._soft_back = 0.0
-        f = state_in.particle_f
+        f = getattr(state_in, "particle_f", None)
         self._call(self.lib.dfsph_step, _dptr(state_out.particle_q), _dptr(state_out.particle_qd), _dptr(f),
                    float(dt), _stream_ptr(self.device), what="SolverDFSPH.step")
 
@@ -400,6 +445,7 @@ class SolverDFSPH:
         iterations used."""
         if self.model.particle_count == 0:
             return 0
+        state = device_state(state, self.device, self.model.particle_count)
         guess = torch.tensor(hydrostatic_guess(self.model, state, self.config.spacing), device=self.device)
         it = ctypes.c_int32(0)
         self._call(self.lib.dfsph_prime, _dptr(state.particle_q), _dptr(state.particle_qd), _dptr(guess), float(dt),
@@ -411,6 +457,7 @@ class SolverDFSPH:
         n = self.model.particle_count
         if n == 0:
             return np.zeros(0), np.zeros(0, dtype=bool)
+        state = device_state(state, self.device, n)
         st = _stream_ptr(self.device)
         self._call(self.lib.dfsph_build_neighbors, _dptr(state.particle_q), st)
         self._call(self.lib.dfsph_boundary_slots, _dptr(state.particle_q), _dptr(state.particle_qd), st)
@@ -635,7 +682,7 @@ class SolverDFSPH:
             enabled = self.buffer(_abi.BUF_EMIT_ENABLED, np.int32, (E,)) if E else None
             S = self.model.shape_count
             meters = self.buffer(_abi.BUF_SHAPE_METER, np.int64, (max(S, 1), 3)) if S else None
-            opts = {o: self.get_option(o) for o in (_abi.OPT_CHECK_INTERVAL, _abi.OPT_GRAPHS, _abi.OPT_PAIR_CACHE)}
+            opts = {o: self.get_option(o) for o in _abi.ALL_OPTIONS}
             self.lib.dfsph_destroy(self._h)
             self._h = None
             self._create_native()
@@ -649,10 +696,14 @@ class SolverDFSPH:
             if meters is not None and self.model.shape_count == S:
                 self._call(self.lib.dfsph_write_buffer, _abi.BUF_SHAPE_METER, meters.ctypes.data_as(ctypes.c_void_p),
                            meters.nbytes)
-            for o, v in opts.items():
-                self.set_option(o, v)
             if self._warm_model is not None:
                 self._call(self.lib.dfsph_set_warm_start_model, _dptr(self._warm_model), None)
+            for o, v in opts.items():
+                try:
+                    self.set_option(o, v)
+                except ValueError:
+                    if o != _abi.OPT_PACKED_ACCEL:  # the one option a new handle may refuse (several masses)
+                        raise
 
     def alloy_shape_work(self, out: torch.Tensor) -> None:
         """Work [J] each shape's boundary did on the liquid in the last step at the wall's point velocity, per model
@@ -668,6 +719,7 @@ class SolverDFSPH:
         m32 = mask.to(device=self.device, dtype=torch.int32).contiguous()
         if tuple(m32.shape) != (n,):
             raise ValueError(f"SolverDFSPH.retire: mask must have {n} entries")
+        state = device_state(state, self.device, n)
         self._checked_state(state, n)
         self._call(self.lib.dfsph_retire, _dptr(m32), _dptr(state.particle_q), _dptr(state.particle_qd),
                    _stream_ptr(self.device), what="SolverDFSPH.retire")

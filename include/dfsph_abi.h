@@ -208,6 +208,9 @@ enum DfsphBuffer {
     DFSPH_BUF_VISC_ITERS,     /* int32  [W] iterations of the last implicit viscosity solve */
     DFSPH_BUF_CSF_SURF,       /* int32  [n] csf: 1 on the surface particles of the last step */
     DFSPH_BUF_CSF_KAPPA,      /* float  [n] csf: the curvature applied at the interface, 0 off the surface */
+    DFSPH_BUF_PW_ROW_COUNT,   /* int32  [n] with particle walls: the wall particles the last slot pass's colour sum of
+                                 each liquid particle visited, -1 when more than its 32-entry row holds (the reaction
+                                 split then walks the cells again) */
     DFSPH_BUF_COUNT_
 };
 

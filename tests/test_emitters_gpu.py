@@ -249,7 +249,7 @@ def test_dense_particle_walls_overflow_the_wall_rows_and_match_the_oracle_bitwis
     the shell has far more than the 32 wall particles within h that the slot pass records per particle, so the
     reaction split walks the cells again for those (particle_walls.py:138-177). 60 steps: the GPU state and the
     walls' fixed-point impulses equal the oracle's bit for bit."""
-    from alloy_amd.sph import scenes
+    from alloy_amd.sph import _abi, scenes
     from alloy_amd.sph.build import PHASE_NONE, add_box_tank, add_fluid_points, bowl_points, lattice_points
     from alloy_amd.sph.config import particle_mass
     from alloy_amd.sph.model import ModelBuilder, ParticleFlags
@@ -276,9 +276,19 @@ def test_dense_particle_walls_overflow_the_wall_rows_and_match_the_oracle_bitwis
     st = sc.model.state()
     o = _oracle_for(s, sc)
     _log(f"[emitters] dense bowl: {len(shell)} wall particles at half spacing, {len(pts)} liquid")
+    n, S = sc.model.particle_count, sc.model.shape_count
+    walked = 0  # steps in which a particle whose row overflowed pushed on the bowl: the cell-walk branch ran for it
     for k in range(60):
         s.step(st, st, None, None, DT)
         o.step(DT)
         if k % 30 == 29:
             _compare(s, o, st, f"step {k + 1}")
+        if k % 10 == 9:
+            rows = s.buffer(_abi.BUF_PW_ROW_COUNT, np.int32, (n,))
+            shape = s.buffer(_abi.BUF_SLOT_SHAPE, np.int32, (n, 4))
+            imp = s.buffer(_abi.BUF_SLOT_IMPULSE, np.float32, (n, 4, 3))
+            pushing = ((shape == S) & np.any(imp != 0.0, axis=2)).any(axis=1)
+            walked += int(np.any(pushing & (rows == -1)))
+    _log(f"[emitters] dense bowl: cell-walk reactions in {walked} of 6 sampled steps")
+    assert walked > 0
     assert np.any(s.pw.particle_impulse() != 0.0)

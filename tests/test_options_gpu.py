@@ -159,8 +159,12 @@ def test_autotune_leaves_the_simulation_unchanged():
     for _ in range(3):
         a.step(sa, sa, None, None, DT)
         b.step(sb, sb, None, None, DT)
+    assert a.pw.count > 0
+    exch = a.pw.particle_impulse()
+    assert np.any(exch != 0.0)  # the liquid already pushed on the bowl in this exchange window
     res = autotune(a, sa, DT, steps=2, warm=1, rounds=1, log=_log)
     _log(f"[autotune] {res['options']} in {res['seconds']:.1f} s")
+    assert np.array_equal(a.pw.particle_impulse(), exch)  # the tuning steps' wall impulses are not kept
     assert set(res["options"]) == {"check_interval", "build_block", "sparse_correction", "packed_accel", "pair_cache",
                                    "graphs"}
     for _ in range(6):
@@ -168,3 +172,4 @@ def test_autotune_leaves_the_simulation_unchanged():
         b.step(sb, sb, None, None, DT)
     torch.cuda.synchronize()
     assert torch.equal(sa.particle_q, sb.particle_q) and torch.equal(sa.particle_qd, sb.particle_qd)
+    assert np.array_equal(a.pw.particle_impulse(), b.pw.particle_impulse())

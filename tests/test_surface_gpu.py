@@ -22,21 +22,36 @@ def _log(msg):
 
 def test_shape_change_rebuilds_like_a_new_solver():
     """After a wall moves, notify_model_changed(SHAPE_PROPERTIES) gives the same steps, bit for bit, as a solver
-    created on the changed model with the same state arrays restored."""
+    created on the changed model with the same state arrays restored. Every tuning option survives the rebuild,
+    among them the rule start chosen over an attached learned network (a rebuild that fell back to the default
+    learned start would follow another trajectory than the new solver with the same options)."""
     import torch
 
-    from alloy_amd.sph import scenes
+    from alloy_amd.sph import _abi, scenes
+    from alloy_amd.sph.learned import DEFAULT_WEIGHTS, WarmStartMLP
     from alloy_amd.sph.model import ModelFlags
 
     sc = scenes.dam_break(8, 0.2, 0, device="cuda")
     _log(f"[surface] shape change: {sc.model.particle_count} particles, {sc.model.shape_count} shapes")
+    net = WarmStartMLP.load(DEFAULT_WEIGHTS, "cuda")
+    opts = {_abi.OPT_CHECK_INTERVAL: 2, _abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_APPLY, _abi.OPT_LEARNED_SOLVES: 0,
+            _abi.OPT_SPARSE_CORRECTION: 0, _abi.OPT_BUILD_BLOCK: 64}
     a = sc.make_solver()
+    net.attach(a)
+    for o, v in opts.items():
+        a.set_option(o, v)
+    before = {o: a.get_option(o) for o in _abi.ALL_OPTIONS}
     st_a = sc.model.state()
     for _ in range(5):
         a.step(st_a, st_a, None, None, DT)
     sc.model.shape_transform[0, 0] += 0.01  # move one wall of the tank
     a.notify_model_changed(int(ModelFlags.SHAPE_PROPERTIES))
+    after = {o: a.get_option(o) for o in _abi.ALL_OPTIONS}
+    assert after == before, (before, after)
     b = sc.make_solver()
+    net.attach(b)
+    for o, v in opts.items():
+        b.set_option(o, v)
     b.alloy_restore_state_arrays(a.alloy_state_arrays(None))
     st_b = sc.model.state()
     st_b.particle_q.copy_(st_a.particle_q)
