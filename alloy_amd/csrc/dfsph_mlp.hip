@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/dfsph_abi.h"
@@ -344,6 +345,30 @@ __global__ void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g, f
 
 thread_local std::string g_mlp_err;
 
+// The gradient partials of dfsph_mlp_grad_noalloc: a file-scope device array (one copy per GPU, held until the process
+// ends) sized for DFSPH_MLP_NOALLOC_MAX_SAMPLES samples. Host code never passes the array's name to a kernel: it is a
+// host-side stand-in, not the device address. Each device's address comes from hipGetSymbolAddress on that
+// device's first call and is kept in g_noalloc_ptr.
+constexpr int64_t NOALLOC_FLOATS = int64_t(4) * ((DFSPH_MLP_NOALLOC_MAX_SAMPLES + 63) / 64) * PART_STRIDE;
+static __device__ __align__(256) float scratch[NOALLOC_FLOATS];
+constexpr int MAX_DEVICES = 64;
+float* g_noalloc_ptr[MAX_DEVICES] = {};
+std::mutex g_noalloc_mu;
+
+// the current device's address of `scratch` (looked up once per device, with that device current)
+int noalloc_scratch(float** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return DFSPH_EDEVICE;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return DFSPH_EDEVICE;
+        g_noalloc_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return DFSPH_OK;
+}
+
 }  // namespace
 
 // launched by the step (dfsph_kernels.hip) inside its captured segment
@@ -377,6 +402,23 @@ int32_t dfsph_mlp_grad(const float* params, const float* X, const float* T, int3
     int blocks = (n + 63) / 64;
     k_mlp_grad<<<blocks, 256, 0, st>>>(params, X, T, n, 1.0f / (2.0f * float(n_total)), scratch);
     k_mlp_reduce<<<(N_PARAMS + 1 + 255) / 256, 256, 0, st>>>(scratch, blocks * 4, grad);
+    return hipGetLastError() == hipSuccess ? DFSPH_OK : DFSPH_EDEVICE;
+}
+
+// dfsph_mlp_grad with the partials in the library's static device array instead of a caller workspace: no
+// allocation at all. n <= DFSPH_MLP_NOALLOC_MAX_SAMPLES. Every call on a device shares its array (calls that overlap
+// on one device race); each launch writes every partial it then reduces, so nothing needs clearing between calls.
+int32_t dfsph_mlp_grad_noalloc(const float* params, const float* X, const float* T, int32_t n, int32_t n_total,
+                               float* grad, void* stream) {
+    if (!params || !X || !T || !grad || n <= 0 || n_total <= 0) return DFSPH_EINVAL;
+    if (n > DFSPH_MLP_NOALLOC_MAX_SAMPLES) return DFSPH_ECAPACITY;
+    float* part = nullptr;
+    int rc = noalloc_scratch(&part);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int blocks = (n + 63) / 64;
+    k_mlp_grad<<<blocks, 256, 0, st>>>(params, X, T, n, 1.0f / (2.0f * float(n_total)), part);
+    k_mlp_reduce<<<(N_PARAMS + 1 + 255) / 256, 256, 0, st>>>(part, blocks * 4, grad);
     return hipGetLastError() == hipSuccess ? DFSPH_OK : DFSPH_EDEVICE;
 }
 

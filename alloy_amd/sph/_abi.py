@@ -374,6 +374,7 @@ def load_library() -> ctypes.CDLL:
         "dfsph_mlp_forward": ([vp, vp, i32, vp, vp], i32),
         "dfsph_mlp_grad_scratch_floats": ([i32], i64),
         "dfsph_mlp_grad": ([vp, vp, vp, i32, i32, vp, i64, vp, vp], i32),
+        "dfsph_mlp_grad_noalloc": ([vp, vp, vp, i32, i32, vp, vp], i32),
         "dfsph_mlp_adam": ([vp, vp, vp, vp, dbl, dbl, dbl, dbl, i32, vp], i32),
         "dfsph_set_logging": ([vp, vp, vp], i32),
         "dfsph_get_buffer": ([vp, i32, P(vp), P(i64)], i32),

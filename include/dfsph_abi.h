@@ -333,6 +333,14 @@ int64_t dfsph_mlp_grad_scratch_floats(int32_t n);
  * order (deterministic); a smaller scratch is rejected with DFSPH_EINVAL. */
 int32_t dfsph_mlp_grad(const float* params, const float* X, const float* T, int32_t n, int32_t n_total,
                        float* scratch, int64_t scratch_floats, float* grad, void* stream);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph capture,
+ * tight training loops, harnesses that forbid hipMalloc): dfsph_mlp_grad with its per-wave partials in a static
+ * device array of the library (about 10.7 MB per GPU, held until the process ends) instead of a caller workspace.
+ * n at most DFSPH_MLP_NOALLOC_MAX_SAMPLES, else DFSPH_ECAPACITY. It allocates nothing; calls that overlap on one GPU
+ * share that array and race. Same results, bit for bit, as dfsph_mlp_grad. */
+#define DFSPH_MLP_NOALLOC_MAX_SAMPLES 4096
+int32_t dfsph_mlp_grad_noalloc(const float* params, const float* X, const float* T, int32_t n, int32_t n_total,
+                               float* grad, void* stream);
 int32_t dfsph_mlp_adam(float* params, const float* grad, float* m, float* v, double lr, double beta1,
                        double beta2, double eps, int32_t t, void* stream);
 /* Training-data logging of dfsph_step: the step's start-of-step features [n][16] and the converged kappa of its
