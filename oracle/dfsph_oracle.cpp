@@ -2639,6 +2639,21 @@ void orc_shape_sdf(int32_t geo, const double* scale, const double* p, double* ou
     out[0] = phi, out[1] = g.x, out[2] = g.y, out[3] = g.z;
 }
 
+// boundary_slots.py:50-102 on one float32 slot row: insert (s, phi, world normal nrm) into the phi-ascending row of
+// n <= 4 slots (shape[4], phi[4], nrm[4 * 3], updated in place); returns 1 if the new wall was kept
+int32_t orc_shape_insert(int32_t* shape, float* phi, float* nrm, int32_t n, int32_t s, float p, const float* n_new,
+                         float near_) {
+    Oracle<float> o;
+    o.s_shape.assign(shape, shape + 4);
+    o.s_phi.assign(phi, phi + 4);
+    o.s_nbl.assign(nrm, nrm + 12);
+    int kept = o.shape_insert(0, n, s, p, V3<float>{n_new[0], n_new[1], n_new[2]}, near_);
+    std::copy(o.s_shape.begin(), o.s_shape.end(), shape);
+    std::copy(o.s_phi.begin(), o.s_phi.end(), phi);
+    std::copy(o.s_nbl.begin(), o.s_nbl.end(), nrm);
+    return kept;
+}
+
 double orc_colour_distance(void* h, double c) {
     double r = 0.0;
     if (static_cast<Handle*>(h)->precision == 64) r = impl<double>(h)->colour_distance(c);

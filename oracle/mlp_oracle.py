@@ -22,11 +22,14 @@ OFF_B4 = OFF_W4 + OUT * H
 N_PARAMS = OFF_B4 + OUT
 
 
-def bf16(x) -> np.ndarray:
+def _bf16(x) -> np.ndarray:
     """Round float32 values to bf16 (nearest even), returned as float64."""
     a = np.ascontiguousarray(np.asarray(x, np.float32)).view(np.uint32).astype(np.uint64)
     a = (a + 0x7FFF + ((a >> 16) & 1)) & 0xFFFF0000
     return a.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+bf16 = _bf16
 
 
 def unpack(p: np.ndarray):
@@ -47,9 +50,16 @@ def _silu_grad(z):
 IN_USED = 13  # the feature rows carry other data in columns 13..15; the network reads columns 0..12
 
 
-def forward(p: np.ndarray, X: np.ndarray):
+def _exact(x) -> np.ndarray:
+    return np.asarray(x, np.float64)
+
+
+def forward(p: np.ndarray, X: np.ndarray, rounding: bool = True):
+    """``rounding`` False: the same network without the bf16 roundings, all float64 (the smooth function whose
+    derivative ``grad`` computes, for finite-difference checks)."""
+    bf16 = _bf16 if rounding else _exact
     W1, b1, W2, b2, W3, b3, W4, b4 = unpack(p)
-    X = np.array(X, np.float32, copy=True)
+    X = np.array(X, np.float32 if rounding else np.float64, copy=True)
     X[:, IN_USED:] = 0.0
     a0 = bf16(X)
     z1 = a0 @ bf16(W1).T + b1
@@ -62,12 +72,13 @@ def forward(p: np.ndarray, X: np.ndarray):
     return y, (a0, z1, a1, z2, a2, z3, a3)
 
 
-def grad(p: np.ndarray, X: np.ndarray, T: np.ndarray, n_total: int | None = None):
+def grad(p: np.ndarray, X: np.ndarray, T: np.ndarray, n_total: int | None = None, rounding: bool = True):
     """(gradient [N_PARAMS], squared-error sum) of sum((y[:, :2] - T)^2) / (2 n_total)."""
+    bf16 = _bf16 if rounding else _exact
     n = X.shape[0]
     n_total = n_total or n
     W1, b1, W2, b2, W3, b3, W4, b4 = unpack(p)
-    y, (a0, z1, a1, z2, a2, z3, a3) = forward(p, X)
+    y, (a0, z1, a1, z2, a2, z3, a3) = forward(p, X, rounding)
     d = np.zeros_like(y)
     d[:, :2] = y[:, :2] - np.asarray(T, np.float64)
     se = float(np.sum(d[:, :2] ** 2))

@@ -65,6 +65,7 @@ def load() -> ctypes.CDLL:
         "orc_halfspace_volume": ([i32, dbl, dbl, dbl, i32, P(dbl)], None),
         "orc_shape_sdf": ([i32, P(dbl), P(dbl), P(dbl)], None),
         "orc_colour_distance": ([vp, dbl], dbl),
+        "orc_shape_insert": ([vp, vp, vp, i32, i32, ctypes.c_float, vp, ctypes.c_float], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -239,6 +240,21 @@ def halfspace_volume(d: float, h: float, H: float = 0.0, kind: int = 0, precisio
     out = np.zeros(2)
     load().orc_halfspace_volume(precision, d, h, H, kind, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return float(out[0]), float(out[1])
+
+
+def shape_insert(slots, new, near: float) -> tuple[int, list[int], np.ndarray]:
+    """Insert ``new`` = (shape, phi, normal) into the slot row ``slots`` (a list of up to 4 (shape, phi, normal),
+    phi-ascending) with the oracle's float32 shape_insert; (kept, shapes after, phis after)."""
+    shape = np.full(4, -1, np.int32)
+    phi = np.zeros(4, np.float32)
+    nrm = np.zeros((4, 3), np.float32)
+    for k, (s_, p_, n_) in enumerate(slots):
+        shape[k], phi[k], nrm[k] = s_, p_, n_
+    nn = np.ascontiguousarray(new[2], np.float32)
+    kept = load().orc_shape_insert(shape.ctypes.data_as(ctypes.c_void_p), phi.ctypes.data_as(ctypes.c_void_p),
+                                   nrm.ctypes.data_as(ctypes.c_void_p), len(slots), int(new[0]), float(new[1]),
+                                   nn.ctypes.data_as(ctypes.c_void_p), float(near))
+    return int(kept), shape.tolist(), phi.copy()
 
 
 def shape_sdf(geo: int, scale, p) -> tuple[float, np.ndarray]:
