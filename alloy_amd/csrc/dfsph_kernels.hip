@@ -129,6 +129,13 @@ struct Ctx {
     float *t_mcount, *t_kraw, *t_ktmp, *t_kappa;           // [n]
     float *t_grad, *t_g1, *t_g2, *t_nhat, *t_nwall, *t_anorm;  // [3n]
     int *t_surf, *t_wslot;                // [n]
+    // LDS-staged halos of the gather passes (DFSPH_OPT_HALO): per chunk the sorted distinct neighbours of its
+    // particles (global ids, at most HALO_MAX, h_count -1 when the chunk's halo does not fit) and per row entry the
+    // neighbour's index in its chunk's halo ([cap + NB_U][n], the rows' layout and order)
+    int use_halo;
+    int* h_count;
+    int* h_ids;
+    uint16_t* h_slot;
     // packed gathers of the pressure acceleration: [3][n] {x, y, z, vec / rho0} for the three vectors k_accel
     // reads (kappa, the CG direction, the CG correction), and the per-world particle mass they stand in for
     int packed;
@@ -209,6 +216,7 @@ __device__ __forceinline__ int nbr_at(const Ctx& c, int base, int i, int k) {
 #define DFSPH_NB_U 4  // neighbour records in flight per lane (build-time tuning; 4 measured fastest of 1..16)
 #endif
 constexpr int NB_U = DFSPH_NB_U;
+constexpr int ROW_PAD = 16;  // zero rows past the row capacity of nbr, nbr_hi, pgrad and h_slot
 template <class T>
 __device__ __forceinline__ float4 load_pos(const T& d) { return d.q; }
 [[maybe_unused]] __device__ __forceinline__ float4 load_pos(float4 d) { return d; }
@@ -219,18 +227,19 @@ __device__ __forceinline__ void put_vr(const Ctx& c, float* vr, int slot, int i,
     vr[i] = v;
     if (c.packed) c.qk[int64_t(slot) * c.n + i].w = v;
 }
+// The row arrays hold ROW_PAD >= NB_U rows beyond `cap` (zero, never written), so a batch starting below the count
+// loads all of its NB_U entries unconditionally: the loads issue back to back instead of one branch-guarded round
+// trip each.
 template <bool WIDE>
 __device__ __forceinline__ void nbr_batch(const Ctx& c, int i, int cnt, int base, int k0, int (&j)[NB_U]) {
+    int e[NB_U];
 #pragma unroll
     for (int u = 0; u < NB_U; ++u) {
-        int k = k0 + u, v = i;
-        if (k < c.cap) {
-            int jj = base + int((c.nbr + int64_t(k) * c.n)[i]);
-            if (WIDE) jj += int((c.nbr_hi + int64_t(k) * c.n)[i]) << 16;
-            v = k < cnt ? jj : i;
-        }
-        j[u] = v;
+        e[u] = int((c.nbr + int64_t(k0 + u) * c.n)[i]);
+        if (WIDE) e[u] += int((c.nbr_hi + int64_t(k0 + u) * c.n)[i]) << 16;
     }
+#pragma unroll
+    for (int u = 0; u < NB_U; ++u) j[u] = k0 + u < cnt ? base + e[u] : i;
 }
 // body(k, j, record)
 template <bool WIDE, class T, class LoadF, class BodyF>
@@ -260,7 +269,7 @@ __device__ __forceinline__ void for_nbrs_g(const Ctx& c, int i, f3 xi, int cnt, 
 #pragma unroll
         for (int u = 0; u < NB_U; ++u) {
             d[u] = load(j[u]);
-            sc[u] = cached && k0 + u < c.cap ? (c.pgrad + int64_t(k0 + u) * c.n)[i] : 0.0f;
+            sc[u] = cached ? (c.pgrad + int64_t(k0 + u) * c.n)[i] : 0.0f;  // padded like the rows
         }
 #pragma unroll
         for (int u = 0; u < NB_U; ++u) {
@@ -1251,6 +1260,173 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
     int i = c.ch_begin[b_] + threadIdx.x; \
     bool valid = i < c.ch_end[b_];
 
+// ----------------------------------------------------------------------------------------------- staged halos
+// The gather passes of the pressure solves read, per pair, a 16-32 B record of the neighbour. Through the vector
+// memory path that is one gather instruction per record and about 20 distinct cache lines per instruction. A chunk's
+// 256 particles share most of their neighbours (their halo is about 4 chunks of records), so the staged passes load
+// the halo's records into LDS once per launch, with loads that walk ascending ids, and gather from LDS by each row
+// entry's halo index. The neighbour order of every sum is the row's, so results are bit for bit those of the direct
+// passes.
+constexpr int HALO_MAX = 1536;   // records a chunk's halo may hold in LDS (48 KB of {q4, acc4} in k_apply_h)
+constexpr int HALO_BITS = 32768; // world-local id range a halo may span (the bitmap of k_halo_build, 4 KB of LDS)
+
+// One block per chunk: the distinct world-local ids of the chunk's row entries by a bitmap over their id range, in
+// ascending order (h_ids, as global ids), and each entry's rank among them (h_slot). A chunk whose entries span more
+// than HALO_BITS ids or hold more than HALO_MAX distinct neighbours gets h_count -1: the passes gather its
+// neighbours directly.
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_halo_build(Ctx c) {
+    __shared__ unsigned bits[HALO_BITS / 32];
+    __shared__ int wpre[HALO_BITS / 32];
+    __shared__ int red[8];
+    CHUNK_PROLOGUE
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int cnt = valid ? c.count[i] : 0;
+    const int w = c.ch_world[b_];
+    auto entry = [&](int k) {
+        int e = int((c.nbr + int64_t(k) * c.n)[i]);
+        if (WIDE) e += int((c.nbr_hi + int64_t(k) * c.n)[i]) << 16;
+        return e;
+    };
+    int lo = INT_MAX, hi = -1;
+    for (int k = 0; k < cnt; ++k) {
+        int e = entry(k);
+        lo = min(lo, e);
+        hi = max(hi, e);
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        lo = min(lo, __shfl_xor(lo, d, 64));
+        hi = max(hi, __shfl_xor(hi, d, 64));
+    }
+    if (lane == 0) {
+        red[wv] = lo;
+        red[4 + wv] = hi;
+    }
+    __syncthreads();
+    lo = min(min(red[0], red[1]), min(red[2], red[3]));
+    hi = max(max(red[4], red[5]), max(red[6], red[7]));
+    if (hi < 0) {  // no row entries: an empty halo
+        if (t == 0) c.h_count[b_] = 0;
+        return;
+    }
+    if (hi - lo + 1 > HALO_BITS) {
+        if (t == 0) c.h_count[b_] = -1;
+        return;
+    }
+    const int nw = (hi - lo + 1 + 31) >> 5;
+    for (int q = t; q < nw; q += 256) bits[q] = 0u;
+    __syncthreads();
+    for (int k = 0; k < cnt; ++k) {
+        int d = entry(k) - lo;
+        atomicOr(&bits[d >> 5], 1u << (d & 31));
+    }
+    __syncthreads();
+    // exclusive prefix of the words' popcounts: thread t owns words 4t .. 4t + 3 of each 1024-word pass
+    int running = 0;
+    for (int q0 = 0; q0 < nw; q0 += 1024) {
+        int pc[4], sum = 0;
+        for (int u = 0; u < 4; ++u) {
+            int q = q0 + 4 * t + u;
+            pc[u] = q < nw ? __popc(bits[q]) : 0;
+            sum += pc[u];
+        }
+        int tot;
+        int ex = block_scan_excl(sum, &tot) + running;
+        for (int u = 0; u < 4; ++u) {
+            int q = q0 + 4 * t + u;
+            if (q < nw) wpre[q] = ex;
+            ex += pc[u];
+        }
+        running += tot;
+    }
+    __syncthreads();
+    if (running > HALO_MAX) {
+        if (t == 0) c.h_count[b_] = -1;
+        return;
+    }
+    const int gbase = c.ws[w] + lo;
+    int* ids = c.h_ids + int64_t(b_) * HALO_MAX;
+    for (int q = t; q < nw; q += 256) {
+        unsigned m = bits[q];
+        int r = wpre[q];
+        while (m) {
+            int b = __ffs(m) - 1;
+            ids[r++] = gbase + 32 * q + b;
+            m &= m - 1u;
+        }
+    }
+    for (int k = 0; k < cnt; ++k) {
+        int d = entry(k) - lo;
+        unsigned below = bits[d >> 5] & ((1u << (d & 31)) - 1u);
+        (c.h_slot + int64_t(k) * c.n)[i] = uint16_t(wpre[d >> 5] + __popc(below));
+    }
+    if (t == 0) c.h_count[b_] = running;
+}
+
+// Neighbour loop over a staged halo: the rows of halo indices (h_slot, padded like the rows) and the records in LDS
+// (load(s), s < h_count); otherwise as for_nbrs_g: body(record, grad W_ij) in ascending k.
+// SU entries per batch (8 measured slower than 4: more registers, fewer waves); the next batch's halo indices and
+// cached scales are loaded before the current batch's arithmetic (the rows hold ROW_PAD >= 2 SU padding rows), so
+// their memory latency overlaps it.
+constexpr int SU = 4;
+static_assert(ROW_PAD >= 2 * SU && ROW_PAD >= NB_U, "row padding");
+template <class T, class LoadF, class BodyF>
+__device__ __forceinline__ void for_slots_g(const Ctx& c, int i, f3 xi, int cnt, bool cached, LoadF load, BodyF body) {
+    const KC K = make_kc(c.h, c.kind);
+    int sl[SU];
+    float sc[SU];
+    auto fetch = [&](int k0, int (&s_)[SU], float (&g_)[SU]) {
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            s_[u] = int((c.h_slot + int64_t(k0 + u) * c.n)[i]);
+            g_[u] = cached ? (c.pgrad + int64_t(k0 + u) * c.n)[i] : 0.0f;
+        }
+    };
+    if (cnt > 0) fetch(0, sl, sc);
+    for (int k0 = 0; k0 < cnt; k0 += SU) {
+        T d[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) d[u] = load(k0 + u < cnt ? sl[u] : 0);
+        int sl_n[SU];
+        float sc_n[SU];
+        fetch(k0 + SU, sl_n, sc_n);
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            if (k0 + u >= cnt) continue;
+            f3 rv = xi - xyz(load_pos(d[u]));
+            body(d[u], cached ? pair_grad(rv, sc[u]) : kernel_grad(rv, K));
+        }
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            sl[u] = sl_n[u];
+            sc[u] = sc_n[u];
+        }
+    }
+}
+
+// Stage the chunk's halo into LDS: every index of the thread's share first, then every record, then the stores, so
+// a block waits for two memory round trips rather than one pair per record.
+constexpr int HALO_PER_THREAD = (HALO_MAX + 255) / 256;
+template <class RecF, class PutF>
+__device__ __forceinline__ void stage_halo(const Ctx& c, int b, int hc, RecF rec, PutF put) {
+    if (hc <= 0) return;
+    const int* ids = c.h_ids + int64_t(b) * HALO_MAX;
+    int jj[HALO_PER_THREAD];
+#pragma unroll
+    for (int u = 0; u < HALO_PER_THREAD; ++u) {
+        int t = int(threadIdx.x) + 256 * u;
+        jj[u] = t < hc ? ids[t] : -1;
+    }
+    using R = decltype(rec(0));
+    R r[HALO_PER_THREAD];
+#pragma unroll
+    for (int u = 0; u < HALO_PER_THREAD; ++u)
+        if (jj[u] >= 0) r[u] = rec(jj[u]);
+#pragma unroll
+    for (int u = 0; u < HALO_PER_THREAD; ++u)
+        if (jj[u] >= 0) put(int(threadIdx.x) + 256 * u, r[u]);
+}
+
 // pressure.py:144-193 for one particle (is_fluid), returning 1 for the chunk partial of `active` (solver.py:241).
 // With FEAT the same neighbour pass also gathers the neighbours' velocities and writes the particle's learned
 // warm-start features [16] (13 network inputs, then the rule's initial kappa of both solves), so the learned start
@@ -1623,6 +1799,176 @@ __global__ void __launch_bounds__(256) k_apply(Ctx c, int mode, float dt, const 
     }
     if (c.prof_bytes) account_bytes(c, KID_APPLY, b_, nb);
     if (reduce) chunk_reduce(c, px, 0.0, valid ? 0.0 : -3.4e38);
+}
+
+// k_accel / k_apply with the chunk's halo staged in LDS (DFSPH_OPT_HALO). The records: PACKED {x, y, z, vec / rho0}
+// from the vector's packed copy (the mass the world's), else the position-and-mass record and the quotient; for the
+// apply the position-and-mass record and the acceleration. A chunk without a halo (h_count -1) gathers directly.
+// The sparse-correction test of the gated pass (sparse_corr) still reads its neighbours' values directly.
+template <bool WIDE, bool PACKED>
+__device__ __forceinline__ void accel_h(const Ctx& c, const float* vec, const float* vecr, int slot, int gated) {
+    __shared__ float4 sq[HALO_MAX];
+    __shared__ float skr[PACKED ? 1 : HALO_MAX];
+    CHUNK_PROLOGUE
+    if (block_skip(c, c.ch_world[b_], gated) || !c.ch_live[b_]) return;
+    const int hc = c.h_count[b_];
+    if (PACKED) {
+        const float4* qk = c.qk + int64_t(slot) * c.n;
+        stage_halo(c, b_, hc, [&](int j) { return qk[j]; }, [&](int t, float4 r) { sq[t] = r; });
+    } else {
+        stage_halo(c, b_, hc, [&](int j) { return QKR{c.q4[j], vecr[j]}; }, [&](int t, QKR r) {
+            sq[t] = r.q;
+            skr[t] = r.kr;
+        });
+    }
+    __syncthreads();
+    unsigned nb = 0;
+    if (valid) {
+        if (!is_fluid(c, i)) {
+            c.acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            nb = 17u;
+        } else {
+            bool full = true;
+            if (gated && c.sparse_corr) {
+                int cnt = c.count[i];
+                full = vec[i] != 0.0f || any_nbr_nonzero<WIDE>(c, i, cnt, c.ws[c.world[i]], vecr);
+                c.touched[i] = full ? 1 : 0;
+                if (!full) {
+                    c.acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    nb = 34u + (WIDE ? 3u : 2u) * unsigned(cnt);
+                }
+            }
+            if (full) {
+                f3 a;
+                if (hc < 0) {
+                    a = pressure_accel<WIDE>(c, vec, vecr, slot, i);
+                } else {  // pressure_accel over the staged records
+                    f3 xi = xyz(c.q4[i]);
+                    float ki = vec[i] / c.rho0[i];
+                    a = f3{0.0f, 0.0f, 0.0f};
+                    const bool cached = (c.pair_cache & DFSPH_PAIR_CACHE_ACCEL) != 0;
+                    if (PACKED) {
+                        const float mw = c.w_mass[c.world[i]];
+                        for_slots_g<float4>(c, i, xi, c.count[i], cached, [&](int s) { return sq[s]; },
+                                            [&](float4 d, f3 g) { a = a - (mw * (ki + d.w)) * g; });
+                    } else {
+                        for_slots_g<QKR>(c, i, xi, c.count[i], cached, [&](int s) { return QKR{sq[s], skr[s]}; },
+                                         [&](QKR d, f3 g) { a = a - (d.q.w * (ki + d.kr)) * g; });
+                    }
+                    int64_t b4 = 4 * int64_t(i);
+                    for (int b = 0; b < 4; ++b)
+                        if (c.s_shape[b4 + b] >= 0) a = a - vec[i] * ld3(c.s_gpsi, b4 + b);
+                }
+                c.acc4[i] = make_float4(a.x, a.y, a.z, 0.0f);
+                nb = pass_bytes<WIDE>(c, i, 69u + (gated && c.sparse_corr ? 1u : 0u), 17u);
+            }
+        }
+    }
+    if (c.prof_bytes) account_bytes(c, KID_ACCEL, b_, nb);
+}
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_accel_hp(Ctx c, const float* vec, const float* vecr, int slot, int gated) {
+    accel_h<WIDE, true>(c, vec, vecr, slot, gated);
+}
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_accel_hn(Ctx c, const float* vec, const float* vecr, int slot, int gated) {
+    accel_h<WIDE, false>(c, vec, vecr, slot, gated);
+}
+
+// PACKED (one particle mass per world, the packed acceleration records' condition): 24 B records {x, y, z, a_x} and
+// {a_y, a_z}, the neighbour's mass the world's; else {x, y, z, m} and the acceleration
+struct QA2 {
+    float4 q;
+    float2 a;
+};
+template <bool WIDE, bool PACKED>
+__device__ __forceinline__ void apply_h(const Ctx& c, int mode, float dt, const float* vec, float* out, int gated,
+                                        int bias, int accum, int reduce) {
+    __shared__ float4 sq[HALO_MAX];
+    __shared__ float4 sa[PACKED ? 1 : HALO_MAX];
+    __shared__ float2 sa2[PACKED ? HALO_MAX : 1];
+    CHUNK_PROLOGUE
+    if (block_skip(c, c.ch_world[b_], gated)) return;
+    if (!c.ch_live[b_]) {
+        if (reduce) chunk_zero(c);
+        return;
+    }
+    const int hc = c.h_count[b_];
+    if (PACKED) {
+        stage_halo(c, b_, hc, [&](int j) {
+            float4 q = c.q4[j], a = c.acc4[j];
+            return QA2{make_float4(q.x, q.y, q.z, a.x), make_float2(a.y, a.z)};
+        }, [&](int t, QA2 r) {
+            sq[t] = r.q;
+            sa2[t] = r.a;
+        });
+    } else {
+        stage_halo(c, b_, hc, [&](int j) { return QA{c.q4[j], c.acc4[j]}; }, [&](int t, QA r) {
+            sq[t] = r.q;
+            sa[t] = r.a;
+        });
+    }
+    __syncthreads();
+    unsigned nb = 0;
+    double px = 0.0;
+    if (valid) {
+        if (!is_fluid(c, i)) {
+            out[i] = 0.0f;
+            nb = 9u;
+        } else {
+            bool full = true;
+            if (gated && c.sparse_corr) {
+                int cnt = c.count[i];
+                full = c.touched[i] != 0 || any_nbr_nonzero<WIDE>(c, i, cnt, c.ws[c.world[i]], c.touched);
+                if (!full) nb = 14u + (WIDE ? 3u : 2u) * unsigned(cnt);
+            }
+            if (full) {
+                float div;
+                if (hc < 0) {
+                    div = accel_divergence<WIDE>(c, i);
+                } else {  // accel_divergence over the staged records
+                    f3 xi = xyz(c.q4[i]), ai = A4(c, i);
+                    float inv_r0 = 1.0f / c.rho0[i];
+                    float ar = 0.0f;
+                    const bool cached = (c.pair_cache & DFSPH_PAIR_CACHE_APPLY) != 0;
+                    if (PACKED) {
+                        const float mw = c.w_mass[c.world[i]];
+                        for_slots_g<QA2>(c, i, xi, c.count[i], cached, [&](int s) { return QA2{sq[s], sa2[s]}; },
+                                         [&](QA2 d, f3 g) {
+                                             ar += (mw * inv_r0) * dot(ai - f3{d.q.w, d.a.x, d.a.y}, g);
+                                         });
+                    } else {
+                        for_slots_g<QA>(c, i, xi, c.count[i], cached, [&](int s) { return QA{sq[s], sa[s]}; },
+                                        [&](QA d, f3 g) { ar += (d.q.w * inv_r0) * dot(ai - xyz(d.a), g); });
+                    }
+                    int64_t b4 = 4 * int64_t(i);
+                    for (int b = 0; b < 4; ++b)
+                        if (c.s_shape[b4 + b] >= 0) ar += dot(ai, ld3(c.s_gpsi, b4 + b));
+                    div = ar;
+                }
+                float sc = mode == 0 ? dt : dt * dt;
+                float v = -sc * (div - c.reg[i] * vec[i]);
+                px = double(vec[i] * v);
+                if (bias) v -= (mode == 0 ? c.src[i] : c.src[i] - 1.0f);
+                if (accum) v += out[i];
+                out[i] = v;
+                nb = pass_bytes<WIDE>(c, i, 77u + (bias ? 4u : 0u) + (accum ? 4u : 0u) +
+                                                (gated && c.sparse_corr ? 1u : 0u), 9u);
+            }
+        }
+    }
+    if (c.prof_bytes) account_bytes(c, KID_APPLY, b_, nb);
+    if (reduce) chunk_reduce(c, px, 0.0, valid ? 0.0 : -3.4e38);
+}
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_apply_hp(Ctx c, int mode, float dt, const float* vec, float* out, int gated,
+                                                  int bias, int accum, int reduce) {
+    apply_h<WIDE, true>(c, mode, dt, vec, out, gated, bias, accum, reduce);
+}
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_apply_hn(Ctx c, int mode, float dt, const float* vec, float* out, int gated,
+                                                  int bias, int accum, int reduce) {
+    apply_h<WIDE, false>(c, mode, dt, vec, out, gated, bias, accum, reduce);
 }
 
 // cg_init_kernel (krylov.py:112-132)
@@ -3133,6 +3479,20 @@ int prof_harvest(Handle* H, bool wait) {
         else LAUNCH_T(kid, kernel<false>, grid, block, ctx, ##__VA_ARGS__);              \
     } while (0)
 
+// the pressure-acceleration and operator-apply passes, LDS-staged when DFSPH_OPT_HALO is on
+#define LAUNCH_ACCEL(ctx, ...)                                                                   \
+    do {                                                                                         \
+        if (!(ctx).use_halo) LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, ctx, __VA_ARGS__);  \
+        else if ((ctx).packed) LAUNCH_WT(KID_ACCEL, k_accel_hp, H->n_chunks, 256, ctx, __VA_ARGS__); \
+        else LAUNCH_WT(KID_ACCEL, k_accel_hn, H->n_chunks, 256, ctx, __VA_ARGS__);               \
+    } while (0)
+#define LAUNCH_APPLY(ctx, ...)                                                                   \
+    do {                                                                                         \
+        if (!(ctx).use_halo) LAUNCH_WT(KID_APPLY, k_apply, H->n_chunks, 256, ctx, __VA_ARGS__);  \
+        else if ((ctx).packed) LAUNCH_WT(KID_APPLY, k_apply_hp, H->n_chunks, 256, ctx, __VA_ARGS__); \
+        else LAUNCH_WT(KID_APPLY, k_apply_hn, H->n_chunks, 256, ctx, __VA_ARGS__);               \
+    } while (0)
+
 template <class T>
 int dalloc(Handle* H, T** p, int64_t count) {
     void* q = nullptr;
@@ -3209,6 +3569,10 @@ int phase_neighbors(Handle* H, hipStream_t st) {
         else k_skin_filter<false><<<blocks_for(c.n), 256, 0, st>>>(c);
     } else {
         launch_build_lists(H, st);
+    }
+    if (c.use_halo) {
+        if (c.wide) k_halo_build<true><<<H->n_chunks, 256, 0, st>>>(c);
+        else k_halo_build<false><<<H->n_chunks, 256, 0, st>>>(c);
     }
     LAUNCH_CHECK();
     return DFSPH_OK;
@@ -3320,8 +3684,8 @@ int enq_solve_head(Handle* H, const SolveSetup& S, hipStream_t st) {
               float(P.warm_start_density), S.relief, (const float*)S.store, S.learned);
     k_begin_solve<<<blocks_for(c.W), 256, 0, st>>>(c);
     if (P.pressure_solver == 0) {
-        LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.kappa, c.kr, VR_KAPPA, 0);
-        LAUNCH_WT(KID_APPLY, k_apply, H->n_chunks, 256, c, S.mode, S.dt, c.kappa, c.wv, 0, 1, 0, 0);
+        LAUNCH_ACCEL(c, c.kappa, c.kr, VR_KAPPA, 0);
+        LAUNCH_APPLY(c, S.mode, S.dt, c.kappa, c.wv, 0, 1, 0, 0);
         k_cg_init<<<H->n_chunks, 256, 0, st>>>(c, S.mode, S.dt);
         k_world<<<c.W, 256, 0, st>>>(c, W_CG_INIT, S.sp);
     }
@@ -3336,12 +3700,12 @@ int enq_solve_batch(Handle* H, const SolveSetup& S, hipStream_t st, int* flag_ou
     float dt = S.dt;
     for (int rep = 0; rep < H->check_every; ++rep) {
         if (H->P.pressure_solver == 0) {
-            LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.pd, c.pdr, VR_DIR, 0);
-            LAUNCH_WT(KID_APPLY, k_apply, H->n_chunks, 256, c, mode, dt, c.pd, c.ap, 0, 0, 0, 1);
+            LAUNCH_ACCEL(c, c.pd, c.pdr, VR_DIR, 0);
+            LAUNCH_APPLY(c, mode, dt, c.pd, c.ap, 0, 0, 0, 1);
             k_world<<<c.W, 256, 0, st>>>(c, W_CG_PAP, S.sp);
             k_cg_step<<<H->n_chunks, 256, 0, st>>>(c);
-            LAUNCH_WT(KID_ACCEL, k_accel, H->n_chunks, 256, c, c.corr, c.corrr, VR_CORR, 1);
-            LAUNCH_WT(KID_APPLY, k_apply, H->n_chunks, 256, c, mode, dt, c.corr, c.wv, 1, 0, 1, 0);
+            LAUNCH_ACCEL(c, c.corr, c.corrr, VR_CORR, 1);
+            LAUNCH_APPLY(c, mode, dt, c.corr, c.wv, 1, 0, 1, 0);
             k_cg_finish<<<H->n_chunks, 256, 0, st>>>(c, mode, dt);
             k_world<<<c.W, 256, 0, st>>>(c, W_CG_BETA, S.sp);
             k_cg_direction<<<H->n_chunks, 256, 0, st>>>(c, rep == H->check_every - 1 ? flag_out : nullptr);
@@ -3372,7 +3736,8 @@ int run_segment(Handle* H, hipStream_t st, int kind, int mode, float dt, F enque
     uint32_t bits;
     std::memcpy(&bits, &dt, 4);
     GraphKey key{kind, mode, H->check_every,
-                 H->c.pair_cache | (H->c.sparse_corr << 8) | (H->build_block << 16) | (H->c.packed << 28), bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
+                 H->c.pair_cache | (H->c.sparse_corr << 8) | (H->build_block << 16) | (H->c.packed << 28) |
+                     (H->c.use_halo << 29), bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
                  H->mlp_params, H->log_feat, H->log_targets};
     auto it = H->graphs.find(key);
     if (it == H->graphs.end()) {
@@ -3752,9 +4117,18 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         // lists
         H->nbr_bytes = int64_t(c.cap) * std::max(n, 1) * 2;
         H->nbr_hi_bytes = c.wide ? int64_t(c.cap) * std::max(n, 1) : 1;
-        TRY(dalloc(H, &c.nbr, int64_t(c.cap) * std::max(n, 1)));
-        TRY(dalloc(H, &c.nbr_hi, c.wide ? int64_t(c.cap) * std::max(n, 1) : 1));
-        TRY(dalloc(H, &c.pgrad, int64_t(c.cap) * std::max(n, 1)));
+        // ROW_PAD zero rows past the capacity: a neighbour batch reads them unconditionally (nbr_batch), and the
+        // staged loops prefetch the next batch's entries (for_slots_g)
+        const int64_t rows = int64_t(c.cap) + ROW_PAD;
+        TRY(dalloc(H, &c.nbr, rows * std::max(n, 1)));
+        TRY(dalloc(H, &c.nbr_hi, c.wide ? rows * std::max(n, 1) : 1));
+        TRY(dalloc(H, &c.pgrad, rows * std::max(n, 1)));
+        TRY(dalloc(H, &c.h_slot, rows * std::max(n, 1)));
+        TRY(dalloc(H, &c.h_ids, int64_t(std::max(H->n_chunks, 1)) * HALO_MAX));
+        TRY(dalloc(H, &c.h_count, std::max(H->n_chunks, 1)));
+        // no halo until the first list build with the option on: every chunk gathers directly
+        k_fill_i32<<<blocks_for(std::max(H->n_chunks, 1)), 256>>>(c.h_count, -1, std::max(H->n_chunks, 1));
+        LAUNCH_CHECK();
         {  // one mass per world (bitwise) over its SPH particles: the packed acceleration gathers may stand in for it
             std::vector<float> hm(std::max(n, 1));
             std::vector<int8_t> hp(std::max(n, 1));
@@ -4150,6 +4524,17 @@ int32_t dfsph_set_option(void* h, int32_t option, int32_t value) {
             H->learned_solves = value;
             apply_learned(H);
             return DFSPH_OK;
+        case DFSPH_OPT_HALO:
+            if (value < 0 || value > 1) return fail(DFSPH_EINVAL, "halo: 0 or 1");
+            if (H->c.use_halo != value) {
+                H->c.use_halo = value;
+                // the halos of the current rows do not exist yet: gather directly until the next list build
+                int nc = std::max(H->n_chunks, 1);
+                k_fill_i32<<<blocks_for(nc), 256>>>(H->c.h_count, -1, nc);
+                LAUNCH_CHECK();
+                HIPCHK(hipDeviceSynchronize());
+            }
+            return DFSPH_OK;
         default:
             return fail(DFSPH_EINVAL, "unknown option");
     }
@@ -4178,6 +4563,9 @@ int32_t dfsph_get_option(void* h, int32_t option, int32_t* value) {
             return DFSPH_OK;
         case DFSPH_OPT_BUILD_BLOCK:
             *value = H->build_block;
+            return DFSPH_OK;
+        case DFSPH_OPT_HALO:
+            *value = H->c.use_halo;
             return DFSPH_OK;
         default:
             return fail(DFSPH_EINVAL, "unknown option");
@@ -4328,6 +4716,7 @@ int32_t dfsph_get_buffer(void* h, int32_t which, void** ptr, int64_t* nb) {
         case DFSPH_BUF_PW_ROW_COUNT:
             if (c.NW == 0) return fail(DFSPH_EINVAL, "no particle walls");
             B(c.wr_n, 4 * n)
+        case DFSPH_BUF_HALO_COUNT: B(c.h_count, 4 * int64_t(std::max(H->n_chunks, 1)))
         default: return fail(DFSPH_EINVAL, "unknown buffer id");
     }
 #undef B

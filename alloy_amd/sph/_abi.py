@@ -33,7 +33,7 @@ DFSPH_OK, DFSPH_EINVAL, DFSPH_ECAPACITY, DFSPH_EDEVICE = 0, -1, -2, -3
  BUF_DIV_ERR, BUF_SHAPE_METER, BUF_FEATURES, BUF_MASS_INITIAL, BUF_MASS_IN, BUF_MASS_OUT, BUF_MASS_ACTIVE, BUF_ESCAPED,
  BUF_EMIT_OVERFLOW, BUF_FLUX_E, BUF_FLUX_G, BUF_EMIT_T, BUF_EMIT_CURSOR, BUF_EMIT_ENABLED, BUF_FLAGS,
  BUF_PHASE, BUF_PW_EXCH, BUF_PW_WORK, BUF_SKIN_FLAG, BUF_VISC_ITERS, BUF_CSF_SURF, BUF_CSF_KAPPA,
- BUF_PW_ROW_COUNT) = range(60)
+ BUF_PW_ROW_COUNT, BUF_HALO_COUNT) = range(61)
 IMP_SCALE = float(2**50)    # impulse fixed point, bodies.py:25
 
 FIX_SCALE = float(2**40)    # mass fixed point, kernels.py:32-33
@@ -43,9 +43,9 @@ FORCE_SCALE = float(2**40)  # bodies.py:23
 # kernel ids of dfsph_profile_read
 # tuning options (dfsph_set_option); none but OPT_LEARNED_SOLVES changes a result bit
 (OPT_CHECK_INTERVAL, OPT_GRAPHS, OPT_PAIR_CACHE, OPT_LEARNED_SOLVES, OPT_SPARSE_CORRECTION, OPT_BUILD_BLOCK,
- OPT_PACKED_ACCEL) = (1, 2, 3, 4, 5, 6, 7)
+ OPT_PACKED_ACCEL, OPT_HALO) = (1, 2, 3, 4, 5, 6, 7, 8)
 ALL_OPTIONS = (OPT_CHECK_INTERVAL, OPT_GRAPHS, OPT_PAIR_CACHE, OPT_LEARNED_SOLVES, OPT_SPARSE_CORRECTION,
-               OPT_BUILD_BLOCK, OPT_PACKED_ACCEL)
+               OPT_BUILD_BLOCK, OPT_PACKED_ACCEL, OPT_HALO)
 PAIR_CACHE_ACCEL, PAIR_CACHE_APPLY, PAIR_CACHE_OTHER, PAIR_CACHE_ALL = 1, 2, 4, 7
 
 (KID_ACCEL, KID_APPLY, KID_BUILD, KID_DENSITY, KID_SLOTS, KID_NONPRESSURE, KID_ADVECT, KID_SOLVE_INIT,

@@ -211,6 +211,8 @@ enum DfsphBuffer {
     DFSPH_BUF_PW_ROW_COUNT,   /* int32  [n] with particle walls: the wall particles the last slot pass's colour sum of
                                  each liquid particle visited, -1 when more than its 32-entry row holds (the reaction
                                  split then walks the cells again) */
+    DFSPH_BUF_HALO_COUNT,     /* int32  [chunks] with DFSPH_OPT_HALO: distinct neighbours of each 256-particle chunk at
+                                 the last list build, -1 where the halo did not fit (direct gathers) */
     DFSPH_BUF_COUNT_
 };
 
@@ -291,10 +293,14 @@ enum DfsphOption {
                                         the others' exact zero results; 0 evaluates every particle. Bit-identical. */
     DFSPH_OPT_BUILD_BLOCK = 6,       /* threads per block of the neighbour-list build: 64, 128 (default) or 256
                                         (256 only without 24-bit rows). Bit-identical. */
-    DFSPH_OPT_PACKED_ACCEL = 7       /* 1 (default when every world's SPH particles share one mass): the pressure
+    DFSPH_OPT_PACKED_ACCEL = 7,      /* 1 (default when every world's SPH particles share one mass): the pressure
                                         acceleration gathers one 16 B record {x, y, z, kappa / rho0} per neighbour
                                         from a packed copy, the mass the world's; 0: positions and quotients apart.
                                         Bit-identical. */
+    DFSPH_OPT_HALO = 8               /* 1: each list build also lists every 256-particle chunk's distinct neighbours
+                                        (its halo, up to 1,536), and the pressure acceleration and operator apply
+                                        stage the halo's records in LDS and gather from there; chunks whose halo does
+                                        not fit gather directly. 0: every chunk gathers directly. Bit-identical. */
 };
 enum DfsphPairCache {
     DFSPH_PAIR_CACHE_ACCEL = 1, /* pressure acceleration (k_accel, post-solve) */
