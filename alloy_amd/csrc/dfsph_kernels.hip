@@ -185,6 +185,9 @@ struct Ctx {
     unsigned long long* meter;
     double* part;
     int* n_unconv;
+    // the device-side solve loop (k_cg_loop): its grid-barrier counter and a fault flag (a barrier wait that ran out)
+    unsigned* loop_bar;
+    int* loop_fault;
     unsigned long long* prof_bytes;  // [2][n_chunks][4] per-wave byte counts while profiling, else null
     // state (bound per call)
     float* q;
@@ -350,14 +353,14 @@ __device__ __forceinline__ double tree_max(double v, double* lds) {
     __syncthreads();
     return r;
 }
-// chunk partial (sum x, sum y, max y) of the block's values; invalid lanes pass (0, 0, -3.4e38)
-__device__ __forceinline__ void chunk_reduce(const Ctx& c, double x, double y, double m) {
+// chunk b_'s partial (sum x, sum y, max y) of the block's values; invalid lanes pass (0, 0, -3.4e38)
+__device__ __forceinline__ void chunk_reduce(const Ctx& c, int b_, double x, double y, double m) {
     __shared__ double lds[CHUNK];
     double sx = tree_sum(x, lds);
     double sy = tree_sum(y, lds);
     double mx = tree_max(m, lds);
     if (threadIdx.x == 0) {
-        double* p = c.part + 3 * int64_t(xcd_block(int(blockIdx.x), int(gridDim.x)));
+        double* p = c.part + 3 * int64_t(b_);
         p[0] = sx;
         p[1] = sy;
         p[2] = mx;
@@ -365,9 +368,9 @@ __device__ __forceinline__ void chunk_reduce(const Ctx& c, double x, double y, d
 }
 
 // The chunk partial of a chunk without liquid: every term is +0 and every max term 0.0 (each chunk has a particle)
-__device__ __forceinline__ void chunk_zero(const Ctx& c) {
+__device__ __forceinline__ void chunk_zero(const Ctx& c, int b_) {
     if (threadIdx.x == 0) {
-        double* p = c.part + 3 * int64_t(xcd_block(int(blockIdx.x), int(gridDim.x)));
+        double* p = c.part + 3 * int64_t(b_);
         p[0] = 0.0;
         p[1] = 0.0;
         p[2] = 0.0;
@@ -389,8 +392,7 @@ __device__ __forceinline__ bool stop_test(const Ctx& c, int w, int it, float mea
 
 // One block per world: tree over the world's chunk partials (thread t sums chunks t, t + 256, ... in order), then
 // the world's control logic in thread 0 (reduce.py:100-164 + pressure.py:370-390 / krylov.py:215-240).
-__global__ void __launch_bounds__(256) k_world(Ctx c, int mode, StopSpec sp) {
-    int w = blockIdx.x;
+__device__ void world_body(const Ctx& c, int w, int mode, const StopSpec& sp) {
     if (mode == W_CG_PAP && w == 0 && threadIdx.x == 0) *c.n_unconv = 0;  // recounted by W_CG_BETA
     if ((mode == W_CG_PAP || mode == W_CG_BETA || mode == W_JACOBI) && c.converged[w] != 0) return;
     if (mode == W_VISC_ALPHA && w == 0 && threadIdx.x == 0) *c.n_unconv = 0;  // recounted by W_VISC_CHECK
@@ -488,6 +490,8 @@ __global__ void __launch_bounds__(256) k_world(Ctx c, int mode, StopSpec sp) {
         }
     }
 }
+
+__global__ void __launch_bounds__(256) k_world(Ctx c, int mode, StopSpec sp) { world_body(c, blockIdx.x, mode, sp); }
 
 // ----------------------------------------------------------------------------------------------- utilities
 __global__ void k_fill_i32(int* p, int v, int64_t n) {
@@ -1533,7 +1537,7 @@ __global__ void __launch_bounds__(256) k_density_alpha(Ctx c) {
             act_v = 1.0;
         }
     }
-    chunk_reduce(c, act_v, 0.0, valid ? act_v : -3.4e38);
+    chunk_reduce(c, b_, act_v, 0.0, valid ? act_v : -3.4e38);
 }
 // the same pass plus the learned warm start's features of the step (dt) in c.feat
 template <bool WIDE>
@@ -1549,7 +1553,7 @@ __global__ void __launch_bounds__(256) k_density_features(Ctx c, float dt) {
             act_v = 1.0;
         }
     }
-    chunk_reduce(c, act_v, 0.0, valid ? act_v : -3.4e38);
+    chunk_reduce(c, b_, act_v, 0.0, valid ? act_v : -3.4e38);
 }
 
 // ----------------------------------------------------------------------------------------------- solves
@@ -1728,8 +1732,10 @@ __global__ void __launch_bounds__(256) k_pack_sync(Ctx c) {
 
 // accel_kernel (pressure.py:280-291) / cg_accel_kernel (krylov.py:77-87)
 template <bool WIDE>
-__global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, const float* vecr, int slot, int gated) {
-    CHUNK_PROLOGUE
+__device__ __forceinline__ void accel_chunk(const Ctx& c, int b_, const float* vec, const float* vecr, int slot,
+                                            int gated) {
+    int i = c.ch_begin[b_] + threadIdx.x;
+    bool valid = i < c.ch_end[b_];
     if (block_skip(c, c.ch_world[b_], gated) || !c.ch_live[b_]) return;
     unsigned nb = 0;  // algorithmic bytes of this particle (profiling)
     if (valid) {
@@ -1759,15 +1765,20 @@ __global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, const fl
     }
     if (c.prof_bytes) account_bytes(c, KID_ACCEL, b_, nb);
 }
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_accel(Ctx c, const float* vec, const float* vecr, int slot, int gated) {
+    accel_chunk<WIDE>(c, xcd_block(int(blockIdx.x), int(gridDim.x)), vec, vecr, slot, gated);
+}
 
 // cg_apply_kernel (krylov.py:90-109); with `reduce` the chunk partial of pair.x = vec_i (A vec)_i
 template <bool WIDE>
-__global__ void __launch_bounds__(256) k_apply(Ctx c, int mode, float dt, const float* vec, float* out, int gated,
-                                               int bias, int accum, int reduce) {
-    CHUNK_PROLOGUE
+__device__ __forceinline__ void apply_chunk(const Ctx& c, int b_, int mode, float dt, const float* vec, float* out,
+                                            int gated, int bias, int accum, int reduce) {
+    int i = c.ch_begin[b_] + threadIdx.x;
+    bool valid = i < c.ch_end[b_];
     if (block_skip(c, c.ch_world[b_], gated)) return;
     if (!c.ch_live[b_]) {
-        if (reduce) chunk_zero(c);
+        if (reduce) chunk_zero(c, b_);
         return;
     }
     unsigned nb = 0;  // algorithmic bytes of this particle (profiling)
@@ -1798,7 +1809,12 @@ __global__ void __launch_bounds__(256) k_apply(Ctx c, int mode, float dt, const 
         }
     }
     if (c.prof_bytes) account_bytes(c, KID_APPLY, b_, nb);
-    if (reduce) chunk_reduce(c, px, 0.0, valid ? 0.0 : -3.4e38);
+    if (reduce) chunk_reduce(c, b_, px, 0.0, valid ? 0.0 : -3.4e38);
+}
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_apply(Ctx c, int mode, float dt, const float* vec, float* out, int gated,
+                                               int bias, int accum, int reduce) {
+    apply_chunk<WIDE>(c, xcd_block(int(blockIdx.x), int(gridDim.x)), mode, dt, vec, out, gated, bias, accum, reduce);
 }
 
 // k_accel / k_apply with the chunk's halo staged in LDS (DFSPH_OPT_HALO). The records: PACKED {x, y, z, vec / rho0}
@@ -1890,7 +1906,7 @@ __device__ __forceinline__ void apply_h(const Ctx& c, int mode, float dt, const 
     CHUNK_PROLOGUE
     if (block_skip(c, c.ch_world[b_], gated)) return;
     if (!c.ch_live[b_]) {
-        if (reduce) chunk_zero(c);
+        if (reduce) chunk_zero(c, b_);
         return;
     }
     const int hc = c.h_count[b_];
@@ -1958,7 +1974,7 @@ __device__ __forceinline__ void apply_h(const Ctx& c, int mode, float dt, const 
         }
     }
     if (c.prof_bytes) account_bytes(c, KID_APPLY, b_, nb);
-    if (reduce) chunk_reduce(c, px, 0.0, valid ? 0.0 : -3.4e38);
+    if (reduce) chunk_reduce(c, b_, px, 0.0, valid ? 0.0 : -3.4e38);
 }
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_apply_hp(Ctx c, int mode, float dt, const float* vec, float* out, int gated,
@@ -1976,7 +1992,7 @@ __global__ void __launch_bounds__(256) k_cg_init(Ctx c, int mode, float dt) {
     CHUNK_PROLOGUE
     if (c.converged[c.ch_world[b_]] != 0) return;
     if (!c.ch_live[b_]) {
-        chunk_zero(c);
+        chunk_zero(c, b_);
         return;
     }
     double px = 0.0;
@@ -2002,12 +2018,13 @@ __global__ void __launch_bounds__(256) k_cg_init(Ctx c, int mode, float dt) {
             px = double(-wi * zi);
         }
     }
-    chunk_reduce(c, px, 0.0, valid ? 0.0 : -3.4e38);
+    chunk_reduce(c, b_, px, 0.0, valid ? 0.0 : -3.4e38);
 }
 
 // cg_step_kernel (krylov.py:153-177)
-__global__ void __launch_bounds__(256) k_cg_step(Ctx c) {
-    CHUNK_PROLOGUE
+__device__ __forceinline__ void cg_step_chunk(const Ctx& c, int b_) {
+    int i = c.ch_begin[b_] + threadIdx.x;
+    bool valid = i < c.ch_end[b_];
     int w = c.ch_world[b_];
     if (c.converged[w] != 0 || !valid || !c.ch_live[b_]) return;
     if (!is_fluid(c, i)) {
@@ -2035,15 +2052,17 @@ __global__ void __launch_bounds__(256) k_cg_step(Ctx c) {
     put_vr(c, c.corrr, VR_CORR, i, cc / c.rho0[i]);
     c.wv[i] = c.wv[i] + a * c.ap[i];
 }
+__global__ void __launch_bounds__(256) k_cg_step(Ctx c) { cg_step_chunk(c, xcd_block(int(blockIdx.x), int(gridDim.x))); }
 
 // cg_finish_kernel (krylov.py:180-212) + the chunk partial of (r.z, error, max error)
-__global__ void __launch_bounds__(256) k_cg_finish(Ctx c, int mode, float dt) {
-    CHUNK_PROLOGUE
+__device__ __forceinline__ void cg_finish_chunk(const Ctx& c, int b_, int mode, float dt) {
+    int i = c.ch_begin[b_] + threadIdx.x;
+    bool valid = i < c.ch_end[b_];
     int w = c.ch_world[b_];
     if (c.converged[w] != 0) return;
     if (!c.ch_live[b_]) {  // the error of a particle that is not liquid is 0
         if (valid) c.err[i] = 0.0f;
-        chunk_zero(c);
+        chunk_zero(c, b_);
         return;
     }
     double px = 0.0, py = 0.0;
@@ -2071,22 +2090,107 @@ __global__ void __launch_bounds__(256) k_cg_finish(Ctx c, int mode, float dt) {
             py = double(e);
         }
     }
-    chunk_reduce(c, px, py, valid ? py : -3.4e38);
+    chunk_reduce(c, b_, px, py, valid ? py : -3.4e38);
+}
+__global__ void __launch_bounds__(256) k_cg_finish(Ctx c, int mode, float dt) {
+    cg_finish_chunk(c, xcd_block(int(blockIdx.x), int(gridDim.x)), mode, dt);
 }
 
 // cg_direction_kernel (krylov.py:243-255)
-// flag_out (pinned host memory, may be null): block 0 stores the unconverged-world count that k_world's stop test
-// just produced, so the host learns it from this batch's completion event without a separate copy
-__global__ void __launch_bounds__(256) k_cg_direction(Ctx c, int* flag_out) {
-    CHUNK_PROLOGUE
-    if (flag_out && blockIdx.x == 0 && threadIdx.x == 0)
-        __hip_atomic_store(flag_out, *c.n_unconv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+__device__ __forceinline__ void cg_direction_chunk(const Ctx& c, int b_) {
+    int i = c.ch_begin[b_] + threadIdx.x;
+    bool valid = i < c.ch_end[b_];
     int w = c.ch_world[b_];
     if (c.converged[w] != 0 || !valid || !c.ch_live[b_] || !is_fluid(c, i)) return;
     float p = 0.0f;
     if (c.act[i] != 0) p = c.z[i] + float(c.stepb[w]) * c.pd[i];
     c.pd[i] = p;
     put_vr(c, c.pdr, VR_DIR, i, p / c.rho0[i]);
+}
+// flag_out (pinned host memory, may be null): block 0 stores the unconverged-world count that k_world's stop test
+// just produced, so the host learns it from this batch's completion event without a separate copy
+__global__ void __launch_bounds__(256) k_cg_direction(Ctx c, int* flag_out) {
+    if (flag_out && blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(flag_out, *c.n_unconv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    cg_direction_chunk(c, xcd_block(int(blockIdx.x), int(gridDim.x)));
+}
+
+// ----------------------------------------------------------------------------------------------- device solve loop
+// The projected-CG iterations of one solve (krylov.py:298-318's capture_while) as one cooperative launch: every
+// block of the grid is resident, the phases of an iteration (the same per-chunk and per-world bodies the separate
+// kernels run, in the same order) are separated by grid barriers, and the loop ends when the stop test of the last
+// iteration left no unconverged world. No host round trip: the host enqueues the head, this kernel and the tail.
+
+constexpr long long LOOP_WAIT_TICKS = 200000000LL;  // 2 s of the 100 MHz real-time counter: no phase takes that long
+
+// Grid barrier: thread 0 of each block adds one arrival to the counter (agent-scope release) and waits for the
+// barrier's target (arrivals of all blocks so far, agent-scope acquire); the block's other threads wait at the
+// workgroup barriers around it. The wait is bounded: past LOOP_WAIT_TICKS, or when another block already gave up,
+// the block sets the fault flag and the call returns false, and the kernel ends. Returns `*watch` as read after the
+// barrier by thread 0 (the same value for every block: no phase writes it between two barriers' reads).
+__device__ __forceinline__ bool grid_barrier(const Ctx& c, unsigned target, const int* watch, int* seen) {
+    __shared__ int s_ok, s_val;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(c.loop_bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const long long t0 = wall_clock64();
+        int ok = 1;
+        while (__hip_atomic_load(c.loop_bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(c.loop_fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                wall_clock64() - t0 > LOOP_WAIT_TICKS) {
+                __hip_atomic_store(c.loop_fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        s_ok = ok;
+        s_val = watch ? __hip_atomic_load(watch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+    }
+    __syncthreads();
+    if (seen) *seen = s_val;
+    return s_ok != 0;
+}
+
+// body(chunk) over this block's share of the chunks: the blocks of XCD x (block b runs on XCD b % 8) take the
+// contiguous x-th eighth of the chunk table, as xcd_block arranges it for the separate launches
+template <class F>
+__device__ __forceinline__ void loop_chunks(const Ctx& c, F body) {
+    const int G = int(gridDim.x), x = int(blockIdx.x) % N_XCD, per = G / N_XCD;
+    const int lo = int((int64_t(c.n_chunks) * x) / N_XCD), hi = int((int64_t(c.n_chunks) * (x + 1)) / N_XCD);
+    for (int b = lo + int(blockIdx.x) / N_XCD; b < hi; b += per) body(b);
+}
+
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_cg_loop(Ctx c, int mode, float dt, StopSpec sp) {
+    const unsigned G = gridDim.x;
+    unsigned target = 0;
+    int unconv = 0;
+    if (!grid_barrier(c, target += G, c.n_unconv, &unconv)) return;
+    for (int it = 0; unconv != 0; ++it) {
+        if (it > 100000) {  // the stop test caps every world at max_iterations: never reached
+            if (threadIdx.x == 0) __hip_atomic_store(c.loop_fault, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        loop_chunks(c, [&](int b) { accel_chunk<WIDE>(c, b, c.pd, c.pdr, VR_DIR, 0); });
+        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        loop_chunks(c, [&](int b) { apply_chunk<WIDE>(c, b, mode, dt, c.pd, c.ap, 0, 0, 0, 1); });
+        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        for (int w = int(blockIdx.x); w < c.W; w += int(G)) world_body(c, w, W_CG_PAP, sp);
+        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        loop_chunks(c, [&](int b) { cg_step_chunk(c, b); });
+        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        loop_chunks(c, [&](int b) { accel_chunk<WIDE>(c, b, c.corr, c.corrr, VR_CORR, 1); });
+        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        loop_chunks(c, [&](int b) { apply_chunk<WIDE>(c, b, mode, dt, c.corr, c.wv, 1, 0, 1, 0); });
+        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        loop_chunks(c, [&](int b) { cg_finish_chunk(c, b, mode, dt); });
+        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        for (int w = int(blockIdx.x); w < c.W; w += int(G)) world_body(c, w, W_CG_BETA, sp);
+        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        loop_chunks(c, [&](int b) { cg_direction_chunk(c, b); });
+        if (!grid_barrier(c, target += G, c.n_unconv, &unconv)) return;
+    }
 }
 
 // jacobi_kernel (pressure.py:313-352) + the chunk partial of (error, -, max error); block (0,0) clears the
@@ -2128,7 +2232,7 @@ __global__ void __launch_bounds__(256) k_jacobi(Ctx c, int mode, float dt) {
             e64 = double(e);
         }
     }
-    chunk_reduce(c, e64, 0.0, valid ? e64 : -3.4e38);
+    chunk_reduce(c, b_, e64, 0.0, valid ? e64 : -3.4e38);
 }
 
 // contact_limit_kernel + apply_pressure_kernel + lone_friction_kernel (pressure.py:393-480) for one particle: a
@@ -2233,7 +2337,7 @@ __global__ void __launch_bounds__(256) k_flow_terms(Ctx c, int which) {
             y = v.y;
         }
     }
-    chunk_reduce(c, x, y, valid ? y : -3.4e38);
+    chunk_reduce(c, b_, x, y, valid ? y : -3.4e38);
 }
 // head_terms_kernel (solve_driver.py:34-46)
 __global__ void __launch_bounds__(256) k_head_terms(Ctx c) {
@@ -2244,7 +2348,7 @@ __global__ void __launch_bounds__(256) k_head_terms(Ctx c) {
         x = k;
         y = double(k * k);
     }
-    chunk_reduce(c, x, y, valid ? y : -3.4e38);
+    chunk_reduce(c, b_, x, y, valid ? y : -3.4e38);
 }
 // tol_scale_kernel (solve_driver.py:63-82)
 __global__ void k_tol_scale(Ctx c, float dt, float speed, float ratio, float tol, float tol_min) {
@@ -2799,7 +2903,7 @@ __global__ void __launch_bounds__(256) k_visc_apply(Ctx c, float dt, const float
             px = double(dot(vi, r));
         }
     }
-    chunk_reduce(c, px, 0.0, valid ? 0.0 : -3.4e38);
+    chunk_reduce(c, b_, px, 0.0, valid ? 0.0 : -3.4e38);
 }
 
 // viscosity.py:74-111 visc_init_kernel: the block preconditioner, r = b - A x, z = M^-1 r, p = z; partial (r.z, b.b)
@@ -2849,7 +2953,7 @@ __global__ void __launch_bounds__(256) k_visc_init(Ctx c, float dt) {
             py = double(dot(bi, bi));
         }
     }
-    chunk_reduce(c, px, py, valid ? 0.0 : -3.4e38);
+    chunk_reduce(c, b_, px, py, valid ? 0.0 : -3.4e38);
 }
 
 // viscosity.py:148-163 visc_update_kernel: x += a p, r -= a Ap, z = M^-1 r; partial (r.z, r.r)
@@ -2871,7 +2975,7 @@ __global__ void __launch_bounds__(256) k_visc_update(Ctx c) {
         px = double(dot(ri, zi));
         py = double(dot(ri, ri));
     }
-    chunk_reduce(c, px, py, valid ? 0.0 : -3.4e38);
+    chunk_reduce(c, b_, px, py, valid ? 0.0 : -3.4e38);
 }
 
 // viscosity.py:188-197 visc_direction_kernel; block 0 stores the unconverged-world count for the host
@@ -3394,6 +3498,8 @@ struct Handle {
     float* learned_div = nullptr;
     float* learned_den = nullptr;
     int check_every = 1;
+    int device_loop = 0;                // DFSPH_OPT_DEVICE_LOOP: the CG iterations as one cooperative launch (k_cg_loop)
+    int loop_grid[2] = {0, 0};          // its grid (co-resident blocks, a multiple of 8) per row width, 0: not yet known
     Prof prof;
     bool use_graphs = false;  // measured: eager launches are faster than graph replay for this step (bench.py --graphs)
     const float* mlp_params = nullptr;  // learned warm start on when set
@@ -3728,6 +3834,34 @@ int enq_solve_tail(Handle* H, const SolveSetup& S, hipStream_t st) {
     return DFSPH_OK;
 }
 
+// The CG iterations of a solve as one cooperative launch of k_cg_loop (every block resident: hipLaunchCooperativeKernel
+// refuses a grid that does not fit), with the barrier counter cleared by a kernel on the same stream first.
+template <bool WIDE>
+int launch_cg_loop_w(Handle* H, const SolveSetup& S, hipStream_t st) {
+    int& grid = H->loop_grid[WIDE ? 1 : 0];
+    if (grid == 0) {
+        int dev = 0, per_cu = 0;
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipGetDeviceProperties(&prop, dev));
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_loop<WIDE>, 256, 0));
+        grid = (per_cu * prop.multiProcessorCount) / N_XCD * N_XCD;
+        if (grid < N_XCD) return fail(DFSPH_EDEVICE, "device solve loop: no co-resident grid");
+    }
+    k_fill_i32<<<1, 64, 0, st>>>(reinterpret_cast<int*>(S.c.loop_bar), 0, 1);
+    Ctx c = S.c;
+    int mode = S.mode;
+    float dt = S.dt;
+    StopSpec sp = S.sp;
+    void* args[] = {&c, &mode, &dt, &sp};
+    HIPCHK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_cg_loop<WIDE>), dim3(grid), dim3(256), args, 0,
+                                      st));
+    return DFSPH_OK;
+}
+int launch_cg_loop(Handle* H, const SolveSetup& S, hipStream_t st) {
+    return S.c.wide ? launch_cg_loop_w<true>(H, S, st) : launch_cg_loop_w<false>(H, S, st);
+}
+
 // A captured segment of the step: enqueued once into a hipGraph on the handle's private capture stream, then
 // replayed on the caller's stream. Kernel arguments (state pointers, dt, iteration batch) are part of the key.
 template <class F>
@@ -3737,7 +3871,7 @@ int run_segment(Handle* H, hipStream_t st, int kind, int mode, float dt, F enque
     std::memcpy(&bits, &dt, 4);
     GraphKey key{kind, mode, H->check_every,
                  H->c.pair_cache | (H->c.sparse_corr << 8) | (H->build_block << 16) | (H->c.packed << 28) |
-                     (H->c.use_halo << 29), bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
+                     (H->c.use_halo << 29) | (H->device_loop << 30), bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
                  H->mlp_params, H->log_feat, H->log_targets};
     auto it = H->graphs.find(key);
     if (it == H->graphs.end()) {
@@ -3770,6 +3904,13 @@ int run_solve(Handle* H, const SolveSetup& S, hipStream_t st, int tail_kind, Ext
     int rc;
     if ((rc = run_segment(H, st, SEG_HEAD, S.mode, S.dt, [&](hipStream_t s) { return enq_solve_head(H, S, s); })))
         return rc;
+    if (H->device_loop && H->P.pressure_solver == 0 && !H->prof.on) {  // no host round trip at all
+        if ((rc = launch_cg_loop(H, S, st))) return rc;
+        return run_segment(H, st, tail_kind, S.mode, S.dt, [&](hipStream_t s) {
+            int r = enq_solve_tail(H, S, s);
+            return r ? r : tail_extra(s);
+        });
+    }
     // a captured batch cannot alternate its slot pointer, so replayed graphs and Jacobi batches copy the count
     bool direct = H->P.pressure_solver == 0 && (!H->use_graphs || H->prof.on);
     int nu = 0;
@@ -4239,6 +4380,8 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         }
         TRY(dalloc(H, &c.part, 3 * int64_t(std::max(H->n_chunks, 1))));
         TRY(dalloc(H, &c.n_unconv, 1));
+        TRY(dalloc(H, &c.loop_bar, 1));
+        TRY(dalloc(H, &c.loop_fault, 1));
         TRY(dalloc(H, &H->qd_saved, 3 * int64_t(n)));
         {
             std::vector<float> ones(W, 1.0f);
@@ -4524,6 +4667,10 @@ int32_t dfsph_set_option(void* h, int32_t option, int32_t value) {
             H->learned_solves = value;
             apply_learned(H);
             return DFSPH_OK;
+        case DFSPH_OPT_DEVICE_LOOP:
+            if (value < 0 || value > 1) return fail(DFSPH_EINVAL, "device loop: 0 or 1");
+            H->device_loop = value;
+            return DFSPH_OK;
         case DFSPH_OPT_HALO:
             if (value < 0 || value > 1) return fail(DFSPH_EINVAL, "halo: 0 or 1");
             if (H->c.use_halo != value) {
@@ -4566,6 +4713,9 @@ int32_t dfsph_get_option(void* h, int32_t option, int32_t* value) {
             return DFSPH_OK;
         case DFSPH_OPT_HALO:
             *value = H->c.use_halo;
+            return DFSPH_OK;
+        case DFSPH_OPT_DEVICE_LOOP:
+            *value = H->device_loop;
             return DFSPH_OK;
         default:
             return fail(DFSPH_EINVAL, "unknown option");
@@ -4774,7 +4924,11 @@ int32_t dfsph_report(void* h, DfsphReport* out, void* stream) {
     };
     for (auto& it : items)
         if (it.dst) HIPCHK(hipMemcpyAsync(it.dst, it.src, it.bytes, hipMemcpyDeviceToHost, st));
+    int fault = 0;
+    HIPCHK(hipMemcpyAsync(&fault, c.loop_fault, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (fault) return fail(DFSPH_EDEVICE, "device solve loop: a grid barrier wait ran out (code " +
+                                              std::to_string(fault) + ")");
     return DFSPH_OK;
 }
 

@@ -2,7 +2,7 @@
 
 Every tuning option (DFSPH_OPT_*: the host convergence-check interval, i.e. the iteration schedule of the pressure
 solves, the neighbour-list block size, the pair-gradient cache, the sparse correction pass, the packed acceleration
-records, the LDS-staged halos, hipGraph replay) gives
+records, the LDS-staged halos, hipGraph replay, the device-side solve loop) gives
 bit-identical results, so the tuner can time them on the live simulation: it snapshots the state, the warm starts,
 the ledgers, the emitters and the shape meters, times short step windows for each candidate (one option at a time,
 each from the snapshot, median of ``rounds``), keeps the fastest value, and restores the snapshot. The caller's
@@ -30,10 +30,12 @@ CANDIDATES = (
     (_abi.OPT_HALO, (0, 1)),
     (_abi.OPT_PAIR_CACHE, (0, _abi.PAIR_CACHE_APPLY, _abi.PAIR_CACHE_ALL)),
     (_abi.OPT_GRAPHS, (0, 1)),
+    (_abi.OPT_DEVICE_LOOP, (0, 1)),
 )
 NAMES = {_abi.OPT_CHECK_INTERVAL: "check_interval", _abi.OPT_BUILD_BLOCK: "build_block",
          _abi.OPT_SPARSE_CORRECTION: "sparse_correction", _abi.OPT_PAIR_CACHE: "pair_cache", _abi.OPT_GRAPHS: "graphs",
-         _abi.OPT_LEARNED_SOLVES: "learned_solves", _abi.OPT_PACKED_ACCEL: "packed_accel", _abi.OPT_HALO: "halo"}
+         _abi.OPT_LEARNED_SOLVES: "learned_solves", _abi.OPT_PACKED_ACCEL: "packed_accel", _abi.OPT_HALO: "halo",
+         _abi.OPT_DEVICE_LOOP: "device_loop"}
 
 
 class _Snapshot:

@@ -297,10 +297,15 @@ enum DfsphOption {
                                         acceleration gathers one 16 B record {x, y, z, kappa / rho0} per neighbour
                                         from a packed copy, the mass the world's; 0: positions and quotients apart.
                                         Bit-identical. */
-    DFSPH_OPT_HALO = 8               /* 1: each list build also lists every 256-particle chunk's distinct neighbours
+    DFSPH_OPT_HALO = 8,              /* 1: each list build also lists every 256-particle chunk's distinct neighbours
                                         (its halo, up to 1,536), and the pressure acceleration and operator apply
                                         stage the halo's records in LDS and gather from there; chunks whose halo does
                                         not fit gather directly. 0: every chunk gathers directly. Bit-identical. */
+    DFSPH_OPT_DEVICE_LOOP = 9        /* 1: the projected-CG iterations of each solve run as one cooperative launch
+                                        that loops on the device until every world's stop test passed (grid barriers
+                                        between the phases, direct gathers), so a step makes no host round trip;
+                                        0: iteration batches with a host check of the unconverged-world count.
+                                        Jacobi and profiled steps use the batches. Bit-identical. */
 };
 enum DfsphPairCache {
     DFSPH_PAIR_CACHE_ACCEL = 1, /* pressure acceleration (k_accel, post-solve) */

@@ -54,7 +54,10 @@ def test_options_are_bit_identical(scene_name):
                  {_abi.OPT_PACKED_ACCEL: 0, _abi.OPT_GRAPHS: 1, _abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ALL},
                  {_abi.OPT_HALO: 1}, {_abi.OPT_HALO: 1, _abi.OPT_PACKED_ACCEL: 0},
                  {_abi.OPT_HALO: 1, _abi.OPT_SPARSE_CORRECTION: 0, _abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ALL},
-                 {_abi.OPT_HALO: 1, _abi.OPT_GRAPHS: 1, _abi.OPT_CHECK_INTERVAL: 2}):
+                 {_abi.OPT_HALO: 1, _abi.OPT_GRAPHS: 1, _abi.OPT_CHECK_INTERVAL: 2},
+                 {_abi.OPT_DEVICE_LOOP: 1}, {_abi.OPT_DEVICE_LOOP: 1, _abi.OPT_GRAPHS: 1},
+                 {_abi.OPT_DEVICE_LOOP: 1, _abi.OPT_HALO: 1, _abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ALL,
+                  _abi.OPT_SPARSE_CORRECTION: 0}):
         q, v, it = _run(sc, 12, opts)
         _log(f"  {opts}: q equal {np.array_equal(q, q0)} qd equal {np.array_equal(v, v0)}")
         assert np.array_equal(q, q0) and np.array_equal(v, v0), opts
@@ -169,7 +172,7 @@ def test_autotune_leaves_the_simulation_unchanged():
     _log(f"[autotune] {res['options']} in {res['seconds']:.1f} s")
     assert np.array_equal(a.pw.particle_impulse(), exch)  # the tuning steps' wall impulses are not kept
     assert set(res["options"]) == {"check_interval", "build_block", "sparse_correction", "packed_accel", "halo",
-                                   "pair_cache", "graphs"}
+                                   "pair_cache", "graphs", "device_loop"}
     for _ in range(6):
         a.step(sa, sa, None, None, DT)
         b.step(sb, sb, None, None, DT)
