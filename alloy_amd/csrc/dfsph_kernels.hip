@@ -2123,27 +2123,46 @@ __global__ void __launch_bounds__(256) k_cg_direction(Ctx c, int* flag_out) {
 
 constexpr long long LOOP_WAIT_TICKS = 200000000LL;  // 2 s of the 100 MHz real-time counter: no phase takes that long
 
-// Grid barrier: thread 0 of each block adds one arrival to the counter (agent-scope release) and waits for the
-// barrier's target (arrivals of all blocks so far, agent-scope acquire); the block's other threads wait at the
-// workgroup barriers around it. The wait is bounded: past LOOP_WAIT_TICKS, or when another block already gave up,
-// the block sets the fault flag and the call returns false, and the kernel ends. Returns `*watch` as read after the
-// barrier by thread 0 (the same value for every block: no phase writes it between two barriers' reads).
-__device__ __forceinline__ bool grid_barrier(const Ctx& c, unsigned target, const int* watch, int* seen) {
+// Grid barrier k (k = 1, 2, ... within one launch), hierarchical over the XCDs (block b runs on XCD b % 8): thread 0
+// of each block adds an arrival to its XCD's counter; the XCD's last arrival adds one to the global counter; the last
+// of those stores k into every XCD's release flag; each block waits for its XCD's flag. The counters and flags sit
+// on cache lines of their own (LOOP_LINE ints apart) and hold monotonic values, so nothing is reset between
+// barriers. Read-modify-writes are acquire-release and the flags release / acquire at agent scope, so every write of
+// the phase before the barrier is visible to every block after it. The wait is bounded: past LOOP_WAIT_TICKS, or
+// when another block already gave up, the block sets the fault flag and the call returns false, and the kernel ends.
+// `*watch` (may be null) is read by thread 0 after the barrier: every block sees the same value when no phase writes
+// it between two barriers.
+constexpr int LOOP_LINE = 32;                        // 128 B
+constexpr int LOOP_WORDS = (2 * N_XCD + 1) * LOOP_LINE;  // XCD counters, the global counter, XCD release flags
+__device__ __forceinline__ bool grid_barrier(const Ctx& c, unsigned k, const int* watch, int* seen) {
     __shared__ int s_ok, s_val;
     __syncthreads();
     if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(c.loop_bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned G = gridDim.x, x = blockIdx.x % N_XCD, per = G / N_XCD;
+        unsigned* xcd_count = c.loop_bar + x * LOOP_LINE;
+        unsigned* all_count = c.loop_bar + N_XCD * LOOP_LINE;
+        unsigned* release = c.loop_bar + (N_XCD + 1 + x) * LOOP_LINE;
+        unsigned old = __hip_atomic_fetch_add(xcd_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1u == per * k) {  // this XCD's last arrival
+            unsigned o2 = __hip_atomic_fetch_add(all_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (o2 + 1u == N_XCD * k)  // the grid's last arrival
+                for (int y = 0; y < N_XCD; ++y)
+                    __hip_atomic_store(c.loop_bar + (N_XCD + 1 + y) * LOOP_LINE, k, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        }
         const long long t0 = wall_clock64();
         int ok = 1;
-        while (__hip_atomic_load(c.loop_bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        // relaxed polling (an acquire per poll would invalidate the caches on every read), one acquire fence after
+        while (__hip_atomic_load(release, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k) {
             if (__hip_atomic_load(c.loop_fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
                 wall_clock64() - t0 > LOOP_WAIT_TICKS) {
                 __hip_atomic_store(c.loop_fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
             }
-            __builtin_amdgcn_s_sleep(1);
+            __builtin_amdgcn_s_sleep(4);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         s_ok = ok;
         s_val = watch ? __hip_atomic_load(watch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
     }
@@ -2164,32 +2183,32 @@ __device__ __forceinline__ void loop_chunks(const Ctx& c, F body) {
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_cg_loop(Ctx c, int mode, float dt, StopSpec sp) {
     const unsigned G = gridDim.x;
-    unsigned target = 0;
+    unsigned target = 0;  // the barrier count
     int unconv = 0;
-    if (!grid_barrier(c, target += G, c.n_unconv, &unconv)) return;
+    if (!grid_barrier(c, ++target, c.n_unconv, &unconv)) return;
     for (int it = 0; unconv != 0; ++it) {
         if (it > 100000) {  // the stop test caps every world at max_iterations: never reached
             if (threadIdx.x == 0) __hip_atomic_store(c.loop_fault, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return;
         }
         loop_chunks(c, [&](int b) { accel_chunk<WIDE>(c, b, c.pd, c.pdr, VR_DIR, 0); });
-        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        if (!grid_barrier(c, ++target, nullptr, nullptr)) return;
         loop_chunks(c, [&](int b) { apply_chunk<WIDE>(c, b, mode, dt, c.pd, c.ap, 0, 0, 0, 1); });
-        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        if (!grid_barrier(c, ++target, nullptr, nullptr)) return;
         for (int w = int(blockIdx.x); w < c.W; w += int(G)) world_body(c, w, W_CG_PAP, sp);
-        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        if (!grid_barrier(c, ++target, nullptr, nullptr)) return;
         loop_chunks(c, [&](int b) { cg_step_chunk(c, b); });
-        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        if (!grid_barrier(c, ++target, nullptr, nullptr)) return;
         loop_chunks(c, [&](int b) { accel_chunk<WIDE>(c, b, c.corr, c.corrr, VR_CORR, 1); });
-        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        if (!grid_barrier(c, ++target, nullptr, nullptr)) return;
         loop_chunks(c, [&](int b) { apply_chunk<WIDE>(c, b, mode, dt, c.corr, c.wv, 1, 0, 1, 0); });
-        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        if (!grid_barrier(c, ++target, nullptr, nullptr)) return;
         loop_chunks(c, [&](int b) { cg_finish_chunk(c, b, mode, dt); });
-        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        if (!grid_barrier(c, ++target, nullptr, nullptr)) return;
         for (int w = int(blockIdx.x); w < c.W; w += int(G)) world_body(c, w, W_CG_BETA, sp);
-        if (!grid_barrier(c, target += G, nullptr, nullptr)) return;
+        if (!grid_barrier(c, ++target, nullptr, nullptr)) return;
         loop_chunks(c, [&](int b) { cg_direction_chunk(c, b); });
-        if (!grid_barrier(c, target += G, c.n_unconv, &unconv)) return;
+        if (!grid_barrier(c, ++target, c.n_unconv, &unconv)) return;
     }
 }
 
@@ -3848,7 +3867,7 @@ int launch_cg_loop_w(Handle* H, const SolveSetup& S, hipStream_t st) {
         grid = (per_cu * prop.multiProcessorCount) / N_XCD * N_XCD;
         if (grid < N_XCD) return fail(DFSPH_EDEVICE, "device solve loop: no co-resident grid");
     }
-    k_fill_i32<<<1, 64, 0, st>>>(reinterpret_cast<int*>(S.c.loop_bar), 0, 1);
+    k_fill_i32<<<1, 256, 0, st>>>(reinterpret_cast<int*>(S.c.loop_bar), 0, LOOP_WORDS);
     Ctx c = S.c;
     int mode = S.mode;
     float dt = S.dt;
@@ -4380,7 +4399,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         }
         TRY(dalloc(H, &c.part, 3 * int64_t(std::max(H->n_chunks, 1))));
         TRY(dalloc(H, &c.n_unconv, 1));
-        TRY(dalloc(H, &c.loop_bar, 1));
+        TRY(dalloc(H, &c.loop_bar, LOOP_WORDS));
         TRY(dalloc(H, &c.loop_fault, 1));
         TRY(dalloc(H, &H->qd_saved, 3 * int64_t(n)));
         {

@@ -201,9 +201,25 @@ def plain_sample(wl: dict, worlds: list[int], steps: int, warmup: int, dt: float
             "solver": "plain DFSPH, one single-world solver per world stepped in turn, no learned warm start"}
 
 
+def cpu_info() -> dict:
+    """The host CPU: model name, logical CPUs (nproc) and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(wl: dict, steps: int, warmup: int, sample_w: int) -> dict:
-    """The CPU oracle (float32, serial C++) on a bounded sample: the whole scene for single-world workloads, an
-    even sample of `sample_w` worlds (each exactly as in the pack) for the batched one."""
+    """The CPU oracle (float32 C++, the same algorithm and operation order) on a bounded sample: the whole scene for
+    single-world workloads, an even sample of `sample_w` worlds (each exactly as in the pack) for the batched ones;
+    timed on one core (the serial build) and on every core the process may use (the OpenMP build, OMP_NUM_THREADS
+    threads, else one per available CPU)."""
     from alloy_amd.sph import _abi, scenes
     from oracle.oracle import OracleDFSPH
 
@@ -212,44 +228,59 @@ def cpu_baseline(wl: dict, steps: int, warmup: int, sample_w: int) -> dict:
     sc = make_scene(wl, "cpu", subset=subset)
     if steps is None:
         steps = wl["steps"] if pour else int(max(2, min(40, CPU_SAMPLE_PARTICLE_STEPS // max(sum(sc.fluid_counts), 1))))
-    o = OracleDFSPH(sc.model, sc.config, 32)
-    if pour:  # the scene's emitters through the same host bookkeeping, and the swaying walls
-        from alloy_amd.sph.emitters import EmitterManager
-
-        def cursors():
-            c = np.asarray(o.buffer(_abi.BUF_EMIT_CURSOR)).copy()
-            return np.concatenate([c, np.zeros(len(mgr.table.world) - len(c), np.int32)])
-
-        mgr = EmitterManager(sc.model, sc.config, o.set_emitters, cursors)
-        for args in sc.emitters:
-            mgr.add_emitter(*args)
-        drive_state = sc.model.state()
-
-        wall_ids = sc.sway["ids"].numpy()
-
-    def step(k):
-        if pour:
-            sc.drive(drive_state, k * scenes.DT)
-            o.qd[wall_ids] = drive_state.particle_qd.numpy()[wall_ids]
-        o.step(scenes.DT)
-
-    for k in range(warmup):
-        step(k)
-    act = lambda: int(np.count_nonzero(((o.buffer(_abi.BUF_FLAGS) & 1) != 0) & (o.buffer(_abi.BUF_PHASE) >= 1)))  # noqa: E731
-    n_ps = 0
-    t = time.perf_counter()
-    for k in range(warmup, warmup + steps):
-        step(k)
-        n_ps += act() if pour else 0
-    dt = time.perf_counter() - t
+    info = cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", info["affinity"]) or info["affinity"])
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     n = sum(sc.fluid_counts)
-    if not pour:
-        n_ps = n * steps
+
+    def timed(parallel: bool, k_steps: int) -> tuple[float, float]:
+        if pour:  # reset the spares the previous pass emitted
+            sc.model.particle_flags.copy_(flags0)
+            sc.model.sph.phase.copy_(phase0)
+        o = OracleDFSPH(sc.model, sc.config, 32, parallel=parallel)
+        step = lambda k: o.step(scenes.DT)  # noqa: E731
+        if pour:  # the scene's emitters through the same host bookkeeping, and the swaying walls
+            from alloy_amd.sph.emitters import EmitterManager
+
+            def cursors():
+                c = np.asarray(o.buffer(_abi.BUF_EMIT_CURSOR)).copy()
+                return np.concatenate([c, np.zeros(len(mgr.table.world) - len(c), np.int32)])
+
+            mgr = EmitterManager(sc.model, sc.config, o.set_emitters, cursors)
+            for args in sc.emitters:
+                mgr.add_emitter(*args)
+            drive_state = sc.model.state()
+            wall_ids = sc.sway["ids"].numpy()
+
+            def step(k):
+                sc.drive(drive_state, k * scenes.DT)
+                o.qd[wall_ids] = drive_state.particle_qd.numpy()[wall_ids]
+                o.step(scenes.DT)
+
+        for k in range(warmup):
+            step(k)
+        act = lambda: int(np.count_nonzero(((o.buffer(_abi.BUF_FLAGS) & 1) != 0) & (o.buffer(_abi.BUF_PHASE) >= 1)))  # noqa: E731
+        n_ps = 0
+        t = time.perf_counter()
+        for k in range(warmup, warmup + k_steps):
+            step(k)
+            n_ps += act() if pour else 0
+        dt = time.perf_counter() - t
+        return (n_ps if pour else n * k_steps) / dt, dt
+
+    flags0, phase0 = sc.model.particle_flags.clone(), sc.model.sph.phase.clone()
+    serial, t1 = timed(False, steps)
+    par_steps = steps if pour else steps * max(1, min(threads, 8))
+    parallel, tp = timed(True, par_steps)
     what = (f"{len(subset)} of the {wl['worlds']} worlds ({n} {'spares' if pour else 'liquid particles'})" if subset
             else f"{n} particles")
-    return {"value": n_ps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"CPU oracle (float32, serial C++ -O3, one core) on {what} of the same workload from its "
-                      f"initial state: {warmup} warm-up + {steps} timed steps ({dt:.1f} s)"}
+    return {"value": parallel, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (float32 C++ -O3, OpenMP over the per-particle loops, {threads} threads) on {what} "
+                      f"of the same workload from its initial state: {warmup} warm-up + {par_steps} timed steps "
+                      f"({tp:.1f} s)",
+            "serial_value": serial, "serial_cores": 1,
+            "serial_sample": f"the serial build on the same sample, {warmup} warm-up + {steps} timed steps ({t1:.1f} s)",
+            **info}
 
 
 def fp64_density_error(scene, q: np.ndarray, qd: np.ndarray) -> float:
@@ -266,12 +297,59 @@ def fp64_density_error(scene, q: np.ndarray, qd: np.ndarray) -> float:
     return float(np.mean(np.maximum(r[a] - 1.0, 0.0))) if a.any() else 0.0
 
 
-def traffic_from_profiles(kernel: str, workload: str):
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
+def options_string(options: dict) -> str:
+    return ",".join(f"{k}={v}" for k, v in sorted(options.items()))
+
+
+def _traffic_record(kernel: str, workload: str, steps: int, warmup: int, options: str):
+    """The committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary of `kernel` on `workload`, if it was measured on the
+    roofline window of a bench run with these step counts and options; (record or None, reason)."""
+    name = f"{kernel.split('<')[0]}_{workload}_traffic.json"
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
-        return None
+        return None, f"no profiles/{name}"
     with open(path) as f:
-        return float(json.load(f)["hbm_bytes_per_launch"])
+        rec = json.load(f)
+    if rec.get("steps") != steps or rec.get("warmup") != warmup:
+        return None, (f"profiles/{name} was measured with --steps {rec.get('steps')} --warmup {rec.get('warmup')}, "
+                      f"not this run's {steps} / {warmup}")
+    if rec.get("options") != options:
+        return None, f"profiles/{name} was measured with options {rec.get('options')}, not this run's {options}"
+    return rec, f"profiles/{name}: {rec.get('window', '')}; command: {rec.get('command', '')}"
+
+
+LEG_1M_WARMUP = 60  # past the first steps of the collapse, where every solve sits at min_iterations whatever its start
+
+
+def learned_1m_leg(net, solves: int, steps: int) -> dict:
+    """The 1,000,000-particle dam break (100^3, seed 3, one world, 24-bit rows): `steps` timed steps after
+    LEG_1M_WARMUP warm-up steps with the learned start and with the rule, the same tolerances; ms per step and
+    iterations."""
+    import torch
+
+    from alloy_amd.sph import _abi, scenes
+
+    out = {"workload": "dam break, 1,000,000 particles (100^3 lattice, seed 3) in a 2 m SDF box tank, 1 world",
+           "steps": steps, "warmup": LEG_1M_WARMUP}
+    for name, use in (("rule", None), ("learned", net)):
+        _log(f"[bench] 1M dam break, {name} start")
+        sc = scenes.dam_break(100, 2.0, 3, device="cuda")
+        s = sc.make_solver()
+        if use is not None:
+            use.attach(s)
+            s.set_option(_abi.OPT_LEARNED_SOLVES, solves)
+        st = sc.model.state()
+        run = Run(sc, s, st)
+        for _ in range(LEG_1M_WARMUP):
+            run.step(scenes.DT)
+        r = timed_run(run, steps, scenes.DT)
+        out[name] = {"ms_per_step": r["total_ms"] / steps, "value": 1.0e6 * steps / (r["total_ms"] * 1e-3),
+                     "iters_density_mean": r["iters_density_mean"], "iters_divergence_mean": r["iters_divergence_mean"],
+                     "density_err_mean": r["density_err_mean"]}
+        del s, st, run, sc
+        torch.cuda.empty_cache()
+    out["learned_vs_rule"] = out["learned"]["value"] / out["rule"]["value"]
+    return out
 
 
 def main() -> int:
@@ -295,13 +373,18 @@ def main() -> int:
     ap.add_argument("--options", default="",
                     help="name=value,... of library options set before the warm-up (e.g. the autotuner's choice, to "
                          "profile exactly the timed configuration with --no-autotune)")
+    ap.add_argument("--no-single-launch", action="store_true",
+                    help="skip the plain single-launch run (rule start, default options) behind vs_single_launch")
+    ap.add_argument("--learned-1m-steps", type=int, default=40,
+                    help="steps of the 1,000,000-particle dam-break leg timing the learned start against the rule "
+                         "(0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=None,
                     help="timed oracle steps (default: about 3e6 particle-steps of the sample)")
     ap.add_argument("--cpu-worlds", type=int, default=4, help="worlds of the batched pack in the CPU sample")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--roofline-sample", type=int, default=16,
-                    help="time one in this many k_accel / k_apply launches of the timed region (pseudo-random)")
+    ap.add_argument("--roofline-sample", type=int, default=1,
+                    help="time one in this many k_accel / k_apply launches of the roofline window (pseudo-random)")
     ap.add_argument("--breakdown-steps", type=int, default=30,
                     help="steps after the timed region with every kernel timed (per-kernel table)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -362,20 +445,29 @@ def main() -> int:
 
         tuned = autotune(solver, state, dt, learned=True, log=_log)
     learned_used = net is not None and solver.get_option(_abi.OPT_LEARNED_SOLVES) != 0
-    roof_mask = (1 << _abi.KID_ACCEL) | (1 << _abi.KID_APPLY)
-    run = timed_run(main_run, args.steps, dt, prof_mask=roof_mask, sample=args.roofline_sample, count_active=pour)
+    options_used = {name: solver.get_option(oid) for name, oid in OPTION_IDS.items()}
+    # the timed region: profiling off, exactly the configuration the options above describe
+    run = timed_run(main_run, args.steps, dt, count_active=pour)
     total_ms = run["total_ms"]
     _log(f"[bench] timed {args.steps} steps in {total_ms:.1f} ms")
-    ks = run["kernels"]
+    particle_steps = run["particle_steps"] if pour else n_fluid * args.steps
+    value = particle_steps / (total_ms * 1e-3)
+    # the roofline window: the next `steps` steps, every k_accel / k_apply launch timed inside its own dispatch and its
+    # algorithmic bytes counted by the kernels (per-launch events cost the host, so this is not the timed region)
+    roof_mask = (1 << _abi.KID_ACCEL) | (1 << _abi.KID_APPLY)
+    roof = timed_run(main_run, args.steps, dt, prof_mask=roof_mask, sample=args.roofline_sample, count_active=pour)
+    ks = roof["kernels"]
     dom = max(ks, key=lambda k: ks[k][1])
     nl, tl, bl = ks[dom]
     achieved = bl / (tl * 1e-3) / 1e9
-    particle_steps = run["particle_steps"] if pour else n_fluid * args.steps
-    value = particle_steps / (total_ms * 1e-3)
+    w0 = args.warmup + args.steps
+    traffic_rec = _traffic_record(dom, args.workload, args.steps, args.warmup, options_string(options_used))
     solver_desc = ("batched, one launch per phase over all worlds" if W > 1 else "single world") + (
         f", learned warm start (trained network; solves {'divergence+density' if solves == 3 else 'density' if solves == 2 else 'divergence'})"
         if learned_used else ", the reference's warm-start rule" + (
             " (the autotuner timed the learned start slower on this pack)" if net is not None else ""))
+    if options_used.get("device_loop"):
+        solver_desc += ", device-side solve loop (no host round trip in a step)"
     out = {
         "metric": METRIC,
         "value": value,
@@ -391,18 +483,29 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": describe(wl, sc), "particles": n_fluid, "worlds": W, "solver": solver_desc},
         "particle_steps": particle_steps,
+        "regime": {"window": f"steps {args.warmup + 1}-{w0} of the run (warm-up {args.warmup} steps, autotuning "
+                             "restored to the warm-up's end state)",
+                   "iters_density_mean": run["iters_density_mean"],
+                   "iters_divergence_mean": run["iters_divergence_mean"],
+                   "density_err_mean": run["density_err_mean"],
+                   "density_err_max_sampled": run["density_err_max_sampled"],
+                   "unconverged_sampled_steps": run["unconverged_sampled_steps"],
+                   "note": "per-world means over the reports every 10 steps; a world's solve stops at its tolerance "
+                           "or at min_iterations = 2"},
         "density_err_mean": run["density_err_mean"],
-        "density_err_max_sampled": run["density_err_max_sampled"],
         "iters_density_mean": run["iters_density_mean"],
         "iters_divergence_mean": run["iters_divergence_mean"],
-        "unconverged_sampled_steps": run["unconverged_sampled_steps"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(dom, args.workload),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_rec[0]["hbm_bytes_per_launch"] if traffic_rec[0] else None,
                      "alg_bytes_per_launch": bl / max(nl, 1), "avg_launch_ms": tl / max(nl, 1), "launches_timed": nl,
-                     "window": f"the timed region, 1 in {args.roofline_sample} launches (pseudo-random)"},
-        "kernel_ms_per_step_timed": {k: round(v[1] / args.steps, 4) for k, v in ks.items()},
+                     "window": f"steps {w0 + 1}-{w0 + args.steps} (the {args.steps} steps after the timed region), "
+                               f"1 in {args.roofline_sample} launches of {dom} timed by HIP events inside its "
+                               "dispatch, algorithmic bytes counted by the kernel",
+                     "traffic_source": traffic_rec[1]},
+        "kernel_ms_per_step_roofline_window": {k: round(v[1] / args.steps, 4) for k, v in ks.items()},
+        "options": options_used,
         "tuning": {"autotuned": tuned is not None,
-                   "options": tuned["options"] if tuned else None,
                    "ms_per_step": tuned["ms_per_step"] if tuned else None,
                    "seconds": round(tuned["seconds"], 2) if tuned else 0.0},
     }
@@ -419,35 +522,65 @@ def main() -> int:
         _log(f"[bench] fp64 density error {out['fp64_density_err']:.3e}")
     del solver, state, main_run
     torch.cuda.empty_cache()
-    if learned_used and not args.no_rule_compare:
-        _log("[bench] the same run with the rule warm start")
+
+    def rerun(options: dict, use_net) -> dict:
+        """The same pack from its initial state with these options and start, the same warm-up and timed steps."""
         if pour:  # the first run emitted from the model's spares: back to the build-time flags and phase
             sc.model.particle_flags.copy_(flags0)
             sc.model.sph.phase.copy_(phase0)
         s2 = sc.make_solver()
-        s2.set_check_interval(args.check_every)
-        s2.set_graphs(args.graphs)
-        if args.pair_cache is not None:
-            s2.set_option(_abi.OPT_PAIR_CACHE, args.pair_cache)
-        for name, v in (tuned["options"].items() if tuned else ()):
-            if name in OPTION_IDS:
-                s2.set_option(OPTION_IDS[name], v)
+        for name, v in options.items():
+            if name != "learned_solves":
+                try:
+                    s2.set_option(OPTION_IDS[name], v)
+                except ValueError:
+                    pass
+        if use_net is not None:
+            use_net.attach(s2)
+            s2.set_option(_abi.OPT_LEARNED_SOLVES, solves)
         st2 = sc.model.state()
         run2 = Run(sc, s2, st2)
         for _ in range(args.warmup):
             run2.step(dt)
         r2 = timed_run(run2, args.steps, dt, count_active=pour)
         ps2 = r2["particle_steps"] if pour else n_fluid * args.steps
-        out["batched_rule_start"] = {
-            "value": ps2 / (r2["total_ms"] * 1e-3), "ms_per_step": r2["total_ms"] / args.steps,
-            "iters_density_mean": r2["iters_density_mean"], "iters_divergence_mean": r2["iters_divergence_mean"],
-            "density_err_mean": r2["density_err_mean"]}
-        out["learned_vs_rule_start"] = value / out["batched_rule_start"]["value"]
+        res = {"value": ps2 / (r2["total_ms"] * 1e-3), "ms_per_step": r2["total_ms"] / args.steps,
+               "iters_density_mean": r2["iters_density_mean"], "iters_divergence_mean": r2["iters_divergence_mean"],
+               "density_err_mean": r2["density_err_mean"]}
         del s2, st2, run2
         torch.cuda.empty_cache()
+        return res
+
+    if net is not None and not args.no_rule_compare:
+        # the other warm start on the same pack with the same options: the learned start's own effect
+        if learned_used:
+            _log("[bench] the same run with the rule warm start")
+            other = rerun(options_used, None)
+            out["batched_rule_start"] = other
+            out["learned_vs_rule_start"] = value / other["value"]
+        else:
+            _log("[bench] the same run with the learned warm start")
+            other = rerun(options_used, net)
+            out["batched_learned_start"] = other
+            out["learned_vs_rule_start"] = other["value"] / value
+        out["learned_vs_rule_note"] = ("learned / rule warm start, batched, same options and steps; the headline uses "
+                                       "the start the autotuner timed faster")
+    if not args.no_single_launch:
+        # the plain single-launch solver: all worlds in one launch per phase (the reference's own structure), the
+        # reference's rule start, the library's default options, no autotuning
+        _log("[bench] plain single-launch DFSPH (rule start, default options)")
+        single = rerun({}, None)
+        out["single_launch_rule_start"] = single
+        out["vs_single_launch"] = value / single["value"]
+        _log(f"[bench] vs single launch {out['vs_single_launch']:.3f}")
+    if args.learned_1m_steps > 0 and net is not None:
+        out["dam_break_1m_learned_vs_rule"] = learned_1m_leg(net, solves, args.learned_1m_steps)
     if wl["kind"] in ("bowl_worlds", "pour_worlds") and not args.no_plain:
         _log(f"[bench] plain single-world DFSPH on {args.plain_worlds} sampled worlds")
         plain = plain_sample(wl, sample_worlds(W, args.plain_worlds), args.steps, args.warmup, dt, args.pair_cache)
+        plain["note"] = ("one single-world solver per sampled world, stepped one after another: the per-launch cost "
+                         "of small worlds, which the reference avoids by launching all worlds at once (that structure "
+                         "is single_launch_rule_start)")
         out["plain"] = plain
         out["speedup_vs_plain"] = value / plain["value"]
         _log(f"[bench] plain {plain['value']:.3e} particle-steps/s, speedup {out['speedup_vs_plain']:.2f}")

@@ -598,7 +598,10 @@ struct Oracle {
         set_groups();
         build_grid();
         T h2 = h * h;
+#pragma omp parallel
+        {
         std::vector<int> cand;
+#pragma omp for schedule(static, 256)
         for (int i = 0; i < n; ++i) {
             int g = group[i];
             if (g < 0) {
@@ -632,8 +635,12 @@ struct Oracle {
                 nbr[size_t(k) * n + i] = uint16_t(off & 65535);
                 if (wide) nbr_hi[size_t(k) * n + i] = uint8_t(off >> 16);
             }
-            if (total > cap) st_overflow[g] += 1;
+            if (total > cap) {
+#pragma omp atomic
+                st_overflow[g] += 1;
+            }
             count[i] = m;
+        }
         }
     }
 
@@ -885,13 +892,16 @@ struct Oracle {
 
     void boundary_slots() {
         // find runs over every particle first (slot_nrm is scratch shared with nbl), then fill
+        #pragma omp parallel for schedule(static, 256)
         for (int i = 0; i < n; ++i) find_slots(i);
+        #pragma omp parallel for schedule(static, 256)
         for (int i = 0; i < n; ++i) fill_slots(i);
     }
 
     // ---------------------------------------------------------------------------- density / alpha
     // pressure.py:144-193
     void density_alpha() {
+#pragma omp parallel for schedule(static, 256)
         for (int i = 0; i < n; ++i) {
             if (!is_fluid(i)) {
                 ratio[i] = alpha[i] = neff[i] = active[i] = reg[i] = T(0);
@@ -953,6 +963,7 @@ struct Oracle {
 
     // pressure.py:214-261
     void solve_init(int mode, T dt, T warm, T relief, const std::vector<T>& store) {
+#pragma omp parallel for schedule(static, 256)
         for (int i = 0; i < n; ++i) {
             if (!is_fluid(i)) {
                 src[i] = kappa[i] = lower[i] = T(0);
@@ -1020,6 +1031,7 @@ struct Oracle {
 
     // krylov.py:77-109 (cg_accel + cg_apply): out (+)= A vec (- b)
     void product(int mode, T dt, const std::vector<T>& vec, std::vector<T>& out, int gated, int bias, int accum) {
+#pragma omp parallel for schedule(static, 256)
         for (int i = 0; i < n; ++i) {
             if (!is_fluid(i)) {
                 put3(acc, i, v3<T>(0, 0, 0));
@@ -1029,6 +1041,7 @@ struct Oracle {
             put3(acc, i, pressure_accel(vec, i));
         }
         T sc = mode == 0 ? dt : dt * dt;
+#pragma omp parallel for schedule(static, 256)
         for (int i = 0; i < n; ++i) {
             if (!is_fluid(i)) {
                 out[i] = T(0);
@@ -1054,6 +1067,7 @@ struct Oracle {
         T sc = mode == 0 ? dt : dt * dt;
         std::fill(restart.begin(), restart.end(), 0);
         product(mode, dt, kappa, wv, 0, 1, 0);
+#pragma omp parallel for schedule(static, 256)
         for (int i = 0; i < n; ++i) {  // cg_init_kernel
             if (!is_fluid(i)) {
                 act[i] = 0;
@@ -1084,6 +1098,7 @@ struct Oracle {
             ++loops;
             product(mode, dt, pd, ap, 0, 0, 0);
             reduce(pair, 2, sums, nullptr);
+#pragma omp parallel for schedule(static, 256)
             for (int i = 0; i < n; ++i) {  // cg_step_kernel
                 if (!is_fluid(i)) {
                     corr[i] = T(0);
@@ -1102,6 +1117,7 @@ struct Oracle {
                         c = lower[i] - k;
                         k = lower[i];
                         act[i] = 0;
+#pragma omp atomic
                         restart[w] += 1;
                     }
                     kappa[i] = k;
@@ -1110,6 +1126,7 @@ struct Oracle {
                 wv[i] = wv[i] + a * ap[i];
             }
             product(mode, dt, corr, wv, 1, 0, 1);
+#pragma omp parallel for schedule(static, 256)
             for (int i = 0; i < n; ++i) {  // cg_finish_kernel
                 if (!is_fluid(i)) {
                     pair[2 * i] = pair[2 * i + 1] = T(0);
@@ -1121,6 +1138,7 @@ struct Oracle {
                 T wvv = wv[i];
                 if (act[i] == 0 && wvv < T(0)) {
                     act[i] = 1;
+#pragma omp atomic
                     restart[w] += 1;
                 }
                 T zi = T(0);
@@ -1153,6 +1171,7 @@ struct Oracle {
                 if (stop_test(w, it, double(T(mean)), tol, tol_max, scale, mi, ma)) converged[w] = 1;
                 else n_unconv += 1;
             }
+#pragma omp parallel for schedule(static, 256)
             for (int i = 0; i < n; ++i) {  // cg_direction_kernel
                 if (!is_fluid(i)) continue;
                 int w = world[i];
@@ -1316,6 +1335,7 @@ struct Oracle {
     double lattice_grad_host() const { return lattice_grad_d; }
 
     void contact_limit(T dt) {
+#pragma omp parallel for schedule(static, 256)
         for (int i = 0; i < n; ++i) {
             if (!is_fluid(i) || count[i] > 0 || kappa[i] <= T(0)) continue;
             V3<T> G{0, 0, 0}, vb{0, 0, 0};
@@ -1335,6 +1355,7 @@ struct Oracle {
     }
 
     void apply_pressure(int mode, T dt, std::vector<T>& store) {
+#pragma omp parallel for schedule(static, 256)
         for (int i = 0; i < n; ++i) {
             if (!is_fluid(i)) continue;
             V3<T> a = pressure_accel(kappa, i);
@@ -1348,6 +1369,7 @@ struct Oracle {
     }
 
     void lone_friction(T dt) {
+#pragma omp parallel for schedule(static, 256)
         for (int i = 0; i < n; ++i) {
             if (!is_fluid(i) || count[i] > 0 || kappa[i] <= T(0)) continue;
             V3<T> G{0, 0, 0}, vb{0, 0, 0};
@@ -1384,6 +1406,7 @@ struct Oracle {
 
     // ---------------------------------------------------------------------------- non-pressure (integrate.py:150-178)
     void nonpressure(T dt) {
+#pragma omp parallel for schedule(static, 256)
         for (int i = 0; i < n; ++i) {
             V3<T> vi = vv(i);
             if (!is_fluid(i)) {

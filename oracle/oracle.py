@@ -20,7 +20,8 @@ from alloy_amd.sph.model import Model
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle_dfsph.so")
-_lib = None
+OMP_LIB_PATH = os.path.join(_HERE, "liboracle_dfsph_omp.so")  # the same oracle, loops over OpenMP threads
+_libs: dict[bool, ctypes.CDLL] = {}
 
 
 def build(force: bool = False) -> str:
@@ -30,13 +31,14 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
-def load() -> ctypes.CDLL:
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
+def load(parallel: bool = False) -> ctypes.CDLL:
+    """The serial oracle, or with ``parallel`` its OpenMP build (OMP_NUM_THREADS threads; the same results)."""
+    if parallel in _libs:
+        return _libs[parallel]
+    path = OMP_LIB_PATH if parallel else LIB_PATH
+    if not os.path.exists(path):
         build()
-    lib = ctypes.CDLL(LIB_PATH)
+    lib = ctypes.CDLL(path)
     vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
     P = ctypes.POINTER
     sig = {
@@ -70,13 +72,13 @@ def load() -> ctypes.CDLL:
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
         fn.argtypes, fn.restype = args, res
-    _lib = lib
+    _libs[parallel] = lib
     return lib
 
 
-def _check(rc: int, what: str) -> None:
+def _check(rc: int, what: str, lib=None) -> None:
     if rc != 0:
-        msg = f"{what}: {load().orc_last_error().decode()}"
+        msg = f"{what}: {(lib or load()).orc_last_error().decode()}"
         raise ValueError(msg) if rc == -1 else RuntimeError(msg)
 
 
@@ -96,8 +98,8 @@ class OracleDFSPH:
     """The CPU oracle for one model: ``precision`` 32 (bitwise parity with the HIP path) or 64 (FP64 reference)."""
 
     def __init__(self, model: Model, cfg: DFSPHConfig, precision: int = 32, pressure_solver: str | None = None,
-                 viscosity: str | None = None):
-        self.lib = load()
+                 viscosity: str | None = None, parallel: bool = False):
+        self.lib = load(parallel)
         self.model, self.cfg, self.precision = model, cfg, precision
         self.real = np.float64 if precision == 64 else np.float32
         self.n, self.W = model.particle_count, model.world_count
