@@ -43,15 +43,20 @@ def content_box(model: Any) -> tuple[np.ndarray, np.ndarray] | None:
 
 
 def auto_box(model: Any, pad_min: float = AUTO_PAD_MIN) -> tuple[Bounds, float] | None:
-    """(the content box padded by max(pad_min, largest extent) on every side, that pad); None for an empty model."""
+    """(The content box grown on every side by its largest extent, at least ``pad_min``; that growth), or None for
+    an empty model."""
     box = content_box(model)
     if box is None:
         return None
-    lo, hi = box
-    pad = max(float(pad_min), float((hi - lo).max()))
-    return (tuple(float(v) for v in lo - pad), tuple(float(v) for v in hi + pad)), pad
+    lo, hi = (np.asarray(c, np.float64) for c in box)
+    grow = float(np.max(hi - lo))
+    if grow < pad_min:
+        grow = float(pad_min)
+    corners = tuple(tuple(float(v) for v in c) for c in (lo - grow, hi + grow))
+    return corners, grow
 
 
 def auto_world_bounds(model: Any, pad_min: float = AUTO_PAD_MIN) -> Bounds | None:
+    """The corners of ``auto_box``; None for an empty model."""
     box = auto_box(model, pad_min)
-    return None if box is None else box[0]
+    return box[0] if box is not None else None

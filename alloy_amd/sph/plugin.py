@@ -23,33 +23,44 @@ log = logging.getLogger("alloy.sph")
 _CONFIG_FIELDS = frozenset(f.name for f in dataclasses.fields(DFSPHConfig))
 
 
+def _split_kwargs(kwargs: Mapping[str, Any]) -> tuple[Any, Any, dict]:
+    """(config, capacity_per_world, DFSPHConfig fields) of the factory's kwargs; a kwarg that is none of these raises
+    TypeError naming it."""
+    rest = dict(kwargs)
+    cfg = rest.pop("config", None)
+    cap = rest.pop("capacity_per_world", None)
+    unknown = sorted(k for k in rest if k not in _CONFIG_FIELDS)
+    if unknown:
+        raise TypeError(f"sph factory: unknown kwargs {unknown}")
+    return cfg, cap, rest
+
+
 def _factory(view: Any, kwargs: Mapping[str, Any]):
-    """Entry factory (plugin.py:36-65): ``config`` or Config fields, plus capacity_per_world; unknown kwargs raise
-    TypeError. Without ``config`` and without a ``world_bounds`` kwarg the escape box comes from the scene's content
-    (``escape_box.auto_box``); an explicit ``world_bounds`` (None too) or a full ``config`` is kept as given. Cloth
-    absorption (plugin.py:46, 60-62) belongs to the cloth coupling, outside this path."""
+    """Entry factory (plugin.py:36-65): ``config`` or DFSPHConfig fields, plus capacity_per_world.
+
+    Fields given next to a ``config`` override it. A call with neither a ``config`` nor a ``world_bounds`` kwarg
+    (the scene assembler's) takes its escape box from the scene's content (``escape_box.auto_box``) and keeps the
+    box's padding on the solver; an explicit ``world_bounds``, None included, is used as given. Cloth absorption
+    (plugin.py:46, 60-62) belongs to the cloth coupling, outside this path, so its kwarg is unknown here."""
     from . import escape_box
     from .solver import SolverDFSPH
 
-    kw = dict(kwargs)
-    cfg = kw.pop("config", None)
-    cap = kw.pop("capacity_per_world", None)
-    fields = {k: kw.pop(k) for k in list(kw) if k in _CONFIG_FIELDS}
-    if kw:
-        raise TypeError(f"sph factory: unknown kwargs {sorted(kw)}")
-    auto = None
-    if cfg is None:
+    cfg, cap, fields = _split_kwargs(kwargs)
+    pad = None
+    if cfg is not None:
+        if fields:
+            cfg = dataclasses.replace(cfg, **fields)
+    else:
         if "world_bounds" not in fields:
-            auto = escape_box.auto_box(view)
-            fields["world_bounds"] = None if auto is None else auto[0]
+            box = escape_box.auto_box(view)
+            bounds, pad = (None, None) if box is None else box
+            fields["world_bounds"] = bounds
             log.info("sph factory: escape box %s from the scene's content (retired particles: report().escaped)",
-                     fields["world_bounds"])
+                     bounds)
         cfg = DFSPHConfig(**fields)
-    elif fields:
-        cfg = dataclasses.replace(cfg, **fields)
     solver = SolverDFSPH(view, cfg, capacity_per_world=cap)
-    if auto is not None:
-        solver.escape_box_pad = auto[1]
+    if pad is not None:
+        solver.escape_box_pad = pad  # an automatic box (a drive would widen it by this much)
     return solver
 
 

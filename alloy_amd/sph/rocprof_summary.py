@@ -2,7 +2,8 @@
 
     python -m alloy_amd.sph.rocprof_summary kernels DIR --command CMD --title T --out FILE.md [--window A-B]
         kernel trace (rocprofv3 --kernel-trace -f csv): calls, mean and total duration and share per kernel; with
-        --window, the same for the dispatches of steps A..B of the run alone (a step starts at its k_groups_hist)
+        --window A-B[,C-D...], the same for the dispatches of steps A..B of the run alone (a step starts at its
+        k_groups_hist)
     python -m alloy_amd.sph.rocprof_summary traffic FETCH_DIR WRITE_DIR --kernel K --window A-B --fetch-scale F
             --write-scale S --steps K --warmup W --options O --command CMD --out FILE.json
         HBM bytes per launch of kernel K over its dispatches in steps A..B (the bench's roofline window) from two
@@ -75,15 +76,16 @@ def kernels(args) -> None:
     if args.note:
         lines += [args.note, ""]
     lines += _table(agg, ["## The whole run", ""])
-    if args.window:
-        ids = _window([(int(r["Dispatch_Id"]), _short(r["Kernel_Name"])) for r in rows], args.window)
+    order = [(int(r["Dispatch_Id"]), _short(r["Kernel_Name"])) for r in rows]
+    for window in filter(None, args.window.split(",")):
+        ids = _window(order, window)
         win = collections.defaultdict(lambda: [0, 0])
         for r in rows:
             if int(r["Dispatch_Id"]) in ids:
                 k = _short(r["Kernel_Name"])
                 win[k][0] += 1
                 win[k][1] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
-        lines += _table(win, [f"## Steps {args.window} alone", ""])
+        lines += _table(win, [f"## Steps {window} alone", ""])
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")
 

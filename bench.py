@@ -301,9 +301,19 @@ def options_string(options: dict) -> str:
     return ",".join(f"{k}={v}" for k, v in sorted(options.items()))
 
 
+# options that leave the roofline window's k_accel / k_apply launches unchanged: the list build's block size, and the
+# graph and device-loop paths, which per-launch timing switches off
+TRAFFIC_NEUTRAL = ("build_block", "device_loop", "graphs")
+
+
+def _traffic_options(options: str) -> str:
+    return ",".join(kv for kv in options.split(",") if kv.split("=")[0] not in TRAFFIC_NEUTRAL)
+
+
 def _traffic_record(kernel: str, workload: str, steps: int, warmup: int, options: str):
     """The committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary of `kernel` on `workload`, if it was measured on the
-    roofline window of a bench run with these step counts and options; (record or None, reason)."""
+    roofline window of a bench run with these step counts and with options that launch the same `kernel` work;
+    (record or None, reason)."""
     name = f"{kernel.split('<')[0]}_{workload}_traffic.json"
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -313,7 +323,7 @@ def _traffic_record(kernel: str, workload: str, steps: int, warmup: int, options
     if rec.get("steps") != steps or rec.get("warmup") != warmup:
         return None, (f"profiles/{name} was measured with --steps {rec.get('steps')} --warmup {rec.get('warmup')}, "
                       f"not this run's {steps} / {warmup}")
-    if rec.get("options") != options:
+    if _traffic_options(rec.get("options", "")) != _traffic_options(options):
         return None, f"profiles/{name} was measured with options {rec.get('options')}, not this run's {options}"
     return rec, f"profiles/{name}: {rec.get('window', '')}; command: {rec.get('command', '')}"
 
@@ -501,7 +511,9 @@ def main() -> int:
                      "alg_bytes_per_launch": bl / max(nl, 1), "avg_launch_ms": tl / max(nl, 1), "launches_timed": nl,
                      "window": f"steps {w0 + 1}-{w0 + args.steps} (the {args.steps} steps after the timed region), "
                                f"1 in {args.roofline_sample} launches of {dom} timed by HIP events inside its "
-                               "dispatch, algorithmic bytes counted by the kernel",
+                               "dispatch, algorithmic bytes counted by the kernel" + (
+                                   "; per-launch timing runs the solve's phases as separate launches, so the "
+                                   "device-side loop is off in this window" if options_used.get("device_loop") else ""),
                      "traffic_source": traffic_rec[1]},
         "kernel_ms_per_step_roofline_window": {k: round(v[1] / args.steps, 4) for k, v in ks.items()},
         "options": options_used,
