@@ -46,11 +46,40 @@ __device__ __forceinline__ void st3(float* p, int64_t k, f3 v) {
 
 constexpr float PI_F = 3.14159265358979323846f;
 
+// ------------------------------------------------------------------------------ correctly rounded pair arithmetic
+// The pair loops spend most of their instructions on one square root and two divisions per pair, which the compiler
+// expands for the whole float range (input scaling for tiny values, special-case fix-ups). For the operand ranges
+// of a smoothing length h in [PAIR_H_MIN, PAIR_H_MAX] the shorter sequences below give the same correctly rounded
+// results (the bounds keep every operand, quotient and residual normal and the quotient-to-divisor exponent gap
+// under the 96 at which the hardware's division pre-scaling begins):
+// - sqrt_rn: the hardware estimate (within 1 ulp) replaced by a neighbour when an exact residual of the midpoint
+//   test says so; exact for x = 0 and x >= 2^-96 (a pair closer than 2^-48 m is inside the 1e-9 h cut-off anyway);
+// - div_h: r / h from the correctly rounded 1 / h with two remainder corrections (Markstein's theorem for the
+//   second, the first bringing the quotient within 1 ulp);
+// - div_rn: a / d from the hardware reciprocal with one Newton step and two remainder corrections, the compiler's
+//   own division sequence without its range scaling and fix-up, which change nothing in this range.
+// tests/pair_math_check.c checks the three sequences against IEEE results on the CPU.
+constexpr float PAIR_H_MIN = 1.0e-5f, PAIR_H_MAX = 100.0f;
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const float s0 = __builtin_amdgcn_sqrtf(x);
+    const float dn = __int_as_float(__float_as_int(s0) - 1), up = __int_as_float(__float_as_int(s0) + 1);
+    const float vp = fmaf(-dn, s0, x), vs = fmaf(-up, s0, x);
+    const float s = vp <= 0.0f ? dn : s0;
+    return vs > 0.0f ? up : s;
+}
+__device__ __forceinline__ float div_rn(float a, float d) {
+    const float y0 = __builtin_amdgcn_rcpf(d);
+    const float y = fmaf(fmaf(-d, y0, 1.0f), y0, y0);
+    const float q0 = a * y;
+    const float q1 = fmaf(fmaf(-d, q0, a), y, q0);
+    return fmaf(fmaf(-d, q1, a), y, q1);
+}
 // The smoothing kernel's launch constants, built once per thread from the same float expressions the per-call
-// forms below evaluate (h^3, the kind's normalisation, the 1e-9 h cut-off), so both give identical bits.
+// forms below evaluate (h^3, the kind's normalisation, the 1e-9 h cut-off), so both give identical bits; `fast`
+// selects the short pair arithmetic above (a launch-uniform branch).
 struct KC {
-    float h, rmin, knorm;
-    int kind;
+    float h, rmin, knorm, inv_h;
+    int kind, fast;
 };
 __device__ __forceinline__ KC make_kc(float h, int kind) {
     float h3 = h * h * h;
@@ -59,12 +88,28 @@ __device__ __forceinline__ KC make_kc(float h, int kind) {
     K.kind = kind;
     K.rmin = 1.0e-9f * h;
     K.knorm = kind == 1 ? (21.0f / (2.0f * PI_F * h3)) : 8.0f / (PI_F * h3);
+    K.inv_h = 1.0f / h;
+    K.fast = h >= PAIR_H_MIN && h <= PAIR_H_MAX;
     return K;
+}
+__device__ __forceinline__ float div_h(float r, const KC& K) {
+    const float q0 = r * K.inv_h;
+    const float q1 = fmaf(fmaf(-K.h, q0, r), K.inv_h, q0);
+    return fmaf(fmaf(-K.h, q1, r), K.inv_h, q1);
+}
+// |rv| and r / h, correctly rounded
+__device__ __forceinline__ float dist(f3 rv, const KC& K) {
+    if (K.fast) return sqrt_rn(dot(rv, rv));
+    return length(rv);
+}
+__device__ __forceinline__ float over_h(float r, const KC& K) {
+    if (K.fast) return div_h(r, K);
+    return r / K.h;
 }
 
 // kernels.py:41-55
 __device__ __forceinline__ float kernel_w(float r, const KC& K) {
-    float q = r / K.h;
+    float q = over_h(r, K);
     if (q >= 1.0f) return 0.0f;
     if (K.kind == 1) {
         float s = 1.0f - q;
@@ -77,7 +122,24 @@ __device__ __forceinline__ float kernel_w(float r, const KC& K) {
 
 // kernels.py:58-77 as a scale: grad W(rv) = rv * dW/dq / (r h), with 0 standing for the zero vector of the cut-off
 // cases (q >= 1 or r <= 1e-9 h). dW/dq vanishes inside (0, 1) nowhere, so a zero scale occurs only there.
+__device__ __forceinline__ float kernel_grad_scale_fast(f3 rv, const KC& K) {
+    // every branch evaluated and the result selected: no divergent control flow per pair
+    const float r = sqrt_rn(dot(rv, rv));
+    const float q = div_h(r, K);
+    const float s = 1.0f - q;
+    float dwdq;
+    if (K.kind == 1) {
+        dwdq = K.knorm * (-20.0f * q * s * s * s);
+    } else {
+        const float near = K.knorm * (18.0f * q * q - 12.0f * q);
+        const float far = K.knorm * (-6.0f * s * s);
+        dwdq = q <= 0.5f ? near : far;
+    }
+    const float g = div_rn(dwdq, r * K.h);
+    return (q >= 1.0f || r <= K.rmin) ? 0.0f : g;
+}
 __device__ __forceinline__ float kernel_grad_scale(f3 rv, const KC& K) {
+    if (K.fast) return kernel_grad_scale_fast(rv, K);
     float r = length(rv);
     float q = r / K.h;
     if (q >= 1.0f || r <= K.rmin) return 0.0f;

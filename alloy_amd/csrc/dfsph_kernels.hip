@@ -140,6 +140,8 @@ struct Ctx {
     // reads (kappa, the CG direction, the CG correction), and the per-world particle mass they stand in for
     int packed;
     float4* qk;
+    float4* qa;  // packed (DFSPH_OPT_PACKED_ACCEL): {x, y, z, acc.x} and {acc.y, acc.z} of the last pressure
+    float2* a2;  // acceleration, the operator apply's 24 B neighbour record instead of q4 and acc4 (32 B)
     const float* w_mass;
     int* cell_fill;
     int* items;
@@ -200,7 +202,6 @@ __device__ __forceinline__ bool is_fluid(const Ctx& c, int i) {
 }
 __device__ __forceinline__ f3 Q(const Ctx& c, int i) { return ld3(c.q, i); }
 __device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
-__device__ __forceinline__ f3 A4(const Ctx& c, int i) { return xyz(c.acc4[i]); }
 __device__ __forceinline__ f3 V(const Ctx& c, int i) { return ld3(c.qd, i); }
 
 template <bool WIDE>
@@ -229,6 +230,23 @@ enum VrSlot { VR_KAPPA = 0, VR_DIR = 1, VR_CORR = 2 };
 __device__ __forceinline__ void put_vr(const Ctx& c, float* vr, int slot, int i, float v) {
     vr[i] = v;
     if (c.packed) c.qk[int64_t(slot) * c.n + i].w = v;
+}
+// the pressure acceleration of particle i: acc4, or with the packed records (c.packed) their acc components only
+// (dfsph_get_buffer unpacks acc4 on demand)
+__device__ __forceinline__ void put_acc(const Ctx& c, int i, float ax, float ay, float az) {
+    if (c.packed) {
+        c.qa[i].w = ax;
+        c.a2[i] = make_float2(ay, az);
+    } else {
+        c.acc4[i] = make_float4(ax, ay, az, 0.0f);
+    }
+}
+__device__ __forceinline__ f3 acc_of(const Ctx& c, int i) {
+    if (c.packed) {
+        const float2 a = c.a2[i];
+        return f3{c.qa[i].w, a.x, a.y};
+    }
+    return xyz(c.acc4[i]);
 }
 // The row arrays hold ROW_PAD >= NB_U rows beyond `cap` (zero, never written), so a batch starting below the count
 // loads all of its NB_U entries unconditionally: the loads issue back to back instead of one branch-guarded round
@@ -549,8 +567,10 @@ __global__ void __launch_bounds__(256) k_groups_hist(Ctx c) {
     if (g >= 0) c.ch_live[c.w_chunk[g] + (i - c.ws[g]) / CHUNK] = 1;
     f3 xi = Q(c, i);
     c.q4[i] = make_float4(xi.x, xi.y, xi.z, c.mass[i]);
-    if (c.packed)  // the w components are written by the vector's producer before any read this step
+    if (c.packed) {  // the w components are written by the vector's producer before any read this step
         for (int s = 0; s < 3; ++s) c.qk[int64_t(s) * c.n + i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
+        c.qa[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
+    }
     int k = -1;
     if (g != -1) {
         int gi = g >= 0 ? g : c.W + (-2 - g);
@@ -693,6 +713,31 @@ __global__ void k_fill_u8(uint8_t* p, int64_t n) {
 constexpr int BUILD_U = DFSPH_BUILD_U;
 // BB: the block size (DFSPH_OPT_BUILD_BLOCK); the row of thread t is row[k * BB + t]. One row: the lowest `cap`
 // world-local ids within sqrt(r2) of particle i (group g >= 0), ascending, in LDS; returns (count, candidates).
+// Accepted candidates are appended unsorted (a branch-free test per candidate, one LDS store per acceptance) and the
+// row is sorted once at the end; a thread whose entries could overflow ROW_BUF within the next batch sorts them and
+// keeps the lowest `cap`, so the result is the lowest `cap` of all candidates for any count. The candidate batches load unclamped: q4s holds
+// BUILD_U records past its end, and records past the z-row's range are rejected by the test.
+#ifndef DFSPH_ROW_BUF
+#define DFSPH_ROW_BUF 80  // row entries collected per thread before a sort (build-time tuning; cap + BUILD_U at least)
+#endif
+constexpr int ROW_BUF = DFSPH_ROW_BUF;
+static_assert(64 + BUILD_U <= ROW_BUF, "a compacted row and one candidate batch must fit the row buffer");
+template <class RowT, int BB>
+__device__ __forceinline__ void sort_row(RowT* L, int nn) {
+    // insertion sort, ascending: the candidates arrive nearly sorted (ids ascending within a cell, cells in x, y, z
+    // order), so most entries stay where they are
+    for (int a = 1; a < nn; ++a) {
+        const RowT v = L[a * BB];
+        int k = a - 1;
+        RowT u = L[k * BB];
+        while (u > v) {
+            L[(k + 1) * BB] = u;
+            if (--k < 0) break;
+            u = L[k * BB];
+        }
+        L[(k + 1) * BB] = v;
+    }
+}
 template <bool WIDE, int BB>
 __device__ __forceinline__ void build_row(const Ctx& c, int i, int g, f3 xi, float r2, int cap,
                                           typename std::conditional<WIDE, int, uint16_t>::type* L, int& nn_out,
@@ -704,9 +749,7 @@ __device__ __forceinline__ void build_row(const Ctx& c, int i, int g, f3 xi, flo
     cell_of(c, g, xi, cx, cy, cz);
     const int* d = c.g_dims + 3 * g;
     int nn = 0, total = 0;
-    // x outer, y, then the contiguous z-row of 3 cells, ids ascending within a cell: for lattice-ordered ids
-    // (x slowest) candidates arrive nearly sorted and most insertions append. Candidates stream from q4s (cell
-    // order) in batches of BUILD_U.
+    // x outer, y, then the contiguous z-row of 3 cells, ids ascending within a cell
     for (int dx = -1; dx <= 1; ++dx) {
         int X = cx + dx;
         if (X < 0 || X >= d[0]) continue;
@@ -717,36 +760,30 @@ __device__ __forceinline__ void build_row(const Ctx& c, int i, int g, f3 xi, flo
             int cb = c.cell_base[g] + (X * d[1] + Y) * d[2];
             int e0 = c.cell_start[cb + z0], e1 = c.cell_start[cb + z1 + 1];
             for (int e = e0; e < e1; e += BUILD_U) {
+                if (nn > ROW_BUF - BUILD_U) {  // room for a whole batch: keep the lowest cap (cap + BUILD_U <= ROW_BUF)
+                    sort_row<RowT, BB>(L, nn);
+                    nn = cap;
+                }
+                const float4* pe = c.q4s + e;
                 float4 pc[BUILD_U];
 #pragma unroll
-                for (int u = 0; u < BUILD_U; ++u) pc[u] = c.q4s[min(e + u, e1 - 1)];
+                for (int u = 0; u < BUILD_U; ++u) pc[u] = pe[u];
 #pragma unroll
                 for (int u = 0; u < BUILD_U; ++u) {
-                    if (e + u >= e1) continue;
-                    int j = __float_as_int(pc[u].w);
-                    if (j == i) continue;
-                    f3 dd = xyz(pc[u]) - xi;
-                    if (dot(dd, dd) < h2) {
+                    const int j = __float_as_int(pc[u].w);
+                    const f3 dd = xyz(pc[u]) - xi;
+                    const bool ok = (e + u < e1) & (j != i) & (dot(dd, dd) < h2);
+                    if (ok) {
                         total += 1;
-                        int jl = j - base;
-                        int pos = nn, k = nn - 1;
-                        while (k >= 0) {
-                            int v = L[k * BB];
-                            if (v < jl) break;
-                            pos = k;
-                            k -= 1;
-                        }
-                        if (pos >= cap) continue;
-                        int lastk = nn >= cap ? cap - 1 : nn;
-                        for (k = lastk; k > pos; --k) L[k * BB] = L[(k - 1) * BB];
-                        L[pos * BB] = RowT(jl);
-                        if (nn < cap) nn += 1;
+                        L[nn * BB] = RowT(j - base);
+                        nn += 1;
                     }
                 }
             }
         }
     }
-    nn_out = nn;
+    sort_row<RowT, BB>(L, nn);
+    nn_out = nn < cap ? nn : cap;
     total_out = total;
 }
 
@@ -765,10 +802,14 @@ __device__ __forceinline__ void write_row(const Ctx& c, int i, int g, int nn, in
 // SUP false: the step's rows (neighbors.py:160-195). SUP true (neighbors_skin.py:57-106): the superset rows within
 // h + skin of the worlds flagged this step, each particle's reference position and group stored; a superset that
 // overflows its capacity is replaced by this step's plain row and forces the world's next rebuild.
+// Threads take the particles in id order, so the row writes of a wave are coalesced. (Taking the liquid in cell
+// order instead, so that the lanes of one cell stream the same candidates, was measured slower: 1.27 against
+// 1.15 ms per build on the 512-world bowl pack, the scattered 2-byte row writes costing more than the shared loads
+// save.)
 template <bool WIDE, int BB, bool SUP>
 __global__ void __launch_bounds__(BB) k_build_lists(Ctx c) {
     using RowT = typename std::conditional<WIDE, int, uint16_t>::type;
-    __shared__ RowT row[64 * BB];
+    __shared__ RowT row[ROW_BUF * BB];
     int i = xcd_particle();
     if (i >= c.n) return;
     int g = c.group[i];
@@ -900,7 +941,7 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv,
                 for (int e = c.cell_start[cc]; e < c.cell_start[cc + 1]; ++e) {
                     const float4 p4 = c.q4s[e];
                     f3 dd = x - f3{p4.x, p4.y, p4.z};
-                    float kw = kernel_w(length(dd), K);
+                    float kw = kernel_w(dist(dd, K), K);
                     if (!(kw > 0.0f)) continue;
                     int j = __float_as_int(p4.w);
                     float vj = c.wvol[j];
@@ -987,7 +1028,7 @@ __global__ void __launch_bounds__(256) k_pw_accumulate(Ctx c) {
                         int cc = c.cell_base[g] + (X * d[1] + Y) * d[2] + Z;
                         for (int e = c.cell_start[cc]; e < c.cell_start[cc + 1]; ++e) {
                             int j = c.items[e];
-                            float wj = c.wvol[j] * kernel_w(length(x - xyz(c.q4[j])), K);
+                            float wj = c.wvol[j] * kernel_w(dist(x - xyz(c.q4[j]), K), K);
                             if (wj > 0.0f) share(c.wcidx[j], wj);
                         }
                     }
@@ -1457,7 +1498,7 @@ __device__ __forceinline__ void density_alpha_one(const Ctx& c, const KC& K, int
         [&](int k, int, QVd d) {
             f3 rv = xi - xyz(d.q);
             float mj = d.q.w;
-            rho += mj * kernel_w(length(rv), K);
+            rho += mj * kernel_w(dist(rv, K), K);
             float sc = kernel_grad_scale(rv, K);
             if (c.pair_cache) c.pgrad[int64_t(k) * c.n + i] = sc;
             f3 g = pair_grad(rv, sc);
@@ -1684,16 +1725,26 @@ __device__ f3 pressure_accel(const Ctx& c, const float* kap, const float* kapr, 
 struct QA {
     float4 q, a;
 };
+struct QA2 {  // the packed record {x, y, z, acc.x}, {acc.y, acc.z}
+    float4 q;
+    float2 a;
+};
 template <bool WIDE>
 __device__ float accel_divergence(const Ctx& c, int i) {
-    f3 xi = xyz(c.q4[i]), ai = A4(c, i);
+    f3 xi = xyz(c.q4[i]), ai = acc_of(c, i);
     float inv_r0 = 1.0f / c.rho0[i];
     float ar = 0.0f;
     int cnt = c.count[i];
     int base = c.ws[c.world[i]];
-    for_nbrs_g<WIDE, QA>(c, i, xi, cnt, base, (c.pair_cache & DFSPH_PAIR_CACHE_APPLY) != 0,
-                         [&](int j) { return QA{c.q4[j], c.acc4[j]}; },
-                         [&](int, QA d, f3 g) { ar += (d.q.w * inv_r0) * dot(ai - xyz(d.a), g); });
+    const bool cached = (c.pair_cache & DFSPH_PAIR_CACHE_APPLY) != 0;
+    if (c.packed) {  // one 16 B and one 8 B gather per neighbour; its mass is the world's
+        const float mw = c.w_mass[c.world[i]];
+        for_nbrs_g<WIDE, QA2>(c, i, xi, cnt, base, cached, [&](int j) { return QA2{c.qa[j], c.a2[j]}; },
+                              [&](int, QA2 d, f3 g) { ar += (mw * inv_r0) * dot(ai - f3{d.q.w, d.a.x, d.a.y}, g); });
+    } else {
+        for_nbrs_g<WIDE, QA>(c, i, xi, cnt, base, cached, [&](int j) { return QA{c.q4[j], c.acc4[j]}; },
+                             [&](int, QA d, f3 g) { ar += (d.q.w * inv_r0) * dot(ai - xyz(d.a), g); });
+    }
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
         if (c.s_shape[b4 + b] >= 0) ar += dot(ai, ld3(c.s_gpsi, b4 + b));
@@ -1721,13 +1772,24 @@ __device__ __forceinline__ bool block_skip(const Ctx& c, int w, int gated) {
     return gated != 0 && c.restart[w] == 0;
 }
 
-// the packed copies {x, y, z, vec / rho0} of the three vectors from q4 and the quotient arrays
+// acc4 from the packed acceleration records
+__global__ void __launch_bounds__(256) k_unpack_acc(Ctx c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const float2 a = c.a2[i];
+    c.acc4[i] = make_float4(c.qa[i].w, a.x, a.y, 0.0f);
+}
+// the packed copies {x, y, z, vec / rho0} of the three vectors from q4 and the quotient arrays, and the packed
+// acceleration records from q4 and acc4
 __global__ void __launch_bounds__(256) k_pack_sync(Ctx c) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.n) return;
     const float4 q = c.q4[i];
     const float* vr[3] = {c.kr, c.pdr, c.corrr};
     for (int s = 0; s < 3; ++s) c.qk[int64_t(s) * c.n + i] = make_float4(q.x, q.y, q.z, vr[s][i]);
+    const float4 a = c.acc4[i];
+    c.qa[i] = make_float4(q.x, q.y, q.z, a.x);
+    c.a2[i] = make_float2(a.y, a.z);
 }
 
 // accel_kernel (pressure.py:280-291) / cg_accel_kernel (krylov.py:77-87)
@@ -1740,7 +1802,7 @@ __device__ __forceinline__ void accel_chunk(const Ctx& c, int b_, const float* v
     unsigned nb = 0;  // algorithmic bytes of this particle (profiling)
     if (valid) {
         if (!is_fluid(c, i)) {
-            c.acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            put_acc(c, i, 0.0f, 0.0f, 0.0f);
             nb = 17u;
         } else {
             bool full = true;
@@ -1751,14 +1813,14 @@ __device__ __forceinline__ void accel_chunk(const Ctx& c, int b_, const float* v
                 full = vec[i] != 0.0f || any_nbr_nonzero<WIDE>(c, i, cnt, c.ws[c.world[i]], vecr);
                 c.touched[i] = full ? 1 : 0;
                 if (!full) {
-                    c.acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    put_acc(c, i, 0.0f, 0.0f, 0.0f);
                     // flags 4, phase 1, vec 4, vec / rho0 4, count 4, world 4, touched 1, acc written 12
                     nb = 34u + (WIDE ? 3u : 2u) * unsigned(cnt);
                 }
             }
             if (full) {
                 f3 a = pressure_accel<WIDE>(c, vec, vecr, slot, i);
-                c.acc4[i] = make_float4(a.x, a.y, a.z, 0.0f);
+                put_acc(c, i, a.x, a.y, a.z);
                 nb = pass_bytes<WIDE>(c, i, 69u + (gated && c.sparse_corr ? 1u : 0u), 17u);
             }
         }
@@ -1841,7 +1903,7 @@ __device__ __forceinline__ void accel_h(const Ctx& c, const float* vec, const fl
     unsigned nb = 0;
     if (valid) {
         if (!is_fluid(c, i)) {
-            c.acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            put_acc(c, i, 0.0f, 0.0f, 0.0f);
             nb = 17u;
         } else {
             bool full = true;
@@ -1850,7 +1912,7 @@ __device__ __forceinline__ void accel_h(const Ctx& c, const float* vec, const fl
                 full = vec[i] != 0.0f || any_nbr_nonzero<WIDE>(c, i, cnt, c.ws[c.world[i]], vecr);
                 c.touched[i] = full ? 1 : 0;
                 if (!full) {
-                    c.acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    put_acc(c, i, 0.0f, 0.0f, 0.0f);
                     nb = 34u + (WIDE ? 3u : 2u) * unsigned(cnt);
                 }
             }
@@ -1875,7 +1937,7 @@ __device__ __forceinline__ void accel_h(const Ctx& c, const float* vec, const fl
                     for (int b = 0; b < 4; ++b)
                         if (c.s_shape[b4 + b] >= 0) a = a - vec[i] * ld3(c.s_gpsi, b4 + b);
                 }
-                c.acc4[i] = make_float4(a.x, a.y, a.z, 0.0f);
+                put_acc(c, i, a.x, a.y, a.z);
                 nb = pass_bytes<WIDE>(c, i, 69u + (gated && c.sparse_corr ? 1u : 0u), 17u);
             }
         }
@@ -1893,10 +1955,6 @@ __global__ void __launch_bounds__(256) k_accel_hn(Ctx c, const float* vec, const
 
 // PACKED (one particle mass per world, the packed acceleration records' condition): 24 B records {x, y, z, a_x} and
 // {a_y, a_z}, the neighbour's mass the world's; else {x, y, z, m} and the acceleration
-struct QA2 {
-    float4 q;
-    float2 a;
-};
 template <bool WIDE, bool PACKED>
 __device__ __forceinline__ void apply_h(const Ctx& c, int mode, float dt, const float* vec, float* out, int gated,
                                         int bias, int accum, int reduce) {
@@ -1911,10 +1969,7 @@ __device__ __forceinline__ void apply_h(const Ctx& c, int mode, float dt, const 
     }
     const int hc = c.h_count[b_];
     if (PACKED) {
-        stage_halo(c, b_, hc, [&](int j) {
-            float4 q = c.q4[j], a = c.acc4[j];
-            return QA2{make_float4(q.x, q.y, q.z, a.x), make_float2(a.y, a.z)};
-        }, [&](int t, QA2 r) {
+        stage_halo(c, b_, hc, [&](int j) { return QA2{c.qa[j], c.a2[j]}; }, [&](int t, QA2 r) {
             sq[t] = r.q;
             sa2[t] = r.a;
         });
@@ -1943,7 +1998,7 @@ __device__ __forceinline__ void apply_h(const Ctx& c, int mode, float dt, const 
                 if (hc < 0) {
                     div = accel_divergence<WIDE>(c, i);
                 } else {  // accel_divergence over the staged records
-                    f3 xi = xyz(c.q4[i]), ai = A4(c, i);
+                    f3 xi = xyz(c.q4[i]), ai = acc_of(c, i);
                     float inv_r0 = 1.0f / c.rho0[i];
                     float ar = 0.0f;
                     const bool cached = (c.pair_cache & DFSPH_PAIR_CACHE_APPLY) != 0;
@@ -2421,7 +2476,7 @@ __global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
             [&](int, int, XS d) {
                 float rho_bar = 0.5f * (rho_i + d.ratio * d.rho0);
                 if (rho_bar > 0.0f) {
-                    float wk = kernel_w(length(xi - xyz(d.q)), K);
+                    float wk = kernel_w(dist(xi - xyz(d.q), K), K);
                     cr = cr + (d.q.w / rho_bar * wk) * (d.v - vi);
                 }
             });
@@ -2842,7 +2897,7 @@ __global__ void __launch_bounds__(256) k_visc_shift(Ctx c) {
     for (int k = 0; k < cnt; ++k) {
         int j = nbr_at<WIDE>(c, base, i, k);
         if (!is_fluid(c, j)) continue;
-        float wj = c.mass[j] / c.rho0[j] * kernel_w(length(xi - xyz(c.q4[j])), K);
+        float wj = c.mass[j] / c.rho0[j] * kernel_w(dist(xi - xyz(c.q4[j]), K), K);
         acc = acc + wj * ld3(c.vraw, j);
         tot = tot + wj;
     }
@@ -3069,7 +3124,7 @@ __global__ void __launch_bounds__(256) k_csf_shepard_vec(Ctx c, const float* vin
     const int cnt = c.count[i], base = c.ws[c.world[i]];
     for (int k = 0; k < cnt; ++k) {
         int j = nbr_at<WIDE>(c, base, i, k);
-        float w = kernel_w(length(xi - xyz(c.q4[j])), K);
+        float w = kernel_w(dist(xi - xyz(c.q4[j]), K), K);
         s = s + w * ld3(vin, j);
         den = den + w;
     }
@@ -3145,7 +3200,7 @@ __global__ void __launch_bounds__(256) k_csf_curvature(Ctx c) {
         int j = nbr_at<WIDE>(c, base, i, k);
         if (c.t_surf[j] == 0) continue;
         const f3 xj = xyz(c.q4[j]);
-        const float w = kernel_w(length(xi - xj), K);
+        const float w = kernel_w(dist(xi - xj, K), K);
         const f3 t = mul33(P, xj - xi);
         const f3 d = mul33(P, ld3(c.t_nhat, j) - ni);
         const float tv[3] = {t.x, t.y, t.z}, dv[3] = {d.x, d.y, d.z};
@@ -3186,7 +3241,7 @@ __global__ void __launch_bounds__(256) k_csf_shepard_surf(Ctx c, const float* ki
     for (int k = 0; k < cnt; ++k) {
         int j = nbr_at<WIDE>(c, base, i, k);
         if (c.t_surf[j] == 0) continue;
-        float w = kernel_w(length(xi - xyz(c.q4[j])), K);
+        float w = kernel_w(dist(xi - xyz(c.q4[j]), K), K);
         num = num + w * kin[j];
         den = den + w;
     }
@@ -4318,7 +4373,11 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
             float* wm_dev = nullptr;
             TRY(dupload(H, &wm_dev, wm.data(), std::max(W, 1)));
             c.w_mass = wm_dev;
-            if (H->packed_ok) TRY(dalloc(H, &c.qk, 3 * int64_t(std::max(n, 1))));
+            if (H->packed_ok) {
+                TRY(dalloc(H, &c.qk, 3 * int64_t(std::max(n, 1))));
+                TRY(dalloc(H, &c.qa, n));
+                TRY(dalloc(H, &c.a2, n));
+            }
             c.packed = H->packed_ok;
         }
         if (P->viscosity_implicit) {  // viscosity.py:213-234
@@ -4372,7 +4431,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         TRY(dalloc(H, &c.acc4, n));
         TRY(dalloc(H, &c.touched, n));
         TRY(dalloc(H, &c.q4, n));
-        TRY(dalloc(H, &c.q4s, n));
+        TRY(dalloc(H, &c.q4s, int64_t(n) + BUILD_U));  // BUILD_U records of padding: the build loads batches unclamped
         TRY(dalloc(H, &c.feat, 16 * int64_t(n)));
         TRY(dalloc(H, &c.mlp_out, 2 * int64_t(n)));
         TRY(dalloc(H, &c.warm_k, 2 * int64_t(n)));
@@ -4673,12 +4732,12 @@ int32_t dfsph_set_option(void* h, int32_t option, int32_t value) {
             if (value == 1 && !H->packed_ok)
                 return fail(DFSPH_EINVAL, "packed accel needs one particle mass per world");
             if (H->c.packed != value) {
+                // bring the layout being switched to up to date, so a phase call before the next step reads it
+                if (value) k_pack_sync<<<blocks_for(H->c.n), 256>>>(H->c);
+                else k_unpack_acc<<<blocks_for(H->c.n), 256>>>(H->c);
+                LAUNCH_CHECK();
+                HIPCHK(hipDeviceSynchronize());
                 H->c.packed = value;
-                if (value) {  // bring the packed copies up to date, so a phase call before the next step reads them
-                    k_pack_sync<<<blocks_for(H->c.n), 256>>>(H->c);
-                    LAUNCH_CHECK();
-                    HIPCHK(hipDeviceSynchronize());
-                }
             }
             return DFSPH_OK;
         case DFSPH_OPT_LEARNED_SOLVES:
@@ -4829,7 +4888,13 @@ int32_t dfsph_get_buffer(void* h, int32_t which, void** ptr, int64_t* nb) {
         case DFSPH_BUF_REG: B(c.reg, 4 * n)
         case DFSPH_BUF_SRC: B(c.src, 4 * n)
         case DFSPH_BUF_KAPPA: B(c.kappa, 4 * n)
-        case DFSPH_BUF_ACC: B(c.acc4, 16 * n)
+        case DFSPH_BUF_ACC:
+            if (c.packed) {  // the packed records hold the acceleration: unpack it
+                k_unpack_acc<<<blocks_for(c.n), 256>>>(c);
+                LAUNCH_CHECK();
+                HIPCHK(hipDeviceSynchronize());
+            }
+            B(c.acc4, 16 * n)
         case DFSPH_BUF_ERR: B(c.err, 4 * n)
         case DFSPH_BUF_LOWER: B(c.lower, 4 * n)
         case DFSPH_BUF_KV: B(c.kv, 4 * n)
@@ -4910,6 +4975,12 @@ int32_t dfsph_write_buffer(void* h, int32_t which, const void* src, int64_t nb) 
     if (nb > have) return fail(DFSPH_ECAPACITY, "write beyond the buffer");
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(p, src, size_t(nb), hipMemcpyHostToDevice));
+    Ctx& c = as_handle(h)->c;
+    if (which == DFSPH_BUF_ACC && c.packed) {  // into the packed records the passes read
+        k_pack_sync<<<blocks_for(c.n), 256>>>(c);
+        LAUNCH_CHECK();
+        HIPCHK(hipDeviceSynchronize());
+    }
     return DFSPH_OK;
 }
 

@@ -118,11 +118,28 @@ def load_learned(mode: str, device: str):
     return WarmStartMLP.load(DEFAULT_WEIGHTS, device), int(solves)
 
 
+def device_view(solver, which: int, typestr: str, count: int):
+    """A torch view (no copy) of a library buffer in device memory."""
+    import torch
+
+    from alloy_amd.sph import _abi
+
+    ptr, nbytes = ctypes.c_void_p(), ctypes.c_int64()
+    _abi.check(solver.lib.dfsph_get_buffer(solver._h, which, ctypes.byref(ptr), ctypes.byref(nbytes)), solver.lib)
+
+    class _Export:
+        __cuda_array_interface__ = {"shape": (count,), "typestr": typestr, "data": (ptr.value, False), "version": 3}
+
+    return torch.as_tensor(_Export(), device=solver.device)
+
+
 def timed_run(run: Run, steps: int, dt: float, prof_mask: int = 0, sample: int = 1, count_active: bool = False) -> dict:
     """`steps` steps timed with HIP events on the current stream, a report (the step's only host sync point) every
     10 steps; over the same region, per-launch timing of a pseudo-random 1 / `sample` of the launches of the kernels
     in `prof_mask` (a timed launch costs the host more than a plain one; sampling keeps that out of the step time).
-    `count_active`: sum the active liquid after every step on the device (particle-steps of a growing pour)."""
+    Both solves' per-world iteration counts are summed on the device after every step (two small adds, no sync), so
+    the iteration means cover every step. `count_active`: sum the active liquid after every step on the device
+    (particle-steps of a growing pour)."""
     import torch
 
     from alloy_amd.sph import _abi
@@ -133,26 +150,29 @@ def timed_run(run: Run, steps: int, dt: float, prof_mask: int = 0, sample: int =
     if prof_mask:
         _abi.check(lib.dfsph_set_profiling(h, prof_mask, sample), lib)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    den, it_den, it_div, unconv = [], [], [], 0
+    den, unconv = [], 0
+    W = solver.model.world_count
+    its = [device_view(solver, b, "<i4", W) for b in (_abi.BUF_ITERS_DEN, _abi.BUF_ITERS_DIV)]
+    it_sum = [torch.zeros(W, dtype=torch.int64, device=solver.device) for _ in its]
     acc = torch.zeros((), dtype=torch.int64, device=solver.device) if count_active else None
     torch.cuda.synchronize()
     e0.record(stream)
     for k in range(steps):
         run.step(dt)
+        for a, v in zip(it_sum, its):
+            a += v
         if acc is not None:
             acc += run.active()
         if k % 10 == 9:
             r = solver.report()
             den.append(float(np.mean(r.density_err)))
-            it_den.append(float(np.mean(r.iters_density)))
-            it_div.append(float(np.mean(r.iters_divergence)))
             unconv += int(np.sum(r.unconverged))
     e1.record(stream)
     torch.cuda.synchronize()
+    it_mean = [float(a.sum().item()) / (W * steps) for a in it_sum]
     out = {"total_ms": e0.elapsed_time(e1), "density_err_mean": float(np.mean(den)) if den else None,
            "density_err_max_sampled": float(np.max(den)) if den else None,
-           "iters_density_mean": float(np.mean(it_den)) if it_den else None,
-           "iters_divergence_mean": float(np.mean(it_div)) if it_div else None, "unconverged_sampled_steps": unconv}
+           "iters_density_mean": it_mean[0], "iters_divergence_mean": it_mean[1], "unconverged_sampled_steps": unconv}
     if acc is not None:
         out["particle_steps"] = int(acc.item())
     if prof_mask:
@@ -500,8 +520,9 @@ def main() -> int:
                    "density_err_mean": run["density_err_mean"],
                    "density_err_max_sampled": run["density_err_max_sampled"],
                    "unconverged_sampled_steps": run["unconverged_sampled_steps"],
-                   "note": "per-world means over the reports every 10 steps; a world's solve stops at its tolerance "
-                           "or at min_iterations = 2"},
+                   "note": "iterations: per-world means over every step of the window; density error: per-world "
+                           "means over the reports every 10 steps; a world's solve stops at its tolerance or at "
+                           "min_iterations = 2"},
         "density_err_mean": run["density_err_mean"],
         "iters_density_mean": run["iters_density_mean"],
         "iters_divergence_mean": run["iters_divergence_mean"],
