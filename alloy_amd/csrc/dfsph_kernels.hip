@@ -204,10 +204,18 @@ __device__ __forceinline__ f3 Q(const Ctx& c, int i) { return ld3(c.q, i); }
 __device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
 __device__ __forceinline__ f3 V(const Ctx& c, int i) { return ld3(c.qd, i); }
 
+// Row storage in quads: entry k of particle i sits at ((k / 4) n + i) 4 + k % 4, so the four entries of a
+// neighbour batch are one 8 B load per lane (4 B for the high bytes), coalesced across the wave. On the texture data
+// path a 2 B load per lane costs as much as a 16 B one, so four separate entry loads cost about as much as the four
+// records they name. dfsph_get_buffer presents the rows in the reference's layout [k][n] (rows_view).
+__device__ __forceinline__ int64_t row_at(const Ctx& c, int i, int k) {
+    return ((int64_t(k >> 2) * c.n + i) << 2) + (k & 3);
+}
 template <bool WIDE>
 __device__ __forceinline__ int nbr_at(const Ctx& c, int base, int i, int k) {
-    int j = base + int(c.nbr[int64_t(k) * c.n + i]);
-    if (WIDE) j += int(c.nbr_hi[int64_t(k) * c.n + i]) << 16;
+    const int64_t r = row_at(c, i, k);
+    int j = base + int(c.nbr[r]);
+    if (WIDE) j += int(c.nbr_hi[r]) << 16;
     return j;
 }
 
@@ -251,16 +259,31 @@ __device__ __forceinline__ f3 acc_of(const Ctx& c, int i) {
 // The row arrays hold ROW_PAD >= NB_U rows beyond `cap` (zero, never written), so a batch starting below the count
 // loads all of its NB_U entries unconditionally: the loads issue back to back instead of one branch-guarded round
 // trip each.
+static_assert(NB_U % 4 == 0, "a neighbour batch is whole row quads");
+// the NB_U world-local entries k0.. (k0 a multiple of 4) of particle i's row
+template <bool WIDE>
+__device__ __forceinline__ void row_entries(const Ctx& c, int i, int k0, unsigned (&e)[NB_U]) {
+#pragma unroll
+    for (int q = 0; q < NB_U / 4; ++q) {
+        const int64_t r = int64_t((k0 >> 2) + q) * c.n + i;
+        const uint2 v = reinterpret_cast<const uint2*>(c.nbr)[r];
+        e[4 * q] = v.x & 0xffffu;
+        e[4 * q + 1] = v.x >> 16;
+        e[4 * q + 2] = v.y & 0xffffu;
+        e[4 * q + 3] = v.y >> 16;
+        if (WIDE) {
+            const unsigned hb = reinterpret_cast<const unsigned*>(c.nbr_hi)[r];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) e[4 * q + u] |= ((hb >> (8 * u)) & 0xffu) << 16;
+        }
+    }
+}
 template <bool WIDE>
 __device__ __forceinline__ void nbr_batch(const Ctx& c, int i, int cnt, int base, int k0, int (&j)[NB_U]) {
-    int e[NB_U];
+    unsigned e[NB_U];
+    row_entries<WIDE>(c, i, k0, e);
 #pragma unroll
-    for (int u = 0; u < NB_U; ++u) {
-        e[u] = int((c.nbr + int64_t(k0 + u) * c.n)[i]);
-        if (WIDE) e[u] += int((c.nbr_hi + int64_t(k0 + u) * c.n)[i]) << 16;
-    }
-#pragma unroll
-    for (int u = 0; u < NB_U; ++u) j[u] = k0 + u < cnt ? base + e[u] : i;
+    for (int u = 0; u < NB_U; ++u) j[u] = k0 + u < cnt ? base + int(e[u]) : i;
 }
 // body(k, j, record)
 template <bool WIDE, class T, class LoadF, class BodyF>
@@ -297,6 +320,83 @@ __device__ __forceinline__ void for_nbrs_g(const Ctx& c, int i, f3 xi, int cnt, 
             if (k0 + u >= cnt) continue;
             f3 rv = xi - xyz(load_pos(d[u]));
             body(j[u], d[u], cached ? pair_grad(rv, sc[u]) : kernel_grad(rv, K));
+        }
+    }
+}
+
+// The pair loop of the solve gathers (pressure acceleration and operator apply), with the addressing taken off the
+// vector ALU, which these passes saturate:
+// - the row entries (and cached scales) come through buffer loads whose row offset is a scalar and whose lane offset
+//   (i times the entry size) is computed once, instead of a 64-bit address per entry;
+// - the records are gathered through buffer loads on a per-world resource (the caller's load(e), see WorldRecs) with
+//   the world-local entry times the record size as the lane offset: one shift per record; a lane past its count
+//   gathers the record its stale entry names (the rows hold world-local ids of the same world, zero when never
+//   written, and the resource's bound returns zeros past the array), and its body is skipped;
+// - the body receives rv * s without the zero-vector select of pair_grad: for a cut-off pair (s = 0) each product is
+//   a signed zero, and the bodies only add such terms to (or subtract them from) sums that start at +0, which leaves
+//   every sum unchanged (x + -0 = x, +0 - +0 = +0, and a sum that starts at +0 never becomes -0 in round to nearest).
+// The arithmetic and its order are those of for_nbrs_g. Offsets stay below 2^32 for n < 2^28 (dfsph_create checks).
+constexpr int BUF_WORD3 = 0x00020000;  // gfx9 buffer resource word 3: untyped access, 32-bit data format
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), short(0), int(bytes), BUF_WORD3);
+}
+// a world's records of one per-particle array, addressed by world-local id
+struct WorldRecs {
+    __amdgpu_buffer_rsrc_t r;
+    template <class T>
+    __device__ __forceinline__ static WorldRecs of(const T* arr, int base, int n) {
+        return WorldRecs{buf_rsrc(arr + base, uint32_t(n - base) * uint32_t(sizeof(T)))};
+    }
+    __device__ __forceinline__ float4 f4(unsigned e) const {
+        return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, e * 16u, 0, 0));
+    }
+    __device__ __forceinline__ float2 f2(unsigned e) const {
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, e * 8u, 0, 0));
+    }
+    __device__ __forceinline__ float f1(unsigned e) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, e * 4u, 0, 0));
+    }
+};
+template <bool WIDE, class T, class LoadF, class BodyF>
+__device__ __forceinline__ void for_rows_g(const Ctx& c, int i, f3 xi, int cnt, bool cached, LoadF load,
+                                           BodyF body) {
+    const KC K = make_kc(c.h, c.kind);
+    const uint32_t n = uint32_t(c.n);
+    const unsigned o4 = 4u * unsigned(i), o8 = 8u * unsigned(i);
+    for (int k0 = 0; k0 < cnt; k0 += NB_U) {
+        unsigned e[NB_U];
+        {  // the batch's row quads
+            const __amdgpu_buffer_rsrc_t r16 = buf_rsrc(c.nbr + int64_t(k0) * n, 2u * NB_U * n);
+            const __amdgpu_buffer_rsrc_t r8 = buf_rsrc(c.nbr_hi + (WIDE ? int64_t(k0) * n : 0), NB_U * n);
+#pragma unroll
+            for (int q = 0; q < NB_U / 4; ++q) {
+                const uint2 v = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r16, o8, int(8u * q * n), 0));
+                e[4 * q] = v.x & 0xffffu;
+                e[4 * q + 1] = v.x >> 16;
+                e[4 * q + 2] = v.y & 0xffffu;
+                e[4 * q + 3] = v.y >> 16;
+                if (WIDE) {
+                    const unsigned hb = __builtin_amdgcn_raw_buffer_load_b32(r8, o4, int(4u * q * n), 0);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) e[4 * q + u] |= ((hb >> (8 * u)) & 0xffu) << 16;
+                }
+            }
+        }
+        float sc[NB_U];
+        if (cached) {
+            const __amdgpu_buffer_rsrc_t rg = buf_rsrc(c.pgrad + int64_t(k0) * n, 4u * NB_U * n);
+#pragma unroll
+            for (int u = 0; u < NB_U; ++u)
+                sc[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rg, o4, int(4u * u * n), 0));
+        }
+        T d[NB_U];
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u) d[u] = load(e[u]);
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u) {
+            if (k0 + u >= cnt) continue;
+            const f3 rv = xi - xyz(load_pos(d[u]));
+            body(d[u], rv * (cached ? sc[u] : kernel_grad_scale(rv, K)));
         }
     }
 }
@@ -703,6 +803,22 @@ __global__ void k_fill_u8(uint8_t* p, int64_t n) {
     for (; i < n; i += stride) p[i] = 0;
 }
 
+// The rows in the reference's layout, entry k of particle i at k n + i (neighbors.py:198-230), from the stored quads
+// (to_view) or back into them: the buffer interface's view of the rows.
+__global__ void k_rows_view(Ctx c, uint16_t* v16, uint8_t* v8, int to_view) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(c.cap) * c.n) return;
+    const int k = int(t / c.n), i = int(t % c.n);
+    const int64_t r = row_at(c, i, k);
+    if (to_view) {
+        v16[t] = c.nbr[r];
+        if (c.wide) v8[t] = c.nbr_hi[r];
+    } else {
+        c.nbr[r] = v16[t];
+        if (c.wide) c.nbr_hi[r] = v8[t];
+    }
+}
+
 // neighbors.py:160-195: rows of the lowest `cap` world-local ids within h (fp32 dot(d, d) < h^2), ascending;
 // overflow counted per world. Each thread keeps its row in LDS (column per thread: conflict-free), as uint16
 // world-local ids unless a world exceeds 65,536 particles (WIDE: 32-bit entries), so the narrow build holds
@@ -790,10 +906,15 @@ __device__ __forceinline__ void build_row(const Ctx& c, int i, int g, f3 xi, flo
 template <bool WIDE, int BB>
 __device__ __forceinline__ void write_row(const Ctx& c, int i, int g, int nn, int total,
                                           const typename std::conditional<WIDE, int, uint16_t>::type* L) {
-    for (int k = 0; k < nn; ++k) {
-        int v = L[k * BB];
-        c.nbr[int64_t(k) * c.n + i] = uint16_t(v & 65535);
-        if (WIDE) c.nbr_hi[int64_t(k) * c.n + i] = uint8_t(v >> 16);
+    for (int k0 = 0; k0 < nn; k0 += 4) {  // whole quads; entries past nn are 0 (a valid world-local id)
+        unsigned v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = k0 + u < nn ? unsigned(L[(k0 + u) * BB]) : 0u;
+        const int64_t r = int64_t(k0 >> 2) * c.n + i;
+        reinterpret_cast<uint2*>(c.nbr)[r] = make_uint2((v[0] & 0xffffu) | (v[1] << 16), (v[2] & 0xffffu) | (v[3] << 16));
+        if (WIDE)
+            reinterpret_cast<unsigned*>(c.nbr_hi)[r] = (v[0] >> 16) | ((v[1] >> 16) << 8) | ((v[2] >> 16) << 16) |
+                                                       ((v[3] >> 16) << 24);
     }
     if (total > c.cap) atomicAdd(c.st_overflow + g, 1);
     c.count[i] = nn;
@@ -900,8 +1021,8 @@ __global__ void __launch_bounds__(256) k_skin_filter(Ctx c) {
         if (dot(dd, dd) < h2) {
             total += 1;
             if (n < c.cap) {
-                c.nbr[int64_t(n) * c.n + i] = uint16_t(jl & 65535);
-                if (WIDE) c.nbr_hi[int64_t(n) * c.n + i] = uint8_t(jl >> 16);
+                c.nbr[row_at(c, i, n)] = uint16_t(jl & 65535);
+                if (WIDE) c.nbr_hi[row_at(c, i, n)] = uint8_t(jl >> 16);
                 n += 1;
             }
         }
@@ -1329,8 +1450,9 @@ __global__ void __launch_bounds__(256) k_halo_build(Ctx c) {
     const int cnt = valid ? c.count[i] : 0;
     const int w = c.ch_world[b_];
     auto entry = [&](int k) {
-        int e = int((c.nbr + int64_t(k) * c.n)[i]);
-        if (WIDE) e += int((c.nbr_hi + int64_t(k) * c.n)[i]) << 16;
+        const int64_t r = row_at(c, i, k);
+        int e = int(c.nbr[r]);
+        if (WIDE) e += int(c.nbr_hi[r]) << 16;
         return e;
     };
     int lo = INT_MAX, hi = -1;
@@ -1699,21 +1821,22 @@ struct QKR {
 // With c.packed (one mass per world) each neighbour is one 16 B gather {x, y, z, kap / rho0} of the vector's packed
 // copy, the mass the world's; otherwise a position-and-mass gather plus the quotient's. The terms are the same floats.
 template <bool WIDE>
-__device__ f3 pressure_accel(const Ctx& c, const float* kap, const float* kapr, int slot, int i) {
+__device__ f3 pressure_accel(const Ctx& c, const float* kap, const float* kapr, int slot, int i, int w) {
     f3 xi = xyz(c.q4[i]);
     float ki = kap[i] / c.rho0[i];
     f3 a{0.0f, 0.0f, 0.0f};
     int cnt = c.count[i];
-    int base = c.ws[c.world[i]];
+    const int base = c.ws[w];
     const bool cached = (c.pair_cache & DFSPH_PAIR_CACHE_ACCEL) != 0;
     if (c.packed) {
-        const float mw = c.w_mass[c.world[i]];
-        const float4* qk = c.qk + int64_t(slot) * c.n;
-        for_nbrs_g<WIDE, float4>(c, i, xi, cnt, base, cached, [&](int j) { return qk[j]; },
-                                 [&](int, float4 d, f3 g) { a = a - (mw * (ki + d.w)) * g; });
+        const float mw = c.w_mass[w];
+        const WorldRecs qk = WorldRecs::of(c.qk + int64_t(slot) * c.n, base, c.n);
+        for_rows_g<WIDE, float4>(c, i, xi, cnt, cached, [&](unsigned e) { return qk.f4(e); },
+                                 [&](float4 d, f3 g) { a = a - (mw * (ki + d.w)) * g; });
     } else {
-        for_nbrs_g<WIDE, QKR>(c, i, xi, cnt, base, cached, [&](int j) { return QKR{c.q4[j], kapr[j]}; },
-                              [&](int, QKR d, f3 g) { a = a - (d.q.w * (ki + d.kr)) * g; });
+        const WorldRecs q4 = WorldRecs::of(c.q4, base, c.n), kr = WorldRecs::of(kapr, base, c.n);
+        for_rows_g<WIDE, QKR>(c, i, xi, cnt, cached, [&](unsigned e) { return QKR{q4.f4(e), kr.f1(e)}; },
+                              [&](QKR d, f3 g) { a = a - (d.q.w * (ki + d.kr)) * g; });
     }
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
@@ -1730,20 +1853,22 @@ struct QA2 {  // the packed record {x, y, z, acc.x}, {acc.y, acc.z}
     float2 a;
 };
 template <bool WIDE>
-__device__ float accel_divergence(const Ctx& c, int i) {
+__device__ float accel_divergence(const Ctx& c, int i, int w) {
     f3 xi = xyz(c.q4[i]), ai = acc_of(c, i);
     float inv_r0 = 1.0f / c.rho0[i];
     float ar = 0.0f;
     int cnt = c.count[i];
-    int base = c.ws[c.world[i]];
+    const int base = c.ws[w];
     const bool cached = (c.pair_cache & DFSPH_PAIR_CACHE_APPLY) != 0;
     if (c.packed) {  // one 16 B and one 8 B gather per neighbour; its mass is the world's
-        const float mw = c.w_mass[c.world[i]];
-        for_nbrs_g<WIDE, QA2>(c, i, xi, cnt, base, cached, [&](int j) { return QA2{c.qa[j], c.a2[j]}; },
-                              [&](int, QA2 d, f3 g) { ar += (mw * inv_r0) * dot(ai - f3{d.q.w, d.a.x, d.a.y}, g); });
+        const float mw = c.w_mass[w];
+        const WorldRecs qa = WorldRecs::of(c.qa, base, c.n), a2 = WorldRecs::of(c.a2, base, c.n);
+        for_rows_g<WIDE, QA2>(c, i, xi, cnt, cached, [&](unsigned e) { return QA2{qa.f4(e), a2.f2(e)}; },
+                              [&](QA2 d, f3 g) { ar += (mw * inv_r0) * dot(ai - f3{d.q.w, d.a.x, d.a.y}, g); });
     } else {
-        for_nbrs_g<WIDE, QA>(c, i, xi, cnt, base, cached, [&](int j) { return QA{c.q4[j], c.acc4[j]}; },
-                             [&](int, QA d, f3 g) { ar += (d.q.w * inv_r0) * dot(ai - xyz(d.a), g); });
+        const WorldRecs q4 = WorldRecs::of(c.q4, base, c.n), acc4 = WorldRecs::of(c.acc4, base, c.n);
+        for_rows_g<WIDE, QA>(c, i, xi, cnt, cached, [&](unsigned e) { return QA{q4.f4(e), acc4.f4(e)}; },
+                             [&](QA d, f3 g) { ar += (d.q.w * inv_r0) * dot(ai - xyz(d.a), g); });
     }
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
@@ -1819,7 +1944,7 @@ __device__ __forceinline__ void accel_chunk(const Ctx& c, int b_, const float* v
                 }
             }
             if (full) {
-                f3 a = pressure_accel<WIDE>(c, vec, vecr, slot, i);
+                f3 a = pressure_accel<WIDE>(c, vec, vecr, slot, i, c.ch_world[b_]);
                 put_acc(c, i, a.x, a.y, a.z);
                 nb = pass_bytes<WIDE>(c, i, 69u + (gated && c.sparse_corr ? 1u : 0u), 17u);
             }
@@ -1860,7 +1985,7 @@ __device__ __forceinline__ void apply_chunk(const Ctx& c, int b_, int mode, floa
             }
             if (full) {
                 float sc = mode == 0 ? dt : dt * dt;
-                float v = -sc * (accel_divergence<WIDE>(c, i) - c.reg[i] * vec[i]);
+                float v = -sc * (accel_divergence<WIDE>(c, i, c.ch_world[b_]) - c.reg[i] * vec[i]);
                 px = double(vec[i] * v);
                 if (bias) v -= (mode == 0 ? c.src[i] : c.src[i] - 1.0f);
                 if (accum) v += out[i];
@@ -1919,7 +2044,7 @@ __device__ __forceinline__ void accel_h(const Ctx& c, const float* vec, const fl
             if (full) {
                 f3 a;
                 if (hc < 0) {
-                    a = pressure_accel<WIDE>(c, vec, vecr, slot, i);
+                    a = pressure_accel<WIDE>(c, vec, vecr, slot, i, c.ch_world[b_]);
                 } else {  // pressure_accel over the staged records
                     f3 xi = xyz(c.q4[i]);
                     float ki = vec[i] / c.rho0[i];
@@ -1996,7 +2121,7 @@ __device__ __forceinline__ void apply_h(const Ctx& c, int mode, float dt, const 
             if (full) {
                 float div;
                 if (hc < 0) {
-                    div = accel_divergence<WIDE>(c, i);
+                    div = accel_divergence<WIDE>(c, i, c.ch_world[b_]);
                 } else {  // accel_divergence over the staged records
                     f3 xi = xyz(c.q4[i]), ai = acc_of(c, i);
                     float inv_r0 = 1.0f / c.rho0[i];
@@ -2280,7 +2405,7 @@ __global__ void __launch_bounds__(256) k_jacobi(Ctx c, int mode, float dt) {
         if (!is_fluid(c, i)) {
             c.err[i] = 0.0f;
         } else {
-            float ar = accel_divergence<WIDE>(c, i) - c.reg[i] * c.kappa[i];
+            float ar = accel_divergence<WIDE>(c, i, w) - c.reg[i] * c.kappa[i];
             float s, scale;
             if (mode == 0) {
                 ar *= dt;
@@ -2335,7 +2460,7 @@ __global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, f
         }
     }
     // apply_pressure_kernel
-    f3 a = pressure_accel<WIDE>(c, c.kappa, c.kr, VR_KAPPA, i);
+    f3 a = pressure_accel<WIDE>(c, c.kappa, c.kr, VR_KAPPA, i, c.world[i]);
     f3 v = V(c, i) + dt * a;
     float kap = c.kappa[i];
     float mk = c.mass[i] * kap * dt;
@@ -3564,6 +3689,8 @@ struct Handle {
     Ctx c;
     int n_chunks = 0, n_scan_blocks = 0;
     int64_t nbr_bytes = 0, nbr_hi_bytes = 0;
+    uint16_t* nbr_view = nullptr;  // the rows in the reference's layout [cap][n], made on request (rows_view)
+    uint8_t* nbr_hi_view = nullptr;
     int* h_unconv = nullptr;  // pinned [2]: unconverged-world counts of the last two iteration batches
     int* d_unconv = nullptr;  // the same memory as the device addresses it
     hipEvent_t ev_unconv[2] = {nullptr, nullptr};
@@ -4152,6 +4279,8 @@ const char* dfsph_last_error(void) { return g_err.c_str(); }
 int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphModel* M, void** out) {
     if (!P || !L || !M || !out) return fail(DFSPH_EINVAL, "dfsph_create: null argument");
     if (L->n < 0 || L->world_count < 1) return fail(DFSPH_EINVAL, "dfsph_create: need n >= 0 and world_count >= 1");
+    if (L->n >= (int64_t(1) << 28))  // the solve gathers' 32-bit buffer offsets (4 B entries of NB_U rows)
+        return fail(DFSPH_EINVAL, "dfsph_create: at most 2^28 - 1 particles per handle");
     if (P->neighbor_capacity < 8 || P->neighbor_capacity > 64)
         return fail(DFSPH_EINVAL, "neighbor_capacity must be in [8, 64]");
     if (!(P->h > 0.0)) return fail(DFSPH_EINVAL, "h must be > 0");
@@ -4335,8 +4464,9 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         // ROW_PAD zero rows past the capacity: a neighbour batch reads them unconditionally (nbr_batch), and the
         // staged loops prefetch the next batch's entries (for_slots_g)
         const int64_t rows = int64_t(c.cap) + ROW_PAD;
-        TRY(dalloc(H, &c.nbr, rows * std::max(n, 1)));
-        TRY(dalloc(H, &c.nbr_hi, c.wide ? rows * std::max(n, 1) : 1));
+        const int64_t quad_rows = (rows + 3) / 4 * 4;  // the neighbour rows are stored in quads (row_at)
+        TRY(dalloc(H, &c.nbr, quad_rows * std::max(n, 1)));
+        TRY(dalloc(H, &c.nbr_hi, c.wide ? quad_rows * std::max(n, 1) : 4));
         TRY(dalloc(H, &c.pgrad, rows * std::max(n, 1)));
         TRY(dalloc(H, &c.h_slot, rows * std::max(n, 1)));
         TRY(dalloc(H, &c.h_ids, int64_t(std::max(H->n_chunks, 1)) * HALO_MAX));
@@ -4856,6 +4986,22 @@ int32_t dfsph_profile_read(void* h, int32_t kid, int32_t* count, double* total_m
     return DFSPH_OK;
 }
 
+// the rows' view in the reference's layout (allocated on the first request), refreshed from the quads (to_view) or
+// written back into them
+static int rows_view(Handle* H, bool to_view) {
+    Ctx& c = H->c;
+    const int64_t count = int64_t(c.cap) * std::max(c.n, 1);
+    if (!H->nbr_view) {
+        if (int rc = dalloc(H, &H->nbr_view, count)) return rc;
+        if (int rc = dalloc(H, &H->nbr_hi_view, c.wide ? count : 1)) return rc;
+    }
+    HIPCHK(hipDeviceSynchronize());
+    k_rows_view<<<blocks_for(int64_t(c.cap) * c.n), 256>>>(c, H->nbr_view, H->nbr_hi_view, to_view ? 1 : 0);
+    LAUNCH_CHECK();
+    HIPCHK(hipDeviceSynchronize());
+    return DFSPH_OK;
+}
+
 int32_t dfsph_get_buffer(void* h, int32_t which, void** ptr, int64_t* nb) {
     if (!h || !ptr || !nb) return fail(DFSPH_EINVAL, "null argument");
     Handle* H = as_handle(h);
@@ -4868,8 +5014,12 @@ int32_t dfsph_get_buffer(void* h, int32_t which, void** ptr, int64_t* nb) {
         return DFSPH_OK;              \
     }
     switch (which) {
-        case DFSPH_BUF_NBR: B(c.nbr, H->nbr_bytes)
-        case DFSPH_BUF_NBR_HI: B(c.nbr_hi, H->nbr_hi_bytes)
+        case DFSPH_BUF_NBR:
+            if (int rc = rows_view(H, true)) return rc;
+            B(H->nbr_view, H->nbr_bytes)
+        case DFSPH_BUF_NBR_HI:
+            if (int rc = rows_view(H, true)) return rc;
+            B(H->nbr_hi_view, H->nbr_hi_bytes)
         case DFSPH_BUF_COUNT: B(c.count, 4 * n)
         case DFSPH_BUF_GROUP: B(c.group, 4 * n)
         case DFSPH_BUF_SLOT_SHAPE: B(c.s_shape, 16 * n)
@@ -4981,6 +5131,7 @@ int32_t dfsph_write_buffer(void* h, int32_t which, const void* src, int64_t nb) 
         LAUNCH_CHECK();
         HIPCHK(hipDeviceSynchronize());
     }
+    if (which == DFSPH_BUF_NBR || which == DFSPH_BUF_NBR_HI) return rows_view(as_handle(h), false);  // into the quads
     return DFSPH_OK;
 }
 

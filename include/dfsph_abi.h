@@ -93,7 +93,7 @@ typedef struct DfsphParams {
 
 /* Static description of the particle pack and its walls (layout.py:47-72, boundary.py:79-95). */
 typedef struct DfsphLayout {
-    int32_t n;                       /* particles in the pack (liquid, spares, wall particles, others) */
+    int32_t n;                       /* particles in the pack (liquid, spares, wall particles, others), < 2^28 */
     int32_t world_count;             /* W */
     const int32_t* world_start;      /* host [W+1], contiguous world ranges */
     /* per-world cell grid of the neighbour search (cell side = h * (1 + 1e-5)), host arrays */
