@@ -205,9 +205,8 @@ __device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
 __device__ __forceinline__ f3 V(const Ctx& c, int i) { return ld3(c.qd, i); }
 
 // Row storage in quads: entry k of particle i sits at ((k / 4) n + i) 4 + k % 4, so the four entries of a
-// neighbour batch are one 8 B load per lane (4 B for the high bytes), coalesced across the wave. On the texture data
-// path a 2 B load per lane costs as much as a 16 B one, so four separate entry loads cost about as much as the four
-// records they name. dfsph_get_buffer presents the rows in the reference's layout [k][n] (rows_view).
+// neighbour batch are one 8 B load per lane (4 B for the high bytes), coalesced across the wave, instead of four 2 B
+// loads. dfsph_get_buffer presents the rows in the reference's layout [k][n] (rows_view).
 __device__ __forceinline__ int64_t row_at(const Ctx& c, int i, int k) {
     return ((int64_t(k >> 2) * c.n + i) << 2) + (k & 3);
 }
@@ -219,11 +218,10 @@ __device__ __forceinline__ int nbr_at(const Ctx& c, int base, int i, int k) {
     return j;
 }
 
-// Neighbour loops with staged loads: NB_U row entries, then the NB_U neighbour records (load(j)), then the
-// arithmetic in ascending k, exactly the sequential order. The loads of a batch are in flight together instead of
-// one dependent index -> record round trip per neighbour. Row k is addressed as a wave-uniform row base plus the
-// lane's i; slots past the lane's count (still inside the allocated rows) read a stale entry that is replaced by
-// i, so every gathered record is in bounds, and their bodies are skipped.
+// Neighbour loops with staged loads: NB_U row entries, then the NB_U neighbour records, then the arithmetic in
+// ascending k, exactly the sequential order. The loads of a batch are in flight together instead of one dependent
+// index -> record round trip per neighbour; entries past the lane's count are read too (stale world-local ids of the
+// same world, or the zero padding rows) and their bodies are skipped.
 #ifndef DFSPH_NB_U
 #define DFSPH_NB_U 4  // neighbour records in flight per lane (build-time tuning; 4 measured fastest of 1..16)
 #endif
@@ -285,67 +283,33 @@ __device__ __forceinline__ void nbr_batch(const Ctx& c, int i, int cnt, int base
 #pragma unroll
     for (int u = 0; u < NB_U; ++u) j[u] = k0 + u < cnt ? base + int(e[u]) : i;
 }
-// body(k, j, record)
-template <bool WIDE, class T, class LoadF, class BodyF>
-__device__ __forceinline__ void for_nbrs(const Ctx& c, int i, int cnt, int base, LoadF load, BodyF body) {
-    for (int k0 = 0; k0 < cnt; k0 += NB_U) {
-        int j[NB_U];
-        nbr_batch<WIDE>(c, i, cnt, base, k0, j);
-        T d[NB_U];
-#pragma unroll
-        for (int u = 0; u < NB_U; ++u) d[u] = load(j[u]);
-#pragma unroll
-        for (int u = 0; u < NB_U; ++u)
-            if (k0 + u < cnt) body(k0 + u, j[u], d[u]);
-    }
-}
-// body(j, record, grad W_ij): with `cached` the gradient comes from the pair's scale stored by the density pass
-// (pair_grad), otherwise it is evaluated (kernel_grad); both give the same bits, so the choice is a tuning option.
-template <bool WIDE, class T, class LoadF, class BodyF>
-__device__ __forceinline__ void for_nbrs_g(const Ctx& c, int i, f3 xi, int cnt, int base, bool cached, LoadF load,
-                                           BodyF body) {
-    const KC K = make_kc(c.h, c.kind);
-    for (int k0 = 0; k0 < cnt; k0 += NB_U) {
-        int j[NB_U];
-        nbr_batch<WIDE>(c, i, cnt, base, k0, j);
-        T d[NB_U];
-        float sc[NB_U];
-#pragma unroll
-        for (int u = 0; u < NB_U; ++u) {
-            d[u] = load(j[u]);
-            sc[u] = cached ? (c.pgrad + int64_t(k0 + u) * c.n)[i] : 0.0f;  // padded like the rows
-        }
-#pragma unroll
-        for (int u = 0; u < NB_U; ++u) {
-            if (k0 + u >= cnt) continue;
-            f3 rv = xi - xyz(load_pos(d[u]));
-            body(j[u], d[u], cached ? pair_grad(rv, sc[u]) : kernel_grad(rv, K));
-        }
-    }
-}
-
-// The pair loop of the solve gathers (pressure acceleration and operator apply), with the addressing taken off the
-// vector ALU, which these passes saturate:
+// The neighbour loops of the per-particle passes (density, source terms, XSPH, the solves' acceleration and operator
+// apply), with the addressing taken off the vector ALU, which the solve passes saturate:
 // - the row entries (and cached scales) come through buffer loads whose row offset is a scalar and whose lane offset
 //   (i times the entry size) is computed once, instead of a 64-bit address per entry;
 // - the records are gathered through buffer loads on a per-world resource (the caller's load(e), see WorldRecs) with
-//   the world-local entry times the record size as the lane offset: one shift per record; a lane past its count
-//   gathers the record its stale entry names (the rows hold world-local ids of the same world, zero when never
-//   written, and the resource's bound returns zeros past the array), and its body is skipped;
-// - the body receives rv * s without the zero-vector select of pair_grad: for a cut-off pair (s = 0) each product is
-//   a signed zero, and the bodies only add such terms to (or subtract them from) sums that start at +0, which leaves
-//   every sum unchanged (x + -0 = x, +0 - +0 = +0, and a sum that starts at +0 never becomes -0 in round to nearest).
-// The arithmetic and its order are those of for_nbrs_g. Offsets stay below 2^32 for n < 2^28 (dfsph_create checks).
+//   the world-local entry times the record size as the lane offset: one shift per record; the caller's block holds
+//   one chunk, so the world and the resource are uniform; a lane past its count gathers the record its stale entry
+//   names (the rows hold world-local ids of the same world, zero when never written, and the resource's bound returns
+//   zeros past the array), and its body is skipped;
+// - for_rows_g's body receives rv * s without the zero-vector select of pair_grad: for a cut-off pair (s = 0) each
+//   product is a signed zero, and the bodies only add such terms to (or subtract them from) sums that start at +0,
+//   which leaves every sum unchanged (x + -0 = x, +0 - +0 = +0, and a sum that starts at +0 never becomes -0 in round
+//   to nearest); a body that needs the exact zero vector forms pair_grad itself.
+// Bodies run in ascending k, the reference's sequential order. Offsets stay below 2^32 for n < 2^28 (dfsph_create
+// checks).
 constexpr int BUF_WORD3 = 0x00020000;  // gfx9 buffer resource word 3: untyped access, 32-bit data format
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), short(0), int(bytes), BUF_WORD3);
 }
-// a world's records of one per-particle array, addressed by world-local id
+// a world's records of one per-particle array, addressed by world-local id; `base` (the world's first particle) is
+// uniform over the wave, which readfirstlane tells the compiler, so the resource lives in scalar registers
 struct WorldRecs {
     __amdgpu_buffer_rsrc_t r;
     template <class T>
     __device__ __forceinline__ static WorldRecs of(const T* arr, int base, int n) {
-        return WorldRecs{buf_rsrc(arr + base, uint32_t(n - base) * uint32_t(sizeof(T)))};
+        const int b = __builtin_amdgcn_readfirstlane(base);
+        return WorldRecs{buf_rsrc(arr + b, uint32_t(n - b) * uint32_t(sizeof(T)))};
     }
     __device__ __forceinline__ float4 f4(unsigned e) const {
         return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, e * 16u, 0, 0));
@@ -356,38 +320,66 @@ struct WorldRecs {
     __device__ __forceinline__ float f1(unsigned e) const {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, e * 4u, 0, 0));
     }
+    // a 12 B record (x, y, z), as 8 + 4 B (the compiler merges them into one 12 B load; its 96-bit load builtin
+    // returned the first word for the third in a probe, so it is not used)
+    __device__ __forceinline__ f3 v3(unsigned e) const {
+        const uint2 a = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, e * 12u, 0, 0));
+        const unsigned b = __builtin_amdgcn_raw_buffer_load_b32(r, e * 12u + 8u, 0, 0);
+        return f3{__builtin_bit_cast(float, a.x), __builtin_bit_cast(float, a.y), __builtin_bit_cast(float, b)};
+    }
 };
+// the NB_U world-local entries k0.. (a multiple of 4) of particle i's row: one buffer load per row quad (plus one for
+// the high bytes), row offsets scalar, the lane offset 8 i
+template <bool WIDE>
+__device__ __forceinline__ void row_batch(const Ctx& c, int i, int k0, unsigned (&e)[NB_U]) {
+    const uint32_t n = uint32_t(c.n);
+    const __amdgpu_buffer_rsrc_t r16 = buf_rsrc(c.nbr + int64_t(k0) * n, 2u * NB_U * n);
+    const __amdgpu_buffer_rsrc_t r8 = buf_rsrc(c.nbr_hi + (WIDE ? int64_t(k0) * n : 0), NB_U * n);
+#pragma unroll
+    for (int q = 0; q < NB_U / 4; ++q) {
+        const uint2 v = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r16, 8u * unsigned(i),
+                                                                                        int(8u * q * n), 0));
+        e[4 * q] = v.x & 0xffffu;
+        e[4 * q + 1] = v.x >> 16;
+        e[4 * q + 2] = v.y & 0xffffu;
+        e[4 * q + 3] = v.y >> 16;
+        if (WIDE) {
+            const unsigned hb = __builtin_amdgcn_raw_buffer_load_b32(r8, 4u * unsigned(i), int(4u * q * n), 0);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) e[4 * q + u] |= ((hb >> (8 * u)) & 0xffu) << 16;
+        }
+    }
+}
+// body(k, record) in ascending k over the world's records load(e)
+template <bool WIDE, class T, class LoadF, class BodyF>
+__device__ __forceinline__ void for_rows(const Ctx& c, int i, int cnt, LoadF load, BodyF body) {
+    for (int k0 = 0; k0 < cnt; k0 += NB_U) {
+        unsigned e[NB_U];
+        row_batch<WIDE>(c, i, k0, e);
+        T d[NB_U];
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u) d[u] = load(e[u]);
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u)
+            if (k0 + u < cnt) body(k0 + u, d[u]);
+    }
+}
+// body(record, grad W_ij) in ascending k, the gradient from the pair cache (cached) or evaluated
 template <bool WIDE, class T, class LoadF, class BodyF>
 __device__ __forceinline__ void for_rows_g(const Ctx& c, int i, f3 xi, int cnt, bool cached, LoadF load,
                                            BodyF body) {
     const KC K = make_kc(c.h, c.kind);
     const uint32_t n = uint32_t(c.n);
-    const unsigned o4 = 4u * unsigned(i), o8 = 8u * unsigned(i);
     for (int k0 = 0; k0 < cnt; k0 += NB_U) {
         unsigned e[NB_U];
-        {  // the batch's row quads
-            const __amdgpu_buffer_rsrc_t r16 = buf_rsrc(c.nbr + int64_t(k0) * n, 2u * NB_U * n);
-            const __amdgpu_buffer_rsrc_t r8 = buf_rsrc(c.nbr_hi + (WIDE ? int64_t(k0) * n : 0), NB_U * n);
-#pragma unroll
-            for (int q = 0; q < NB_U / 4; ++q) {
-                const uint2 v = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r16, o8, int(8u * q * n), 0));
-                e[4 * q] = v.x & 0xffffu;
-                e[4 * q + 1] = v.x >> 16;
-                e[4 * q + 2] = v.y & 0xffffu;
-                e[4 * q + 3] = v.y >> 16;
-                if (WIDE) {
-                    const unsigned hb = __builtin_amdgcn_raw_buffer_load_b32(r8, o4, int(4u * q * n), 0);
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) e[4 * q + u] |= ((hb >> (8 * u)) & 0xffu) << 16;
-                }
-            }
-        }
+        row_batch<WIDE>(c, i, k0, e);
         float sc[NB_U];
         if (cached) {
             const __amdgpu_buffer_rsrc_t rg = buf_rsrc(c.pgrad + int64_t(k0) * n, 4u * NB_U * n);
 #pragma unroll
             for (int u = 0; u < NB_U; ++u)
-                sc[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rg, o4, int(4u * u * n), 0));
+                sc[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rg, 4u * unsigned(i),
+                                                                                       int(4u * u * n), 0));
         }
         T d[NB_U];
 #pragma unroll
@@ -1531,7 +1523,7 @@ __global__ void __launch_bounds__(256) k_halo_build(Ctx c) {
 }
 
 // Neighbour loop over a staged halo: the rows of halo indices (h_slot, padded like the rows) and the records in LDS
-// (load(s), s < h_count); otherwise as for_nbrs_g: body(record, grad W_ij) in ascending k.
+// (load(s), s < h_count); otherwise as for_rows_g: body(record, grad W_ij) in ascending k.
 // SU entries per batch (8 measured slower than 4: more registers, fewer waves); the next batch's halo indices and
 // cached scales are loaded before the current batch's arithmetic (the rows hold ROW_PAD >= 2 SU padding rows), so
 // their memory latency overlaps it.
@@ -1604,7 +1596,7 @@ struct QVd {
     f3 v;
 };
 template <bool WIDE, bool FEAT>
-__device__ __forceinline__ void density_alpha_one(const Ctx& c, const KC& K, int i, float dt) {
+__device__ __forceinline__ void density_alpha_one(const Ctx& c, const KC& K, int i, int w, float dt) {
     float4 qi = c.q4[i];
     f3 xi = xyz(qi);
     float mi = qi.w;
@@ -1614,10 +1606,11 @@ __device__ __forceinline__ void density_alpha_one(const Ctx& c, const KC& K, int
     float sq = 0.0f, rate = 0.0f;
     if (FEAT) vi = V(c, i);
     int cnt = c.count[i];
-    int base = c.ws[c.world[i]];
-    for_nbrs<WIDE, QVd>(
-        c, i, cnt, base, [&](int j) { return QVd{c.q4[j], FEAT ? V(c, j) : f3{0.0f, 0.0f, 0.0f}}; },
-        [&](int k, int, QVd d) {
+    const int base = c.ws[w];
+    const WorldRecs q4 = WorldRecs::of(c.q4, base, c.n), qd = WorldRecs::of(reinterpret_cast<const f3*>(c.qd), base, c.n);
+    for_rows<WIDE, QVd>(
+        c, i, cnt, [&](unsigned e) { return QVd{q4.f4(e), FEAT ? qd.v3(e) : f3{0.0f, 0.0f, 0.0f}}; },
+        [&](int k, QVd d) {
             f3 rv = xi - xyz(d.q);
             float mj = d.q.w;
             rho += mj * kernel_w(dist(rv, K), K);
@@ -1652,7 +1645,7 @@ __device__ __forceinline__ void density_alpha_one(const Ctx& c, const KC& K, int
     c.active[i] = 1.0f;
     if (FEAT) {
         float* F = c.feat + 16 * int64_t(i);
-        f3 gw = ld3(c.gravity, c.world[i]);
+        f3 gw = ld3(c.gravity, w);
         float gl = length(gw);
         f3 gh = gl > 0.0f ? gw / gl : f3{0.0f, 0.0f, -1.0f};
         float ks = c.kscale;
@@ -1696,7 +1689,7 @@ __global__ void __launch_bounds__(256) k_density_alpha(Ctx c) {
         if (!is_fluid(c, i)) {
             density_alpha_other<false>(c, i);
         } else {
-            density_alpha_one<WIDE, false>(c, K, i, 0.0f);
+            density_alpha_one<WIDE, false>(c, K, i, c.ch_world[b_], 0.0f);
             act_v = 1.0;
         }
     }
@@ -1712,7 +1705,7 @@ __global__ void __launch_bounds__(256) k_density_features(Ctx c, float dt) {
         if (!is_fluid(c, i)) {
             density_alpha_other<true>(c, i);
         } else {
-            density_alpha_one<WIDE, true>(c, K, i, dt);
+            density_alpha_one<WIDE, true>(c, K, i, c.ch_world[b_], dt);
             act_v = 1.0;
         }
     }
@@ -1724,16 +1717,18 @@ struct QV {
     float4 q;
     f3 v;
 };
+// w: the particle's world, block-uniform (the caller runs one chunk per block)
 template <bool WIDE>
-__device__ float density_rate(const Ctx& c, int i) {
+__device__ float density_rate(const Ctx& c, int i, int w) {
     f3 xi = xyz(c.q4[i]), vi = V(c, i);
     float inv_r0 = 1.0f / c.rho0[i];
     float rate = 0.0f;
     int cnt = c.count[i];
-    int base = c.ws[c.world[i]];
-    for_nbrs_g<WIDE, QV>(c, i, xi, cnt, base, (c.pair_cache & DFSPH_PAIR_CACHE_OTHER) != 0,
-                         [&](int j) { return QV{c.q4[j], V(c, j)}; },
-                         [&](int, QV d, f3 g) { rate += (d.q.w * inv_r0) * dot(vi - d.v, g); });
+    const int base = c.ws[w];
+    const WorldRecs q4 = WorldRecs::of(c.q4, base, c.n), qd = WorldRecs::of(reinterpret_cast<const f3*>(c.qd), base, c.n);
+    for_rows_g<WIDE, QV>(c, i, xi, cnt, (c.pair_cache & DFSPH_PAIR_CACHE_OTHER) != 0,
+                         [&](unsigned e) { return QV{q4.f4(e), qd.v3(e)}; },
+                         [&](QV d, f3 g) { rate += (d.q.w * inv_r0) * dot(vi - d.v, g); });
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
         if (c.s_shape[b4 + b] >= 0) rate += dot(vi - ld3(c.s_vb, b4 + b), ld3(c.s_gpsi, b4 + b));
@@ -1745,8 +1740,8 @@ template <bool WIDE>
 __global__ void __launch_bounds__(256) k_solve_init(Ctx c, int mode, float dt, float warm, float warm_den,
                                                     float relief, const float* store, const float* learned) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *c.n_unconv = 0;
-    int i = xcd_particle();
-    if (i >= c.n) return;
+    CHUNK_PROLOGUE
+    if (!valid) return;
     if (!is_fluid(c, i)) {
         c.src[i] = 0.0f;
         c.kappa[i] = 0.0f;
@@ -1754,7 +1749,7 @@ __global__ void __launch_bounds__(256) k_solve_init(Ctx c, int mode, float dt, f
         c.lower[i] = 0.0f;
         return;
     }
-    float rate = density_rate<WIDE>(c, i);
+    float rate = density_rate<WIDE>(c, i, c.ch_world[b_]);
     if (mode == 0) {
         float kdv = 0.0f;
         if (c.two_sided) kdv = fmx(warm_den * fmn(c.kd[i], 2.5e-4f) / (dt * dt), 0.0f);
@@ -2439,8 +2434,8 @@ __global__ void __launch_bounds__(256) k_jacobi(Ctx c, int mode, float dt) {
 // accelerations read kappa changes nothing for them.
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, float* store) {
-    int i = xcd_particle();
-    if (i >= c.n || !is_fluid(c, i)) return;
+    CHUNK_PROLOGUE
+    if (!valid || !is_fluid(c, i)) return;
     int64_t b4 = 4 * int64_t(i);
     bool lone = c.count[i] == 0;
     if (lone && c.kappa[i] > 0.0f) {  // contact_limit_kernel
@@ -2460,7 +2455,7 @@ __global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, f
         }
     }
     // apply_pressure_kernel
-    f3 a = pressure_accel<WIDE>(c, c.kappa, c.kr, VR_KAPPA, i, c.world[i]);
+    f3 a = pressure_accel<WIDE>(c, c.kappa, c.kr, VR_KAPPA, i, c.ch_world[b_]);
     f3 v = V(c, i) + dt * a;
     float kap = c.kappa[i];
     float mk = c.mass[i] * kap * dt;
@@ -2579,14 +2574,14 @@ struct XS {
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
     const KC K = make_kc(c.h, c.kind);
-    int i = xcd_particle();
-    if (i >= c.n) return;
+    CHUNK_PROLOGUE
+    if (!valid) return;
     f3 vi = V(c, i);
     if (!is_fluid(c, i)) {
         st3(c.vtmp, i, vi);
         return;
     }
-    int w = c.world[i];
+    const int w = c.ch_world[b_];
     f3 a = ld3(c.gravity, w);
     if (c.f) a = a + ld3(c.f, i) / c.mass[i];
     f3 v = vi + dt * a;
@@ -2595,10 +2590,13 @@ __global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
         f3 cr{0.0f, 0.0f, 0.0f};
         float rho_i = c.ratio[i] * c.rho0[i];
         int cnt = c.count[i];
-        int base = c.ws[w];
-        for_nbrs<WIDE, XS>(
-            c, i, cnt, base, [&](int j) { return XS{c.q4[j], V(c, j), c.ratio[j], c.rho0[j]}; },
-            [&](int, int, XS d) {
+        const int base = c.ws[w];
+        const WorldRecs q4 = WorldRecs::of(c.q4, base, c.n), ratio = WorldRecs::of(c.ratio, base, c.n),
+                        rho0 = WorldRecs::of(c.rho0, base, c.n),
+                        qd = WorldRecs::of(reinterpret_cast<const f3*>(c.qd), base, c.n);
+        for_rows<WIDE, XS>(
+            c, i, cnt, [&](unsigned e) { return XS{q4.f4(e), qd.v3(e), ratio.f1(e), rho0.f1(e)}; },
+            [&](int, XS d) {
                 float rho_bar = 0.5f * (rho_i + d.ratio * d.rho0);
                 if (rho_bar > 0.0f) {
                     float wk = kernel_w(dist(xi - xyz(d.q), K), K);
@@ -3987,7 +3985,7 @@ int enq_solve_head(Handle* H, const SolveSetup& S, hipStream_t st) {
                                                       float(P.rest_jitter_flow_ratio), float(P.max_density_error),
                                                       float(std::min(P.min_density_error, P.max_density_error)));
     }
-    LAUNCH_WT(KID_SOLVE_INIT, k_solve_init, blocks_for(c.n), 256, c, S.mode, S.dt, S.warm,
+    LAUNCH_WT(KID_SOLVE_INIT, k_solve_init, H->n_chunks, 256, c, S.mode, S.dt, S.warm,
               float(P.warm_start_density), S.relief, (const float*)S.store, S.learned);
     k_begin_solve<<<blocks_for(c.W), 256, 0, st>>>(c);
     if (P.pressure_solver == 0) {
@@ -4029,7 +4027,7 @@ int enq_solve_batch(Handle* H, const SolveSetup& S, hipStream_t st, int* flag_ou
 // contact limit, apply pressure, lone friction; stats (solve_driver.py:151-162)
 int enq_solve_tail(Handle* H, const SolveSetup& S, hipStream_t st) {
     const Ctx& c = S.c;
-    LAUNCH_WT(KID_POST, k_post_solve, blocks_for(c.n), 256, c, S.mode, S.dt, S.store);
+    LAUNCH_WT(KID_POST, k_post_solve, H->n_chunks, 256, c, S.mode, S.dt, S.store);
     k_record<<<blocks_for(c.W), 256, 0, st>>>(c, S.mode, float(S.tolh));
     LAUNCH_CHECK();
     return DFSPH_OK;
@@ -4169,7 +4167,7 @@ int run_csf(Handle* H, float dt, hipStream_t st) {
 int phase_nonpressure(Handle* H, double dt, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
-    LAUNCH_WT(KID_NONPRESSURE, k_nonpressure, blocks_for(c.n), 256, c, float(dt));
+    LAUNCH_WT(KID_NONPRESSURE, k_nonpressure, H->n_chunks, 256, c, float(dt));
     k_copy_f32<<<blocks_for(3 * int64_t(c.n)), 256, 0, st>>>(c.qd, c.vtmp, 3 * int64_t(c.n));
     LAUNCH_CHECK();
     return run_csf(H, float(dt), st);  // solver.py:304-305: surface tension after the non-pressure update
