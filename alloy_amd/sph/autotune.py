@@ -24,6 +24,7 @@ from . import _abi
 # (option, candidate values) in the order they are tuned; the first value of each is the library default
 CANDIDATES = (
     (_abi.OPT_CHECK_INTERVAL, (1, 2, 3)),
+    (_abi.OPT_BUILD_STAGED, (0, 1)),
     (_abi.OPT_BUILD_BLOCK, (128, 64, 256)),
     (_abi.OPT_SPARSE_CORRECTION, (1, 0)),
     (_abi.OPT_PACKED_ACCEL, (1, 0)),
@@ -35,7 +36,7 @@ CANDIDATES = (
 NAMES = {_abi.OPT_CHECK_INTERVAL: "check_interval", _abi.OPT_BUILD_BLOCK: "build_block",
          _abi.OPT_SPARSE_CORRECTION: "sparse_correction", _abi.OPT_PAIR_CACHE: "pair_cache", _abi.OPT_GRAPHS: "graphs",
          _abi.OPT_LEARNED_SOLVES: "learned_solves", _abi.OPT_PACKED_ACCEL: "packed_accel", _abi.OPT_HALO: "halo",
-         _abi.OPT_DEVICE_LOOP: "device_loop"}
+         _abi.OPT_DEVICE_LOOP: "device_loop", _abi.OPT_BUILD_STAGED: "build_staged"}
 
 
 class _Snapshot:

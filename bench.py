@@ -50,7 +50,7 @@ WORKLOADS = {  # steps: the timed steps each workload is specified with
 }
 CPU_SAMPLE_PARTICLE_STEPS = 3.0e6  # about 20 s of the serial oracle
 OPTION_IDS = {"check_interval": 1, "graphs": 2, "pair_cache": 3, "learned_solves": 4, "sparse_correction": 5,
-              "build_block": 6, "packed_accel": 7, "halo": 8, "device_loop": 9}
+              "build_block": 6, "packed_accel": 7, "halo": 8, "device_loop": 9, "build_staged": 10}
 
 
 def make_scene(wl: dict, device: str, subset=None):
@@ -323,7 +323,7 @@ def options_string(options: dict) -> str:
 
 # options that leave the roofline window's k_accel / k_apply launches unchanged: the list build's block size, and the
 # graph and device-loop paths, which per-launch timing switches off
-TRAFFIC_NEUTRAL = ("build_block", "device_loop", "graphs")
+TRAFFIC_NEUTRAL = ("build_block", "build_staged", "device_loop", "graphs")
 
 
 def _traffic_options(options: str) -> str:

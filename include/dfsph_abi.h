@@ -278,7 +278,7 @@ int32_t dfsph_set_warm_start_model(void* handle, const void* weights, void* stre
 int32_t dfsph_set_warm_start_kappa(void* handle, const float* kappa_div, const float* kappa_den);
 /* Host convergence check every `every` iterations (1..64); converged worlds' blocks exit at once in between. */
 int32_t dfsph_set_check_interval(void* handle, int32_t every);
-/* Tuning options. Options 1-3, 5 and 6 give bit-identical results and only trade time; option 4 chooses which
+/* Tuning options. Every option except 4 gives bit-identical results and only trades time; option 4 chooses which
  * pressure solves start from the learned warm start (both still iterate to the same tolerance). */
 enum DfsphOption {
     DFSPH_OPT_CHECK_INTERVAL = 1, /* as dfsph_set_check_interval */
@@ -301,11 +301,15 @@ enum DfsphOption {
                                         (its halo, up to 1,536), and the pressure acceleration and operator apply
                                         stage the halo's records in LDS and gather from there; chunks whose halo does
                                         not fit gather directly. 0: every chunk gathers directly. Bit-identical. */
-    DFSPH_OPT_DEVICE_LOOP = 9        /* 1: the projected-CG iterations of each solve run as one cooperative launch
+    DFSPH_OPT_DEVICE_LOOP = 9,       /* 1: the projected-CG iterations of each solve run as one cooperative launch
                                         that loops on the device until every world's stop test passed (grid barriers
                                         between the phases, direct gathers), so a step makes no host round trip;
                                         0: iteration batches with a host check of the unconverged-world count.
                                         Jacobi and profiled steps use the batches. Bit-identical. */
+    DFSPH_OPT_BUILD_STAGED = 10      /* 1: the plain neighbour-list build stages each block's box of grid cells in
+                                        LDS (64 or 128 threads by DFSPH_OPT_BUILD_BLOCK, large boxes in z-slabs) and
+                                        every lane walks its 27 cells there; 0 (default): every lane walks its cells
+                                        from global memory. Bit-identical. */
 };
 enum DfsphPairCache {
     DFSPH_PAIR_CACHE_ACCEL = 1, /* pressure acceleration (k_accel, post-solve) */

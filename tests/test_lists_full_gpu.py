@@ -64,6 +64,10 @@ def test_dam_break_125k_lists_every_50_steps():
             _compare_lists(s, o, st, cap, True, f"step {k}")
         if k < 500:
             s.step(st, st, None, None, DT)
+    from alloy_amd.sph import _abi
+
+    s.set_option(_abi.OPT_BUILD_STAGED, 1)  # the LDS-staged build of the 24-bit rows: the same rows
+    _compare_lists(s, o, st, cap, True, "step 500, staged build")
 
 
 def test_bowl_worlds_512_lists_after_100_steps():
@@ -78,3 +82,10 @@ def test_bowl_worlds_512_lists_after_100_steps():
         s.step(st, st, None, None, DT)
     o = OracleDFSPH(sc.model, sc.config, 32)
     _compare_lists(s, o, st, sc.config.neighbor_capacity, False, "step 100")
+    # the LDS-staged build (its boxes staged whole and in z-slabs) on the same state: the same rows
+    from alloy_amd.sph import _abi
+
+    for block in (64, 128):
+        s.set_option(_abi.OPT_BUILD_STAGED, 1)
+        s.set_option(_abi.OPT_BUILD_BLOCK, block)
+        _compare_lists(s, o, st, sc.config.neighbor_capacity, False, f"step 100, staged build, {block} threads")

@@ -51,6 +51,7 @@ def test_options_are_bit_identical(scene_name):
                  {_abi.OPT_CHECK_INTERVAL: 3}, {_abi.OPT_GRAPHS: 1, _abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ACCEL},
                  {_abi.OPT_SPARSE_CORRECTION: 0}, {_abi.OPT_SPARSE_CORRECTION: 0, _abi.OPT_GRAPHS: 1},
                  {_abi.OPT_BUILD_BLOCK: 64}, {_abi.OPT_BUILD_BLOCK: 256}, {_abi.OPT_PACKED_ACCEL: 0},
+                 {_abi.OPT_BUILD_STAGED: 1}, {_abi.OPT_BUILD_STAGED: 1, _abi.OPT_BUILD_BLOCK: 64},
                  {_abi.OPT_PACKED_ACCEL: 0, _abi.OPT_GRAPHS: 1, _abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ALL},
                  {_abi.OPT_HALO: 1}, {_abi.OPT_HALO: 1, _abi.OPT_PACKED_ACCEL: 0},
                  {_abi.OPT_HALO: 1, _abi.OPT_SPARSE_CORRECTION: 0, _abi.OPT_PAIR_CACHE: _abi.PAIR_CACHE_ALL},
@@ -171,8 +172,8 @@ def test_autotune_leaves_the_simulation_unchanged():
     res = autotune(a, sa, DT, steps=2, warm=1, rounds=1, log=_log)
     _log(f"[autotune] {res['options']} in {res['seconds']:.1f} s")
     assert np.array_equal(a.pw.particle_impulse(), exch)  # the tuning steps' wall impulses are not kept
-    assert set(res["options"]) == {"check_interval", "build_block", "sparse_correction", "packed_accel", "halo",
-                                   "pair_cache", "graphs", "device_loop"}
+    assert set(res["options"]) == {"check_interval", "build_staged", "build_block", "sparse_correction",
+                                   "packed_accel", "halo", "pair_cache", "graphs", "device_loop"}
     for _ in range(6):
         a.step(sa, sa, None, None, DT)
         b.step(sb, sb, None, None, DT)
