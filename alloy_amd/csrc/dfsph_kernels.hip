@@ -32,7 +32,8 @@
 
 using namespace dfsph;
 
-void dfsph_launch_mlp_forward(const float* params, const float* X, int n, float* Y, hipStream_t st);
+void dfsph_launch_mlp_forward(const float* params, const float* X, int n, float* Y, const uint8_t* tile_on,
+                              hipStream_t st);
 
 namespace {
 
@@ -171,6 +172,8 @@ struct Ctx {
     float* feat;     // [n][16] learned warm-start features (13 used)
     float* mlp_out;  // [n][2] network outputs
     float* warm_k;   // [2][n] learned initial kappa of the divergence / density solve
+    int* learn_on;   // [W] the step's per-world gate of the learned start (DFSPH_OPT_LEARNED_GATE)
+    int learn_gate = 0;  // 1: the network's features, outputs and start only for worlds with learn_on
     float kscale;    // |g| h: the kappa scale of the features and targets [m^2/s^2]
     float warm_v, warm_d;  // the reference warm-start factors of the divergence / density solve
     float lat_diag;  // interior rest-lattice DFSPH diagonal [1/m^2]
@@ -1897,10 +1900,13 @@ __device__ __forceinline__ void density_alpha_other(const Ctx& c, int i) {
         for (int k = 0; k < 16; ++k) c.feat[16 * int64_t(i) + k] = 0.0f;
 }
 
+// gated (the learned start's per-world gate): only the chunks of worlds whose learned start is off; the features pass
+// takes the others
 template <bool WIDE>
-__global__ void __launch_bounds__(256) k_density_alpha(Ctx c) {
+__global__ void __launch_bounds__(256) k_density_alpha(Ctx c, int gated) {
     const KC K = make_kc(c.h, c.kind);
     CHUNK_PROLOGUE
+    if (gated && c.learn_on[c.ch_world[b_]]) return;
     double act_v = 0.0;
     if (valid) {
         if (!is_fluid(c, i)) {
@@ -1914,9 +1920,10 @@ __global__ void __launch_bounds__(256) k_density_alpha(Ctx c) {
 }
 // the same pass plus the learned warm start's features of the step (dt) in c.feat
 template <bool WIDE>
-__global__ void __launch_bounds__(256) k_density_features(Ctx c, float dt) {
+__global__ void __launch_bounds__(256) k_density_features(Ctx c, float dt, int gated) {
     const KC K = make_kc(c.h, c.kind);
     CHUNK_PROLOGUE
+    if (gated && !c.learn_on[c.ch_world[b_]]) return;
     double act_v = 0.0;
     if (valid) {
         if (!is_fluid(c, i)) {
@@ -1980,14 +1987,14 @@ __global__ void __launch_bounds__(256) k_solve_init(Ctx c, int mode, float dt, f
         c.src[i] = rr;
         c.lower[i] = lb;
         c.kappa[i] = fmx(warm * fmn(store[i], 0.5f) / dt, lb);
-        if (learned) c.kappa[i] = fmx(learned[i], lb);
+        if (learned && (!c.learn_gate || c.learn_on[c.ch_world[b_]])) c.kappa[i] = fmx(learned[i], lb);
     } else {
         float rs = c.ratio[i] + dt * rate;
         if (c.ratio[i] - 1.0f > relief) rs = 1.0f + relief + dt * rate;
         c.src[i] = rs;
         c.lower[i] = 0.0f;
         c.kappa[i] = warm * fmn(store[i], 2.5e-4f) / (dt * dt);
-        if (learned) c.kappa[i] = fmx(learned[i], 0.0f);
+        if (learned && (!c.learn_gate || c.learn_on[c.ch_world[b_]])) c.kappa[i] = fmx(learned[i], 0.0f);
     }
     put_vr(c, c.kr, VR_KAPPA, i, c.kappa[i] / c.rho0[i]);
 }
@@ -3847,6 +3854,7 @@ __global__ void __launch_bounds__(256) k_active_mass(Ctx c, long long* out) {
 __global__ void k_warm_from_mlp(Ctx c, float dt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.n) return;
+    if (c.learn_gate && (c.world[i] < 0 || !c.learn_on[c.world[i]])) return;
     const float* F = c.feat + 16 * int64_t(i);
     float y0 = c.mlp_out[2 * int64_t(i)], y1 = c.mlp_out[2 * int64_t(i) + 1];
     float kv = c.kscale * sinhf(F[13] + y0), kd = c.kscale * sinhf(F[14] + y1);
@@ -3854,6 +3862,24 @@ __global__ void k_warm_from_mlp(Ctx c, float dt) {
     if (!isfinite(kd)) kd = 0.0f;
     c.warm_k[i] = fmn(kv, 0.5f / dt);
     c.warm_k[int64_t(c.n) + i] = fmx(fmn(kd, 2.5e-4f / (dt * dt)), 0.0f);
+}
+
+// The learned start's per-world gate (DFSPH_OPT_LEARNED_GATE): a world takes the network's start at a step when the
+// last step's solve that the network feeds (fed: bit 0 divergence, bit 1 density) needed more than min_iterations
+// iterations. A solve held at the min_iterations floor cannot save an iteration, so there the network would only
+// cost; such a world (and every world at its first step) starts from the reference's rule.
+__global__ void k_learn_flags(Ctx c, int fed, int min_it) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= c.W) return;
+    c.learn_on[w] = ((fed & 1) && c.it_div[w] > min_it) || ((fed & 2) && c.it_den[w] > min_it) ? 1 : 0;
+}
+// per 64-row tile of the network's input rows: 1 when a particle of the tile belongs to a world with the learned start
+__global__ void __launch_bounds__(256) k_learn_tiles(Ctx c, uint8_t* tile_on) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = i < c.n ? c.world[i] : -1;
+    const bool on = w >= 0 && c.learn_on[w] != 0;
+    const unsigned long long any = __ballot(on);
+    if ((threadIdx.x & 63) == 0 && i < c.n) tile_on[i >> 6] = any != 0ull ? 1 : 0;
 }
 
 // training targets of the step just taken, [n][2]: asinh(converged kappa / ks) of both solves minus the rule's
@@ -3920,6 +3946,8 @@ struct Handle {
     bool use_graphs = false;  // measured: eager launches are faster than graph replay for this step (bench.py --graphs)
     const float* mlp_params = nullptr;  // learned warm start on when set
     int learned_solves = 3;             // DFSPH_OPT_LEARNED_SOLVES: bit 0 divergence, bit 1 density
+    int learned_gate = 1;               // DFSPH_OPT_LEARNED_GATE
+    uint8_t* learn_tiles = nullptr;     // [(n + 63) / 64] tiles of the network's rows with a gated-on world
     int build_block = 128;              // DFSPH_OPT_BUILD_BLOCK
     int build_staged = 0;               // DFSPH_OPT_BUILD_STAGED
     int packed_ok = 0;                  // every world's SPH particles share one mass: DFSPH_OPT_PACKED_ACCEL allowed
@@ -4134,8 +4162,12 @@ int phase_slots(Handle* H, hipStream_t st) {
 int phase_density(Handle* H, hipStream_t st, float features_dt = 0.0f) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
-    if (features_dt > 0.0f) LAUNCH_WT(KID_DENSITY, k_density_features, H->n_chunks, 256, c, features_dt);
-    else LAUNCH_WT(KID_DENSITY, k_density_alpha, H->n_chunks, 256, c);
+    if (features_dt > 0.0f) {
+        LAUNCH_WT(KID_DENSITY, k_density_features, H->n_chunks, 256, c, features_dt, c.learn_gate);
+        if (c.learn_gate) LAUNCH_WT(KID_DENSITY, k_density_alpha, H->n_chunks, 256, c, 1);
+    } else {
+        LAUNCH_WT(KID_DENSITY, k_density_alpha, H->n_chunks, 256, c, 0);
+    }
     k_world<<<c.W, 256, 0, st>>>(c, W_NACTIVE, StopSpec{});
     LAUNCH_CHECK();
     return DFSPH_OK;
@@ -4309,7 +4341,8 @@ int run_segment(Handle* H, hipStream_t st, int kind, int mode, float dt, F enque
     std::memcpy(&bits, &dt, 4);
     GraphKey key{kind, mode, H->check_every,
                  H->c.pair_cache | (H->c.sparse_corr << 8) | (H->build_block << 16) | (H->c.packed << 28) |
-                     (H->c.use_halo << 29) | (H->device_loop << 30), bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
+                     (H->c.use_halo << 29) | (H->device_loop << 30) | int(uint32_t(H->c.learn_gate) << 31),
+                 bits, H->c.q, H->c.qd, H->c.f, H->learned_div, H->learned_den,
                  H->mlp_params, H->log_feat, H->log_targets};
     auto it = H->graphs.find(key);
     if (it == H->graphs.end()) {
@@ -4802,6 +4835,8 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         TRY(dalloc(H, &c.feat, 16 * int64_t(n)));
         TRY(dalloc(H, &c.mlp_out, 2 * int64_t(n)));
         TRY(dalloc(H, &c.warm_k, 2 * int64_t(n)));
+        TRY(dalloc(H, &c.learn_on, std::max(W, 1)));
+        TRY(dalloc(H, &H->learn_tiles, (int64_t(n) + 63) / 64));
         TRY(dalloc(H, &c.vtmp, 3 * int64_t(n)));
         TRY(dalloc(H, &c.act, n));
         for (int** p : {&c.converged, &c.iters, &c.restart, &c.st_overflow, &c.st_projected, &c.st_swept,
@@ -4950,17 +4985,25 @@ int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void
     if (H->c.n == 0) return DFSPH_OK;
     if (H->prof.on && prof_harvest(H, false)) return DFSPH_EDEVICE;
     float dtf = float(dt);
+    const bool learned = (H->mlp_params && H->learned_solves != 0) || H->log_feat;
+    // the per-world gate applies to the network's own start (not to logging or to given kappa fields); it holds for
+    // this step's launches only
+    const bool mlp_start = H->mlp_params && (H->learned_div == H->c.warm_k || H->learned_den == H->c.warm_k + H->c.n);
+    H->c.learn_gate = (learned && mlp_start && H->learned_gate && !H->log_feat) ? 1 : 0;
     int rc = run_segment(H, st, SEG_PRE, 0, dtf, [&](hipStream_t s) {
         int r;
         if ((r = phase_neighbors(H, s))) return r;
         if ((r = phase_slots(H, s))) return r;
-        const bool learned = (H->mlp_params && H->learned_solves != 0) || H->log_feat;
+        if (H->c.learn_gate) {
+            k_learn_flags<<<blocks_for(H->c.W), 256, 0, s>>>(H->c, H->learned_solves, H->P.min_iterations);
+            k_learn_tiles<<<blocks_for(H->c.n), 256, 0, s>>>(H->c, H->learn_tiles);
+        }
         if ((r = phase_density(H, s, learned ? dtf : 0.0f))) return r;
         if ((r = phase_flow(H, s))) return r;
         if (learned) {  // learned warm start: features (written by the density pass) -> MLP -> initial kappa
             Ctx& c = H->c;
             if (H->mlp_params) {
-                dfsph_launch_mlp_forward(H->mlp_params, c.feat, c.n, c.mlp_out, s);
+                dfsph_launch_mlp_forward(H->mlp_params, c.feat, c.n, c.mlp_out, c.learn_gate ? H->learn_tiles : nullptr, s);
                 k_warm_from_mlp<<<blocks_for(c.n), 256, 0, s>>>(c, dtf);
             }
             if (H->log_feat)
@@ -4985,6 +5028,7 @@ int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void
         if (r == 0) r = phase_advect(H, dt, s);
         return r ? r : phase_emitters(H, dt, s);
     });
+    H->c.learn_gate = 0;
     return rc;
 }
 
@@ -5120,6 +5164,10 @@ int32_t dfsph_set_option(void* h, int32_t option, int32_t value) {
             if (value < 0 || value > 1) return fail(DFSPH_EINVAL, "staged build: 0 or 1");
             H->build_staged = value;
             return DFSPH_OK;
+        case DFSPH_OPT_LEARNED_GATE:
+            if (value < 0 || value > 1) return fail(DFSPH_EINVAL, "learned gate: 0 or 1");
+            H->learned_gate = value;
+            return DFSPH_OK;
         case DFSPH_OPT_HALO:
             if (value < 0 || value > 1) return fail(DFSPH_EINVAL, "halo: 0 or 1");
             if (H->c.use_halo != value) {
@@ -5168,6 +5216,9 @@ int32_t dfsph_get_option(void* h, int32_t option, int32_t* value) {
             return DFSPH_OK;
         case DFSPH_OPT_BUILD_STAGED:
             *value = H->build_staged;
+            return DFSPH_OK;
+        case DFSPH_OPT_LEARNED_GATE:
+            *value = H->learned_gate;
             return DFSPH_OK;
         default:
             return fail(DFSPH_EINVAL, "unknown option");

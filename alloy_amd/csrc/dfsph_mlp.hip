@@ -188,10 +188,11 @@ __device__ void load_inputs(const float* __restrict__ X, int n, int row0, uint16
 }
 
 // Y = MLP(X) for rows [0, n): the block stages the weights in LDS once and then walks 64-row tiles (16 per wave)
-// with a grid stride, so the 42 KB of weights are read once per block rather than once per 64 rows.
+// with a grid stride, so the 42 KB of weights are read once per block rather than once per 64 rows. With tile_on, the
+// tiles whose flag is 0 are skipped (their rows of Y are left as they are).
 constexpr int FWD_MAX_BLOCKS = 2048;
 __global__ void __launch_bounds__(256) k_mlp_forward(const float* __restrict__ p, const float* __restrict__ X, int n,
-                                                     float* __restrict__ Y) {
+                                                     float* __restrict__ Y, const uint8_t* __restrict__ tile_on) {
     __shared__ uint16_t sW[SW_TOTAL];
     __shared__ WaveTiles tiles[4];
     stage_weights(p, sW);
@@ -199,6 +200,7 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float* __restrict__ p
     WaveTiles& T = tiles[w];
     const int n_tiles = (n + 63) / 64;
     for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {  // block-uniform trip count
+        if (tile_on && !tile_on[tile]) continue;  // block-uniform
         int row0 = tile * 64 + 16 * w;
         __syncthreads();  // weights staged (first pass); the previous tile's LDS reads done
         load_inputs(X, n, row0, T.a0, lane);
@@ -372,8 +374,9 @@ int noalloc_scratch(float** out) {
 }  // namespace
 
 // launched by the step (dfsph_kernels.hip) inside its captured segment
-void dfsph_launch_mlp_forward(const float* params, const float* X, int n, float* Y, hipStream_t st) {
-    if (n > 0) k_mlp_forward<<<fwd_blocks(n), 256, 0, st>>>(params, X, n, Y);
+void dfsph_launch_mlp_forward(const float* params, const float* X, int n, float* Y, const uint8_t* tile_on,
+                              hipStream_t st) {
+    if (n > 0) k_mlp_forward<<<fwd_blocks(n), 256, 0, st>>>(params, X, n, Y, tile_on);
 }
 
 extern "C" {
@@ -384,7 +387,7 @@ int32_t dfsph_mlp_param_count(void) { return N_PARAMS; }
 int32_t dfsph_mlp_forward(const float* params, const float* X, int32_t n, float* Y, void* stream) {
     if (!params || !X || !Y || n < 0) return DFSPH_EINVAL;
     if (n == 0) return DFSPH_OK;
-    k_mlp_forward<<<fwd_blocks(n), 256, 0, static_cast<hipStream_t>(stream)>>>(params, X, n, Y);
+    k_mlp_forward<<<fwd_blocks(n), 256, 0, static_cast<hipStream_t>(stream)>>>(params, X, n, Y, nullptr);
     return hipGetLastError() == hipSuccess ? DFSPH_OK : DFSPH_EDEVICE;
 }
 

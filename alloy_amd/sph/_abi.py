@@ -41,11 +41,12 @@ WORK_SCALE = float(2**50)   # energy fixed point, bodies.py:24
 FORCE_SCALE = float(2**40)  # bodies.py:23
 
 # kernel ids of dfsph_profile_read
-# tuning options (dfsph_set_option); none but OPT_LEARNED_SOLVES changes a result bit
+# tuning options (dfsph_set_option); none but OPT_LEARNED_SOLVES and OPT_LEARNED_GATE (which start each solve takes)
+# changes a result bit
 (OPT_CHECK_INTERVAL, OPT_GRAPHS, OPT_PAIR_CACHE, OPT_LEARNED_SOLVES, OPT_SPARSE_CORRECTION, OPT_BUILD_BLOCK,
- OPT_PACKED_ACCEL, OPT_HALO, OPT_DEVICE_LOOP, OPT_BUILD_STAGED) = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10)
+ OPT_PACKED_ACCEL, OPT_HALO, OPT_DEVICE_LOOP, OPT_BUILD_STAGED, OPT_LEARNED_GATE) = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11)
 ALL_OPTIONS = (OPT_CHECK_INTERVAL, OPT_GRAPHS, OPT_PAIR_CACHE, OPT_LEARNED_SOLVES, OPT_SPARSE_CORRECTION,
-               OPT_BUILD_BLOCK, OPT_PACKED_ACCEL, OPT_HALO, OPT_DEVICE_LOOP, OPT_BUILD_STAGED)
+               OPT_BUILD_BLOCK, OPT_PACKED_ACCEL, OPT_HALO, OPT_DEVICE_LOOP, OPT_BUILD_STAGED, OPT_LEARNED_GATE)
 PAIR_CACHE_ACCEL, PAIR_CACHE_APPLY, PAIR_CACHE_OTHER, PAIR_CACHE_ALL = 1, 2, 4, 7
 
 (KID_ACCEL, KID_APPLY, KID_BUILD, KID_DENSITY, KID_SLOTS, KID_NONPRESSURE, KID_ADVECT, KID_SOLVE_INIT,

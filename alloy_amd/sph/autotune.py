@@ -36,7 +36,8 @@ CANDIDATES = (
 NAMES = {_abi.OPT_CHECK_INTERVAL: "check_interval", _abi.OPT_BUILD_BLOCK: "build_block",
          _abi.OPT_SPARSE_CORRECTION: "sparse_correction", _abi.OPT_PAIR_CACHE: "pair_cache", _abi.OPT_GRAPHS: "graphs",
          _abi.OPT_LEARNED_SOLVES: "learned_solves", _abi.OPT_PACKED_ACCEL: "packed_accel", _abi.OPT_HALO: "halo",
-         _abi.OPT_DEVICE_LOOP: "device_loop", _abi.OPT_BUILD_STAGED: "build_staged"}
+         _abi.OPT_DEVICE_LOOP: "device_loop", _abi.OPT_BUILD_STAGED: "build_staged",
+         _abi.OPT_LEARNED_GATE: "learned_gate"}
 
 
 class _Snapshot:

@@ -50,7 +50,8 @@ WORKLOADS = {  # steps: the timed steps each workload is specified with
 }
 CPU_SAMPLE_PARTICLE_STEPS = 3.0e6  # about 20 s of the serial oracle
 OPTION_IDS = {"check_interval": 1, "graphs": 2, "pair_cache": 3, "learned_solves": 4, "sparse_correction": 5,
-              "build_block": 6, "packed_accel": 7, "halo": 8, "device_loop": 9, "build_staged": 10}
+              "build_block": 6, "packed_accel": 7, "halo": 8, "device_loop": 9, "build_staged": 10,
+              "learned_gate": 11}
 
 
 def make_scene(wl: dict, device: str, subset=None):

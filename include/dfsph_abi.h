@@ -306,10 +306,17 @@ enum DfsphOption {
                                         between the phases, direct gathers), so a step makes no host round trip;
                                         0: iteration batches with a host check of the unconverged-world count.
                                         Jacobi and profiled steps use the batches. Bit-identical. */
-    DFSPH_OPT_BUILD_STAGED = 10      /* 1: the plain neighbour-list build stages each block's box of grid cells in
+    DFSPH_OPT_BUILD_STAGED = 10,     /* 1: the plain neighbour-list build stages each block's box of grid cells in
                                         LDS (64 or 128 threads by DFSPH_OPT_BUILD_BLOCK, large boxes in z-slabs) and
                                         every lane walks its 27 cells there; 0 (default): every lane walks its cells
                                         from global memory. Bit-identical. */
+    DFSPH_OPT_LEARNED_GATE = 11      /* with the warm-start network set (dfsph_set_warm_start_model): 1 (default): a
+                                        world takes the network's start at a step when the last step's solve that
+                                        the network feeds needed more than min_iterations iterations, and only such
+                                        worlds pay for the features and the network; the others (and every world at
+                                        its first step) start from the reference's rule. 0: every world takes the
+                                        network's start. Like option 4 it chooses starts, so results differ in bits;
+                                        every solve still iterates to the same tolerance. */
 };
 enum DfsphPairCache {
     DFSPH_PAIR_CACHE_ACCEL = 1, /* pressure acceleration (k_accel, post-solve) */
