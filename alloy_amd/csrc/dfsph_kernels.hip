@@ -168,6 +168,9 @@ struct Ctx {
     float *ratio, *alpha, *neff, *active, *reg, *src, *kappa, *err, *lower, *kv, *kd, *vtmp;
     float4* q4;    // (x, y, z, mass) per particle, refreshed from q at every neighbour build: one 16 B gather
     float4* q4s;   // (x, y, z, id bits) in cell order: the list build streams candidates from it
+    int* items_u;     // the scatter's unordered cell occupants, then placed in ascending id order (k_place_cells)
+    float4* q4s_u;
+    float4* vrho;     // (v, ratio rho0) per particle: the XSPH pass's one 16 B record besides the position
     float4* acc4;  // pressure acceleration (x, y, z, 0)
     float* feat;     // [n][16] learned warm-start features (13 used)
     float* mlp_out;  // [n][2] network outputs
@@ -747,32 +750,24 @@ __global__ void __launch_bounds__(256) k_scatter(Ctx c) {
     int k = c.key[i];
     if (k < 0) return;
     int pos = atomicAdd(c.cell_fill + k, 1);
-    c.items[pos] = i;
+    c.items_u[pos] = i;
     float4 p = c.q4[i];
-    c.q4s[pos] = make_float4(p.x, p.y, p.z, __int_as_float(i));
+    c.q4s_u[pos] = make_float4(p.x, p.y, p.z, __int_as_float(i));
 }
 
 // ascending ids within each cell: the fixed visit order of the colour sums (wall cells) and a nearly sorted
-// arrival order for the list build (fluid cells). One thread per occupant of the items array; the one at its cell's
-// first slot sorts that cell (empty cells, most of a grid, cost nothing).
-__global__ void __launch_bounds__(256) k_sort_cells(Ctx c) {
-    int a0 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a0 >= c.cell_start[c.n_cells]) return;
-    int cc = c.key[c.items[a0]];
-    int b = c.cell_start[cc], e = c.cell_start[cc + 1];
-    if (a0 != b) return;
-    for (int a = b + 1; a < e; ++a) {
-        int v = c.items[a];
-        float4 pv = c.q4s[a];
-        int k = a - 1;
-        while (k >= b && c.items[k] > v) {
-            c.items[k + 1] = c.items[k];
-            c.q4s[k + 1] = c.q4s[k];
-            --k;
-        }
-        c.items[k + 1] = v;
-        c.q4s[k + 1] = pv;
-    }
+// arrival order for the list build (fluid cells). One thread per occupant of the scatter's unordered slots: its rank
+// in its cell is the number of the cell's occupants with a lower id, and it moves its id and record there.
+__global__ void __launch_bounds__(256) k_place_cells(Ctx c) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= c.cell_start[c.n_cells]) return;
+    const int v = c.items_u[a];
+    const int cc = c.key[v];
+    const int b = c.cell_start[cc], e = c.cell_start[cc + 1];
+    int rank = 0;
+    for (int k = b; k < e; ++k) rank += c.items_u[k] < v ? 1 : 0;
+    c.items[b + rank] = v;
+    c.q4s[b + rank] = c.q4s_u[a];
 }
 
 // Every wall particle marks the cells of its wall grid within one cell of its own (the block pw_sample walks), so a
@@ -2791,10 +2786,15 @@ __global__ void k_tol_scale(Ctx c, float dt, float speed, float ratio, float tol
 // ----------------------------------------------------------------------------------------------- non-pressure
 // nonpressure_kernel (integrate.py:150-178)
 struct XS {
-    float4 q;
-    f3 v;
-    float ratio, rho0;
+    float4 q, vr;  // position and mass; velocity and density ratio * rho0 (k_pack_vrho)
 };
+// the XSPH records (v, ratio rho0): the product is the one the XSPH term forms for the neighbour, rounded once here
+__global__ void __launch_bounds__(256) k_pack_vrho(Ctx c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const f3 v = V(c, i);
+    c.vrho[i] = make_float4(v.x, v.y, v.z, c.ratio[i] * c.rho0[i]);
+}
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
     const KC K = make_kc(c.h, c.kind);
@@ -2815,16 +2815,14 @@ __global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
         float rho_i = c.ratio[i] * c.rho0[i];
         int cnt = c.count[i];
         const int base = c.ws[w];
-        const WorldRecs q4 = WorldRecs::of(c.q4, base, c.n), ratio = WorldRecs::of(c.ratio, base, c.n),
-                        rho0 = WorldRecs::of(c.rho0, base, c.n),
-                        qd = WorldRecs::of(reinterpret_cast<const f3*>(c.qd), base, c.n);
+        const WorldRecs q4 = WorldRecs::of(c.q4, base, c.n), vr = WorldRecs::of(c.vrho, base, c.n);
         for_rows<WIDE, XS>(
-            c, i, cnt, [&](unsigned e) { return XS{q4.f4(e), qd.v3(e), ratio.f1(e), rho0.f1(e)}; },
+            c, i, cnt, [&](unsigned e) { return XS{q4.f4(e), vr.f4(e)}; },
             [&](int, XS d) {
-                float rho_bar = 0.5f * (rho_i + d.ratio * d.rho0);
+                float rho_bar = 0.5f * (rho_i + d.vr.w);
                 if (rho_bar > 0.0f) {
                     float wk = kernel_w(dist(xi - xyz(d.q), K), K);
-                    cr = cr + (d.q.w / rho_bar * wk) * (d.v - vi);
+                    cr = cr + (d.q.w / rho_bar * wk) * (xyz(d.vr) - vi);
                 }
             });
         v = v + c.xsph * cr;
@@ -4127,7 +4125,7 @@ int phase_neighbors(Handle* H, hipStream_t st) {
     k_scan_top<<<1, 256, 0, st>>>(c.scan_part, H->n_scan_blocks, c.cell_start, c.n_cells);
     k_scan_apply<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part, c.cell_start, c.cell_fill);
     k_scatter<<<blocks_for(c.n), 256, 0, st>>>(c);
-    k_sort_cells<<<blocks_for(c.n), 256, 0, st>>>(c);  // ascending ids in every cell
+    k_place_cells<<<blocks_for(c.n), 256, 0, st>>>(c);  // ascending ids in every cell
     if (c.NW > 0) {
         int64_t nw_cells = int64_t(c.n_cells) - c.wall_first;
         k_fill_u8<<<std::min(blocks_for(nw_cells), 4096), 256, 0, st>>>(c.wall_near, nw_cells);
@@ -4439,6 +4437,7 @@ int run_csf(Handle* H, float dt, hipStream_t st) {
 int phase_nonpressure(Handle* H, double dt, hipStream_t st) {
     Ctx& c = H->c;
     if (c.n == 0) return DFSPH_OK;
+    if (c.xsph > 0.0f) k_pack_vrho<<<blocks_for(c.n), 256, 0, st>>>(c);
     LAUNCH_WT(KID_NONPRESSURE, k_nonpressure, H->n_chunks, 256, c, float(dt));
     k_copy_f32<<<blocks_for(3 * int64_t(c.n)), 256, 0, st>>>(c.qd, c.vtmp, 3 * int64_t(c.n));
     LAUNCH_CHECK();
@@ -4708,6 +4707,9 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         H->n_scan_blocks = blocks_for(c.n_cells, 1024);
         TRY(dalloc(H, &c.scan_part, H->n_scan_blocks));
         TRY(dalloc(H, &c.items, n));
+        TRY(dalloc(H, &c.items_u, n));
+        TRY(dalloc(H, &c.q4s_u, n));
+        TRY(dalloc(H, &c.vrho, n));
         TRY(dalloc(H, &c.key, n));
         // shapes
         std::vector<int> ss(L->shape_start, L->shape_start + W + 1);
