@@ -3849,10 +3849,9 @@ __global__ void __launch_bounds__(256) k_active_mass(Ctx c, long long* out) {
 // ----------------------------------------------------------------------------------------------- learned warm start
 // The network predicts the correction y to the rule in asinh(kappa / ks) units: kappa0 = ks sinh(rule + y), so a
 // zero output reproduces the reference's warm start.
-__global__ void k_warm_from_mlp(Ctx c, float dt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    if (c.learn_gate && (c.world[i] < 0 || !c.learn_on[c.world[i]])) return;
+__global__ void __launch_bounds__(256) k_warm_from_mlp(Ctx c, float dt) {
+    CHUNK_PROLOGUE
+    if (!valid || (c.learn_gate && !c.learn_on[c.ch_world[b_]])) return;
     const float* F = c.feat + 16 * int64_t(i);
     float y0 = c.mlp_out[2 * int64_t(i)], y1 = c.mlp_out[2 * int64_t(i) + 1];
     float kv = c.kscale * sinhf(F[13] + y0), kd = c.kscale * sinhf(F[14] + y1);
@@ -5006,7 +5005,7 @@ int32_t dfsph_step(void* h, float* q, float* qd, const float* f, double dt, void
             Ctx& c = H->c;
             if (H->mlp_params) {
                 dfsph_launch_mlp_forward(H->mlp_params, c.feat, c.n, c.mlp_out, c.learn_gate ? H->learn_tiles : nullptr, s);
-                k_warm_from_mlp<<<blocks_for(c.n), 256, 0, s>>>(c, dtf);
+                k_warm_from_mlp<<<H->n_chunks, 256, 0, s>>>(c, dtf);
             }
             if (H->log_feat)
                 k_copy_f32<<<blocks_for(16 * int64_t(c.n)), 256, 0, s>>>(H->log_feat, c.feat, 16 * int64_t(c.n));

@@ -30,8 +30,9 @@ CANDIDATES = (
     (_abi.OPT_PACKED_ACCEL, (1, 0)),
     (_abi.OPT_HALO, (0, 1)),
     (_abi.OPT_PAIR_CACHE, (0, _abi.PAIR_CACHE_APPLY, _abi.PAIR_CACHE_ALL)),
-    (_abi.OPT_GRAPHS, (0, 1)),
-    (_abi.OPT_DEVICE_LOOP, (0, 1)),
+    # graph replay and the device-side solve loop are tuned together: the loop takes the host's per-batch wait off a
+    # step, and only then does replaying the step's segments as graphs save the launches' host cost
+    ((_abi.OPT_GRAPHS, _abi.OPT_DEVICE_LOOP), ((0, 0), (1, 0), (0, 1), (1, 1))),
 )
 NAMES = {_abi.OPT_CHECK_INTERVAL: "check_interval", _abi.OPT_BUILD_BLOCK: "build_block",
          _abi.OPT_SPARSE_CORRECTION: "sparse_correction", _abi.OPT_PAIR_CACHE: "pair_cache", _abi.OPT_GRAPHS: "graphs",
@@ -78,44 +79,58 @@ def _window_ms(solver, state, dt: float, warm: int, steps: int) -> float:
 
 
 def autotune(solver, state, dt: float, *, steps: int = 8, warm: int = 2, rounds: int = 2, candidates=CANDIDATES,
-             learned: bool = False, log: Callable[[str], None] | None = None) -> dict:
+             learned: bool = False, margin: float = 0.005, log: Callable[[str], None] | None = None) -> dict:
     """Tune ``solver``'s options on ``state`` in place (the state and the solver come back unchanged) and return
     {"options": {name: value}, "ms_per_step": {name: {value: median ms}}, "seconds": tuning time}.
 
     ``learned``: with a learned warm start attached, also time it against the reference's rule (DFSPH_OPT_LEARNED_SOLVES
     0). That choice is not bit-identical: both starts converge to the same tolerance, but the iterates differ, so the
-    simulation after tuning follows the chosen start (it is restored to the snapshot either way)."""
+    simulation after tuning follows the chosen start (it is restored to the snapshot either way).
+
+    ``margin``: an option keeps its current value unless another is faster by more than this fraction, so that
+    timing noise between near-equal values does not move the choice from run to run."""
     t0 = time.perf_counter()
     if learned and getattr(solver, "_warm_model", None) is not None:
         cur = solver.get_option(_abi.OPT_LEARNED_SOLVES)
         if cur:
             candidates = tuple(candidates) + ((_abi.OPT_LEARNED_SOLVES, (cur, 0)),)
     snap = _Snapshot(solver, state)
-    current = {opt: solver.get_option(opt) for opt, _ in candidates}
+    # an entry tunes one option (opt, values) or several together ((opt, ...), ((value, ...), ...))
+    group = [(tuple(o) if isinstance(o, tuple) else (o,), [tuple(v) if isinstance(v, tuple) else (v,) for v in vals])
+             for o, vals in candidates]
+    current = {opt: solver.get_option(opt) for opts, _ in group for opt in opts}
     table: dict[str, dict] = {}
-    for opt, values in candidates:
-        best_v, best_t, row = current[opt], math.inf, {}
+    for opts, values in group:
+        incumbent = tuple(current[o] for o in opts)
+        best_v, best_t, row, t_inc = incumbent, math.inf, {}, math.inf
+        label = "+".join(NAMES.get(o, str(o)) for o in opts)
         for v in values:
+            trial = {**current, **dict(zip(opts, v))}
             times = []
             for _ in range(rounds):
                 snap.restore(solver, state)
                 try:
-                    for o, val in current.items():
-                        solver.set_option(o, v if o == opt else val)
+                    for o, val in trial.items():
+                        solver.set_option(o, val)
                 except ValueError:  # not applicable to this pack (e.g. 256-thread builds with 24-bit rows)
                     times = []
                     break
                 times.append(_window_ms(solver, state, dt, warm, steps))
             if not times:
                 continue
-            row[v] = float(np.median(times))
-            if row[v] < best_t:
-                best_v, best_t = v, row[v]
-        current[opt] = best_v
-        table[NAMES.get(opt, str(opt))] = row
+            key = v[0] if len(v) == 1 else ",".join(str(x) for x in v)
+            row[key] = float(np.median(times))
+            if v == incumbent:
+                t_inc = row[key]
+            if row[key] < best_t:
+                best_v, best_t = v, row[key]
+        if best_v != incumbent and t_inc < math.inf and best_t > t_inc * (1.0 - margin):
+            best_v = incumbent  # within the noise of the current value
+        current.update(zip(opts, best_v))
+        table[label] = row
         if log:
-            log(f"autotune {NAMES.get(opt, opt)}: " + ", ".join(f"{k} {t:.3f} ms" for k, t in row.items())
-                + f" -> {best_v}")
+            log(f"autotune {label}: " + ", ".join(f"{k} {t:.3f} ms" for k, t in row.items())
+                + f" -> {best_v if len(best_v) > 1 else best_v[0]}")
     for o, val in current.items():
         solver.set_option(o, val)
     snap.restore(solver, state)

@@ -8,7 +8,8 @@
             --write-scale S --steps K --warmup W --options O --command CMD --out FILE.json
         HBM bytes per launch of kernel K over its dispatches in steps A..B (the bench's roofline window) from two
         --pmc runs (FETCH_SIZE, WRITE_SIZE in KiB per dispatch, k_groups_hist included to mark the steps), each
-        scaled by what the counter reads for a plain copy kernel of known size
+        scaled by what the counter reads for a plain copy kernel of known size; with --append the file keeps a list
+        of such records, one per step counts and options
     python -m alloy_amd.sph.rocprof_summary counters DIR --kernels A,B --top FRAC --command CMD --out FILE.md
         mean per dispatch of every --pmc counter of the named kernels, over their dispatches doing at least FRAC of the
         largest one's work (the launches in which every world still iterates)
@@ -120,8 +121,16 @@ def traffic(args) -> None:
            "fetch_kib_per_launch": f_kib, "write_kib_per_launch": w_kib,
            "fetch_scale": args.fetch_scale, "write_scale": args.write_scale,
            "hbm_bytes_per_launch": 1024.0 * (f_kib * args.fetch_scale + w_kib * args.write_scale)}
+    out = rec
+    if args.append and os.path.exists(args.out):  # one record per step counts and options: replace or add
+        with open(args.out) as f:
+            old = json.load(f)
+        recs = old if isinstance(old, list) else [old]
+        same = lambda r: (r.get("steps"), r.get("warmup"), r.get("options")) == (rec["steps"], rec["warmup"],
+                                                                                   rec["options"])
+        out = [r for r in recs if not same(r)] + [rec]
     with open(args.out, "w") as f:
-        json.dump(rec, f, indent=1)
+        json.dump(out, f, indent=1)
         f.write("\n")
 
 
@@ -175,6 +184,7 @@ def main() -> None:
     b.add_argument("--options", default="")
     b.add_argument("--command", required=True)
     b.add_argument("--out", required=True)
+    b.add_argument("--append", action="store_true", help="keep the file's records of other step counts or options")
     c = sub.add_parser("counters")
     c.add_argument("dir")
     c.add_argument("--kernels", required=True)

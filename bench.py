@@ -348,13 +348,14 @@ def _traffic_record(kernel: str, workload: str, steps: int, warmup: int, options
     if not os.path.exists(path):
         return None, f"no profiles/{name}"
     with open(path) as f:
-        rec = json.load(f)
-    if rec.get("steps") != steps or rec.get("warmup") != warmup:
-        return None, (f"profiles/{name} was measured with --steps {rec.get('steps')} --warmup {rec.get('warmup')}, "
-                      f"not this run's {steps} / {warmup}")
-    if _traffic_options(rec.get("options", "")) != _traffic_options(options):
-        return None, f"profiles/{name} was measured with options {rec.get('options')}, not this run's {options}"
-    return rec, f"profiles/{name}: {rec.get('window', '')}; command: {rec.get('command', '')}"
+        recs = json.load(f)
+    recs = recs if isinstance(recs, list) else [recs]  # one record per step counts and options
+    for rec in recs:
+        if (rec.get("steps"), rec.get("warmup")) == (steps, warmup) and \
+                _traffic_options(rec.get("options", "")) == _traffic_options(options):
+            return rec, f"profiles/{name}: {rec.get('window', '')}; command: {rec.get('command', '')}"
+    have = "; ".join(f"--steps {r.get('steps')} --warmup {r.get('warmup')} options {r.get('options')}" for r in recs)
+    return None, f"profiles/{name} holds {have}, not this run's --steps {steps} --warmup {warmup} options {options}"
 
 
 LEG_1M_WARMUP = 60  # past the first steps of the collapse, where every solve sits at min_iterations whatever its start
