@@ -4,20 +4,25 @@ batched learned-warm-start solver against plain single-world DFSPH.
 Default workload: 512 batched worlds in one pack, each a box tank of side U(0.3, 0.6) m with a static two-layer
 particle-wall bowl (hemisphere, radius 0.08 m) and a seeded lattice block of U{2,000..20,000} liquid particles
 (5,672,431 in all, plus 549,888 wall particles), fp32 physics, dt 1 ms, 300 timed steps after 20 warm-up steps,
-every world one pressure solve per phase launch. The learned warm start (the committed trained network) predicts
-both solves' initial kappa when its weights are present.
+every world one pressure solve per phase launch. The learned warm start (the committed trained network) feeds the
+solves its training run chose (the density solve), gated per world, when its weights are present; the autotuner,
+run between the warm-up and the timed region, times it against the reference's rule and keeps the faster.
 
 `value` is the batched (+learned) run's liquid particle-steps per second: inputs resident in HBM, the timed region
 bracketed by device synchronisations and timed with HIP events on the solver's stream. The line also carries
+- "regime": the steps the timed region covered, the solves' mean iterations and density errors there;
 - "plain": the same worlds stepped as plain single-world DFSPH (one solver, i.e. one launch per phase, per world,
   no learned start), on an even sample of the worlds, and `speedup_vs_plain`;
-- "batched_rule_start": the same batched run with the reference's warm-start rule instead of the network, so the
-  learned start's own effect is on the line;
-- "roofline": the dominant kernel's algorithmic bytes, counted by the kernels themselves over the timed region,
-  divided by its summed launch durations (HIP events inside each launch's dispatch, so they agree with
+- "batched_rule_start" / "batched_learned_start": the same batched run with the other warm start, so the learned
+  start's own effect is on the line; "single_launch_rule_start": one batched launch per phase with the library's
+  default options and the rule, and `vs_single_launch`;
+- "dam_break_1m_learned_vs_rule": the learned start against the rule on the 1,000,000-particle dam break;
+- "roofline": the dominant kernel's algorithmic bytes, counted by the kernels themselves over the window after the
+  timed region, divided by its summed launch durations (HIP events inside each launch's dispatch, so they agree with
   rocprofv3's kernel trace); "traffic" is the HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs
-  (profiles/), or null without one;
-- "cpu_baseline": the CPU oracle (the same algorithm, float32, one core) on a bounded sample of the workload;
+  (profiles/), or null without one for this run's steps and options;
+- "cpu_baseline": the CPU oracle (the same algorithm, float32) on the box's cores (OpenMP) and on one core, on a
+  bounded sample of the workload;
 - "fp64_density_err": mean max(rho/rho0 - 1, 0) of the final state's liquid, evaluated by the FP64 oracle.
 `--workload dam_break_125k` (the 125,000-particle dam break) and the others measure a single world;
 `--workload pour_worlds_2048` the batched pour pack (2,048 worlds, each a spout filling a swaying particle-wall bowl
