@@ -122,13 +122,15 @@ __device__ __forceinline__ float kernel_w(float r, const KC& K) {
 
 // kernels.py:58-77 as a scale: grad W(rv) = rv * dW/dq / (r h), with 0 standing for the zero vector of the cut-off
 // cases (q >= 1 or r <= 1e-9 h). dW/dq vanishes inside (0, 1) nowhere, so a zero scale occurs only there.
-__device__ __forceinline__ float kernel_grad_scale_fast(f3 rv, const KC& K) {
+// KIND: the kernel kind when the caller knows it (0 cubic, 1 Wendland), -1 to read K.kind
+template <int KIND>
+__device__ __forceinline__ float kernel_grad_scale_fast_kind(f3 rv, const KC& K) {
     // every branch evaluated and the result selected: no divergent control flow per pair
     const float r = sqrt_rn(dot(rv, rv));
     const float q = div_h(r, K);
     const float s = 1.0f - q;
     float dwdq;
-    if (K.kind == 1) {
+    if (KIND == 1 || (KIND < 0 && K.kind == 1)) {
         dwdq = K.knorm * (-20.0f * q * s * s * s);
     } else {
         const float near = K.knorm * (18.0f * q * q - 12.0f * q);
@@ -137,6 +139,9 @@ __device__ __forceinline__ float kernel_grad_scale_fast(f3 rv, const KC& K) {
     }
     const float g = div_rn(dwdq, r * K.h);
     return (q >= 1.0f || r <= K.rmin) ? 0.0f : g;
+}
+__device__ __forceinline__ float kernel_grad_scale_fast(f3 rv, const KC& K) {
+    return kernel_grad_scale_fast_kind<-1>(rv, K);
 }
 __device__ __forceinline__ float kernel_grad_scale(f3 rv, const KC& K) {
     if (K.fast) return kernel_grad_scale_fast(rv, K);

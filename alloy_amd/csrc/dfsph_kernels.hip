@@ -233,6 +233,7 @@ __device__ __forceinline__ int nbr_at(const Ctx& c, int base, int i, int k) {
 #endif
 constexpr int NB_U = DFSPH_NB_U;
 constexpr int ROW_PAD = 16;  // zero rows past the row capacity of nbr, nbr_hi, pgrad and h_slot
+static_assert(NB_U % 4 == 0 && ROW_PAD >= 2 * NB_U, "a neighbour loop loads the next batch's row quads ahead");
 template <class T>
 __device__ __forceinline__ float4 load_pos(const T& d) { return d.q; }
 [[maybe_unused]] __device__ __forceinline__ float4 load_pos(float4 d) { return d; }
@@ -334,53 +335,84 @@ struct WorldRecs {
         return f3{__builtin_bit_cast(float, a.x), __builtin_bit_cast(float, a.y), __builtin_bit_cast(float, b)};
     }
 };
-// the NB_U world-local entries k0.. (a multiple of 4) of particle i's row: one buffer load per row quad (plus one for
-// the high bytes), row offsets scalar, the lane offset 8 i
+// The NB_U world-local entries k0.. (a multiple of 4) of particle i's row: one buffer load per row quad (plus one for
+// the high bytes), row offsets scalar, the lane offset 8 i. row_raw issues the loads, row_unpack takes the entries
+// out of the loaded words, and row_batch does both.
+struct RowRaw {
+    uint2 v[NB_U / 4];
+    unsigned hb[NB_U / 4];
+};
 template <bool WIDE>
-__device__ __forceinline__ void row_batch(const Ctx& c, int i, int k0, unsigned (&e)[NB_U]) {
+__device__ __forceinline__ RowRaw row_raw(const Ctx& c, int i, int k0) {
     const uint32_t n = uint32_t(c.n);
     const __amdgpu_buffer_rsrc_t r16 = buf_rsrc(c.nbr + int64_t(k0) * n, 2u * NB_U * n);
     const __amdgpu_buffer_rsrc_t r8 = buf_rsrc(c.nbr_hi + (WIDE ? int64_t(k0) * n : 0), NB_U * n);
+    RowRaw w;
 #pragma unroll
     for (int q = 0; q < NB_U / 4; ++q) {
-        const uint2 v = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r16, 8u * unsigned(i),
-                                                                                        int(8u * q * n), 0));
-        e[4 * q] = v.x & 0xffffu;
-        e[4 * q + 1] = v.x >> 16;
-        e[4 * q + 2] = v.y & 0xffffu;
-        e[4 * q + 3] = v.y >> 16;
-        if (WIDE) {
-            const unsigned hb = __builtin_amdgcn_raw_buffer_load_b32(r8, 4u * unsigned(i), int(4u * q * n), 0);
+        w.v[q] = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r16, 8u * unsigned(i),
+                                                                                 int(8u * q * n), 0));
+        w.hb[q] = WIDE ? __builtin_amdgcn_raw_buffer_load_b32(r8, 4u * unsigned(i), int(4u * q * n), 0) : 0u;
+    }
+    return w;
+}
+template <bool WIDE>
+__device__ __forceinline__ void row_unpack(const RowRaw& w, unsigned (&e)[NB_U]) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) e[4 * q + u] |= ((hb >> (8 * u)) & 0xffu) << 16;
+    for (int q = 0; q < NB_U / 4; ++q) {
+        e[4 * q] = w.v[q].x & 0xffffu;
+        e[4 * q + 1] = w.v[q].x >> 16;
+        e[4 * q + 2] = w.v[q].y & 0xffffu;
+        e[4 * q + 3] = w.v[q].y >> 16;
+        if (WIDE) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) e[4 * q + u] |= ((w.hb[q] >> (8 * u)) & 0xffu) << 16;
         }
     }
 }
-// body(k, record) in ascending k over the world's records load(e)
+template <bool WIDE>
+__device__ __forceinline__ void row_batch(const Ctx& c, int i, int k0, unsigned (&e)[NB_U]) {
+    row_unpack<WIDE>(row_raw<WIDE>(c, i, k0), e);
+}
+// body(k, record) in ascending k over the world's records load(e). The row entries of batch k0 + NB_U are loaded
+// while batch k0 computes (at the last batch from the ROW_PAD rows past the capacity), and every load of a batch is
+// issued before its per-pair branches (the compiler would otherwise sink a load into the one branch that uses it,
+// behind the previous pair's arithmetic).
 template <bool WIDE, class T, class LoadF, class BodyF>
 __device__ __forceinline__ void for_rows(const Ctx& c, int i, int cnt, LoadF load, BodyF body) {
+    RowRaw raw = row_raw<WIDE>(c, i, 0);
     for (int k0 = 0; k0 < cnt; k0 += NB_U) {
         unsigned e[NB_U];
-        row_batch<WIDE>(c, i, k0, e);
+        row_unpack<WIDE>(raw, e);
         T d[NB_U];
 #pragma unroll
         for (int u = 0; u < NB_U; ++u) d[u] = load(e[u]);
+        raw = row_raw<WIDE>(c, i, k0 + NB_U);
+        __asm__ volatile("" ::: "memory");
 #pragma unroll
         for (int u = 0; u < NB_U; ++u)
             if (k0 + u < cnt) body(k0 + u, d[u]);
     }
 }
-// body(record, grad W_ij) in ascending k, the gradient from the pair cache (cached) or evaluated
-template <bool WIDE, class T, class LoadF, class BodyF>
-__device__ __forceinline__ void for_rows_g(const Ctx& c, int i, f3 xi, int cnt, bool cached, LoadF load,
-                                           BodyF body) {
-    const KC K = make_kc(c.h, c.kind);
+// body(record, grad W_ij) in ascending k, the gradient from the pair cache (cached) or evaluated. The gradient's
+// source is chosen once per loop (GS), not per pair: each version of the loop is then straight-line code over its
+// NB_U pairs, which the compiler can interleave, where a per-pair launch-uniform branch would split it into blocks.
+enum GradSource { GS_CACHED = 0, GS_CUBIC, GS_ANY };
+template <int GS>
+__device__ __forceinline__ float grad_scale_of(f3 rv, const KC& K) {
+    if (GS == GS_CUBIC) return kernel_grad_scale_fast_kind<0>(rv, K);
+    return kernel_grad_scale(rv, K);
+}
+template <bool WIDE, int GS, class T, class LoadF, class BodyF>
+__device__ __forceinline__ void rows_g_loop(const Ctx& c, int i, f3 xi, int cnt, const KC& K, LoadF& load,
+                                            BodyF& body) {
     const uint32_t n = uint32_t(c.n);
+    RowRaw raw = row_raw<WIDE>(c, i, 0);  // pipelined as in for_rows
     for (int k0 = 0; k0 < cnt; k0 += NB_U) {
         unsigned e[NB_U];
-        row_batch<WIDE>(c, i, k0, e);
+        row_unpack<WIDE>(raw, e);
         float sc[NB_U];
-        if (cached) {
+        if (GS == GS_CACHED) {
             const __amdgpu_buffer_rsrc_t rg = buf_rsrc(c.pgrad + int64_t(k0) * n, 4u * NB_U * n);
 #pragma unroll
             for (int u = 0; u < NB_U; ++u)
@@ -390,13 +422,23 @@ __device__ __forceinline__ void for_rows_g(const Ctx& c, int i, f3 xi, int cnt, 
         T d[NB_U];
 #pragma unroll
         for (int u = 0; u < NB_U; ++u) d[u] = load(e[u]);
+        raw = row_raw<WIDE>(c, i, k0 + NB_U);
+        __asm__ volatile("" ::: "memory");
 #pragma unroll
         for (int u = 0; u < NB_U; ++u) {
             if (k0 + u >= cnt) continue;
             const f3 rv = xi - xyz(load_pos(d[u]));
-            body(d[u], rv * (cached ? sc[u] : kernel_grad_scale(rv, K)));
+            body(d[u], rv * (GS == GS_CACHED ? sc[u] : grad_scale_of<GS>(rv, K)));
         }
     }
+}
+template <bool WIDE, class T, class LoadF, class BodyF>
+__device__ __forceinline__ void for_rows_g(const Ctx& c, int i, f3 xi, int cnt, bool cached, LoadF load,
+                                           BodyF body) {
+    const KC K = make_kc(c.h, c.kind);
+    if (cached) rows_g_loop<WIDE, GS_CACHED, T>(c, i, xi, cnt, K, load, body);
+    else if (K.fast && K.kind == 0) rows_g_loop<WIDE, GS_CUBIC, T>(c, i, xi, cnt, K, load, body);
+    else rows_g_loop<WIDE, GS_ANY, T>(c, i, xi, cnt, K, load, body);
 }
 
 #define LAUNCH_W(kernel, grid, block, ...)                                        \
