@@ -144,7 +144,7 @@ struct Ctx {
     float4* qa;  // packed (DFSPH_OPT_PACKED_ACCEL): {x, y, z, acc.x} and {acc.y, acc.z} of the last pressure
     float2* a2;  // acceleration, the operator apply's 24 B neighbour record instead of q4 and acc4 (32 B)
     const float* w_mass;
-    int* cell_fill;
+    int* arrive;  // [n] a grouped particle's arrival slot in its cell (the count's atomic returns it), for the scatter
     int* items;
     int* key;
     int* scan_part;
@@ -716,7 +716,7 @@ __global__ void __launch_bounds__(256) k_groups_hist(Ctx c) {
         int gi = g >= 0 ? g : c.W + (-2 - g);
         int cx, cy, cz;
         k = cell_of(c, gi, xi, cx, cy, cz);
-        atomicAdd(c.cell_count + k, 1);
+        c.arrive[i] = atomicAdd(c.cell_count + k, 1);
     }
     c.key[i] = k;
 }
@@ -768,7 +768,7 @@ __global__ void __launch_bounds__(256) k_scan_top(int* part, int nb, int* out, i
     }
     if (threadIdx.x == 0) out[n] = running;
 }
-__global__ void __launch_bounds__(256) k_scan_apply(const int* in, int n, const int* part, int* out, int* out2) {
+__global__ void __launch_bounds__(256) k_scan_apply(const int* in, int n, const int* part, int* out) {
     int base = blockIdx.x * 1024 + threadIdx.x * 4;
     int v[4], s = 0;
     for (int k = 0; k < 4; ++k) {
@@ -778,10 +778,7 @@ __global__ void __launch_bounds__(256) k_scan_apply(const int* in, int n, const 
     int tot;
     int ex = block_scan_excl(s, &tot) + part[blockIdx.x];
     for (int k = 0; k < 4; ++k) {
-        if (base + k < n) {
-            out[base + k] = ex;
-            out2[base + k] = ex;
-        }
+        if (base + k < n) out[base + k] = ex;
         ex += v[k];
     }
 }
@@ -791,7 +788,7 @@ __global__ void __launch_bounds__(256) k_scatter(Ctx c) {
     if (i >= c.n) return;
     int k = c.key[i];
     if (k < 0) return;
-    int pos = atomicAdd(c.cell_fill + k, 1);
+    const int pos = c.cell_start[k] + c.arrive[i];
     c.items_u[pos] = i;
     float4 p = c.q4[i];
     c.q4s_u[pos] = make_float4(p.x, p.y, p.z, __int_as_float(i));
@@ -1299,6 +1296,9 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv,
     }
     int nrec = 0;
     const int* d = c.g_dims + 3 * w;
+    // |d|^2 > 1.0001 h^2 (rounded either way) puts |d| above h (1 + 4e-5) and its rounded quotient by h above 1, where
+    // the kernel is 0: such a candidate is skipped before the kernel's square root and quotient
+    const float out2 = 1.0001f * (c.h * c.h);
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
@@ -1311,6 +1311,7 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv,
                 for (int e = c.cell_start[cc]; e < c.cell_start[cc + 1]; ++e) {
                     const float4 p4 = c.q4s[e];
                     f3 dd = x - f3{p4.x, p4.y, p4.z};
+                    if (dot(dd, dd) > out2) continue;  // kernel_w is 0 there
                     float kw = kernel_w(dist(dd, K), K);
                     if (!(kw > 0.0f)) continue;
                     int j = __float_as_int(p4.w);
@@ -4164,7 +4165,7 @@ int phase_neighbors(Handle* H, hipStream_t st) {
     k_groups_hist<<<blocks_for(c.n), 256, 0, st>>>(c);
     k_scan_reduce<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part);
     k_scan_top<<<1, 256, 0, st>>>(c.scan_part, H->n_scan_blocks, c.cell_start, c.n_cells);
-    k_scan_apply<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part, c.cell_start, c.cell_fill);
+    k_scan_apply<<<H->n_scan_blocks, 256, 0, st>>>(c.cell_count, c.n_cells, c.scan_part, c.cell_start);
     k_scatter<<<blocks_for(c.n), 256, 0, st>>>(c);
     k_place_cells<<<blocks_for(c.n), 256, 0, st>>>(c);  // ascending ids in every cell
     if (c.NW > 0) {
@@ -4727,7 +4728,6 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         TRY(dupload(H, (int**)&c.cell_base, base.data(), 2 * W + 1));
         TRY(dalloc(H, &c.cell_count, int64_t(c.n_cells) + 1));
         TRY(dalloc(H, &c.cell_start, int64_t(c.n_cells) + 1));
-        TRY(dalloc(H, &c.cell_fill, int64_t(c.n_cells) + 1));
         if (c.NW > 0) TRY(dalloc(H, &c.wall_near, int64_t(c.n_cells) - c.wall_first));
         {  // compact wall index (ascending ids), the exchange window and the walls' work
             std::vector<float> wv(std::max(n, 1), 0.0f);
@@ -4752,6 +4752,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
         TRY(dalloc(H, &c.q4s_u, n));
         TRY(dalloc(H, &c.vrho, n));
         TRY(dalloc(H, &c.key, n));
+        TRY(dalloc(H, &c.arrive, n));
         // shapes
         std::vector<int> ss(L->shape_start, L->shape_start + W + 1);
         TRY(dupload(H, (int**)&c.sh_start, ss.data(), W + 1));
