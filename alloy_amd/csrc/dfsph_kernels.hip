@@ -918,11 +918,10 @@ __device__ __forceinline__ void build_row(const Ctx& c, int i, int g, f3 xi, flo
                     const int j = __float_as_int(pc[u].w);
                     const f3 dd = xyz(pc[u]) - xi;
                     const bool ok = (e + u < e1) & (j != i) & (dot(dd, dd) < h2);
-                    if (ok) {
-                        total += 1;
-                        L[nn * BB] = RowT(j - base);
-                        nn += 1;
-                    }
+                    // written whether accepted or not (a rejected entry is overwritten by the next), no branch
+                    L[nn * BB] = RowT(j - base);
+                    total += ok ? 1 : 0;
+                    nn += ok ? 1 : 0;
                 }
             }
         }
