@@ -244,6 +244,12 @@ __device__ __forceinline__ void put_vr(const Ctx& c, float* vr, int slot, int i,
     vr[i] = v;
     if (c.packed) c.qk[int64_t(slot) * c.n + i].w = v;
 }
+// the filled boundary slots of particle i as a 4-bit mask (bit b: slot b holds a wall), for a pass to read before
+// its neighbour loop: the loop's loads are ordered before what follows them, so a load after the loop would wait
+__device__ __forceinline__ unsigned slot_mask(const Ctx& c, int i) {
+    const int4 sh = *reinterpret_cast<const int4*>(c.s_shape + 4 * int64_t(i));
+    return (sh.x >= 0 ? 1u : 0u) | (sh.y >= 0 ? 2u : 0u) | (sh.z >= 0 ? 4u : 0u) | (sh.w >= 0 ? 8u : 0u);
+}
 // the pressure acceleration of particle i: acc4, or with the packed records (c.packed) their acc components only
 // (dfsph_get_buffer unpacks acc4 on demand)
 __device__ __forceinline__ void put_acc(const Ctx& c, int i, float ax, float ay, float az) {
@@ -1864,6 +1870,7 @@ __device__ __forceinline__ void density_alpha_one(const Ctx& c, const KC& K, int
     if (FEAT) vi = V(c, i);
     int cnt = c.count[i];
     const int base = c.ws[w];
+    const unsigned sm = slot_mask(c, i);
     const WorldRecs q4 = WorldRecs::of(c.q4, base, c.n), qd = WorldRecs::of(reinterpret_cast<const f3*>(c.qd), base, c.n);
     for_rows<WIDE, QVd>(
         c, i, cnt, [&](unsigned e) { return QVd{q4.f4(e), FEAT ? qd.v3(e) : f3{0.0f, 0.0f, 0.0f}}; },
@@ -1881,13 +1888,13 @@ __device__ __forceinline__ void density_alpha_one(const Ctx& c, const KC& K, int
     float psi = 0.0f, lam = 0.0f;
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b) {
-        if (c.s_shape[b4 + b] < 0) continue;
+        if (!((sm >> b) & 1u)) continue;
         psi += c.s_psi[b4 + b];
         f3 gb = ld3(c.s_gpsi, b4 + b);
         G = G + gb;
         if (FEAT) rate += dot(vi - ld3(c.s_vb, b4 + b), gb);
         for (int cc = b + 1; cc < 4; ++cc)
-            if (c.s_shape[b4 + cc] >= 0) lam += fmx(-2.0f * dot(gb, ld3(c.s_gpsi, b4 + cc)), 0.0f);
+            if ((sm >> cc) & 1u) lam += fmx(-2.0f * dot(gb, ld3(c.s_gpsi, b4 + cc)), 0.0f);
     }
     float tot = dot(G, G) + sq + lam;
     lam += fmx(c.tot_min - tot, 0.0f);
@@ -1986,13 +1993,14 @@ __device__ float density_rate(const Ctx& c, int i, int w) {
     float rate = 0.0f;
     int cnt = c.count[i];
     const int base = c.ws[w];
+    const unsigned sm = slot_mask(c, i);
     const WorldRecs q4 = WorldRecs::of(c.q4, base, c.n), qd = WorldRecs::of(reinterpret_cast<const f3*>(c.qd), base, c.n);
     for_rows_g<WIDE, QV>(c, i, xi, cnt, (c.pair_cache & DFSPH_PAIR_CACHE_OTHER) != 0,
                          [&](unsigned e) { return QV{q4.f4(e), qd.v3(e)}; },
                          [&](QV d, f3 g) { rate += (d.q.w * inv_r0) * dot(vi - d.v, g); });
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
-        if (c.s_shape[b4 + b] >= 0) rate += dot(vi - ld3(c.s_vb, b4 + b), ld3(c.s_gpsi, b4 + b));
+        if ((sm >> b) & 1u) rate += dot(vi - ld3(c.s_vb, b4 + b), ld3(c.s_gpsi, b4 + b));
     return rate;
 }
 
@@ -2079,11 +2087,13 @@ struct QKR {
 template <bool WIDE>
 __device__ f3 pressure_accel(const Ctx& c, const float* kap, const float* kapr, int slot, int i, int w) {
     f3 xi = xyz(c.q4[i]);
-    float ki = kap[i] / c.rho0[i];
+    const float kap_i = kap[i];
+    float ki = kap_i / c.rho0[i];
     f3 a{0.0f, 0.0f, 0.0f};
     int cnt = c.count[i];
     const int base = c.ws[w];
     const bool cached = (c.pair_cache & DFSPH_PAIR_CACHE_ACCEL) != 0;
+    const unsigned sm = slot_mask(c, i);
     if (c.packed) {
         const float mw = c.w_mass[w];
         const WorldRecs qk = WorldRecs::of(c.qk + int64_t(slot) * c.n, base, c.n);
@@ -2096,7 +2106,7 @@ __device__ f3 pressure_accel(const Ctx& c, const float* kap, const float* kapr, 
     }
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
-        if (c.s_shape[b4 + b] >= 0) a = a - kap[i] * ld3(c.s_gpsi, b4 + b);
+        if ((sm >> b) & 1u) a = a - kap_i * ld3(c.s_gpsi, b4 + b);
     return a;
 }
 
@@ -2116,6 +2126,7 @@ __device__ float accel_divergence(const Ctx& c, int i, int w) {
     int cnt = c.count[i];
     const int base = c.ws[w];
     const bool cached = (c.pair_cache & DFSPH_PAIR_CACHE_APPLY) != 0;
+    const unsigned sm = slot_mask(c, i);
     if (c.packed) {  // one 16 B and one 8 B gather per neighbour; its mass is the world's
         const float mw = c.w_mass[w];
         const WorldRecs qa = WorldRecs::of(c.qa, base, c.n), a2 = WorldRecs::of(c.a2, base, c.n);
@@ -2128,7 +2139,7 @@ __device__ float accel_divergence(const Ctx& c, int i, int w) {
     }
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b)
-        if (c.s_shape[b4 + b] >= 0) ar += dot(ai, ld3(c.s_gpsi, b4 + b));
+        if ((sm >> b) & 1u) ar += dot(ai, ld3(c.s_gpsi, b4 + b));
     return ar;
 }
 
@@ -2699,11 +2710,16 @@ __global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, f
     if (!valid || !is_fluid(c, i)) return;
     int64_t b4 = 4 * int64_t(i);
     bool lone = c.count[i] == 0;
-    if (lone && c.kappa[i] > 0.0f) {  // contact_limit_kernel
+    // read before pressure_accel's neighbour loop, which orders its loads before what follows it
+    const f3 v0 = V(c, i);
+    const float m_i = c.mass[i];
+    const unsigned sm = slot_mask(c, i);
+    float kap = c.kappa[i];
+    if (lone && kap > 0.0f) {  // contact_limit_kernel
         f3 G{0.0f, 0.0f, 0.0f}, vb{0.0f, 0.0f, 0.0f};
         float wsum = 0.0f;
         for (int b = 0; b < 4; ++b)
-            if (c.s_shape[b4 + b] >= 0) {
+            if ((sm >> b) & 1u) {
                 G = G + ld3(c.s_gpsi, b4 + b);
                 vb = vb + c.s_psi[b4 + b] * ld3(c.s_vb, b4 + b);
                 wsum += c.s_psi[b4 + b];
@@ -2711,17 +2727,17 @@ __global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, f
         float g = length(G);
         if (g > 0.0f) {
             if (wsum > 0.0f) vb = vb / wsum;
-            float u = dot(V(c, i) - vb, (-G) / g);
-            c.kappa[i] = fmn(c.kappa[i], fmx(c.v_sep - u, 0.0f) / (dt * g));
+            float u = dot(v0 - vb, (-G) / g);
+            kap = fmn(kap, fmx(c.v_sep - u, 0.0f) / (dt * g));
+            c.kappa[i] = kap;
         }
     }
     // apply_pressure_kernel
     f3 a = pressure_accel<WIDE>(c, c.kappa, c.kr, VR_KAPPA, i, c.ch_world[b_]);
-    f3 v = V(c, i) + dt * a;
-    float kap = c.kappa[i];
-    float mk = c.mass[i] * kap * dt;
+    f3 v = v0 + dt * a;
+    float mk = m_i * kap * dt;
     for (int b = 0; b < 4; ++b)
-        if (c.s_shape[b4 + b] >= 0) st3(c.s_imp, b4 + b, ld3(c.s_imp, b4 + b) + mk * ld3(c.s_gpsi, b4 + b));
+        if ((sm >> b) & 1u) st3(c.s_imp, b4 + b, ld3(c.s_imp, b4 + b) + mk * ld3(c.s_gpsi, b4 + b));
     store[i] = mode == 0 ? kap * dt : kap * dt * dt;
     // lone_friction_kernel
     if (c.friction && lone && kap > 0.0f) {
@@ -2729,7 +2745,7 @@ __global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, f
         float wsum = 0.0f, pbest = 0.0f;
         int best = -1;
         for (int b = 0; b < 4; ++b) {
-            if (c.s_shape[b4 + b] >= 0) {
+            if ((sm >> b) & 1u) {
                 float ps = c.s_psi[b4 + b];
                 G = G + ld3(c.s_gpsi, b4 + b);
                 vb = vb + ps * ld3(c.s_vb, b4 + b);
@@ -2753,7 +2769,7 @@ __global__ void __launch_bounds__(256) k_post_solve(Ctx c, int mode, float dt, f
                     if (lt > 0.0f) {
                         f3 dv = fmn(1.0f, mu * dt * kap * g / lt) * vt;
                         v = v - dv;
-                        st3(c.s_imp, b4 + best, ld3(c.s_imp, b4 + best) + c.mass[i] * dv);
+                        st3(c.s_imp, b4 + best, ld3(c.s_imp, b4 + best) + m_i * dv);
                     }
                 }
             }
