@@ -745,11 +745,32 @@ __device__ __forceinline__ int block_scan_excl(int v, int* tot) {
     return off + x - v;
 }
 
+// The cell scan in three passes over blocks of SCAN_B cells (SCAN_T consecutive cells per thread, loaded as int4
+// where the block lies inside the array): block totals, an exclusive scan of the totals in one block, then each
+// block's exclusive prefix.
+constexpr int SCAN_T = 16, SCAN_B = 256 * SCAN_T;
+__device__ __forceinline__ void scan_load(const int* in, int n, int base, int (&v)[SCAN_T]) {
+    if (base + SCAN_T <= n) {
+        const int4* p = reinterpret_cast<const int4*>(in + base);
+#pragma unroll
+        for (int q = 0; q < SCAN_T / 4; ++q) {
+            const int4 x = p[q];
+            v[4 * q] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < SCAN_T; ++k) v[k] = base + k < n ? in[base + k] : 0;
+    }
+}
 __global__ void __launch_bounds__(256) k_scan_reduce(const int* in, int n, int* part) {
-    int base = blockIdx.x * 1024 + threadIdx.x * 4;
-    int s = 0;
-    for (int k = 0; k < 4; ++k)
-        if (base + k < n) s += in[base + k];
+    const int base = blockIdx.x * SCAN_B + threadIdx.x * SCAN_T;
+    int v[SCAN_T], s = 0;
+    scan_load(in, n, base, v);
+#pragma unroll
+    for (int k = 0; k < SCAN_T; ++k) s += v[k];
     int tot;
     block_scan_excl(s, &tot);
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
@@ -775,17 +796,33 @@ __global__ void __launch_bounds__(256) k_scan_top(int* part, int nb, int* out, i
     if (threadIdx.x == 0) out[n] = running;
 }
 __global__ void __launch_bounds__(256) k_scan_apply(const int* in, int n, const int* part, int* out) {
-    int base = blockIdx.x * 1024 + threadIdx.x * 4;
-    int v[4], s = 0;
-    for (int k = 0; k < 4; ++k) {
-        v[k] = base + k < n ? in[base + k] : 0;
-        s += v[k];
-    }
+    const int base = blockIdx.x * SCAN_B + threadIdx.x * SCAN_T;
+    int v[SCAN_T], s = 0;
+    scan_load(in, n, base, v);
+#pragma unroll
+    for (int k = 0; k < SCAN_T; ++k) s += v[k];
     int tot;
     int ex = block_scan_excl(s, &tot) + part[blockIdx.x];
-    for (int k = 0; k < 4; ++k) {
-        if (base + k < n) out[base + k] = ex;
-        ex += v[k];
+    if (base + SCAN_T <= n) {
+        int4* p = reinterpret_cast<int4*>(out + base);
+#pragma unroll
+        for (int q = 0; q < SCAN_T / 4; ++q) {
+            int4 x;
+            x.x = ex;
+            ex += v[4 * q];
+            x.y = ex;
+            ex += v[4 * q + 1];
+            x.z = ex;
+            ex += v[4 * q + 2];
+            x.w = ex;
+            ex += v[4 * q + 3];
+            p[q] = x;
+        }
+    } else {
+        for (int k = 0; k < SCAN_T; ++k) {
+            if (base + k < n) out[base + k] = ex;
+            ex += v[k];
+        }
     }
 }
 
@@ -1287,6 +1324,8 @@ __global__ void __launch_bounds__(256) k_skin_filter(Ctx c) {
 constexpr int PW_ROW = 32;  // wall particles recorded per liquid particle for the reaction split
 
 // rec >= 0: record the visited wall particles of liquid particle rec (wr_id / wr_w / wr_n)
+constexpr int PW_U = 4;  // wall candidates loaded together in pw_sample (q4s holds BUILD_U >= PW_U padding records)
+static_assert(BUILD_U >= PW_U, "pw_sample reads its batches unclamped");
 __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv, int rec) {
     const KC K = make_kc(c.h, c.kind);
     col = 0.0f;
@@ -1311,26 +1350,35 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv,
                 if (X < 0 || Y < 0 || Z < 0 || X >= d[0] || Y >= d[1] || Z >= d[2]) continue;
                 int cc = c.cell_base[g] + (X * d[1] + Y) * d[2] + Z;
                 // q4s: the grid's cell-ordered copy of q4 with the particle id in w (the same positions; one
-                // contiguous load per candidate). Outside the support W = 0 makes V_j W zero, so V_j is read for
-                // the candidates within h only.
-                for (int e = c.cell_start[cc]; e < c.cell_start[cc + 1]; ++e) {
-                    const float4 p4 = c.q4s[e];
-                    f3 dd = x - f3{p4.x, p4.y, p4.z};
-                    if (dot(dd, dd) > out2) continue;  // kernel_w is 0 there
-                    float kw = kernel_w(dist(dd, K), K);
-                    if (!(kw > 0.0f)) continue;
-                    int j = __float_as_int(p4.w);
-                    float vj = c.wvol[j];
-                    float wj = vj * kw;
-                    if (wj > 0.0f) {
-                        col += wj;
-                        gc = gc + vj * kernel_grad(dd, K);
-                        mv = mv + wj * V(c, j);
-                        if (rec >= 0 && nrec < PW_ROW) {
-                            c.wr_id[int64_t(nrec) * c.n + rec] = c.wcidx[j];
-                            c.wr_w[int64_t(nrec) * c.n + rec] = wj;
+                // contiguous load per candidate, PW_U of them in flight, read unclamped from the padded array, then
+                // taken in order). Outside the support W = 0 makes V_j W zero, so V_j is read for the candidates
+                // within h only.
+                const int e1 = c.cell_start[cc + 1];
+                for (int e0 = c.cell_start[cc]; e0 < e1; e0 += PW_U) {
+                    float4 pb[PW_U];
+#pragma unroll
+                    for (int u = 0; u < PW_U; ++u) pb[u] = c.q4s[e0 + u];
+#pragma unroll
+                    for (int u = 0; u < PW_U; ++u) {
+                        if (e0 + u >= e1) continue;
+                        const float4 p4 = pb[u];
+                        f3 dd = x - f3{p4.x, p4.y, p4.z};
+                        if (dot(dd, dd) > out2) continue;  // kernel_w is 0 there
+                        float kw = kernel_w(dist(dd, K), K);
+                        if (!(kw > 0.0f)) continue;
+                        int j = __float_as_int(p4.w);
+                        float vj = c.wvol[j];
+                        float wj = vj * kw;
+                        if (wj > 0.0f) {
+                            col += wj;
+                            gc = gc + vj * kernel_grad(dd, K);
+                            mv = mv + wj * V(c, j);
+                            if (rec >= 0 && nrec < PW_ROW) {
+                                c.wr_id[int64_t(nrec) * c.n + rec] = c.wcidx[j];
+                                c.wr_w[int64_t(nrec) * c.n + rec] = wj;
+                            }
+                            nrec += 1;
                         }
-                        nrec += 1;
                     }
                 }
             }
@@ -2018,30 +2066,37 @@ __global__ void __launch_bounds__(256) k_solve_init(Ctx c, int mode, float dt, f
         c.lower[i] = 0.0f;
         return;
     }
+    // read before density_rate's neighbour loop, which orders its loads before what follows it
+    const bool use_learned = learned && (!c.learn_gate || c.learn_on[c.ch_world[b_]]);
+    const float store_i = store[i], learned_i = use_learned ? learned[i] : 0.0f, rho0_i = c.rho0[i];
+    const float kd_i = mode == 0 && c.two_sided ? c.kd[i] : 0.0f, neff_i = mode == 0 ? c.neff[i] : 0.0f;
+    const float ratio_i = mode == 0 ? 0.0f : c.ratio[i];
     float rate = density_rate<WIDE>(c, i, c.ch_world[b_]);
+    float kap;
     if (mode == 0) {
         float kdv = 0.0f;
-        if (c.two_sided) kdv = fmx(warm_den * fmn(c.kd[i], 2.5e-4f) / (dt * dt), 0.0f);
+        if (c.two_sided) kdv = fmx(warm_den * fmn(kd_i, 2.5e-4f) / (dt * dt), 0.0f);
         float lb = -kdv;
         float rr = rate;
         if (kdv <= 0.0f) rr = fmx(rate, 0.0f);
-        if (c.neff[i] < c.min_nb) {
+        if (neff_i < c.min_nb) {
             rr = 0.0f;
             lb = 0.0f;
         }
         c.src[i] = rr;
         c.lower[i] = lb;
-        c.kappa[i] = fmx(warm * fmn(store[i], 0.5f) / dt, lb);
-        if (learned && (!c.learn_gate || c.learn_on[c.ch_world[b_]])) c.kappa[i] = fmx(learned[i], lb);
+        kap = fmx(warm * fmn(store_i, 0.5f) / dt, lb);
+        if (use_learned) kap = fmx(learned_i, lb);
     } else {
-        float rs = c.ratio[i] + dt * rate;
-        if (c.ratio[i] - 1.0f > relief) rs = 1.0f + relief + dt * rate;
+        float rs = ratio_i + dt * rate;
+        if (ratio_i - 1.0f > relief) rs = 1.0f + relief + dt * rate;
         c.src[i] = rs;
         c.lower[i] = 0.0f;
-        c.kappa[i] = warm * fmn(store[i], 2.5e-4f) / (dt * dt);
-        if (learned && (!c.learn_gate || c.learn_on[c.ch_world[b_]])) c.kappa[i] = fmx(learned[i], 0.0f);
+        kap = warm * fmn(store_i, 2.5e-4f) / (dt * dt);
+        if (use_learned) kap = fmx(learned_i, 0.0f);
     }
-    put_vr(c, c.kr, VR_KAPPA, i, c.kappa[i] / c.rho0[i]);
+    c.kappa[i] = kap;
+    put_vr(c, c.kr, VR_KAPPA, i, kap / rho0_i);
 }
 
 // pressure.py:355-367
@@ -2252,10 +2307,13 @@ __device__ __forceinline__ void apply_chunk(const Ctx& c, int b_, int mode, floa
             }
             if (full) {
                 float sc = mode == 0 ? dt : dt * dt;
-                float v = -sc * (accel_divergence<WIDE>(c, i, c.ch_world[b_]) - c.reg[i] * vec[i]);
-                px = double(vec[i] * v);
-                if (bias) v -= (mode == 0 ? c.src[i] : c.src[i] - 1.0f);
-                if (accum) v += out[i];
+                // read before accel_divergence's neighbour loop, which orders its loads before what follows it
+                const float vec_i = vec[i], reg_i = c.reg[i];
+                const float src_i = bias ? c.src[i] : 0.0f, out_i = accum ? out[i] : 0.0f;
+                float v = -sc * (accel_divergence<WIDE>(c, i, c.ch_world[b_]) - reg_i * vec_i);
+                px = double(vec_i * v);
+                if (bias) v -= (mode == 0 ? src_i : src_i - 1.0f);
+                if (accum) v += out_i;
                 out[i] = v;
                 nb = pass_bytes<WIDE>(c, i, 77u + (bias ? 4u : 0u) + (accum ? 4u : 0u) +
                                                 (gated && c.sparse_corr ? 1u : 0u), 9u);
@@ -2672,26 +2730,28 @@ __global__ void __launch_bounds__(256) k_jacobi(Ctx c, int mode, float dt) {
         if (!is_fluid(c, i)) {
             c.err[i] = 0.0f;
         } else {
-            float ar = accel_divergence<WIDE>(c, i, w) - c.reg[i] * c.kappa[i];
+            // read before accel_divergence's neighbour loop, which orders its loads before what follows it
+            const float kap_i = c.kappa[i], reg_i = c.reg[i], src_i = c.src[i], lb = c.lower[i];
+            const float alpha_i = c.alpha[i], rho0_i = c.rho0[i], neff_i = c.neff[i];
+            float ar = accel_divergence<WIDE>(c, i, w) - reg_i * kap_i;
             float s, scale;
             if (mode == 0) {
                 ar *= dt;
-                s = -c.src[i];
+                s = -src_i;
                 scale = 1.0f / dt;
             } else {
                 ar *= dt * dt;
-                s = 1.0f - c.src[i];
+                s = 1.0f - src_i;
                 scale = 1.0f / (dt * dt);
             }
             float rr = s - ar;
-            float lb = c.lower[i];
-            float k_new = fmx(c.kappa[i] - c.omega * rr * c.alpha[i] * scale, lb);
+            float k_new = fmx(kap_i - c.omega * rr * alpha_i * scale, lb);
             c.kappa[i] = k_new;
-            put_vr(c, c.kr, VR_KAPPA, i, k_new / c.rho0[i]);
+            put_vr(c, c.kr, VR_KAPPA, i, k_new / rho0_i);
             float e = fmx(-rr, 0.0f);
             if (k_new > lb) e = fabsf(rr);
             if (mode == 0) {
-                if (c.neff[i] < c.min_nb) e = 0.0f;
+                if (neff_i < c.min_nb) e = 0.0f;
                 e *= dt;
             }
             c.err[i] = e;
@@ -4760,7 +4820,7 @@ int32_t dfsph_create(const DfsphParams* P, const DfsphLayout* L, const DfsphMode
             TRY(dalloc(H, &c.wr_id, nw > 0 ? int64_t(PW_ROW) * n : 1));
             TRY(dalloc(H, &c.wr_w, nw > 0 ? int64_t(PW_ROW) * n : 1));
         }
-        H->n_scan_blocks = blocks_for(c.n_cells, 1024);
+        H->n_scan_blocks = blocks_for(c.n_cells, SCAN_B);
         TRY(dalloc(H, &c.scan_part, H->n_scan_blocks));
         TRY(dalloc(H, &c.items, n));
         TRY(dalloc(H, &c.items_u, n));
