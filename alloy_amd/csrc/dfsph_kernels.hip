@@ -937,35 +937,40 @@ __device__ __forceinline__ void build_row(const Ctx& c, int i, int g, f3 xi, flo
     cell_of(c, g, xi, cx, cy, cz);
     const int* d = c.g_dims + 3 * g;
     int nn = 0, total = 0;
-    // x outer, y, then the contiguous z-row of 3 cells, ids ascending within a cell
-    for (int dx = -1; dx <= 1; ++dx) {
-        int X = cx + dx;
-        if (X < 0 || X >= d[0]) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            int Y = cy + dy;
-            if (Y < 0 || Y >= d[1]) continue;
-            int z0 = max(cz - 1, 0), z1 = min(cz + 1, d[2] - 1);
-            int cb = c.cell_base[g] + (X * d[1] + Y) * d[2];
-            int e0 = c.cell_start[cb + z0], e1 = c.cell_start[cb + z1 + 1];
-            for (int e = e0; e < e1; e += BUILD_U) {
-                if (nn > ROW_BUF - BUILD_U) {  // room for a whole batch: keep the lowest cap (cap + BUILD_U <= ROW_BUF)
-                    sort_row<RowT, BB>(L, nn);
-                    nn = cap;
-                }
-                const float4* pe = c.q4s + e;
-                float4 pc[BUILD_U];
+    // x outer, y, then the contiguous z-row of 3 cells, ids ascending within a cell: the nine rows r = 3 (dx + 1) +
+    // (dy + 1) in order, the range of row r + 1 loaded before row r's candidates (an empty range off the grid)
+    const int z0 = max(cz - 1, 0), z1 = min(cz + 1, d[2] - 1);
+    auto row_range = [&](int r, int& a, int& b) {
+        const int X = cx + r / 3 - 1, Y = cy + r % 3 - 1;
+        a = b = 0;
+        if (X < 0 || X >= d[0] || Y < 0 || Y >= d[1]) return;
+        const int cb = c.cell_base[g] + (X * d[1] + Y) * d[2];
+        a = c.cell_start[cb + z0];
+        b = c.cell_start[cb + z1 + 1];
+    };
+    int n0, n1;
+    row_range(0, n0, n1);
+    for (int r = 0; r < 9; ++r) {
+        const int e0 = n0, e1 = n1;
+        if (r < 8) row_range(r + 1, n0, n1);
+        for (int e = e0; e < e1; e += BUILD_U) {
+            if (nn > ROW_BUF - BUILD_U) {  // room for a whole batch: keep the lowest cap (cap + BUILD_U <= ROW_BUF)
+                sort_row<RowT, BB>(L, nn);
+                nn = cap;
+            }
+            const float4* pe = c.q4s + e;
+            float4 pc[BUILD_U];
 #pragma unroll
-                for (int u = 0; u < BUILD_U; ++u) pc[u] = pe[u];
+            for (int u = 0; u < BUILD_U; ++u) pc[u] = pe[u];
 #pragma unroll
-                for (int u = 0; u < BUILD_U; ++u) {
-                    const int j = __float_as_int(pc[u].w);
-                    const f3 dd = xyz(pc[u]) - xi;
-                    const bool ok = (e + u < e1) & (j != i) & (dot(dd, dd) < h2);
-                    // written whether accepted or not (a rejected entry is overwritten by the next), no branch
-                    L[nn * BB] = RowT(j - base);
-                    total += ok ? 1 : 0;
-                    nn += ok ? 1 : 0;
-                }
+            for (int u = 0; u < BUILD_U; ++u) {
+                const int j = __float_as_int(pc[u].w);
+                const f3 dd = xyz(pc[u]) - xi;
+                const bool ok = (e + u < e1) & (j != i) & (dot(dd, dd) < h2);
+                // written whether accepted or not (a rejected entry is overwritten by the next), no branch
+                L[nn * BB] = RowT(j - base);
+                total += ok ? 1 : 0;
+                nn += ok ? 1 : 0;
             }
         }
     }
