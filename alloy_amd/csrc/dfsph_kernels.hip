@@ -362,6 +362,24 @@ __device__ __forceinline__ RowRaw row_raw(const Ctx& c, int i, int k0) {
     }
     return w;
 }
+// the next batch's quads (k0 + NB_U..) where `next`, else batch k0's again: a lane that has no next batch re-reads
+// lines it has just read instead of fetching a row past its count (within the ROW_PAD rows past the capacity)
+template <bool WIDE>
+__device__ __forceinline__ RowRaw row_raw_next(const Ctx& c, int i, int k0, bool next) {
+    const uint32_t n = uint32_t(c.n);
+    const __amdgpu_buffer_rsrc_t r16 = buf_rsrc(c.nbr + int64_t(k0) * n, 4u * NB_U * n);
+    const __amdgpu_buffer_rsrc_t r8 = buf_rsrc(c.nbr_hi + (WIDE ? int64_t(k0) * n : 0), 2u * NB_U * n);
+    const unsigned step = next ? unsigned(NB_U / 4) * n : 0u;  // quads of the next batch
+    RowRaw w;
+#pragma unroll
+    for (int q = 0; q < NB_U / 4; ++q) {
+        w.v[q] = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r16, 8u * (unsigned(i) + step),
+                                                                                 int(8u * q * n), 0));
+        w.hb[q] = WIDE ? __builtin_amdgcn_raw_buffer_load_b32(r8, 4u * (unsigned(i) + step), int(4u * q * n), 0)
+                       : 0u;
+    }
+    return w;
+}
 template <bool WIDE>
 __device__ __forceinline__ void row_unpack(const RowRaw& w, unsigned (&e)[NB_U]) {
 #pragma unroll
@@ -393,7 +411,7 @@ __device__ __forceinline__ void for_rows(const Ctx& c, int i, int cnt, LoadF loa
         T d[NB_U];
 #pragma unroll
         for (int u = 0; u < NB_U; ++u) d[u] = load(e[u]);
-        raw = row_raw<WIDE>(c, i, k0 + NB_U);
+        raw = row_raw_next<WIDE>(c, i, k0, k0 + NB_U < cnt);
         __asm__ volatile("" ::: "memory");
 #pragma unroll
         for (int u = 0; u < NB_U; ++u)
@@ -428,7 +446,7 @@ __device__ __forceinline__ void rows_g_loop(const Ctx& c, int i, f3 xi, int cnt,
         T d[NB_U];
 #pragma unroll
         for (int u = 0; u < NB_U; ++u) d[u] = load(e[u]);
-        raw = row_raw<WIDE>(c, i, k0 + NB_U);
+        raw = row_raw_next<WIDE>(c, i, k0, k0 + NB_U < cnt);
         __asm__ volatile("" ::: "memory");
 #pragma unroll
         for (int u = 0; u < NB_U; ++u) {
