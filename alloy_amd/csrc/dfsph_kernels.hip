@@ -731,7 +731,7 @@ __global__ void __launch_bounds__(256) k_groups_hist(Ctx c) {
     if (g >= 0) c.ch_live[c.w_chunk[g] + (i - c.ws[g]) / CHUNK] = 1;
     f3 xi = Q(c, i);
     c.q4[i] = make_float4(xi.x, xi.y, xi.z, c.mass[i]);
-    if (c.packed) {  // the w components are written by the vector's producer before any read this step
+    if (c.packed && g >= 0) {  // liquid only (the rows list liquid); w is written by the vector's producer before any read
         for (int s = 0; s < 3; ++s) c.qk[int64_t(s) * c.n + i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
         c.qa[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
     }
@@ -1424,9 +1424,8 @@ __device__ float colour_distance(const Ctx& c, float col) {
 // particle_walls.py:138-177 _accumulate_kernel: each particle-wall slot's impulse J, split over the world's wall
 // particles by V_j W_ij / c_i in int64 at 2^50 (the rounding remainder to the largest share, so the parts sum exactly
 // to the slot's fixed-point impulse), and the walls' work -J.v_b
-__global__ void __launch_bounds__(256) k_pw_accumulate(Ctx c) {
-    int i = xcd_particle();
-    if (i >= c.n) return;
+template <class AddF>
+__device__ __forceinline__ void pw_accumulate_one(const Ctx& c, int i, AddF add) {
     const KC K = make_kc(c.h, c.kind);
     const double IMP = 1125899906842624.0;
     int64_t b4 = 4 * int64_t(i);
@@ -1449,11 +1448,8 @@ __global__ void __launch_bounds__(256) k_pw_accumulate(Ctx c) {
             if (sj > 0.0f) {
                 long long part[3] = {llround(double(J.x * sj) * IMP), llround(double(J.y * sj) * IMP),
                                      llround(double(J.z * sj) * IMP)};
-                for (int q = 0; q < 3; ++q) {
-                    atomicAdd(reinterpret_cast<unsigned long long*>(c.pw_exch) + 3 * int64_t(cj) + q,
-                              static_cast<unsigned long long>(part[q]));
-                    left[q] -= part[q];
-                }
+                add(cj, part);
+                for (int q = 0; q < 3; ++q) left[q] -= part[q];
                 if (sj > sbest) {
                     sbest = sj;
                     best = cj;
@@ -1480,12 +1476,47 @@ __global__ void __launch_bounds__(256) k_pw_accumulate(Ctx c) {
                         }
                     }
         }
-        if (best >= 0)
-            for (int q = 0; q < 3; ++q)
-                atomicAdd(reinterpret_cast<unsigned long long*>(c.pw_exch) + 3 * int64_t(best) + q,
-                          static_cast<unsigned long long>(left[q]));
+        if (best >= 0) add(best, left);
     }
 }
+// The wall particles' shares go through a per-block table in LDS (open addressing on the compact wall index, int64
+// sums), flushed with one global atomic per touched wall particle and component: a block's liquid pushes on a few
+// hundred wall particles many times over. Integer sums do not depend on their order, so the totals are those of
+// one global atomic per share; a share whose probe finds no free slot goes to global memory directly.
+constexpr int EXCH_T = 1024;  // slots of the block's table (28 KB of LDS)
+__global__ void __launch_bounds__(256) k_pw_accumulate(Ctx c) {
+    __shared__ int tkey[EXCH_T];
+    __shared__ unsigned long long tval[3 * EXCH_T];
+    for (int t = threadIdx.x; t < EXCH_T; t += blockDim.x) {
+        tkey[t] = -1;
+        tval[3 * t] = tval[3 * t + 1] = tval[3 * t + 2] = 0ull;
+    }
+    __syncthreads();
+    unsigned long long* gx = reinterpret_cast<unsigned long long*>(c.pw_exch);
+    const int i = xcd_particle();
+    if (i < c.n)
+        pw_accumulate_one(c, i, [&](int cj, const long long (&v)[3]) {
+            const unsigned h = (unsigned(cj) * 2654435761u) >> 22;  // 10 bits
+            for (int p = 0; p < 16; ++p) {
+                const int t = int((h + unsigned(p)) & unsigned(EXCH_T - 1));
+                const int k = atomicCAS(&tkey[t], -1, cj);
+                if (k == -1 || k == cj) {
+                    for (int q = 0; q < 3; ++q) atomicAdd(&tval[3 * t + q], static_cast<unsigned long long>(v[q]));
+                    return;
+                }
+            }
+            for (int q = 0; q < 3; ++q) atomicAdd(gx + 3 * int64_t(cj) + q, static_cast<unsigned long long>(v[q]));
+        });
+    __syncthreads();
+    for (int t = threadIdx.x; t < EXCH_T; t += blockDim.x) {
+        const int cj = tkey[t];
+        if (cj < 0) continue;
+        for (int q = 0; q < 3; ++q)
+            if (tval[3 * t + q] != 0ull) atomicAdd(gx + 3 * int64_t(cj) + q, tval[3 * t + q]);
+    }
+}
+
+
 
 // ----------------------------------------------------------------------------------------------- boundary slots
 struct SlotRow {
