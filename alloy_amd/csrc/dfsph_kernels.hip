@@ -1366,45 +1366,53 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv,
     // |d|^2 > 1.0001 h^2 (rounded either way) puts |d| above h (1 + 4e-5) and its rounded quotient by h above 1, where
     // the kernel is 0: such a candidate is skipped before the kernel's square root and quotient
     const float out2 = 1.0001f * (c.h * c.h);
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                int X = cx + dx, Y = cy + dy, Z = cz + dz;
-                if (X < 0 || Y < 0 || Z < 0 || X >= d[0] || Y >= d[1] || Z >= d[2]) continue;
-                int cc = c.cell_base[g] + (X * d[1] + Y) * d[2] + Z;
-                // q4s: the grid's cell-ordered copy of q4 with the particle id in w (the same positions; one
-                // contiguous load per candidate, PW_U of them in flight, read unclamped from the padded array, then
-                // taken in order). Outside the support W = 0 makes V_j W zero, so V_j is read for the candidates
-                // within h only.
-                const int e1 = c.cell_start[cc + 1];
-                for (int e0 = c.cell_start[cc]; e0 < e1; e0 += PW_U) {
-                    float4 pb[PW_U];
+    // the 27 cells in the order z, y, x (x fastest), r = 9 (dz + 1) + 3 (dy + 1) + (dx + 1); the range of cell r + 1
+    // is loaded before cell r's candidates (an empty range off the grid)
+    auto cell_range = [&](int r, int& a0, int& a1) {
+        const int X = cx + r % 3 - 1, Y = cy + (r / 3) % 3 - 1, Z = cz + r / 9 - 1;
+        a0 = a1 = 0;
+        if (X < 0 || Y < 0 || Z < 0 || X >= d[0] || Y >= d[1] || Z >= d[2]) return;
+        const int cc = c.cell_base[g] + (X * d[1] + Y) * d[2] + Z;
+        a0 = c.cell_start[cc];
+        a1 = c.cell_start[cc + 1];
+    };
+    int n0, n1;
+    cell_range(0, n0, n1);
+    for (int r = 0; r < 27; ++r) {
+        const int b0 = n0, e1 = n1;
+        if (r < 26) cell_range(r + 1, n0, n1);
+        // q4s: the grid's cell-ordered copy of q4 with the particle id in w (the same positions; one
+        // contiguous load per candidate, PW_U of them in flight, read unclamped from the padded array, then
+        // taken in order). Outside the support W = 0 makes V_j W zero, so V_j is read for the candidates
+        // within h only.
+        for (int e0 = b0; e0 < e1; e0 += PW_U) {
+            float4 pb[PW_U];
 #pragma unroll
-                    for (int u = 0; u < PW_U; ++u) pb[u] = c.q4s[e0 + u];
+            for (int u = 0; u < PW_U; ++u) pb[u] = c.q4s[e0 + u];
 #pragma unroll
-                    for (int u = 0; u < PW_U; ++u) {
-                        if (e0 + u >= e1) continue;
-                        const float4 p4 = pb[u];
-                        f3 dd = x - f3{p4.x, p4.y, p4.z};
-                        if (dot(dd, dd) > out2) continue;  // kernel_w is 0 there
-                        float kw = kernel_w(dist(dd, K), K);
-                        if (!(kw > 0.0f)) continue;
-                        int j = __float_as_int(p4.w);
-                        float vj = c.wvol[j];
-                        float wj = vj * kw;
-                        if (wj > 0.0f) {
-                            col += wj;
-                            gc = gc + vj * kernel_grad(dd, K);
-                            mv = mv + wj * V(c, j);
-                            if (rec >= 0 && nrec < PW_ROW) {
-                                c.wr_id[int64_t(nrec) * c.n + rec] = c.wcidx[j];
-                                c.wr_w[int64_t(nrec) * c.n + rec] = wj;
-                            }
-                            nrec += 1;
-                        }
+            for (int u = 0; u < PW_U; ++u) {
+                if (e0 + u >= e1) continue;
+                const float4 p4 = pb[u];
+                f3 dd = x - f3{p4.x, p4.y, p4.z};
+                if (dot(dd, dd) > out2) continue;  // kernel_w is 0 there
+                float kw = kernel_w(dist(dd, K), K);
+                if (!(kw > 0.0f)) continue;
+                int j = __float_as_int(p4.w);
+                float vj = c.wvol[j];
+                float wj = vj * kw;
+                if (wj > 0.0f) {
+                    col += wj;
+                    gc = gc + vj * kernel_grad(dd, K);
+                    mv = mv + wj * V(c, j);
+                    if (rec >= 0 && nrec < PW_ROW) {
+                        c.wr_id[int64_t(nrec) * c.n + rec] = c.wcidx[j];
+                        c.wr_w[int64_t(nrec) * c.n + rec] = wj;
                     }
+                    nrec += 1;
                 }
             }
+        }
+    }
     if (rec >= 0) c.wr_n[rec] = nrec <= PW_ROW ? nrec : -1;
     if (col > 0.0f) mv = mv / col;
 }
