@@ -134,12 +134,27 @@ def traffic(args) -> None:
         f.write("\n")
 
 
+def _split_names(spec: str) -> list[str]:
+    """Kernel names separated by commas outside template brackets ("k_a<false>,k_b<false, 128, false>")."""
+    out, cur, depth = [], "", 0
+    for ch in spec:
+        depth += (ch == "<") - (ch == ">")
+        if ch == "," and depth == 0:
+            out.append(cur.strip())
+            cur = ""
+        else:
+            cur += ch
+    if cur.strip():
+        out.append(cur.strip())
+    return out
+
+
 def counters(args) -> None:
     rows = _rows(args.dir, "counter_collection.csv")
     lines = [f"# {args.title}", "", f"Command: `{args.command}`", ""]
     if args.note:
         lines += [args.note, ""]
-    for k in args.kernels.split(","):
+    for k in _split_names(args.kernels):
         per = _per_dispatch(rows, k)
         if not per:
             continue
