@@ -1666,7 +1666,12 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
         }
         return;
     }
-    for (int k = 0; k < 4; ++k) st3(c.s_imp, b4 + k, f3{0.0f, 0.0f, 0.0f});  // _begin_step (solver.py:289-291)
+    // A row whose four slots were empty is clear (as above): its impulses are still zero, so _begin_step's clear
+    // (solver.py:289-291) is needed only for a row that had a slot, and a row that stays empty is not written again.
+    const int4 sh_old = *reinterpret_cast<const int4*>(c.s_shape + b4);
+    const bool was_empty = sh_old.x == -1 && sh_old.y == -1 && sh_old.z == -1 && sh_old.w == -1;
+    if (!was_empty)
+        for (int k = 0; k < 4; ++k) st3(c.s_imp, b4 + k, f3{0.0f, 0.0f, 0.0f});
     int w = c.world[i];
     f3 x = xyz(c.q4[i]);
     int nn = 0;
@@ -1698,6 +1703,7 @@ __global__ void __launch_bounds__(256) k_slots(Ctx c) {
         if (phi < c.reach && phi + c.offset < c.h)
             if (slot_insert(R, nn, c.S, phi) >= 0 && nn < 4) nn += 1;
     }
+    if (nn == 0 && was_empty) return;  // the row stays empty: its shapes, psi and grad psi are stored already
     // fill (boundary.py:290-333)
     float psi[4];
     f3 gp[4];
