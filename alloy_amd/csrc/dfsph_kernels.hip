@@ -549,6 +549,20 @@ __device__ __forceinline__ void chunk_reduce(const Ctx& c, int b_, double x, dou
     }
 }
 
+// chunk_reduce(c, b_, x, 0.0, valid ? 0.0 : -3.4e38) with one tree: the second sum of +0 terms is +0, and the max is
+// 0.0 when a lane of the block is valid, else -3.4e38
+__device__ __forceinline__ void chunk_reduce_x(const Ctx& c, int b_, double x, bool valid) {
+    __shared__ double lds[CHUNK];
+    const double sx = tree_sum(x, lds);
+    const bool any = __syncthreads_or(valid ? 1 : 0) != 0;
+    if (threadIdx.x == 0) {
+        double* p = c.part + 3 * int64_t(b_);
+        p[0] = sx;
+        p[1] = 0.0;
+        p[2] = any ? 0.0 : -3.4e38;
+    }
+}
+
 // The chunk partial of a chunk without liquid: every term is +0 and every max term 0.0 (each chunk has a particle)
 __device__ __forceinline__ void chunk_zero(const Ctx& c, int b_) {
     if (threadIdx.x == 0) {
@@ -2389,7 +2403,7 @@ __device__ __forceinline__ void apply_chunk(const Ctx& c, int b_, int mode, floa
         }
     }
     if (c.prof_bytes) account_bytes(c, KID_APPLY, b_, nb);
-    if (reduce) chunk_reduce(c, b_, px, 0.0, valid ? 0.0 : -3.4e38);
+    if (reduce) chunk_reduce_x(c, b_, px, valid);
 }
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_apply(Ctx c, int mode, float dt, const float* vec, float* out, int gated,
@@ -2547,7 +2561,7 @@ __device__ __forceinline__ void apply_h(const Ctx& c, int mode, float dt, const 
         }
     }
     if (c.prof_bytes) account_bytes(c, KID_APPLY, b_, nb);
-    if (reduce) chunk_reduce(c, b_, px, 0.0, valid ? 0.0 : -3.4e38);
+    if (reduce) chunk_reduce_x(c, b_, px, valid);
 }
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_apply_hp(Ctx c, int mode, float dt, const float* vec, float* out, int gated,
@@ -2591,7 +2605,7 @@ __global__ void __launch_bounds__(256) k_cg_init(Ctx c, int mode, float dt) {
             px = double(-wi * zi);
         }
     }
-    chunk_reduce(c, b_, px, 0.0, valid ? 0.0 : -3.4e38);
+    chunk_reduce_x(c, b_, px, valid);
 }
 
 // cg_step_kernel (krylov.py:153-177)
@@ -3508,7 +3522,7 @@ __global__ void __launch_bounds__(256) k_visc_apply(Ctx c, float dt, const float
             px = double(dot(vi, r));
         }
     }
-    chunk_reduce(c, b_, px, 0.0, valid ? 0.0 : -3.4e38);
+    chunk_reduce_x(c, b_, px, valid);
 }
 
 // viscosity.py:74-111 visc_init_kernel: the block preconditioner, r = b - A x, z = M^-1 r, p = z; partial (r.z, b.b)
