@@ -3000,11 +3000,8 @@ __global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
     const KC K = make_kc(c.h, c.kind);
     CHUNK_PROLOGUE
     if (!valid) return;
+    if (!is_fluid(c, i)) return;  // the velocity of a particle that is not liquid is left as it is
     f3 vi = V(c, i);
-    if (!is_fluid(c, i)) {
-        st3(c.vtmp, i, vi);
-        return;
-    }
     const int w = c.ch_world[b_];
     f3 a = ld3(c.gravity, w);
     if (c.f) a = a + ld3(c.f, i) / c.mass[i];
@@ -3027,6 +3024,8 @@ __global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
             });
         v = v + c.xsph * cr;
     }
+    // in place: the XSPH sums gather the velocities from the packed copy k_pack_vrho made before this pass
+    st3(c.qd, i, v);
     st3(c.vtmp, i, v);
 }
 
@@ -4638,7 +4637,6 @@ int phase_nonpressure(Handle* H, double dt, hipStream_t st) {
     if (c.n == 0) return DFSPH_OK;
     if (c.xsph > 0.0f) k_pack_vrho<<<blocks_for(c.n), 256, 0, st>>>(c);
     LAUNCH_WT(KID_NONPRESSURE, k_nonpressure, H->n_chunks, 256, c, float(dt));
-    k_copy_f32<<<blocks_for(3 * int64_t(c.n)), 256, 0, st>>>(c.qd, c.vtmp, 3 * int64_t(c.n));
     LAUNCH_CHECK();
     return run_csf(H, float(dt), st);  // solver.py:304-305: surface tension after the non-pressure update
 }

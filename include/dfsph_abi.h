@@ -171,7 +171,8 @@ enum DfsphBuffer {
     DFSPH_BUF_LOWER,
     DFSPH_BUF_KV,             /* warm start k dt (divergence) */
     DFSPH_BUF_KD,             /* warm start k dt^2 (density) */
-    DFSPH_BUF_VTMP,           /* f32    [n, 3] */
+    DFSPH_BUF_VTMP,           /* f32    [n, 3] the liquid's velocities after the last non-pressure update (the rows
+                                 of other particles are not written) */
     DFSPH_BUF_CONVERGED,      /* int32  [W] */
     DFSPH_BUF_ITERS,          /* int32  [W] */
     DFSPH_BUF_ERR_MEAN,       /* f32    [W] */
