@@ -2614,9 +2614,14 @@ __device__ __forceinline__ void cg_step_chunk(const Ctx& c, int b_) {
     bool valid = i < c.ch_end[b_];
     int w = c.ch_world[b_];
     if (c.converged[w] != 0 || !valid || !c.ch_live[b_]) return;
+    // corr and its quotients are written together, so where corr is +0 and stays +0 they hold +0 already (the
+    // packed record's w is also set to +0 at each list build) and are not written again
+    const bool corr_zero = __float_as_uint(c.corr[i]) == 0u;
     if (!is_fluid(c, i)) {
-        c.corr[i] = 0.0f;
-        put_vr(c, c.corrr, VR_CORR, i, 0.0f);
+        if (!corr_zero) {
+            c.corr[i] = 0.0f;
+            put_vr(c, c.corrr, VR_CORR, i, 0.0f);
+        }
         return;
     }
     double pap = c.sums[2 * w];
@@ -2635,8 +2640,10 @@ __device__ __forceinline__ void cg_step_chunk(const Ctx& c, int b_) {
         c.kappa[i] = k;
         put_vr(c, c.kr, VR_KAPPA, i, k / c.rho0[i]);
     }
-    c.corr[i] = cc;
-    put_vr(c, c.corrr, VR_CORR, i, cc / c.rho0[i]);
+    if (__float_as_uint(cc) != 0u || !corr_zero) {
+        c.corr[i] = cc;
+        put_vr(c, c.corrr, VR_CORR, i, cc / c.rho0[i]);
+    }
     c.wv[i] = c.wv[i] + a * c.ap[i];
 }
 __global__ void __launch_bounds__(256) k_cg_step(Ctx c) { cg_step_chunk(c, xcd_block(int(blockIdx.x), int(gridDim.x))); }
