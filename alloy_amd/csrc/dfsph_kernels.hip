@@ -735,28 +735,41 @@ __device__ __forceinline__ int cell_of(const Ctx& c, int g, f3 x, int& cx, int& 
 
 // neighbors.py:36-51 group per particle, then its cell key and the cell histogram
 __global__ void __launch_bounds__(256) k_groups_hist(Ctx c) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    int g = -1;
-    int fl = c.flags[i];
-    if ((fl & ACTIVE) != 0 && int(c.phase[i]) >= 1) g = c.world[i];
-    else if (c.NW > 0 && c.wvol[i] > 0.0f && (fl & ACTIVE) != 0 && c.world[i] >= 0) g = -2 - c.world[i];
-    c.group[i] = g;
-    if (g >= 0) c.ch_live[c.w_chunk[g] + (i - c.ws[g]) / CHUNK] = 1;
-    f3 xi = Q(c, i);
-    c.q4[i] = make_float4(xi.x, xi.y, xi.z, c.mass[i]);
-    if (c.packed && g >= 0) {  // liquid only (the rows list liquid); w is written by the vector's producer before any read
-        for (int s = 0; s < 3; ++s) c.qk[int64_t(s) * c.n + i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
-        c.qa[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
-    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int k = -1;
-    if (g != -1) {
-        int gi = g >= 0 ? g : c.W + (-2 - g);
-        int cx, cy, cz;
-        k = cell_of(c, gi, xi, cx, cy, cz);
-        c.arrive[i] = atomicAdd(c.cell_count + k, 1);
+    if (i < c.n) {
+        int g = -1;
+        int fl = c.flags[i];
+        if ((fl & ACTIVE) != 0 && int(c.phase[i]) >= 1) g = c.world[i];
+        else if (c.NW > 0 && c.wvol[i] > 0.0f && (fl & ACTIVE) != 0 && c.world[i] >= 0) g = -2 - c.world[i];
+        c.group[i] = g;
+        if (g >= 0) c.ch_live[c.w_chunk[g] + (i - c.ws[g]) / CHUNK] = 1;
+        f3 xi = Q(c, i);
+        c.q4[i] = make_float4(xi.x, xi.y, xi.z, c.mass[i]);
+        if (c.packed && g >= 0) {  // liquid only (the rows list liquid); w is written by the producer before any read
+            for (int s = 0; s < 3; ++s) c.qk[int64_t(s) * c.n + i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
+            c.qa[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
+        }
+        if (g != -1) {
+            int gi = g >= 0 ? g : c.W + (-2 - g);
+            int cx, cy, cz;
+            k = cell_of(c, gi, xi, cx, cy, cz);
+        }
+        c.key[i] = k;
     }
-    c.key[i] = k;
+    // arrival slots: consecutive lanes in one cell (neighbouring ids often share a cell) form a run whose first lane
+    // takes the run's slots with one atomic; each lane's slot is the run's base plus its place in the run
+    const int lane = int(threadIdx.x) & 63;
+    const int prev = __shfl_up(k, 1, 64);
+    const bool head = lane == 0 || prev != k;
+    const unsigned long long heads = __ballot(head);
+    const int first = 63 - __clzll(static_cast<long long>(heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull))));
+    const unsigned long long after = first == 63 ? 0ull : heads & ~((2ull << first) - 1ull);
+    const int end = after ? __ffsll(static_cast<long long>(after)) - 1 : 64;
+    int base = 0;
+    if (head && k >= 0) base = atomicAdd(c.cell_count + k, end - first);
+    base = __shfl(base, first, 64);
+    if (k >= 0) c.arrive[i] = base + (lane - first);
 }
 
 // exclusive scan of a block's 1024 elements (256 threads x 4); returns the thread's exclusive prefix, total in *tot
