@@ -2709,8 +2709,12 @@ __device__ __forceinline__ void cg_direction_chunk(const Ctx& c, int b_) {
     bool valid = i < c.ch_end[b_];
     int w = c.ch_world[b_];
     if (c.converged[w] != 0 || !valid || !c.ch_live[b_] || !is_fluid(c, i)) return;
+    const float p_old = c.pd[i];
     float p = 0.0f;
-    if (c.act[i] != 0) p = c.z[i] + float(c.stepb[w]) * c.pd[i];
+    if (c.act[i] != 0) p = c.z[i] + float(c.stepb[w]) * p_old;
+    // p and its quotients are written together, so a stored +0 that stays +0 is not written again (the packed
+    // record's w is also set to +0 at each list build)
+    if (__float_as_uint(p) == 0u && __float_as_uint(p_old) == 0u) return;
     c.pd[i] = p;
     put_vr(c, c.pdr, VR_DIR, i, p / c.rho0[i]);
 }
