@@ -160,6 +160,25 @@ __device__ __forceinline__ float kernel_grad_scale(f3 rv, const KC& K) {
     }
     return dwdq / (r * K.h);
 }
+// kernel_w(dist(rv, K), K) and kernel_grad_scale(rv, K) with the variant known to the caller (KV 0: the cubic kernel
+// at a regular h, K.fast and K.kind 0; KV 1: any), so a neighbour loop chooses once instead of per pair; the same
+// operations either way
+template <int KV>
+__device__ __forceinline__ float kernel_w_rv(f3 rv, const KC& K) {
+    if (KV == 0) {
+        const float q = div_h(sqrt_rn(dot(rv, rv)), K);
+        if (q >= 1.0f) return 0.0f;
+        if (q <= 0.5f) return K.knorm * (6.0f * q * q * q - 6.0f * q * q + 1.0f);
+        const float s = 1.0f - q;
+        return K.knorm * 2.0f * s * s * s;
+    }
+    return kernel_w(dist(rv, K), K);
+}
+template <int KV>
+__device__ __forceinline__ float kernel_grad_scale_v(f3 rv, const KC& K) {
+    if (KV == 0) return kernel_grad_scale_fast_kind<0>(rv, K);
+    return kernel_grad_scale(rv, K);
+}
 // the gradient from its scale: the same products kernel_grad forms, the exact zero vector for a zero scale
 __device__ __forceinline__ f3 pair_grad(f3 rv, float s) { return s == 0.0f ? f3{0.0f, 0.0f, 0.0f} : rv * s; }
 __device__ __forceinline__ f3 kernel_grad(f3 rv, const KC& K) { return pair_grad(rv, kernel_grad_scale(rv, K)); }

@@ -2015,19 +2015,24 @@ __device__ __forceinline__ void density_alpha_one(const Ctx& c, const KC& K, int
     const int base = c.ws[w];
     const unsigned sm = slot_mask(c, i);
     const WorldRecs q4 = WorldRecs::of(c.q4, base, c.n), qd = WorldRecs::of(reinterpret_cast<const f3*>(c.qd), base, c.n);
-    for_rows<WIDE, QVd>(
-        c, i, cnt, [&](unsigned e) { return QVd{q4.f4(e), FEAT ? qd.v3(e) : f3{0.0f, 0.0f, 0.0f}}; },
-        [&](int k, QVd d) {
-            f3 rv = xi - xyz(d.q);
-            float mj = d.q.w;
-            rho += mj * kernel_w(dist(rv, K), K);
-            float sc = kernel_grad_scale(rv, K);
-            if (c.pair_cache) c.pgrad[int64_t(k) * c.n + i] = sc;
-            f3 g = pair_grad(rv, sc);
-            G = G + (mj * inv_r0) * g;
-            sq += (mi * mj * inv_r0 * inv_r0) * dot(g, g);
-            if (FEAT) rate += (mj * inv_r0) * dot(vi - d.v, g);
-        });
+    auto loop = [&](auto kv) {  // the kernel variant chosen once for the loop (kernel_w_rv)
+        constexpr int KV = decltype(kv)::value;
+        for_rows<WIDE, QVd>(
+            c, i, cnt, [&](unsigned e) { return QVd{q4.f4(e), FEAT ? qd.v3(e) : f3{0.0f, 0.0f, 0.0f}}; },
+            [&](int k, QVd d) {
+                f3 rv = xi - xyz(d.q);
+                float mj = d.q.w;
+                rho += mj * kernel_w_rv<KV>(rv, K);
+                float sc = kernel_grad_scale_v<KV>(rv, K);
+                if (c.pair_cache) c.pgrad[int64_t(k) * c.n + i] = sc;
+                f3 g = pair_grad(rv, sc);
+                G = G + (mj * inv_r0) * g;
+                sq += (mi * mj * inv_r0 * inv_r0) * dot(g, g);
+                if (FEAT) rate += (mj * inv_r0) * dot(vi - d.v, g);
+            });
+    };
+    if (K.fast && K.kind == 0) loop(std::integral_constant<int, 0>{});
+    else loop(std::integral_constant<int, 1>{});
     float psi = 0.0f, lam = 0.0f;
     int64_t b4 = 4 * int64_t(i);
     for (int b = 0; b < 4; ++b) {
@@ -3037,15 +3042,20 @@ __global__ void __launch_bounds__(256) k_nonpressure(Ctx c, float dt) {
         int cnt = c.count[i];
         const int base = c.ws[w];
         const WorldRecs q4 = WorldRecs::of(c.q4, base, c.n), vr = WorldRecs::of(c.vrho, base, c.n);
-        for_rows<WIDE, XS>(
-            c, i, cnt, [&](unsigned e) { return XS{q4.f4(e), vr.f4(e)}; },
-            [&](int, XS d) {
-                float rho_bar = 0.5f * (rho_i + d.vr.w);
-                if (rho_bar > 0.0f) {
-                    float wk = kernel_w(dist(xi - xyz(d.q), K), K);
-                    cr = cr + (d.q.w / rho_bar * wk) * (xyz(d.vr) - vi);
-                }
-            });
+        auto loop = [&](auto kv) {  // the kernel variant chosen once for the loop (kernel_w_rv)
+            constexpr int KV = decltype(kv)::value;
+            for_rows<WIDE, XS>(
+                c, i, cnt, [&](unsigned e) { return XS{q4.f4(e), vr.f4(e)}; },
+                [&](int, XS d) {
+                    float rho_bar = 0.5f * (rho_i + d.vr.w);
+                    if (rho_bar > 0.0f) {
+                        float wk = kernel_w_rv<KV>(xi - xyz(d.q), K);
+                        cr = cr + (d.q.w / rho_bar * wk) * (xyz(d.vr) - vi);
+                    }
+                });
+        };
+        if (K.fast && K.kind == 0) loop(std::integral_constant<int, 0>{});
+        else loop(std::integral_constant<int, 1>{});
         v = v + c.xsph * cr;
     }
     // in place: the XSPH sums gather the velocities from the packed copy k_pack_vrho made before this pass
