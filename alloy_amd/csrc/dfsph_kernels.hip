@@ -1404,42 +1404,47 @@ __device__ void pw_sample(const Ctx& c, f3 x, int w, float& col, f3& gc, f3& mv,
         a1 = c.cell_start[cc + 1];
     };
     int n0, n1;
-    cell_range(0, n0, n1);
-    for (int r = 0; r < 27; ++r) {
-        const int b0 = n0, e1 = n1;
-        if (r < 26) cell_range(r + 1, n0, n1);
-        // q4s: the grid's cell-ordered copy of q4 with the particle id in w (the same positions; one
-        // contiguous load per candidate, PW_U of them in flight, read unclamped from the padded array, then
-        // taken in order). Outside the support W = 0 makes V_j W zero, so V_j is read for the candidates
-        // within h only.
-        for (int e0 = b0; e0 < e1; e0 += PW_U) {
-            float4 pb[PW_U];
-#pragma unroll
-            for (int u = 0; u < PW_U; ++u) pb[u] = c.q4s[e0 + u];
-#pragma unroll
-            for (int u = 0; u < PW_U; ++u) {
-                if (e0 + u >= e1) continue;
-                const float4 p4 = pb[u];
-                f3 dd = x - f3{p4.x, p4.y, p4.z};
-                if (dot(dd, dd) > out2) continue;  // kernel_w is 0 there
-                float kw = kernel_w(dist(dd, K), K);
-                if (!(kw > 0.0f)) continue;
-                int j = __float_as_int(p4.w);
-                float vj = c.wvol[j];
-                float wj = vj * kw;
-                if (wj > 0.0f) {
-                    col += wj;
-                    gc = gc + vj * kernel_grad(dd, K);
-                    mv = mv + wj * V(c, j);
-                    if (rec >= 0 && nrec < PW_ROW) {
-                        c.wr_id[int64_t(nrec) * c.n + rec] = c.wcidx[j];
-                        c.wr_w[int64_t(nrec) * c.n + rec] = wj;
+    auto walk = [&](auto kv) {  // the kernel variant chosen once for the walk (kernel_w_rv)
+        constexpr int KV = decltype(kv)::value;
+        cell_range(0, n0, n1);
+        for (int r = 0; r < 27; ++r) {
+            const int b0 = n0, e1 = n1;
+            if (r < 26) cell_range(r + 1, n0, n1);
+            // q4s: the grid's cell-ordered copy of q4 with the particle id in w (the same positions; one
+            // contiguous load per candidate, PW_U of them in flight, read unclamped from the padded array, then
+            // taken in order). Outside the support W = 0 makes V_j W zero, so V_j is read for the candidates
+            // within h only.
+            for (int e0 = b0; e0 < e1; e0 += PW_U) {
+                float4 pb[PW_U];
+    #pragma unroll
+                for (int u = 0; u < PW_U; ++u) pb[u] = c.q4s[e0 + u];
+    #pragma unroll
+                for (int u = 0; u < PW_U; ++u) {
+                    if (e0 + u >= e1) continue;
+                    const float4 p4 = pb[u];
+                    f3 dd = x - f3{p4.x, p4.y, p4.z};
+                    if (dot(dd, dd) > out2) continue;  // kernel_w is 0 there
+                    float kw = kernel_w_rv<KV>(dd, K);
+                    if (!(kw > 0.0f)) continue;
+                    int j = __float_as_int(p4.w);
+                    float vj = c.wvol[j];
+                    float wj = vj * kw;
+                    if (wj > 0.0f) {
+                        col += wj;
+                        gc = gc + vj * pair_grad(dd, kernel_grad_scale_v<KV>(dd, K));
+                        mv = mv + wj * V(c, j);
+                        if (rec >= 0 && nrec < PW_ROW) {
+                            c.wr_id[int64_t(nrec) * c.n + rec] = c.wcidx[j];
+                            c.wr_w[int64_t(nrec) * c.n + rec] = wj;
+                        }
+                        nrec += 1;
                     }
-                    nrec += 1;
                 }
             }
         }
-    }
+    };
+    if (K.fast && K.kind == 0) walk(std::integral_constant<int, 0>{});
+    else walk(std::integral_constant<int, 1>{});
     if (rec >= 0) c.wr_n[rec] = nrec <= PW_ROW ? nrec : -1;
     if (col > 0.0f) mv = mv / col;
 }
