@@ -600,9 +600,13 @@ __device__ void world_body(const Ctx& c, int w, int mode, const StopSpec& sp) {
         b += c.part[3 * int64_t(k) + 1];
         m = fmax(m, c.part[3 * int64_t(k) + 2]);
     }
+    // the second sum and the max only for the modes that read them (the chunk partials of the others carry +0 there)
+    const bool need_b = !(mode == W_NACTIVE || mode == W_CG_INIT || mode == W_CG_PAP || mode == W_JACOBI ||
+                          mode == W_VISC_ALPHA);
+    const bool need_m = mode == W_CG_BETA || mode == W_JACOBI;
     double sa = tree_sum(a, lds);
-    double sb = tree_sum(b, lds);
-    double mx = tree_max(m, lds);
+    double sb = need_b ? tree_sum(b, lds) : 0.0;
+    double mx = need_m ? tree_max(m, lds) : -3.4e38;
     if (threadIdx.x != 0) return;
     switch (mode) {
         case W_NACTIVE:
