@@ -183,7 +183,7 @@ struct Ctx {
     float *pd, *z, *wv, *ap, *corr;
     float *kr, *pdr, *corrr;  // kappa / rho0, pd / rho0, corr / rho0: the quotients pressure_accel gathers
     uint8_t* touched;         // gated accel pass: 1 where the acceleration was evaluated (not the exact-zero skip)
-    int* act;
+    uint8_t* act;  // [n] the projected CG's active set (1: the particle's kappa is free), per solve
     // per world
     int *converged, *iters, *restart;
     float *err_mean, *err_max, *tol_scale;
